@@ -1,0 +1,203 @@
+/*
+ * rgp.h - C ABI of the MI355X (gfx950) palette-weight Gaussian rasterizer.
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream, and
+ * returns RGP_OK (0) or a negative error code; rgp_last_error() describes the
+ * last failure of the calling thread.  No entry point allocates device memory:
+ * buffers beyond the outputs come in through a caller-provided workspace
+ * (rgp_bin_workspace_bytes).  All kernels are enqueued on the given stream and
+ * no entry point synchronises the host, so sequences of calls can be captured
+ * into a HIP graph.  Pointer arguments marked "nullable" may be NULL.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (tedchao/ReparamGS-Palette, gsplatcu/ext.cpp bindings implemented in
+ * gsplatcu/gausplat.cu and gsplatcu/kernel.cu).
+ */
+#ifndef RGP_H_
+#define RGP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RGP_OK 0
+#define RGP_ERR_ARG (-1)    /* invalid argument (size, K, pointer) */
+#define RGP_ERR_LAUNCH (-2) /* a kernel launch failed */
+
+#define RGP_MAX_DIM 8 /* MAX_SPLAT_DIM (kernel.cu:12): channels per Gaussian */
+
+/* Camera for the fused entry points, resident in device memory so that a
+ * captured graph can be replayed with a new pose.  Fields follow the
+ * reference Camera (gsplat/gausplat_dataset.py:21-33); tan_fovx/y are
+ * width/(2 fx) and height/(2 fy) as gausplat.cu:357-358 computes them. */
+typedef struct rgp_camera {
+  float Rcw[9];
+  float tcw[3];
+  float twc[3];
+  float fx, fy, cx, cy;
+  float tan_fovx, tan_fovy;
+  int32_t width, height;
+  int32_t pad[2];
+} rgp_camera;
+
+const char* rgp_last_error(void);
+int rgp_version(void);
+
+/* ---- per-Gaussian geometry (palette path calls them with calc_J = False) -- */
+
+/* gsc.project, ext.cpp:94-101 / gausplat.cu:381-419 / kernel.cu:618-663.
+ * pws (N,3); Rcw (3,3), tcw (3) device.  Outputs us (N,2), pcs (N,3),
+ * depths (N) must be zero-filled by the caller (the reference allocates them
+ * with torch::full 0); culled Gaussians get depth -1 and keep zeros. */
+int rgp_project(int n, const float* pws, const float* Rcw, const float* tcw, float fx, float fy,
+                float cx, float cy, float* us, float* pcs, float* depths, hipStream_t stream);
+
+/* gsc.computeCov3D, ext.cpp:78-81 / gausplat.cu:298-331 / kernel.cu:422-501.
+ * rots (N,4) (w,x,y,z), scales (N,3) linear -> cov3ds (N,6) (zero-filled). */
+int rgp_compute_cov3d(int n, const float* rots, const float* scales, const float* depths,
+                      float* cov3ds, hipStream_t stream);
+
+/* gsc.computeCov2D, ext.cpp:83-91 / gausplat.cu:333-379 / kernel.cu:508-611. */
+int rgp_compute_cov2d(int n, const float* cov3ds, const float* pcs, const float* Rcw,
+                      const float* depths, float fx, float fy, float width, float height,
+                      float* cov2ds, hipStream_t stream);
+
+/* gsc.inverseCov2D, ext.cpp:73-75 / gausplat.cu:590-617 / kernel.cu:379-415.
+ * depths is updated in place (NaN determinant -> -1). */
+int rgp_inverse_cov2d(int n, const float* cov2ds, float* depths, float* cinv2ds, int32_t* areas,
+                      hipStream_t stream);
+
+/* gsc.sh2Weight, ext.cpp:103-107 / gausplat.cu:425-462 / kernel.cu:670-870.
+ * shs (N, sh_dim*K) band-major; twc (3) device.  basis (N, sh_dim) nullable
+ * (the calc_J output dweight_dshs); the discarded dW/dpw is not produced. */
+int rgp_sh2weight(int n, int K, int sh_dim, const float* shs, const float* pws, const float* twc,
+                  float* weights, float* basis, hipStream_t stream);
+
+/* gsc.sh2Weight_fast, ext.cpp:116-124 / gausplat.cu:469-509 / kernel.cu:884-987,
+ * generalised from rank 1 to rank r: A (N,P,r), B (N,r,Q), P*Q = 15*K.
+ * dweight_ddc (N,K) and ylm (N,15) nullable (calc_J outputs). */
+int rgp_sh2weight_fast(int n, int K, int P, int Q, int rank, const float* dc, const float* A,
+                       const float* B, const float* pws, const float* twc, float* weights,
+                       float* dweight_ddc, float* ylm, hipStream_t stream);
+
+/* gsc.sh2Weight_fast_B, ext.cpp:127-133 / gausplat.cu:512-541 /
+ * kernel.cu:994-1032, rank r.  dA (N,P,r), dB (N,r,Q) fully overwritten. */
+int rgp_sh2weight_fast_backward(int n, int K, int P, int Q, int rank, const float* A,
+                                const float* B, const float* ylm, const float* dloss_dw,
+                                float* dA, float* dB, hipStream_t stream);
+
+/* gsc.sh2Color, ext.cpp:109-112 / gausplat.cu:547-584 / kernel.cu:1040-1169
+ * (colour path, used to render frozen targets). */
+int rgp_sh2color(int n, int sh_dim, const float* shs, const float* pws, const float* twc,
+                 float* colors, hipStream_t stream);
+
+/* ---- binning: getRects + scan + createKeys + sort + getRanges ----------- */
+
+/* Bytes of workspace for n Gaussians, a pair capacity and a tile grid. */
+size_t rgp_bin_workspace_bytes(int n, int64_t capacity, int width, int height);
+
+/* getRects (kernel.cu:127-162, gausplat.cu:189-197) into the workspace:
+ * rects and per-Gaussian tile counts; culls in place (depths -1, areas 0). */
+int rgp_bin_rects(int n, const float* us, int32_t* areas, float* depths, int width, int height,
+                  void* ws, int64_t capacity, hipStream_t stream);
+
+/* inclusive scan of the counts (gausplat.cu:199-201).  The pair count M and
+ * an overflow flag (M > capacity) are left in the workspace header:
+ * rgp_bin_header() gives their device addresses. */
+int rgp_bin_scan(int n, int width, int height, void* ws, int64_t capacity, hipStream_t stream);
+
+/* createKeys + stable (tile, uint32(depth*1000)) sort + getRanges
+ * (kernel.cu:96-187, gausplat.cu:203-226).  ids_out (capacity) receives the
+ * Gaussian id of every sorted pair, keys_out (capacity, nullable) the sorted
+ * keys, ranges_out (tiles, 2) the [start, end) of every tile ((0,0) when the
+ * tile is empty, and for every tile when M == 1 as in the reference).  When
+ * M > capacity all ranges are (0,0) and the overflow flag is set. */
+int rgp_bin_sort(int n, const float* depths, int width, int height, void* ws, int64_t capacity,
+                 int32_t* ids_out, uint64_t* keys_out, int32_t* ranges_out, hipStream_t stream);
+
+/* Device addresses of the header words: [0] M (u32), [1] overflow flag. */
+const uint32_t* rgp_bin_header(const void* ws);
+
+/* ---- splat records and compositing --------------------------------------- */
+
+/* Floats per splat record for dim channels: 8 + 4*ceil(dim/4). */
+int rgp_record_floats(int dim);
+
+/* Pack us (N,2), cinv2ds (N,3), alphas (N), feat (N,dim) into records. */
+int rgp_pack_records(int n, int dim, const float* us, const float* cinv2ds, const float* alphas,
+                     const float* feat, float* records, hipStream_t stream);
+
+/* draw_weight / draw_color (kernel.cu:194-372, gausplat.cu:89-100, 228-240):
+ * per-tile front-to-back compositing of dim channels.  Writes every pixel:
+ * layer (dim,H,W), contrib (H,W), final_tau (H,W); empty tiles get 0, 0 and
+ * empty_tau (0 for splat_weight, gausplat.cu:178; 1 for splat_color, :39). */
+int rgp_draw_forward(int width, int height, int dim, const int32_t* ranges, const int32_t* ids,
+                     const float* records, float empty_tau, float* layer, int32_t* contrib,
+                     float* final_tau, hipStream_t stream);
+
+/* draw_weight_B (kernel.cu:1291-1379, gausplat.cu:252-292): reverse-order
+ * backward of the channel compositing.  dfeat (N, dim) must be zero-filled;
+ * gradients are added with wavefront reductions then one atomic per
+ * (wavefront, Gaussian, channel). */
+int rgp_draw_backward(int width, int height, int dim, const int32_t* ranges, const int32_t* ids,
+                      const float* records, const int32_t* contrib, const float* final_tau,
+                      const float* dlayer, float* dfeat, hipStream_t stream);
+
+/* ---- fused paths (no reference counterpart as single calls; they produce
+ * the same values as the reference chains they replace) ------------------ */
+
+/* project + computeCov3D + computeCov2D + sh2Weight(_fast) + inverseCov2D +
+ * getRects of WeightGSFunctionTest/Fast.forward (gsmodel.py:107-118 /
+ * 155-170) in one pass.  mode 0: full rank, shs (N, P*K) with P the number
+ * of SH coefficients per channel (1, 4, 9 or 16; Q and rank unused); mode 1:
+ * low rank, dc (N,K), A (N,P,r), B (N,r,Q).  Writes depths (N, with -1 culls), records
+ * (N, rgp_record_floats(K)), and rects/counts into ws.  us/cinv2ds/areas/
+ * weights (nullable) receive the unfused outputs for inspection. */
+int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int rank,
+                          const rgp_camera* cam, const float* pws, const float* rots,
+                          const float* scales, const float* alphas, const float* shs,
+                          const float* dc, const float* A, const float* B, float* depths,
+                          float* records, void* ws, int64_t capacity, int width, int height,
+                          float* us, float* cinv2ds, int32_t* areas, float* weights,
+                          hipStream_t stream);
+
+/* Full-rank SH backward of gsmodel.py:140-143: dshs[n, lm*K+k] =
+ * dw[n,k] * Y_lm(n), Y recomputed from pws and twc (device camera). */
+int rgp_sh_backward_full(int n, int K, int sh_dim, const rgp_camera* cam, const float* pws,
+                         const float* dw, float* dshs, hipStream_t stream);
+
+/* Low-rank SH backward of gsmodel.py:194-198 (dL/ddc = dw * SH_C0 and
+ * sh2Weight_fast_B), Y recomputed from pws and twc. */
+int rgp_sh_backward_lowrank(int n, int K, int P, int Q, int rank, const rgp_camera* cam,
+                            const float* pws, const float* A, const float* B, const float* dw,
+                            float* ddc, float* dA, float* dB, hipStream_t stream);
+
+/* ---- palette mix (no reference kernel; equals bake_palette_edit +
+ * render_vanilla, gsmodel.py:488-509 + 13-26, by linearity) -------------- */
+
+/* rgb (3,H,W) = sum_k palette[k] * W[k] for a batch of `count` images:
+ * W pointers, rgb pointers, pixel counts and palettes (count, K, 3) in
+ * device arrays.  One launch serves every image of the batch. */
+int rgp_mix_batched(int count, int K, const float* const* W, float* const* rgb,
+                    const int64_t* hw, const float* palettes, int64_t max_hw, hipStream_t stream);
+
+/* Single image convenience form. palette (K,3) device. */
+int rgp_mix(int K, int64_t hw, const float* W, const float* palette, float* rgb,
+            hipStream_t stream);
+
+/* Backward of the mix: dW[k,p] = sum_c palette[k,c] drgb[c,p];
+ * dpalette[k,c] += sum_p W[k,p] drgb[c,p] (dpalette zero-filled by caller). */
+int rgp_mix_backward(int K, int64_t hw, const float* W, const float* palette, const float* drgb,
+                     float* dW, float* dpalette, hipStream_t stream);
+
+/* Fill n 32-bit words with a value (graph-capture safe, no memset node). */
+int rgp_fill_u32(void* dst, uint32_t value, int64_t n, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RGP_H_ */

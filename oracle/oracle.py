@@ -1,0 +1,321 @@
+"""CPU oracle of the palette-weight rasterizer: TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may
+import this module.  It wraps oracle/rgp_oracle.cpp (a C++ restatement of the
+reference gsplatcu arithmetic, FP32 and FP64) through ctypes and composes the
+reference call chains of gsplat/gsmodel.py:104-200 on numpy arrays.
+
+How each function is pinned (tests/test_oracle.py):
+  rgp_expf            against libm exp in double: max error <= 2 ulp over a
+                      dense sweep of [-104, 88.7]; exact at 0 and on 2^n.
+  project             closed form: an unprojection of (u, depth) returns the
+                      camera-space point; z < 0.2 culls; FP32 vs FP64.
+  cov3d               det(Sigma) = (s0 s1 s2)^2 for unit quaternions, Sigma
+                      equals R diag(s^2) R^T built by numpy in float64, PSD.
+  cov2d               equals J W Sigma W^T J^T + 0.3 I built by numpy in
+                      float64 (with the +-1.3 tan_fov clamp), symmetric PSD.
+  inverse_cov2d       cinv . cov = I; areas = ceil(3 sqrt(diag)); a NaN
+                      determinant culls (depth -1) but det = 0 does not.
+  sh2weight           SH basis: addition theorem sum_m Y_lm^2 = (2l+1)/4pi per
+                      band; weights equal basis @ coefficients in float64.
+  sh2weight_fast      equals sh2weight on cat(dc, reconstruct(A, B)) (the
+                      reference's fused/unfused pair, gsmodel.py:303-316) and,
+                      at rank 1, A (N,P,1) / B (N,1,Q) reduce to (N,P) / (N,Q).
+  sh2weight_fast_bwd  adjoint identity <dw, J dtheta> = <J^T dw, dtheta> and
+                      float64 central finite differences.
+  get_rects/bin_sort  brute force on tiny inputs: per-Gaussian tile sets from
+                      an independent enumeration, a Python stable sort of
+                      (tile, uint32(depth*1000)) keys, ranges by scanning,
+                      including M = 0 and M = 1 (the reference leaves (0,0)).
+  draw (forward)      pure-Python per-pixel recurrence on tiny scenes (bit
+                      exact in float32); with all weights 1 every channel
+                      equals 1 - final_tau up to rounding; FP32 vs FP64.
+  draw_backward       float64 central finite differences of sum(dL * layer)
+                      w.r.t. the per-Gaussian weights (exactly linear).
+  mix                 linear identity mix(render(w)) = render(bake(w, P))
+                      (gsmodel.py:488-509) and a numpy einsum.
+No function of this module is left unpinned.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "librgp_oracle.so")
+_lib = None
+
+f32, f64, i32, u32, u64, i64 = np.float32, np.float64, np.int32, np.uint32, np.uint64, np.int64
+BLOCK = 16
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        _lib = ctypes.CDLL(_SO)
+        _lib.orc_expf.restype = ctypes.c_float
+        _lib.orc_expf.argtypes = [ctypes.c_float]
+        for s in ("f", "d"):
+            getattr(_lib, "orc_bin_sort_" + s).restype = ctypes.c_int64
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _sfx(dt):
+    return "f" if dt == f32 else "d"
+
+
+def _scal(dt, v):
+    return ctypes.c_float(v) if dt == f32 else ctypes.c_double(v)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def set_threads(t):
+    lib().orc_set_num_threads(int(t))
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def expf(x):
+    x = _c(x, f32)
+    y = np.empty_like(x)
+    lib().orc_expf_array(ctypes.c_int64(x.size), _p(x), _p(y))
+    return y
+
+
+# --------------------------------------------------------------------------
+# per-stage wrappers (shapes follow gsplatcu/ext.cpp:10-124)
+# --------------------------------------------------------------------------
+def project(pws, Rcw, tcw, fx, fy, cx, cy, dt=f32):
+    pws, R, t = _c(pws, dt), _c(Rcw, dt), _c(tcw, dt)
+    n = pws.shape[0]
+    us, pcs, depths = np.zeros((n, 2), dt), np.zeros((n, 3), dt), np.zeros(n, dt)
+    getattr(lib(), "orc_project_" + _sfx(dt))(n, _p(pws), _p(R), _p(t), _scal(dt, fx), _scal(dt, fy),
+                                              _scal(dt, cx), _scal(dt, cy), _p(us), _p(pcs), _p(depths))
+    return us, pcs, depths
+
+
+def cov3d(rots, scales, depths, dt=f32):
+    rots, scales, depths = _c(rots, dt), _c(scales, dt), _c(depths, dt)
+    n = rots.shape[0]
+    out = np.zeros((n, 6), dt)
+    getattr(lib(), "orc_cov3d_" + _sfx(dt))(n, _p(rots), _p(scales), _p(depths), _p(out))
+    return out
+
+
+def tan_fov(width, height, fx, fy, dt=f32):
+    """gausplat.cu:357-358: width / (2 * focal_x) in the arithmetic type."""
+    return dt(dt(width) / (dt(2) * dt(fx))), dt(dt(height) / (dt(2) * dt(fy)))
+
+
+def cov2d(cov3ds, pcs, Rcw, depths, fx, fy, width, height, dt=f32):
+    cov3ds, pcs, R, depths = _c(cov3ds, dt), _c(pcs, dt), _c(Rcw, dt), _c(depths, dt)
+    n = pcs.shape[0]
+    out = np.zeros((n, 3), dt)
+    tfx, tfy = tan_fov(width, height, fx, fy, dt)
+    getattr(lib(), "orc_cov2d_" + _sfx(dt))(n, _p(cov3ds), _p(pcs), _p(R), _p(depths), _scal(dt, fx),
+                                            _scal(dt, fy), _scal(dt, float(tfx)), _scal(dt, float(tfy)), _p(out))
+    return out
+
+
+def inverse_cov2d(cov2ds, depths, dt=f32):
+    """Returns cinv2ds, areas; depths is updated in place (kernel.cu:397)."""
+    cov2ds = _c(cov2ds, dt)
+    assert depths.dtype == dt and depths.flags.c_contiguous
+    n = cov2ds.shape[0]
+    cinv, areas = np.zeros((n, 3), dt), np.zeros((n, 2), i32)
+    getattr(lib(), "orc_inverse_cov2d_" + _sfx(dt))(n, _p(cov2ds), _p(depths), _p(cinv), _p(areas))
+    return cinv, areas
+
+
+def sh2weight(shs, pws, twc, K, dt=f32):
+    shs, pws, twc = _c(shs, dt), _c(pws, dt), _c(twc, dt)
+    n = pws.shape[0]
+    sh_dim = shs.shape[1] // K
+    w, basis = np.zeros((n, K), dt), np.zeros((n, sh_dim), dt)
+    getattr(lib(), "orc_sh2weight_" + _sfx(dt))(n, K, sh_dim, _p(shs), _p(pws), _p(twc), _p(w), _p(basis))
+    return w, basis
+
+
+def _factor_shapes(A, B):
+    if A.ndim == 2:
+        A = A[:, :, None]
+    if B.ndim == 2:
+        B = B[:, None, :]
+    return A, B
+
+
+def sh2weight_fast(dc, A, B, pws, twc, K, Q, dt=f32):
+    A, B = _factor_shapes(A, B)
+    dc, A, B, pws, twc = _c(dc, dt), _c(A, dt), _c(B, dt), _c(pws, dt), _c(twc, dt)
+    n, P, r = A.shape
+    L = P * Q // K
+    w, ylm = np.zeros((n, K), dt), np.zeros((n, L), dt)
+    getattr(lib(), "orc_sh2weight_fast_" + _sfx(dt))(n, K, P, Q, r, _p(dc), _p(A), _p(B), _p(pws), _p(twc),
+                                                     _p(w), _p(ylm))
+    return w, ylm
+
+
+def sh2weight_fast_backward(A, B, ylm, dw, K, Q, dt=f32):
+    A2, B2 = _factor_shapes(A, B)
+    A2, B2, ylm, dw = _c(A2, dt), _c(B2, dt), _c(ylm, dt), _c(dw, dt)
+    n, P, r = A2.shape
+    dA, dB = np.zeros_like(A2), np.zeros_like(B2)
+    getattr(lib(), "orc_sh2weight_fast_backward_" + _sfx(dt))(n, K, P, Q, r, _p(A2), _p(B2), _p(ylm), _p(dw),
+                                                              _p(dA), _p(dB))
+    return dA.reshape(A.shape), dB.reshape(B.shape)
+
+
+def sh2color(shs, pws, twc, dt=f32):
+    shs, pws, twc = _c(shs, dt), _c(pws, dt), _c(twc, dt)
+    n = pws.shape[0]
+    sh_dim = shs.shape[1] // 3
+    out = np.zeros((n, 3), dt)
+    getattr(lib(), "orc_sh2color_" + _sfx(dt))(n, sh_dim, _p(shs), _p(pws), _p(twc), _p(out))
+    return out
+
+
+def grid_dims(width, height):
+    return (width + BLOCK - 1) // BLOCK, (height + BLOCK - 1) // BLOCK
+
+
+def get_rects(us, areas, depths, width, height, dt=f32):
+    """rects (N,4) u32 and counts (N,) u32; areas/depths updated in place."""
+    us = _c(us, dt)
+    assert areas.dtype == i32 and depths.dtype == dt
+    n = us.shape[0]
+    gx, gy = grid_dims(width, height)
+    rects, counts = np.zeros((n, 4), u32), np.zeros(n, u32)
+    getattr(lib(), "orc_get_rects_" + _sfx(dt))(n, _p(us), _p(areas), _p(depths), gx, gy, _p(rects), _p(counts))
+    return rects, counts
+
+
+def bin_sort(depths, rects, counts, width, height, dt=f32):
+    """Returns (keys (M,) u64, ids (M,) i32, ranges (T,2) i32)."""
+    depths = _c(depths, dt)
+    n = depths.shape[0]
+    gx, gy = grid_dims(width, height)
+    M = int(counts.astype(np.uint64).sum())
+    keys, ids = np.zeros(max(M, 1), u64), np.zeros(max(M, 1), i32)
+    ranges = np.zeros((gx * gy, 2), i32)
+    got = getattr(lib(), "orc_bin_sort_" + _sfx(dt))(n, _p(depths), _p(_c(rects, u32)), _p(_c(counts, u32)), gx, gy,
+                                                     _p(keys), _p(ids), ctypes.c_int64(M), _p(ranges))
+    assert got == M
+    return keys[:M], ids[:M], ranges
+
+
+def draw(width, height, ranges, ids, us, cinv, alphas, feat, final_tau_init=0.0, dt=f32):
+    """draw_weight (feat (N,K)) / draw_color (feat (N,3)).  Returns
+    (layer (dim,H,W), contrib (H,W), final_tau (H,W), n_eval, n_contrib)."""
+    feat = _c(feat, dt)
+    dim = feat.shape[1]
+    layer = np.zeros((dim, height, width), dt)
+    contrib = np.zeros((height, width), i32)
+    final_tau = np.full((height, width), final_tau_init, dt)
+    ne, nc = ctypes.c_int64(0), ctypes.c_int64(0)
+    getattr(lib(), "orc_draw_" + _sfx(dt))(width, height, dim, _p(_c(ranges, i32)), _p(_c(ids, i32)), _p(_c(us, dt)),
+                                           _p(_c(cinv, dt)), _p(_c(alphas, dt).reshape(-1)), _p(feat), _p(layer),
+                                           _p(contrib), _p(final_tau), ctypes.byref(ne), ctypes.byref(nc))
+    return layer, contrib, final_tau, ne.value, nc.value
+
+
+def draw_backward(width, height, K, ranges, ids, us, cinv, alphas, contrib, final_tau, dlayer, dt=f32, mode=0):
+    """Returns (grad (N,K) float64, gabs (N,K) float64) for mode 0; for mode 1
+    the gradient accumulated with atomic adds in the arithmetic type."""
+    n = us.shape[0]
+    grad, gabs = np.zeros((n, K), f64), np.zeros((n, K), f64)
+    grad_t = np.zeros((n, K), dt)
+    getattr(lib(), "orc_draw_backward_" + _sfx(dt))(
+        width, height, K, _p(_c(ranges, i32)), _p(_c(ids, i32)), _p(_c(us, dt)), _p(_c(cinv, dt)),
+        _p(_c(alphas, dt).reshape(-1)), _p(_c(contrib, i32)), _p(_c(final_tau, dt)), _p(_c(dlayer, dt)), mode,
+        _p(grad), _p(gabs), _p(grad_t))
+    return (grad, gabs) if mode == 0 else grad_t
+
+
+def mix(palette, W, dt=f32):
+    W = _c(W, dt)
+    K = W.shape[0]
+    HW = W.shape[1] * W.shape[2]
+    rgb = np.zeros((3,) + W.shape[1:], dt)
+    getattr(lib(), "orc_mix_" + _sfx(dt))(K, ctypes.c_int64(HW), _p(_c(palette, dt)), _p(W), _p(rgb))
+    return rgb
+
+
+# --------------------------------------------------------------------------
+# reference call chains (gsplat/gsmodel.py:104-200)
+# --------------------------------------------------------------------------
+def forward_weight(scene, cam, dt=f32, fast=None):
+    """The forward of WeightGSFunctionTest (full rank) or WeightGSFunctionFast
+    (low rank): project -> cov3d -> cov2d -> sh2Weight(_fast) -> inverseCov2D
+    -> splat_weight.  Returns a dict with every intermediate."""
+    K = scene.K
+    if fast is None:
+        fast = scene.lowrank
+    us, pcs, depths = project(scene.pws, cam.Rcw, cam.tcw, cam.fx, cam.fy, cam.cx, cam.cy, dt)
+    c3 = cov3d(scene.rots, scene.scales, depths, dt)
+    c2 = cov2d(c3, pcs, cam.Rcw, depths, cam.fx, cam.fy, cam.width, cam.height, dt)
+    out = dict(K=K, width=cam.width, height=cam.height, dt=dt, fast=fast)
+    if fast:
+        w, ylm = sh2weight_fast(scene.dc, scene.A, scene.B, scene.pws, cam.twc, K, scene.Q, dt)
+        out["ylm"] = ylm
+    else:
+        shs = scene.shs if not scene.lowrank else np.concatenate(
+            [scene.dc, _recon(scene)], axis=1)
+        w, basis = sh2weight(shs, scene.pws, cam.twc, K, dt)
+        out["basis"] = basis
+    cinv, areas = inverse_cov2d(c2, depths, dt)
+    rects, counts = get_rects(us, areas, depths, cam.width, cam.height, dt)
+    keys, ids, ranges = bin_sort(depths, rects, counts, cam.width, cam.height, dt)
+    layer, contrib, final_tau, ne, nc = draw(cam.width, cam.height, ranges, ids, us, cinv, scene.alphas, w, 0.0, dt)
+    out.update(us=us, pcs=pcs, depths=depths, cov3ds=c3, cov2ds=c2, weights=w, cinv=cinv, areas=areas,
+               rects=rects, counts=counts, keys=keys, ids=ids, ranges=ranges, layer=layer, contrib=contrib,
+               final_tau=final_tau, mask=depths > f32(0.2), n_eval=ne, n_contrib=nc)
+    return out
+
+
+def _recon(scene):
+    A, B = _factor_shapes(scene.A, scene.B)
+    n, P, r = A.shape
+    C = np.zeros((n, P, B.shape[2]), f32)
+    for j in range(r):
+        C = C + A[:, :, j:j + 1] * B[:, j:j + 1, :]
+    return C.reshape(n, -1)
+
+
+def backward_weight(scene, fwd, dlayer):
+    """The backward of WeightGSFunctionTest / Fast: splat_weight_B, then the
+    SH chain.  Returns a dict: dweights (N,K) float64 plus its abs-sum bound,
+    and dshs (full rank) or ddc, dA, dB (low rank), in float64."""
+    dt = fwd["dt"]
+    K = fwd["K"]
+    g, gabs = draw_backward(fwd["width"], fwd["height"], K, fwd["ranges"], fwd["ids"], fwd["us"], fwd["cinv"],
+                            scene.alphas, fwd["contrib"], fwd["final_tau"], dlayer, dt)
+    out = dict(dweights=g, dweights_abs=gabs)
+    if fwd["fast"]:
+        C0 = f64(0.28209479177387814) if dt == f64 else f64(f32(0.28209479177387814))
+        out["ddc"] = g * C0
+        out["ddc_abs"] = gabs * C0
+        dA, dB = sh2weight_fast_backward(scene.A, scene.B, fwd["ylm"], g, K, scene.Q, f64)
+        aA, aB = sh2weight_fast_backward(np.abs(scene.A), np.abs(scene.B), np.abs(fwd["ylm"]), gabs, K, scene.Q, f64)
+        out.update(dA=dA, dB=dB, dA_abs=aA, dB_abs=aB)
+    else:
+        basis = fwd["basis"].astype(f64)
+        n, L = basis.shape
+        out["dshs"] = (g[:, None, :] * basis[:, :, None]).reshape(n, L * K)
+        out["dshs_abs"] = (gabs[:, None, :] * np.abs(basis)[:, :, None]).reshape(n, L * K)
+    return out

@@ -1,0 +1,697 @@
+// CPU oracle for the palette-weight Gaussian rasterizer.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, on the CPU, the arithmetic of
+// the reference `gsplatcu` extension (gsplatcu/kernel.cu, gsplatcu/gausplat.cu,
+// gsplatcu/common.cuh, gsplatcu/matrix.cuh) so that the HIP kernels can be
+// checked against it.  Only tests, the smoke check and the CPU-baseline leg of
+// the benchmark may load it; the product path never does.
+//
+// Every function is a template over the arithmetic type T:
+//   T = float  : the reference's FP32 arithmetic, operation by operation, in
+//                source order, without contraction (built with
+//                -ffp-contract=off).  Where the reference multiplies a float by
+//                a `double` SH constant (common.cuh:29-44 are double literals),
+//                the product is formed in double and rounded to float, exactly
+//                as the C++ usual arithmetic conversions prescribe.
+//   T = double : the same formulas in FP64, used as the accuracy reference.
+// exp(): the reference calls the float overload of exp (kernel.cu:352, 1367).
+// The FP32 oracle substitutes rgp_expf below (Cody-Waite reduction + degree-7
+// Taylor polynomial evaluated with fmaf), which the HIP kernels implement with
+// the same operations, so FP32 CPU and GPU agree bit for bit.  Its accuracy
+// against libm is pinned by the tests.
+//
+// Parity status per function is recorded in the header of oracle/oracle.py.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// constants (common.cuh:13-15, 29-44; kernel.cu:10-12)
+// ---------------------------------------------------------------------------
+static const int BLOCK = 16;
+static const int BLOCK_SIZE = BLOCK * BLOCK;
+static const double MIN_DEPTH = 0.2;  // a double literal: comparisons promote
+static const float BAD_MARKER = -1.f;
+static const int MAX_SPLAT_DIM = 8;
+
+static const double SH_C0_0 = 0.28209479177387814;
+static const double SH_C1_0 = -0.4886025119029199;
+static const double SH_C1_1 = 0.4886025119029199;
+static const double SH_C1_2 = -0.4886025119029199;
+static const double SH_C2_0 = 1.0925484305920792;
+static const double SH_C2_1 = -1.0925484305920792;
+static const double SH_C2_2 = 0.31539156525252005;
+static const double SH_C2_3 = -1.0925484305920792;
+static const double SH_C2_4 = 0.5462742152960396;
+static const double SH_C3_0 = -0.5900435899266435;
+static const double SH_C3_1 = 2.890611442640554;
+static const double SH_C3_2 = -0.4570457994644658;
+static const double SH_C3_3 = 0.3731763325901154;
+static const double SH_C3_4 = -0.4570457994644658;
+static const double SH_C3_5 = 1.445305721320277;
+static const double SH_C3_6 = -0.5900435899266435;
+
+// (T)(c * x) with the product formed in double: `SH_Cx * y` in the reference.
+template <typename T> static inline T dmul(double c, T x) { return (T)(c * (double)x); }
+// (T)((c * x) * y) in double: `SH_Cx * y * (expr)` in the reference.
+template <typename T> static inline T dmul2(double c, T x, T y) {
+  return (T)((c * (double)x) * (double)y);
+}
+
+// ---------------------------------------------------------------------------
+// exp
+// ---------------------------------------------------------------------------
+// Cody & Waite (1980) argument reduction x = n ln2 + r, |r| <= ln2/2, with ln2
+// split into a 16-bit head and a tail, then e^r by its Taylor series to degree 7
+// (remainder < 8e-9 relative on |r| <= 0.3466), Horner form with fmaf, and an
+// exact scaling by 2^n assembled from the exponent bits.
+static inline float bits2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+float rgp_expf(float x) {
+  if (!(x > -104.0f)) return (x != x) ? x : 0.0f;
+  if (x > 88.72283f) return bits2f(0x7f800000u);
+  const float LOG2E = 1.44269504088896341f;
+  const float LN2_HI = 0.693145751953125f;        // 0x3f317200
+  const float LN2_LO = 1.42860676533018570e-06f;  // ln2 - LN2_HI
+  float n = rintf(x * LOG2E);
+  float r = fmaf(n, -LN2_HI, x);
+  r = fmaf(n, -LN2_LO, r);
+  float p = 1.98412698e-4f;                 // 1/5040
+  p = fmaf(p, r, 1.38888889e-3f);           // 1/720
+  p = fmaf(p, r, 8.33333377e-3f);           // 1/120
+  p = fmaf(p, r, 4.16666679e-2f);           // 1/24
+  p = fmaf(p, r, 1.66666672e-1f);           // 1/6
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  int ni = (int)n;
+  if (ni < -126) { p = p * 5.42101086e-20f; ni += 64; }  // 2^-64
+  if (ni > 127) { p = p * 2.0f; ni -= 1; }
+  return p * bits2f((uint32_t)(ni + 127) << 23);
+}
+
+template <typename T> static inline T ex(T x);
+template <> inline float ex<float>(float x) { return rgp_expf(x); }
+template <> inline double ex<double>(double x) { return std::exp(x); }
+
+// Float -> integer conversions that saturate and map NaN to 0, the GPU
+// convention the reference relies on (e.g. kernel.cu:119, 147-150, 400).
+template <typename T> static inline int32_t sat_i32(T x) {
+  if (x != x) return 0;
+  if (x >= (T)2147483648.0) return INT32_MAX;
+  if (x <= (T)-2147483648.0) return INT32_MIN;
+  return (int32_t)x;
+}
+template <typename T> static inline uint32_t sat_u32(T x) {
+  if (x != x) return 0;
+  if (x >= (T)4294967296.0) return UINT32_MAX;
+  if (x <= (T)0) return 0;
+  return (uint32_t)x;
+}
+
+// fminf / fmaxf semantics (a NaN operand yields the other operand).
+template <typename T> static inline T fmn(T a, T b) { return std::fmin(a, b); }
+template <typename T> static inline T fmx(T a, T b) { return std::fmax(a, b); }
+
+// ---------------------------------------------------------------------------
+// A1 project (kernel.cu:618-663; matrix.cuh:53-70 accumulation order)
+// ---------------------------------------------------------------------------
+template <typename T>
+void project(int n, const T* pws, const T* R, const T* t, T fx, T fy, T cx, T cy,
+             T* us, T* pcs, T* depths) {
+  for (int i = 0; i < n; ++i) {
+    T pc[3];
+    for (int r = 0; r < 3; ++r) {
+      T s = 0;
+      for (int k = 0; k < 3; ++k) s += R[r * 3 + k] * pws[i * 3 + k];
+      pc[r] = s + t[r];
+    }
+    const T x = pc[0], y = pc[1], z = pc[2];
+    if ((double)z < MIN_DEPTH) { depths[i] = (T)BAD_MARKER; continue; }
+    const T z_inv = (T)1 / z;
+    const T x_fx = x * fx;
+    const T y_fy = y * fy;
+    us[i * 2 + 0] = x_fx * z_inv + cx;
+    us[i * 2 + 1] = y_fy * z_inv + cy;
+    pcs[i * 3 + 0] = x; pcs[i * 3 + 1] = y; pcs[i * 3 + 2] = z;
+    depths[i] = z;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A2 computeCov3D (kernel.cu:422-462)
+// ---------------------------------------------------------------------------
+template <typename T>
+void cov3d(int n, const T* rots, const T* scales, const T* depths, T* cov3ds) {
+  for (int i = 0; i < n; ++i) {
+    if ((double)depths[i] < MIN_DEPTH) continue;
+    const T w = rots[i * 4 + 0], x = rots[i * 4 + 1], y = rots[i * 4 + 2], z = rots[i * 4 + 3];
+    const T s0 = scales[i * 3 + 0], s1 = scales[i * 3 + 1], s2 = scales[i * 3 + 2];
+    const T xx = x * x, yy = y * y, zz = z * z;
+    const T xy = x * y, xz = x * z, yz = y * z;
+    const T xw = x * w, yw = y * w, zw = z * w;
+    const T one = 1, two = 2;
+    T R[9] = {one - two * (yy + zz), two * (xy - zw), two * (xz + yw),
+              two * (xy + zw), one - two * (xx + zz), two * (yz - xw),
+              two * (xz - yw), two * (yz + xw), one - two * (xx + yy)};
+    T M[9];
+    const T s[3] = {s0, s1, s2};
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) M[r * 3 + c] = R[r * 3 + c] * s[c];
+    T S[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        T acc = 0;
+        for (int k = 0; k < 3; ++k) acc += M[r * 3 + k] * M[c * 3 + k];
+        S[r * 3 + c] = acc;
+      }
+    T* o = cov3ds + i * 6;
+    o[0] = S[0]; o[1] = S[1]; o[2] = S[2]; o[3] = S[4]; o[4] = S[5]; o[5] = S[8];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A3 computeCov2D (kernel.cu:508-555; tan_fov from gausplat.cu:357-358)
+// ---------------------------------------------------------------------------
+template <typename T>
+void cov2d(int n, const T* cov3ds, const T* pcs, const T* Rcw, const T* depths,
+           T fx, T fy, T tan_fovx, T tan_fovy, T* cov2ds) {
+  for (int i = 0; i < n; ++i) {
+    if ((double)depths[i] < MIN_DEPTH) continue;
+    T x = pcs[i * 3 + 0], y = pcs[i * 3 + 1];
+    const T z = pcs[i * 3 + 2];
+    const T* c3 = cov3ds + i * 6;
+    const T a = c3[0], b = c3[1], c = c3[2], d = c3[3], e = c3[4], f = c3[5];
+    const T limx = (T)1.3f * tan_fovx;
+    const T limy = (T)1.3f * tan_fovy;
+    x = fmn(limx, fmx(-limx, x / z)) * z;
+    y = fmn(limy, fmx(-limy, y / z)) * z;
+    const T z2 = z * z;
+    const T J[6] = {fx / z, (T)0, -(fx * x) / z2, (T)0, fy / z, -(fy * y) / z2};
+    const T S[9] = {a, b, c, b, d, e, c, e, f};
+    T M[6];
+    for (int r = 0; r < 2; ++r)
+      for (int cc = 0; cc < 3; ++cc) {
+        T acc = 0;
+        for (int k = 0; k < 3; ++k) acc += J[r * 3 + k] * Rcw[k * 3 + cc];
+        M[r * 3 + cc] = acc;
+      }
+    T MS[6];
+    for (int r = 0; r < 2; ++r)
+      for (int cc = 0; cc < 3; ++cc) {
+        T acc = 0;
+        for (int k = 0; k < 3; ++k) acc += M[r * 3 + k] * S[k * 3 + cc];
+        MS[r * 3 + cc] = acc;
+      }
+    T SP[4];
+    for (int r = 0; r < 2; ++r)
+      for (int cc = 0; cc < 2; ++cc) {
+        T acc = 0;
+        for (int k = 0; k < 3; ++k) acc += MS[r * 3 + k] * M[cc * 3 + k];
+        SP[r * 2 + cc] = acc;
+      }
+    cov2ds[i * 3 + 0] = SP[0] + (T)0.3f;
+    cov2ds[i * 3 + 1] = SP[1];
+    cov2ds[i * 3 + 2] = SP[3] + (T)0.3f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A4 inverseCov2D (kernel.cu:379-400)
+// ---------------------------------------------------------------------------
+template <typename T>
+void inverse_cov2d(int n, const T* cov2ds, T* depths, T* cinv2ds, int32_t* areas) {
+  for (int i = 0; i < n; ++i) {
+    if ((double)depths[i] < MIN_DEPTH) continue;
+    const T a = cov2ds[i * 3 + 0], b = cov2ds[i * 3 + 1], c = cov2ds[i * 3 + 2];
+    const T det_inv = (T)1 / (a * c - b * b);
+    if (std::isnan(det_inv)) { depths[i] = (T)BAD_MARKER; continue; }
+    cinv2ds[i * 3 + 0] = det_inv * c;
+    cinv2ds[i * 3 + 1] = -det_inv * b;
+    cinv2ds[i * 3 + 2] = det_inv * a;
+    areas[i * 2 + 0] = sat_i32(std::ceil((T)3 * std::sqrt(std::fabs(a))));
+    areas[i * 2 + 1] = sat_i32(std::ceil((T)3 * std::sqrt(std::fabs(c))));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SH basis of the view direction (kernel.cu:716-760 / 930-957)
+// Y[0] = SH_C0_0 is not produced here; Y[1..15] are the 15 higher terms.
+// ---------------------------------------------------------------------------
+template <typename T>
+static void sh_dir_basis(const T* pw, const T* twc, int sh_dim, T* Y /*[16]*/) {
+  const T d0 = pw[0] - twc[0], d1 = pw[1] - twc[1], d2 = pw[2] - twc[2];
+  const T normd_inv = (T)1 / std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  const T x = d0 * normd_inv, y = d1 * normd_inv, z = d2 * normd_inv;
+  Y[1] = dmul(SH_C1_0, y);
+  Y[2] = dmul(SH_C1_1, z);
+  Y[3] = dmul(SH_C1_2, x);
+  if (sh_dim > 4) {
+    const T xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    Y[4] = dmul(SH_C2_0, xy);
+    Y[5] = dmul(SH_C2_1, yz);
+    Y[6] = dmul(SH_C2_2, (T)2 * zz - xx - yy);
+    Y[7] = dmul(SH_C2_3, xz);
+    Y[8] = dmul(SH_C2_4, xx - yy);
+    if (sh_dim > 9) {
+      Y[9] = dmul2(SH_C3_0, y, (T)3 * xx - yy);
+      Y[10] = dmul2(SH_C3_1, xy, z);
+      Y[11] = dmul2(SH_C3_2, y, (T)4 * zz - xx - yy);
+      Y[12] = dmul2(SH_C3_3, z, (T)2 * zz - (T)3 * xx - (T)3 * yy);
+      Y[13] = dmul2(SH_C3_4, x, (T)4 * zz - xx - yy);
+      Y[14] = dmul2(SH_C3_5, z, xx - yy);
+      Y[15] = dmul2(SH_C3_6, x, xx - (T)3 * yy);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A5 sh2Weight, full rank (kernel.cu:670-790).  basis (N,16): dweight_dshs.
+// ---------------------------------------------------------------------------
+template <typename T>
+void sh2weight(int n, int K, int sh_dim, const T* shs, const T* pws, const T* twc,
+               T* weights, T* basis /*nullable, (N, sh_dim)*/) {
+  for (int i = 0; i < n; ++i) {
+    const T* sh = shs + (size_t)i * sh_dim * K;
+    T w[MAX_SPLAT_DIM];
+    for (int k = 0; k < K; ++k) w[k] = (T)((double)(T)0 + SH_C0_0 * (double)sh[k]);
+    T Y[16] = {0};
+    if (sh_dim > 1) {
+      sh_dir_basis(pws + i * 3, twc, sh_dim, Y);
+      for (int k = 0; k < K; ++k)
+        w[k] += Y[1] * sh[1 * K + k] + Y[2] * sh[2 * K + k] + Y[3] * sh[3 * K + k];
+      if (sh_dim > 4) {
+        for (int k = 0; k < K; ++k)
+          w[k] += Y[4] * sh[4 * K + k] + Y[5] * sh[5 * K + k] + Y[6] * sh[6 * K + k] +
+                  Y[7] * sh[7 * K + k] + Y[8] * sh[8 * K + k];
+        if (sh_dim > 9) {
+          for (int k = 0; k < K; ++k)
+            w[k] += Y[9] * sh[9 * K + k] + Y[10] * sh[10 * K + k] + Y[11] * sh[11 * K + k] +
+                    Y[12] * sh[12 * K + k] + Y[13] * sh[13 * K + k] + Y[14] * sh[14 * K + k] +
+                    Y[15] * sh[15 * K + k];
+        }
+      }
+    }
+    for (int k = 0; k < K; ++k) weights[(size_t)i * K + k] = w[k];
+    if (basis) {
+      T* b = basis + (size_t)i * sh_dim;
+      b[0] = (T)SH_C0_0;
+      for (int l = 1; l < sh_dim; ++l) b[l] = Y[l];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A6 sh2Weight_fast generalised to rank r (kernel.cu:884-987).
+// A: (N, P, r) row-major, B: (N, r, Q).  C[row][col] = sum_r A[row][r] B[r][col],
+// accumulated over r in ascending order; for r = 1 this is A[row]*B[col], the
+// reference expression, bit for bit.
+// ---------------------------------------------------------------------------
+template <typename T>
+void sh2weight_fast(int n, int K, int P, int Q, int rank, const T* dc, const T* A,
+                    const T* B, const T* pws, const T* twc, T* weights, T* ylm /*nullable (N,L)*/) {
+  const int L = P * Q / K;
+  const int sh_dim = L + 1;
+  for (int i = 0; i < n; ++i) {
+    T w[MAX_SPLAT_DIM];
+    for (int k = 0; k < K; ++k) w[k] = dmul(SH_C0_0, dc[(size_t)i * K + k]);
+    if (sh_dim <= 1) {
+      for (int k = 0; k < K; ++k) weights[(size_t)i * K + k] = w[k];
+      continue;
+    }
+    T Y[16] = {0};
+    sh_dir_basis(pws + i * 3, twc, sh_dim, Y);
+    if (ylm)
+      for (int l = 0; l < L; ++l) ylm[(size_t)i * L + l] = Y[l + 1];
+    const T* a = A + (size_t)i * P * rank;
+    const T* b = B + (size_t)i * rank * Q;
+    for (int k = 0; k < K; ++k) {
+      T acc = 0;
+      for (int lm = 0; lm < L; ++lm) {
+        const int g = lm * K + k, row = g / Q, col = g % Q;
+        T c = a[row * rank + 0] * b[0 * Q + col];
+        for (int r = 1; r < rank; ++r) c += a[row * rank + r] * b[r * Q + col];
+        acc += c * Y[lm + 1];
+      }
+      w[k] += acc;
+    }
+    for (int k = 0; k < K; ++k) weights[(size_t)i * K + k] = w[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A16 sh2Weight_fast_B generalised to rank r (kernel.cu:994-1032).  The
+// reference's atomics hit thread-private rows, so its sums run in loop order
+// (k outer, lm inner), which is the order used here.
+// ---------------------------------------------------------------------------
+template <typename T>
+void sh2weight_fast_backward(int n, int K, int P, int Q, int rank, const T* A, const T* B,
+                             const T* ylm, const T* dw, T* dA, T* dB) {
+  const int L = P * Q / K;
+  for (int i = 0; i < n; ++i) {
+    const T* a = A + (size_t)i * P * rank;
+    const T* b = B + (size_t)i * rank * Q;
+    const T* y = ylm + (size_t)i * L;
+    T* da = dA + (size_t)i * P * rank;
+    T* db = dB + (size_t)i * rank * Q;
+    for (int j = 0; j < P * rank; ++j) da[j] = 0;
+    for (int j = 0; j < rank * Q; ++j) db[j] = 0;
+    for (int k = 0; k < K; ++k) {
+      const T dwk = dw[(size_t)i * K + k];
+      for (int lm = 0; lm < L; ++lm) {
+        const int g = lm * K + k, row = g / Q, col = g % Q;
+        for (int r = 0; r < rank; ++r) {
+          da[row * rank + r] += dwk * b[r * Q + col] * y[lm];
+          db[r * Q + col] += dwk * a[row * rank + r] * y[lm];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sh2Color (kernel.cu:1040-1110).  `SH_C0_0 * sh0` binds the double constant
+// to the `const float&` of operator*(float, float3), so it is rounded to float
+// first; the other band terms use the double-formed basis values.
+// ---------------------------------------------------------------------------
+template <typename T>
+void sh2color(int n, int sh_dim, const T* shs, const T* pws, const T* twc, T* colors) {
+  for (int i = 0; i < n; ++i) {
+    const T* sh = shs + (size_t)i * sh_dim * 3;
+    const T c0 = (T)SH_C0_0;
+    T col[3];
+    for (int c = 0; c < 3; ++c) col[c] = c0 * sh[c];
+    if (sh_dim > 1) {
+      T Y[16] = {0};
+      sh_dir_basis(pws + i * 3, twc, sh_dim, Y);
+      for (int c = 0; c < 3; ++c)
+        col[c] += Y[1] * sh[3 + c] + Y[2] * sh[6 + c] + Y[3] * sh[9 + c];
+      if (sh_dim > 4) {
+        for (int c = 0; c < 3; ++c)
+          col[c] += Y[4] * sh[12 + c] + Y[5] * sh[15 + c] + Y[6] * sh[18 + c] +
+                    Y[7] * sh[21 + c] + Y[8] * sh[24 + c];
+        if (sh_dim > 9) {
+          for (int c = 0; c < 3; ++c)
+            col[c] += Y[9] * sh[27 + c] + Y[10] * sh[30 + c] + Y[11] * sh[33 + c] +
+                      Y[12] * sh[36 + c] + Y[13] * sh[39 + c] + Y[14] * sh[42 + c] +
+                      Y[15] * sh[45 + c];
+        }
+      }
+    }
+    for (int c = 0; c < 3; ++c) colors[(size_t)i * 3 + c] = col[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A7 getRects (kernel.cu:127-162): rect = (x0, y0, x1, y1) and tile count,
+// culling empty rects in place.  DIV_ROUND_UP on a float argument expands to
+// ((v + 16) - 1) / 16 in float.
+// ---------------------------------------------------------------------------
+template <typename T>
+void get_rects(int n, const T* us, int32_t* areas, T* depths, int gx, int gy,
+               uint32_t* rects /*(N,4)*/, uint32_t* counts) {
+  for (int i = 0; i < n; ++i) {
+    counts[i] = 0;
+    if ((double)depths[i] < MIN_DEPTH) continue;
+    const T ux = us[2 * i], uy = us[2 * i + 1];
+    const T xs = (T)areas[2 * i], ys = (T)areas[2 * i + 1];
+    const T B = (T)BLOCK;
+    auto clampu = [](int32_t v, int g) -> uint32_t {
+      int32_t m = std::max((int32_t)0, v);
+      return std::min((uint32_t)g, (uint32_t)m);
+    };
+    uint32_t r0 = clampu(sat_i32((ux - xs) / B), gx);
+    uint32_t r1 = clampu(sat_i32((uy - ys) / B), gy);
+    uint32_t r2 = clampu(sat_i32((((ux + xs) + B) - (T)1) / B), gx);
+    uint32_t r3 = clampu(sat_i32((((uy + ys) + B) - (T)1) / B), gy);
+    uint32_t cnt = (r3 - r1) * (r2 - r0);
+    if (cnt == 0) {
+      depths[i] = (T)BAD_MARKER;
+      areas[2 * i] = 0; areas[2 * i + 1] = 0;
+      continue;
+    }
+    rects[4 * i + 0] = r0; rects[4 * i + 1] = r1; rects[4 * i + 2] = r2; rects[4 * i + 3] = r3;
+    counts[i] = cnt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A8-A11: inclusive scan, createKeys (kernel.cu:96-124), stable sort of the
+// u64 keys (gausplat.cu:216, a stable radix sort), getRanges (kernel.cu:165-187)
+// including its M == 1 behaviour.  Returns M.
+// ---------------------------------------------------------------------------
+template <typename T>
+int64_t bin_sort(int n, const T* depths, const uint32_t* rects, const uint32_t* counts,
+                 int gx, int gy, uint64_t* keys_out /*cap*/, int32_t* ids_out /*cap*/,
+                 int64_t cap, int32_t* ranges /*(T,2)*/) {
+  std::vector<uint64_t> off(n > 0 ? n : 1);
+  uint64_t run = 0;
+  for (int i = 0; i < n; ++i) { run += counts[i]; off[i] = run; }
+  const int64_t M = (int64_t)run;
+  const int tiles = gx * gy;
+  for (int t = 0; t < 2 * tiles; ++t) ranges[t] = 0;
+  if (M > cap) return M;
+  std::vector<uint64_t> keys(M);
+  std::vector<int32_t> ids(M);
+  for (int i = 0; i < n; ++i) {
+    const T depth = depths[i];
+    if ((double)depth < MIN_DEPTH) continue;
+    uint64_t o = (i == 0) ? 0 : off[i - 1];
+    const uint32_t* r = rects + 4 * i;
+    for (uint32_t y = r[1]; y < r[3]; ++y)
+      for (uint32_t x = r[0]; x < r[2]; ++x) {
+        uint64_t key = (uint64_t)(y * (uint32_t)gx + x);
+        key <<= 32;
+        key |= (uint64_t)sat_u32(depth * (T)1000);
+        keys[o] = key; ids[o] = i; ++o;
+      }
+  }
+  std::vector<int64_t> perm(M);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](int64_t a, int64_t b) { return keys[a] < keys[b]; });
+  for (int64_t p = 0; p < M; ++p) { keys_out[p] = keys[perm[p]]; ids_out[p] = ids[perm[p]]; }
+  for (int64_t p = 0; p < M; ++p) {
+    const int64_t prv = p == 0 ? 0 : p - 1;
+    const uint32_t cur_t = (uint32_t)(keys_out[p] >> 32);
+    const uint32_t prv_t = (uint32_t)(keys_out[prv] >> 32);
+    if (p == 0) ranges[2 * cur_t + 0] = 0;
+    else if (p == M - 1) ranges[2 * cur_t + 1] = (int32_t)M;
+    if (prv_t != cur_t) { ranges[2 * prv_t + 1] = (int32_t)p; ranges[2 * cur_t + 0] = (int32_t)p; }
+  }
+  return M;
+}
+
+// ---------------------------------------------------------------------------
+// A12 draw_weight (kernel.cu:283-372) and draw_color (kernel.cu:194-276): one
+// pixel at a time over its tile list.  `dim` channels of per-Gaussian feature
+// (K weights, or 3 colours).  Outputs are written for every pixel of a tile
+// with a non-empty list; the caller initialises the rest (final_tau 0 for the
+// weight path, gausplat.cu:178; 1 for the colour path, gausplat.cu:39).
+// Also returns per-pixel evaluation and contribution counts (for the flop
+// model of the benchmark).
+// ---------------------------------------------------------------------------
+template <typename T>
+void draw(int width, int height, int dim, const int32_t* ranges, const int32_t* ids,
+          const T* us, const T* cinv, const T* alphas, const T* feat, T* layer,
+          int32_t* contrib, T* final_tau, int64_t* n_eval, int64_t* n_contrib) {
+  const int gx = (width + BLOCK - 1) / BLOCK, gy = (height + BLOCK - 1) / BLOCK;
+  const size_t HW = (size_t)width * height;
+  int64_t tot_e = 0, tot_c = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : tot_e, tot_c)
+  for (int tile = 0; tile < gx * gy; ++tile) {
+    const int tx = tile % gx, ty = tile / gx;
+    const int r0 = ranges[2 * tile], r1 = ranges[2 * tile + 1];
+    const int gs_num = r1 - r0;
+    if (gs_num == 0) continue;
+    for (int py = ty * BLOCK; py < std::min(height, ty * BLOCK + BLOCK); ++py)
+      for (int px = tx * BLOCK; px < std::min(width, tx * BLOCK + BLOCK); ++px) {
+        T acc[MAX_SPLAT_DIM];
+        for (int k = 0; k < dim; ++k) acc[k] = 0;
+        T tau = 1;
+        int cont = 0, cont_tmp = 0;
+        for (int j = r0; j < r1; ++j) {
+          const int g = ids[j];
+          const T dx = us[2 * g] - (T)px, dy = us[2 * g + 1] - (T)py;
+          const T cx = cinv[3 * g], cy = cinv[3 * g + 1], cz = cinv[3 * g + 2];
+          ++cont_tmp;
+          ++tot_e;
+          const T maha = fmx((T)0, cx * dx * dx + cz * dy * dy + (T)2 * cy * dx * dy);
+          const T ap = fmn((T)0.99f, alphas[g] * ex((T)-0.5f * maha));
+          if (ap < (T)0.002f) continue;
+          ++tot_c;
+          for (int k = 0; k < dim; ++k) acc[k] += tau * ap * feat[(size_t)g * dim + k];
+          cont = cont_tmp;
+          tau *= ((T)1 - ap);
+          if (tau < (T)0.0001f) break;
+        }
+        const size_t pix = (size_t)py * width + px;
+        for (int k = 0; k < dim; ++k) layer[k * HW + pix] = acc[k];
+        contrib[pix] = cont;
+        final_tau[pix] = tau;
+      }
+  }
+  if (n_eval) *n_eval = tot_e;
+  if (n_contrib) *n_contrib = tot_c;
+}
+
+// ---------------------------------------------------------------------------
+// A14 draw_weight_B (kernel.cu:1291-1379).  Per (pixel, contributing entry)
+// the term dL/dW_k * alpha' * tau is formed exactly as in the reference, in
+// type T.  mode 0: terms are summed serially in double into grad (N,K) and
+// their absolute values into gabs (N,K), which bounds the reordering error of
+// any atomic summation.  mode 1: OpenMP over tiles, atomic adds in T.
+// ---------------------------------------------------------------------------
+template <typename T>
+void draw_backward(int width, int height, int K, const int32_t* ranges, const int32_t* ids,
+                   const T* us, const T* cinv, const T* alphas, const int32_t* contrib,
+                   const T* final_tau, const T* dlayer, int mode, double* grad, double* gabs,
+                   T* grad_t) {
+  const int gx = (width + BLOCK - 1) / BLOCK, gy = (height + BLOCK - 1) / BLOCK;
+  const size_t HW = (size_t)width * height;
+#pragma omp parallel for schedule(dynamic, 4) if (mode == 1)
+  for (int tile = 0; tile < gx * gy; ++tile) {
+    const int tx = tile % gx, ty = tile / gx;
+    const int r0 = ranges[2 * tile], r1 = ranges[2 * tile + 1];
+    const int gs_num = r1 - r0;
+    if (gs_num == 0) continue;
+    for (int py = ty * BLOCK; py < std::min(height, ty * BLOCK + BLOCK); ++py)
+      for (int px = tx * BLOCK; px < std::min(width, tx * BLOCK + BLOCK); ++px) {
+        const size_t pix = (size_t)py * width + px;
+        T dl[MAX_SPLAT_DIM];
+        for (int k = 0; k < K; ++k) dl[k] = dlayer[k * HW + pix];
+        T tau = final_tau[pix];
+        const int cont = contrib[pix];
+        for (int i = 0; i < gs_num; ++i) {
+          if (i < gs_num - cont) continue;
+          const int g = ids[r1 - i - 1];
+          const T dx = us[2 * g] - (T)px, dy = us[2 * g + 1] - (T)py;
+          const T cx = cinv[3 * g], cy = cinv[3 * g + 1], cz = cinv[3 * g + 2];
+          const T maha = fmx((T)0, cx * dx * dx + cz * dy * dy + (T)2 * cy * dx * dy);
+          const T gg = ex((T)-0.5f * maha);
+          const T ap = fmn((T)0.99f, alphas[g] * gg);
+          if (ap < (T)0.002f) continue;
+          tau /= ((T)1 - ap);
+          const T dgdw = ap * tau;
+          for (int k = 0; k < K; ++k) {
+            const T term = dl[k] * dgdw;
+            if (mode == 0) {
+              grad[(size_t)g * K + k] += (double)term;
+              gabs[(size_t)g * K + k] += std::fabs((double)term);
+            } else {
+#pragma omp atomic
+              grad_t[(size_t)g * K + k] += term;
+            }
+          }
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A13 palette mix: RGB[c, p] = sum_k P[k, c] W[k, p], k ascending, no
+// contraction.  Equal in exact arithmetic to baking the palette into the
+// per-Gaussian colours and re-rendering (gsmodel.py:488-509 + 13-26).
+// ---------------------------------------------------------------------------
+template <typename T>
+void mix(int K, int64_t HW, const T* pal /*(K,3)*/, const T* W, T* rgb) {
+  for (int c = 0; c < 3; ++c)
+    for (int64_t p = 0; p < HW; ++p) {
+      T acc = pal[0 * 3 + c] * W[0 * HW + p];
+      for (int k = 1; k < K; ++k) acc = acc + pal[k * 3 + c] * W[k * HW + p];
+      rgb[c * HW + p] = acc;
+    }
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C entry points (ctypes); suffix f = FP32, d = FP64.
+// ---------------------------------------------------------------------------
+#define ORC_BOTH(NAME, T, S)                                                                      \
+  void orc_project_##S(int n, const T* pws, const T* R, const T* t, T fx, T fy, T cx, T cy,       \
+                       T* us, T* pcs, T* depths) {                                                \
+    orc::project<T>(n, pws, R, t, fx, fy, cx, cy, us, pcs, depths);                               \
+  }                                                                                               \
+  void orc_cov3d_##S(int n, const T* rots, const T* scales, const T* depths, T* cov3ds) {         \
+    orc::cov3d<T>(n, rots, scales, depths, cov3ds);                                               \
+  }                                                                                               \
+  void orc_cov2d_##S(int n, const T* cov3ds, const T* pcs, const T* Rcw, const T* depths, T fx,   \
+                     T fy, T tfx, T tfy, T* cov2ds) {                                             \
+    orc::cov2d<T>(n, cov3ds, pcs, Rcw, depths, fx, fy, tfx, tfy, cov2ds);                         \
+  }                                                                                               \
+  void orc_inverse_cov2d_##S(int n, const T* cov2ds, T* depths, T* cinv2ds, int32_t* areas) {     \
+    orc::inverse_cov2d<T>(n, cov2ds, depths, cinv2ds, areas);                                     \
+  }                                                                                               \
+  void orc_sh2weight_##S(int n, int K, int sh_dim, const T* shs, const T* pws, const T* twc,      \
+                         T* weights, T* basis) {                                                  \
+    orc::sh2weight<T>(n, K, sh_dim, shs, pws, twc, weights, basis);                               \
+  }                                                                                               \
+  void orc_sh2weight_fast_##S(int n, int K, int P, int Q, int rank, const T* dc, const T* A,      \
+                              const T* B, const T* pws, const T* twc, T* weights, T* ylm) {       \
+    orc::sh2weight_fast<T>(n, K, P, Q, rank, dc, A, B, pws, twc, weights, ylm);                   \
+  }                                                                                               \
+  void orc_sh2weight_fast_backward_##S(int n, int K, int P, int Q, int rank, const T* A,          \
+                                       const T* B, const T* ylm, const T* dw, T* dA, T* dB) {     \
+    orc::sh2weight_fast_backward<T>(n, K, P, Q, rank, A, B, ylm, dw, dA, dB);                     \
+  }                                                                                               \
+  void orc_sh2color_##S(int n, int sh_dim, const T* shs, const T* pws, const T* twc, T* colors) { \
+    orc::sh2color<T>(n, sh_dim, shs, pws, twc, colors);                                           \
+  }                                                                                               \
+  void orc_get_rects_##S(int n, const T* us, int32_t* areas, T* depths, int gx, int gy,           \
+                         uint32_t* rects, uint32_t* counts) {                                     \
+    orc::get_rects<T>(n, us, areas, depths, gx, gy, rects, counts);                               \
+  }                                                                                               \
+  int64_t orc_bin_sort_##S(int n, const T* depths, const uint32_t* rects, const uint32_t* counts, \
+                           int gx, int gy, uint64_t* keys, int32_t* ids, int64_t cap,             \
+                           int32_t* ranges) {                                                     \
+    return orc::bin_sort<T>(n, depths, rects, counts, gx, gy, keys, ids, cap, ranges);            \
+  }                                                                                               \
+  void orc_draw_##S(int w, int h, int dim, const int32_t* ranges, const int32_t* ids,             \
+                    const T* us, const T* cinv, const T* alphas, const T* feat, T* layer,         \
+                    int32_t* contrib, T* final_tau, int64_t* ne, int64_t* nc) {                   \
+    orc::draw<T>(w, h, dim, ranges, ids, us, cinv, alphas, feat, layer, contrib, final_tau, ne,  \
+                 nc);                                                                             \
+  }                                                                                               \
+  void orc_draw_backward_##S(int w, int h, int K, const int32_t* ranges, const int32_t* ids,      \
+                             const T* us, const T* cinv, const T* alphas, const int32_t* contrib, \
+                             const T* final_tau, const T* dlayer, int mode, double* grad,         \
+                             double* gabs, T* grad_t) {                                           \
+    orc::draw_backward<T>(w, h, K, ranges, ids, us, cinv, alphas, contrib, final_tau, dlayer,    \
+                          mode, grad, gabs, grad_t);                                              \
+  }                                                                                               \
+  void orc_mix_##S(int K, int64_t HW, const T* pal, const T* W, T* rgb) {                         \
+    orc::mix<T>(K, HW, pal, W, rgb);                                                              \
+  }
+
+extern "C" {
+ORC_BOTH(f, float, f)
+ORC_BOTH(d, double, d)
+float orc_expf(float x) { return orc::rgp_expf(x); }
+void orc_expf_array(int64_t n, const float* x, float* y) {
+  for (int64_t i = 0; i < n; ++i) y[i] = orc::rgp_expf(x[i]);
+}
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+void orc_set_num_threads(int t) {
+#ifdef _OPENMP
+  omp_set_num_threads(t);
+#else
+  (void)t;
+#endif
+}
+}

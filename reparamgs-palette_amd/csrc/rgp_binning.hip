@@ -1,0 +1,451 @@
+// Tile binning for gfx950: inclusive scan of per-Gaussian tile counts, key
+// duplication, a stable LSD radix sort of the (tile, uint32(depth*1000)) keys
+// and per-tile ranges.  Replaces thrust::inclusive_scan, createKeys,
+// thrust::sort_by_key and getRanges of gsplatcu/gausplat.cu:199-226 /
+// kernel.cu:96-187, producing the same order: tile ascending, quantised depth
+// ascending, Gaussian id ascending (the input order a stable sort keeps).
+//
+// Nothing here synchronises the host.  Sizes that the reference reads back
+// (the pair count M, gausplat.cu:201) stay on the device: every kernel is
+// launched for a caller-given pair capacity and reads M from the workspace
+// header; if M exceeds the capacity the overflow flag is raised and all
+// downstream kernels see zero pairs.
+//
+// Sort: 8-bit digits, one pass per key byte; passes whose digit is the same
+// for every key (depth bytes above the largest quantised depth, tile bytes
+// above the tile count) are skipped, decided on the device.  Each pass is
+// histogram -> per-digit scan over blocks -> scatter, where the scatter ranks
+// keys stably inside the block with wavefront ballots, reorders them in LDS
+// and writes coalesced runs.
+#include "rgp_internal.h"
+
+namespace rgp {
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan over the block (blockDim.x <= 1024, multiple of 64).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total,
+                                                    uint32_t* lds /*16 words*/) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) lds[w] = inc;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (int j = 0; j < nw; ++j) {
+    const uint32_t x = lds[j];
+    if (j < w) before += x;
+    all += x;
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* dst, uint32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// scan
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_scan_reduce(int n, const uint32_t* __restrict__ counts,
+                                                     uint32_t* __restrict__ partials) {
+  __shared__ uint32_t lds[16];
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanChunk / 256; ++j) {
+    const int64_t i = base + j * 256 + threadIdx.x;
+    if (i < n) s += counts[i];
+  }
+  uint32_t tot;
+  block_excl_scan(s, &tot, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// Scans the block sums, publishes M, the overflow flag and the sort plan.
+__global__ __launch_bounds__(1024) void k_scan_top(int nscan, uint32_t* __restrict__ partials,
+                                                   uint32_t* __restrict__ hdr, int64_t cap,
+                                                   int tile_bytes) {
+  __shared__ uint32_t lds[16];
+  uint32_t carry = 0;
+  for (int base = 0; base < nscan; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    const uint32_t v = i < nscan ? partials[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(v, &tot, lds);
+    if (i < nscan) partials[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t M = carry;
+    const bool ovf = (int64_t)M > cap;
+    const uint32_t meff = ovf ? 0u : M;
+    hdr[kHdrM] = M;
+    hdr[kHdrOverflow] = ovf ? 1u : 0u;
+    hdr[kHdrMEff] = meff;
+    const uint32_t dq = hdr[kHdrDqMax];
+    int depth_bytes = 0;
+    while (depth_bytes < 4 && (dq >> (8 * depth_bytes)) != 0) ++depth_bytes;
+    uint32_t cur = 0;
+    int last = -1;
+    for (int p = 0; p < 8; ++p) {
+      bool skip = p < 4 ? (p >= depth_bytes) : (p - 4 >= tile_bytes);
+      if (meff <= 1) skip = true;
+      hdr[kHdrCur + p] = cur;
+      hdr[kHdrSkip + p] = skip ? 1u : 0u;
+      if (!skip) { cur ^= 1u; last = p; }
+    }
+    hdr[kHdrCur + 8] = cur;
+    hdr[kHdrLast] = (uint32_t)last;
+    hdr[kHdrNblk] = (meff + kSortTile - 1) / kSortTile;
+  }
+}
+
+// counts -> inclusive offsets, in place.
+__global__ __launch_bounds__(256) void k_scan_down(int n, uint32_t* __restrict__ counts,
+                                                   const uint32_t* __restrict__ partials) {
+  __shared__ uint32_t lds[16];
+  constexpr int PER = kScanChunk / 256;
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * PER;
+  uint32_t v[PER];
+  uint32_t s = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    v[j] = (base + j < n) ? counts[base + j] : 0u;
+    s += v[j];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan(s, &tot, lds) + partials[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    run += v[j];
+    if (base + j < n) counts[base + j] = run;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// createKeys (kernel.cu:96-124), load balanced: a block expands the pairs of
+// its 256 Gaussians cooperatively, so one large splat does not serialise a
+// thread.  Pair p of Gaussian i (row-major over its rect) goes to
+// offset[i-1] + p, as in the reference.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dup(int n, const uint32_t* __restrict__ offs,
+                                             const uint2* __restrict__ rects,
+                                             const float* __restrict__ depths, int gx,
+                                             const uint32_t* __restrict__ hdr,
+                                             uint64_t* __restrict__ keys,
+                                             int32_t* __restrict__ vals) {
+  if (hdr[kHdrMEff] == 0) return;
+  __shared__ uint32_t s_incl[256];
+  __shared__ uint2 s_rect[256];
+  __shared__ uint32_t s_dq[256];
+  const int t = threadIdx.x;
+  const int i0 = blockIdx.x * 256;
+  const uint32_t base = i0 == 0 ? 0u : offs[i0 - 1];
+  const int i = i0 + t;
+  const int cnt_blk = min(256, n - i0);
+  uint32_t incl = 0;
+  if (i < n) {
+    incl = offs[i] - base;
+    const uint32_t excl = (i == 0 ? 0u : offs[i - 1]) - base;
+    if (incl > excl) {
+      s_rect[t] = rects[i];
+      s_dq[t] = sat_u32(depths[i] * 1000.f);
+    }
+  }
+  s_incl[t] = incl;
+  __syncthreads();
+  const uint32_t total = s_incl[cnt_blk - 1];
+  for (uint32_t p = t; p < total; p += 256) {
+    // smallest g with s_incl[g] > p
+    int lo = 0, hi = cnt_blk - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_incl[mid] > p) hi = mid; else lo = mid + 1;
+    }
+    const uint32_t li = p - (lo == 0 ? 0u : s_incl[lo - 1]);
+    const uint2 r = s_rect[lo];
+    const uint32_t x0 = r.x & 0xffffu, y0 = r.x >> 16, x1 = r.y & 0xffffu;
+    const uint32_t w = x1 - x0;
+    const uint32_t yy = li / w, xx = li - yy * w;
+    const uint32_t tile = (y0 + yy) * (uint32_t)gx + (x0 + xx);
+    const uint32_t gp = base + p;
+    keys[gp] = ((uint64_t)tile << 32) | (uint64_t)s_dq[lo];
+    vals[gp] = i0 + lo;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// radix sort passes
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t digit_of(uint64_t k, int p) {
+  return (uint32_t)(k >> (8 * p)) & 0xffu;
+}
+
+__global__ __launch_bounds__(256) void k_sort_hist(int p, const uint32_t* __restrict__ hdr,
+                                                   const uint64_t* __restrict__ k0,
+                                                   const uint64_t* __restrict__ k1,
+                                                   uint32_t* __restrict__ hist, int nblk_cap) {
+  if (hdr[kHdrSkip + p]) return;
+  const uint32_t M = hdr[kHdrMEff];
+  const int b = blockIdx.x;
+  if ((uint32_t)b >= hdr[kHdrNblk]) return;
+  const uint64_t* src = hdr[kHdrCur + p] ? k1 : k0;
+  __shared__ uint32_t h[4][256];
+  const int t = threadIdx.x, w = t >> 6;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) h[q][t] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)b * kSortTile;
+#pragma unroll 4
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = base + j * 256 + t;
+    if (e < M) atomicAdd(&h[w][digit_of(src[e], p)], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)t * nblk_cap + b] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
+}
+
+// One block per digit: exclusive scan of its counts over the sort blocks.
+__global__ __launch_bounds__(256) void k_sort_scan(int p, const uint32_t* __restrict__ hdr,
+                                                   uint32_t* __restrict__ hist,
+                                                   uint32_t* __restrict__ totals, int nblk_cap) {
+  if (hdr[kHdrSkip + p]) return;
+  __shared__ uint32_t lds[16];
+  const int nblk = (int)hdr[kHdrNblk];
+  uint32_t* row = hist + (size_t)blockIdx.x * nblk_cap;
+  uint32_t carry = 0;
+  for (int base = 0; base < nblk; base += 256) {
+    const int i = base + threadIdx.x;
+    const uint32_t v = i < nblk ? row[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(v, &tot, lds);
+    if (i < nblk) row[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(int p, const uint32_t* __restrict__ hdr,
+                                                      uint64_t* __restrict__ k0,
+                                                      uint64_t* __restrict__ k1,
+                                                      int32_t* __restrict__ v0,
+                                                      int32_t* __restrict__ v1,
+                                                      int32_t* __restrict__ ids_out,
+                                                      const uint32_t* __restrict__ hist,
+                                                      const uint32_t* __restrict__ totals,
+                                                      int nblk_cap) {
+  if (hdr[kHdrSkip + p]) return;
+  const uint32_t M = hdr[kHdrMEff];
+  const int b = blockIdx.x;
+  if ((uint32_t)b >= hdr[kHdrNblk]) return;
+  const bool from1 = hdr[kHdrCur + p] != 0;
+  const uint64_t* sk = from1 ? k1 : k0;
+  const int32_t* sv = from1 ? v1 : v0;
+  uint64_t* dk = from1 ? k0 : k1;
+  int32_t* dv = ((int)hdr[kHdrLast] == p) ? ids_out : (from1 ? v0 : v1);
+
+  __shared__ uint64_t s_key[kSortTile];
+  __shared__ int32_t s_val[kSortTile];
+  __shared__ uint32_t s_wcnt[4][256];
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_start[256];
+  __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t lds[16];
+
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // global start of every digit bucket for this block
+  {
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(totals[t], &tot, lds);
+    s_gbase[t] = ex + hist[(size_t)t * nblk_cap + b];
+  }
+  s_run[t] = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) s_wcnt[q][t] = 0;
+  __syncthreads();
+
+  const int64_t base = (int64_t)b * kSortTile;
+  uint64_t key[kSortItems];
+  int32_t val[kSortItems];
+  uint32_t off[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = base + j * 256 + t;
+    const bool valid = e < M;
+    key[j] = valid ? sk[e] : 0ull;
+    val[j] = valid ? sv[e] : 0;
+    const uint32_t d = digit_of(key[j], p);
+    // peers: lanes of this wavefront holding the same digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool on = (d >> bit) & 1u;
+      const uint64_t bb = __ballot(on);
+      peers &= on ? bb : ~bb;
+    }
+    const uint32_t rank = __popcll(peers & lt_mask);
+    if (valid && rank == 0) s_wcnt[w][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t o = s_run[d] + rank;
+      for (int q = 0; q < w; ++q) o += s_wcnt[q][d];
+      off[j] = o;
+    }
+    __syncthreads();
+    s_run[t] += s_wcnt[0][t] + s_wcnt[1][t] + s_wcnt[2][t] + s_wcnt[3][t];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s_wcnt[q][t] = 0;
+    __syncthreads();
+  }
+  {
+    uint32_t tot;
+    s_start[t] = block_excl_scan(s_run[t], &tot, lds);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t e = base + j * 256 + t;
+    if (e < M) {
+      const uint32_t pos = s_start[digit_of(key[j], p)] + off[j];
+      s_key[pos] = key[j];
+      s_val[pos] = val[j];
+    }
+  }
+  __syncthreads();
+  const int cnt = (int)min((int64_t)kSortTile, (int64_t)M - base);
+  for (int e = t; e < cnt; e += 256) {
+    const uint64_t k = s_key[e];
+    const uint32_t d = digit_of(k, p);
+    const uint32_t g = s_gbase[d] + (uint32_t)e - s_start[d];
+    dk[g] = k;
+    dv[g] = s_val[e];
+  }
+}
+
+// Final buffers: ids when no pass ran, keys on request.
+__global__ __launch_bounds__(256) void k_sort_finish(const uint32_t* __restrict__ hdr,
+                                                     const uint64_t* __restrict__ k0,
+                                                     const uint64_t* __restrict__ k1,
+                                                     const int32_t* __restrict__ v0,
+                                                     int32_t* __restrict__ ids_out,
+                                                     uint64_t* __restrict__ keys_out) {
+  const uint32_t M = hdr[kHdrMEff];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M) return;
+  if ((int)hdr[kHdrLast] < 0) ids_out[e] = v0[e];
+  if (keys_out) keys_out[e] = hdr[kHdrCur + 8] ? k1[e] : k0[e];
+}
+
+// getRanges (kernel.cu:165-187) by binary search per tile; (0,0) for empty
+// tiles and, as in the reference, for every tile when M == 1.
+__global__ __launch_bounds__(256) void k_ranges(int tiles, const uint32_t* __restrict__ hdr,
+                                                const uint64_t* __restrict__ k0,
+                                                const uint64_t* __restrict__ k1,
+                                                int2* __restrict__ ranges) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tiles) return;
+  const uint32_t M = hdr[kHdrMEff];
+  if (M <= 1) { ranges[t] = make_int2(0, 0); return; }
+  const uint64_t* keys = hdr[kHdrCur + 8] ? k1 : k0;
+  auto lower = [&](uint32_t tile) {
+    uint32_t lo = 0, hi = M;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if ((uint32_t)(keys[mid] >> 32) < tile) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
+  const uint32_t a = lower((uint32_t)t), e = lower((uint32_t)t + 1);
+  ranges[t] = e > a ? make_int2((int)a, (int)e) : make_int2(0, 0);
+}
+
+int bin_reset_header(void* ws, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, (uint32_t*)ws, 0u, (int64_t)kHdrWords);
+  return check_launch("bin_reset_header");
+}
+
+}  // namespace rgp
+
+using namespace rgp;
+
+extern "C" size_t rgp_bin_workspace_bytes(int n, int64_t capacity, int width, int height) {
+  if (n < 0 || width <= 0 || height <= 0) return 0;
+  return bin_layout(n, capacity, width, height).total;
+}
+
+extern "C" const uint32_t* rgp_bin_header(const void* ws) { return (const uint32_t*)ws; }
+
+extern "C" int rgp_bin_scan(int n, int width, int height, void* ws, int64_t capacity,
+                            hipStream_t s) {
+  if (n < 0 || width <= 0 || height <= 0) return set_error(RGP_ERR_ARG, "rgp_bin_scan: bad size");
+  BinLayout L = bin_layout(n, capacity, width, height);
+  char* base = (char*)ws;
+  uint32_t* hdr = (uint32_t*)(base + L.header);
+  int tile_bytes = 0;
+  while (tile_bytes < 4 && ((uint32_t)(L.tiles - 1) >> (8 * tile_bytes)) != 0) ++tile_bytes;
+  if (L.nscan > 0)
+    hipLaunchKernelGGL(k_scan_reduce, dim3(L.nscan), dim3(256), 0, s, n,
+                       (const uint32_t*)(base + L.counts), (uint32_t*)(base + L.partials));
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, L.nscan,
+                     (uint32_t*)(base + L.partials), hdr, L.cap, tile_bytes);
+  if (L.nscan > 0)
+    hipLaunchKernelGGL(k_scan_down, dim3(L.nscan), dim3(256), 0, s, n,
+                       (uint32_t*)(base + L.counts), (const uint32_t*)(base + L.partials));
+  return check_launch("rgp_bin_scan");
+}
+
+extern "C" int rgp_bin_sort(int n, const float* depths, int width, int height, void* ws,
+                            int64_t capacity, int32_t* ids_out, uint64_t* keys_out,
+                            int32_t* ranges_out, hipStream_t s) {
+  if (n < 0 || width <= 0 || height <= 0) return set_error(RGP_ERR_ARG, "rgp_bin_sort: bad size");
+  if (capacity > (int64_t)INT32_MAX)
+    return set_error(RGP_ERR_ARG, "rgp_bin_sort: capacity above 2^31 - 1");
+  BinLayout L = bin_layout(n, capacity, width, height);
+  char* base = (char*)ws;
+  const uint32_t* hdr = (const uint32_t*)(base + L.header);
+  uint64_t* k0 = (uint64_t*)(base + L.keys0);
+  uint64_t* k1 = (uint64_t*)(base + L.keys1);
+  int32_t* v0 = (int32_t*)(base + L.vals0);
+  int32_t* v1 = (int32_t*)(base + L.vals1);
+  uint32_t* hist = (uint32_t*)(base + L.hist);
+  uint32_t* totals = (uint32_t*)(base + L.totals);
+  if (n > 0 && L.cap > 0)
+    hipLaunchKernelGGL(k_dup, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                       (const uint32_t*)(base + L.counts), (const uint2*)(base + L.rects), depths,
+                       L.gx, hdr, k0, v0);
+  if (L.nblk_cap > 0) {
+    for (int p = 0; p < 8; ++p) {
+      hipLaunchKernelGGL(k_sort_hist, dim3(L.nblk_cap), dim3(256), 0, s, p, hdr, k0, k1, hist,
+                         L.nblk_cap);
+      hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, s, p, hdr, hist, totals,
+                         L.nblk_cap);
+      hipLaunchKernelGGL(k_sort_scatter, dim3(L.nblk_cap), dim3(256), 0, s, p, hdr, k0, k1, v0,
+                         v1, ids_out, hist, totals, L.nblk_cap);
+    }
+    hipLaunchKernelGGL(k_sort_finish, dim3((unsigned)((L.cap + 255) / 256)), dim3(256), 0, s, hdr,
+                       k0, k1, v0, ids_out, keys_out);
+  }
+  hipLaunchKernelGGL(k_ranges, dim3((L.tiles + 255) / 256), dim3(256), 0, s, L.tiles, hdr, k0, k1,
+                     (int2*)ranges_out);
+  return check_launch("rgp_bin_sort");
+}
+
+extern "C" int rgp_fill_u32(void* dst, uint32_t value, int64_t n, hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_fill_u32: n < 0");
+  if (n == 0) return RGP_OK;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     (uint32_t*)dst, value, n);
+  return check_launch("rgp_fill_u32");
+}

@@ -1,0 +1,332 @@
+// Device-side helpers shared by the gfx950 kernels of the palette-weight
+// rasterizer.  All code here is compiled with -ffp-contract=off and correctly
+// rounded division and square root, so every FP32 operation is rounded once,
+// in source order: the arithmetic of the reference kernels
+// (gsplatcu/kernel.cu) written out without contraction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rgp.h"
+
+namespace rgp {
+
+constexpr int kTile = 16;               // BLOCK (common.cuh:13)
+constexpr int kTilePix = kTile * kTile;  // BLOCK_SIZE
+constexpr double kMinDepth = 0.2;       // MIN_DEPTH, a double literal (kernel.cu:10)
+constexpr float kBadMarker = -1.f;      // BAD_MARKER (kernel.cu:11)
+constexpr int kMaxDim = 8;              // MAX_SPLAT_DIM (kernel.cu:12)
+
+// Real SH constants, degree <= 3 (common.cuh:29-44).  Kept in double: the
+// reference multiplies them with floats in double precision.
+constexpr double SH_C0_0 = 0.28209479177387814;
+constexpr double SH_C1_0 = -0.4886025119029199;
+constexpr double SH_C1_1 = 0.4886025119029199;
+constexpr double SH_C1_2 = -0.4886025119029199;
+constexpr double SH_C2_0 = 1.0925484305920792;
+constexpr double SH_C2_1 = -1.0925484305920792;
+constexpr double SH_C2_2 = 0.31539156525252005;
+constexpr double SH_C2_3 = -1.0925484305920792;
+constexpr double SH_C2_4 = 0.5462742152960396;
+constexpr double SH_C3_0 = -0.5900435899266435;
+constexpr double SH_C3_1 = 2.890611442640554;
+constexpr double SH_C3_2 = -0.4570457994644658;
+constexpr double SH_C3_3 = 0.3731763325901154;
+constexpr double SH_C3_4 = -0.4570457994644658;
+constexpr double SH_C3_5 = 1.445305721320277;
+constexpr double SH_C3_6 = -0.5900435899266435;
+
+__device__ __forceinline__ float dmul(double c, float x) { return (float)(c * (double)x); }
+__device__ __forceinline__ float dmul2(double c, float x, float y) {
+  return (float)((c * (double)x) * (double)y);
+}
+
+// exp(x) for the compositing kernels: Cody-Waite reduction x = n ln2 + r
+// (ln2 split in a 16-bit head and a tail), e^r by its Taylor series to degree
+// 7 in Horner form with fused multiply-adds, then exact scaling by 2^n.  Max
+// error about 2 ulp; identical results on any IEEE-754 machine with fma.
+__device__ __forceinline__ float exp_cw(float x) {
+  if (!(x > -104.0f)) return (x != x) ? x : 0.0f;
+  if (x > 88.72283f) return __uint_as_float(0x7f800000u);
+  float n = __builtin_rintf(x * 1.44269504088896341f);
+  float r = __builtin_fmaf(n, -0.693145751953125f, x);
+  r = __builtin_fmaf(n, -1.42860676533018570e-06f, r);
+  float p = 1.98412698e-4f;
+  p = __builtin_fmaf(p, r, 1.38888889e-3f);
+  p = __builtin_fmaf(p, r, 8.33333377e-3f);
+  p = __builtin_fmaf(p, r, 4.16666679e-2f);
+  p = __builtin_fmaf(p, r, 1.66666672e-1f);
+  p = __builtin_fmaf(p, r, 0.5f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  int ni = (int)n;
+  if (ni < -126) { p = p * 5.42101086e-20f; ni += 64; }
+  if (ni > 127) { p = p * 2.0f; ni -= 1; }
+  return p * __uint_as_float((uint32_t)(ni + 127) << 23);
+}
+
+// Float -> int conversions that saturate and send NaN to 0.
+__device__ __forceinline__ int32_t sat_i32(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0f) return INT32_MAX;
+  if (x <= -2147483648.0f) return INT32_MIN;
+  return (int32_t)x;
+}
+__device__ __forceinline__ uint32_t sat_u32(float x) {
+  if (x != x) return 0u;
+  if (x >= 4294967296.0f) return UINT32_MAX;
+  if (x <= 0.0f) return 0u;
+  return (uint32_t)x;
+}
+
+// ---------------------------------------------------------------------------
+// per-Gaussian geometry (kernel.cu:379-663)
+// ---------------------------------------------------------------------------
+struct Proj {
+  float ux, uy, x, y, z;
+  bool ok;
+};
+
+// project (kernel.cu:635-654): pc = R pw + t, accumulated from 0 as in
+// Matrix::operator* (matrix.cuh:53-70).
+__device__ __forceinline__ Proj project_one(const float* R, const float* t, float px, float py,
+                                            float pz, float fx, float fy, float cx, float cy) {
+  const float pw[3] = {px, py, pz};
+  float pc[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s = s + R[r * 3 + k] * pw[k];
+    pc[r] = s + t[r];
+  }
+  Proj o;
+  o.x = pc[0]; o.y = pc[1]; o.z = pc[2];
+  o.ok = !((double)o.z < kMinDepth);
+  const float z_inv = 1.f / o.z;
+  o.ux = (o.x * fx) * z_inv + cx;
+  o.uy = (o.y * fy) * z_inv + cy;
+  return o;
+}
+
+// computeCov3D (kernel.cu:435-461) -> (S00, S01, S02, S11, S12, S22)
+__device__ __forceinline__ void cov3d_one(float w, float x, float y, float z, float s0, float s1,
+                                          float s2, float* c) {
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float xy = x * y, xz = x * z, yz = y * z;
+  const float xw = x * w, yw = y * w, zw = z * w;
+  const float R[9] = {1.f - 2.f * (yy + zz), 2.f * (xy - zw), 2.f * (xz + yw),
+                      2.f * (xy + zw), 1.f - 2.f * (xx + zz), 2.f * (yz - xw),
+                      2.f * (xz - yw), 2.f * (yz + xw), 1.f - 2.f * (xx + yy)};
+  const float s[3] = {s0, s1, s2};
+  float M[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) M[r * 3 + k] = R[r * 3 + k] * s[k];
+  float S[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc = acc + M[r * 3 + k] * M[q * 3 + k];
+      S[r * 3 + q] = acc;
+    }
+  c[0] = S[0]; c[1] = S[1]; c[2] = S[2]; c[3] = S[4]; c[4] = S[5]; c[5] = S[8];
+}
+
+// computeCov2D (kernel.cu:526-555) -> (a, b, c) with the 0.3 low-pass
+__device__ __forceinline__ void cov2d_one(const float* c3, float x, float y, float z,
+                                          const float* R, float fx, float fy, float tfx,
+                                          float tfy, float* c2) {
+  const float a = c3[0], b = c3[1], c = c3[2], d = c3[3], e = c3[4], f = c3[5];
+  const float limx = 1.3f * tfx;
+  const float limy = 1.3f * tfy;
+  x = fminf(limx, fmaxf(-limx, x / z)) * z;
+  y = fminf(limy, fmaxf(-limy, y / z)) * z;
+  const float z2 = z * z;
+  const float J[6] = {fx / z, 0.0f, -(fx * x) / z2, 0.0f, fy / z, -(fy * y) / z2};
+  const float S[9] = {a, b, c, b, d, e, c, e, f};
+  float M[6];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc = acc + J[r * 3 + k] * R[k * 3 + q];
+      M[r * 3 + q] = acc;
+    }
+  float MS[6];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc = acc + M[r * 3 + k] * S[k * 3 + q];
+      MS[r * 3 + q] = acc;
+    }
+  float SP[4];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc = acc + MS[r * 3 + k] * M[q * 3 + k];
+      SP[r * 2 + q] = acc;
+    }
+  c2[0] = SP[0] + 0.3f;
+  c2[1] = SP[1];
+  c2[2] = SP[3] + 0.3f;
+}
+
+// inverseCov2D (kernel.cu:391-400).  Returns false when the inverse
+// determinant is NaN (the Gaussian is culled).
+__device__ __forceinline__ bool inverse_one(float a, float b, float c, float* ci, int* ar) {
+  const float det_inv = 1.f / (a * c - b * b);
+  if (det_inv != det_inv) return false;
+  ci[0] = det_inv * c;
+  ci[1] = -det_inv * b;
+  ci[2] = det_inv * a;
+  ar[0] = sat_i32(ceilf(3.f * sqrtf(fabsf(a))));
+  ar[1] = sat_i32(ceilf(3.f * sqrtf(fabsf(c))));
+  return true;
+}
+
+// SH basis Y[1..15] of the view direction (kernel.cu:718-755, 924-956).
+__device__ __forceinline__ void sh_basis(float px, float py, float pz, const float* twc, int sh_dim,
+                                         float* Y) {
+  const float d0 = px - twc[0], d1 = py - twc[1], d2 = pz - twc[2];
+  const float normd_inv = 1.f / sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+  const float x = d0 * normd_inv, y = d1 * normd_inv, z = d2 * normd_inv;
+  Y[1] = dmul(SH_C1_0, y);
+  Y[2] = dmul(SH_C1_1, z);
+  Y[3] = dmul(SH_C1_2, x);
+  if (sh_dim > 4) {
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    Y[4] = dmul(SH_C2_0, xy);
+    Y[5] = dmul(SH_C2_1, yz);
+    Y[6] = dmul(SH_C2_2, 2.f * zz - xx - yy);
+    Y[7] = dmul(SH_C2_3, xz);
+    Y[8] = dmul(SH_C2_4, xx - yy);
+    if (sh_dim > 9) {
+      Y[9] = dmul2(SH_C3_0, y, 3.f * xx - yy);
+      Y[10] = dmul2(SH_C3_1, xy, z);
+      Y[11] = dmul2(SH_C3_2, y, 4.f * zz - xx - yy);
+      Y[12] = dmul2(SH_C3_3, z, 2.f * zz - 3.f * xx - 3.f * yy);
+      Y[13] = dmul2(SH_C3_4, x, 4.f * zz - xx - yy);
+      Y[14] = dmul2(SH_C3_5, z, xx - yy);
+      Y[15] = dmul2(SH_C3_6, x, xx - 3.f * yy);
+    }
+  }
+}
+
+// Full-rank weights (kernel.cu:712-767).  sh: band-major coefficients of one
+// Gaussian, sh[lm * K + k].
+template <int K>
+__device__ __forceinline__ void sh_weights_full(const float* __restrict__ sh, const float* Y,
+                                                int sh_dim, float* w) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) w[k] = (float)(0.0 + SH_C0_0 * (double)sh[k]);
+  if (sh_dim > 1) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      w[k] = w[k] + ((Y[1] * sh[1 * K + k] + Y[2] * sh[2 * K + k]) + Y[3] * sh[3 * K + k]);
+    if (sh_dim > 4) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        w[k] = w[k] + ((((Y[4] * sh[4 * K + k] + Y[5] * sh[5 * K + k]) + Y[6] * sh[6 * K + k]) +
+                        Y[7] * sh[7 * K + k]) +
+                       Y[8] * sh[8 * K + k]);
+      if (sh_dim > 9) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          w[k] = w[k] + ((((((Y[9] * sh[9 * K + k] + Y[10] * sh[10 * K + k]) +
+                             Y[11] * sh[11 * K + k]) +
+                            Y[12] * sh[12 * K + k]) +
+                           Y[13] * sh[13 * K + k]) +
+                          Y[14] * sh[14 * K + k]) +
+                         Y[15] * sh[15 * K + k]);
+      }
+    }
+  }
+}
+
+// Rank-r low-rank weights (kernel.cu:966-981 generalised): a (P, r), b (r, Q).
+template <int K>
+__device__ __forceinline__ void sh_weights_lowrank(const float* __restrict__ dc,
+                                                   const float* __restrict__ a,
+                                                   const float* __restrict__ b, const float* Y,
+                                                   int L, int Q, int rank, float* w) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) w[k] = dmul(SH_C0_0, dc[k]);
+  if (L <= 0) return;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    float acc = 0.f;
+    int g = k;
+    int row = g / Q, col = g - row * Q;
+    for (int lm = 0; lm < L; ++lm) {
+      float c = a[row * rank] * b[col];
+      for (int r = 1; r < rank; ++r) c = c + a[row * rank + r] * b[r * Q + col];
+      acc = acc + c * Y[lm + 1];
+      col += K;  // g += K
+      while (col >= Q) { col -= Q; ++row; }
+    }
+    w[k] = w[k] + acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// splat record: what the compositing kernels read per list entry.
+//   f4[0] = (u.x, u.y, cinv.x, 2*cinv.y)   f4[1] = (cinv.z, alpha, mcut, 0)
+//   f4[2..] = the dim feature channels (weights or colours), zero padded.
+// 2*cinv.y is exact, so the Mahalanobis distance below is bit-identical to
+// the reference expression.  mcut bounds the distance beyond which
+// alpha * exp(-maha/2) < 0.002 with a margin far above rounding, so a skip on
+// maha > mcut never changes a result; NaN (alpha NaN) disables the skip.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float mcut_of(float alpha) {
+  if (alpha != alpha) return alpha;  // NaN: never skip
+  if (alpha < 0.002f) return -1.0f;  // alpha * e <= alpha < 0.002 for every e <= 1
+  return 2.0f * logf(alpha / 0.002f) * 1.00001f + 1e-3f;
+}
+
+template <int DIM>
+struct RecStride {
+  static constexpr int f4 = 2 + (DIM + 3) / 4;
+};
+
+// Mahalanobis distance of mahaSqDist (common.cuh:86-89), clamped at 0.
+__device__ __forceinline__ float maha_dist(float4 g0, float cz, float dx, float dy) {
+  const float m = (g0.z * dx * dx + cz * dy * dy) + g0.w * dx * dy;
+  return fmaxf(0.0f, m);
+}
+
+// Blocks of one launch go round-robin to the 8 XCDs; give each XCD a
+// contiguous run of tiles so neighbouring tiles share its L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+  const int q = nblocks >> 3, rem = nblocks & 7;
+  const int x = b & 7, l = b >> 3;
+  return x * q + (x < rem ? x : rem) + l;
+}
+
+// Sum over the 64 lanes of a wavefront with DPP row operations; the total
+// lands in lane 63.
+#define RGP_DPP(v, ctrl, rmask, bmask, bound)                                               \
+  __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, rmask, bmask, bound))
+__device__ __forceinline__ float wave_sum_to63(float v) {
+  v = v + RGP_DPP(v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+  v = v + RGP_DPP(v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+  v = v + RGP_DPP(v, 0x124, 0xF, 0xF, false);  // row_ror:4
+  v = v + RGP_DPP(v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  v = v + RGP_DPP(v, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1, 3
+  v = v + RGP_DPP(v, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+
+}  // namespace rgp

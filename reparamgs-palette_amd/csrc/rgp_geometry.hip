@@ -1,0 +1,638 @@
+// Per-Gaussian kernels of the palette-weight rasterizer for gfx950: the
+// unfused gsplatcu operators (project, computeCov3D, computeCov2D,
+// inverseCov2D, sh2Weight, sh2Weight_fast, sh2Weight_fast_B, sh2Color), the
+// fused preprocess that replaces their chain in WeightGSFunctionTest/Fast
+// (gsplat/gsmodel.py:107-118, 155-170), record packing, and the SH backward.
+// One thread per Gaussian; every stage is HBM-bound.
+#include "rgp_internal.h"
+
+namespace rgp {
+
+// ---------------------------------------------------------------------------
+// unfused operators
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_project(int n, const float* __restrict__ pws,
+                                                 const float* __restrict__ Rcw,
+                                                 const float* __restrict__ tcw, float fx, float fy,
+                                                 float cx, float cy, float* __restrict__ us,
+                                                 float* __restrict__ pcs,
+                                                 float* __restrict__ depths) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float R[9], t[3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = Rcw[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) t[k] = tcw[k];
+  const Proj p = project_one(R, t, pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], fx, fy, cx, cy);
+  if (!p.ok) { depths[i] = kBadMarker; return; }
+  us[2 * i] = p.ux; us[2 * i + 1] = p.uy;
+  pcs[3 * i] = p.x; pcs[3 * i + 1] = p.y; pcs[3 * i + 2] = p.z;
+  depths[i] = p.z;
+}
+
+__global__ __launch_bounds__(256) void k_cov3d(int n, const float4* __restrict__ rots,
+                                               const float* __restrict__ scales,
+                                               const float* __restrict__ depths,
+                                               float* __restrict__ cov3ds) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if ((double)depths[i] < kMinDepth) return;
+  const float4 q = rots[i];
+  float c[6];
+  cov3d_one(q.x, q.y, q.z, q.w, scales[3 * i], scales[3 * i + 1], scales[3 * i + 2], c);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) cov3ds[6 * i + k] = c[k];
+}
+
+__global__ __launch_bounds__(256) void k_cov2d(int n, const float* __restrict__ cov3ds,
+                                               const float* __restrict__ pcs,
+                                               const float* __restrict__ Rcw,
+                                               const float* __restrict__ depths, float fx,
+                                               float fy, float tfx, float tfy,
+                                               float* __restrict__ cov2ds) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if ((double)depths[i] < kMinDepth) return;
+  float R[9], c3[6], c2[3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = Rcw[k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) c3[k] = cov3ds[6 * i + k];
+  cov2d_one(c3, pcs[3 * i], pcs[3 * i + 1], pcs[3 * i + 2], R, fx, fy, tfx, tfy, c2);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) cov2ds[3 * i + k] = c2[k];
+}
+
+__global__ __launch_bounds__(256) void k_inverse(int n, const float* __restrict__ cov2ds,
+                                                 float* __restrict__ depths,
+                                                 float* __restrict__ cinv,
+                                                 int32_t* __restrict__ areas) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if ((double)depths[i] < kMinDepth) return;
+  float ci[3];
+  int ar[2];
+  if (!inverse_one(cov2ds[3 * i], cov2ds[3 * i + 1], cov2ds[3 * i + 2], ci, ar)) {
+    depths[i] = kBadMarker;
+    return;
+  }
+  cinv[3 * i] = ci[0]; cinv[3 * i + 1] = ci[1]; cinv[3 * i + 2] = ci[2];
+  areas[2 * i] = ar[0]; areas[2 * i + 1] = ar[1];
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_sh2weight(int n, int sh_dim, const float* __restrict__ shs,
+                                                   const float* __restrict__ pws,
+                                                   const float* __restrict__ twc,
+                                                   float* __restrict__ weights,
+                                                   float* __restrict__ basis) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float Y[16];
+#pragma unroll
+  for (int l = 0; l < 16; ++l) Y[l] = 0.f;
+  const float tw[3] = {twc[0], twc[1], twc[2]};
+  if (sh_dim > 1) sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], tw, sh_dim, Y);
+  float w[K];
+  sh_weights_full<K>(shs + (size_t)i * sh_dim * K, Y, sh_dim, w);
+#pragma unroll
+  for (int k = 0; k < K; ++k) weights[(size_t)i * K + k] = w[k];
+  if (basis) {
+    basis[(size_t)i * sh_dim] = (float)SH_C0_0;
+    for (int l = 1; l < sh_dim; ++l) basis[(size_t)i * sh_dim + l] = Y[l];
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_sh2weight_fast(int n, int P, int Q, int rank,
+                                                        const float* __restrict__ dc,
+                                                        const float* __restrict__ A,
+                                                        const float* __restrict__ B,
+                                                        const float* __restrict__ pws,
+                                                        const float* __restrict__ twc,
+                                                        float* __restrict__ weights,
+                                                        float* __restrict__ dweight_ddc,
+                                                        float* __restrict__ ylm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int L = P * Q / K;
+  float Y[16];
+#pragma unroll
+  for (int l = 0; l < 16; ++l) Y[l] = 0.f;
+  const float tw[3] = {twc[0], twc[1], twc[2]};
+  if (L > 0) sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], tw, L + 1, Y);
+  float w[K];
+  sh_weights_lowrank<K>(dc + (size_t)i * K, A + (size_t)i * P * rank, B + (size_t)i * rank * Q, Y,
+                        L, Q, rank, w);
+#pragma unroll
+  for (int k = 0; k < K; ++k) weights[(size_t)i * K + k] = w[k];
+  if (dweight_ddc) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) dweight_ddc[(size_t)i * K + k] = (float)SH_C0_0;
+  }
+  if (ylm)
+    for (int l = 0; l < L; ++l) ylm[(size_t)i * L + l] = Y[l + 1];
+}
+
+__global__ __launch_bounds__(256) void k_sh2color(int n, int sh_dim, const float* __restrict__ shs,
+                                                  const float* __restrict__ pws,
+                                                  const float* __restrict__ twc,
+                                                  float* __restrict__ colors) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* sh = shs + (size_t)i * sh_dim * 3;
+  const float c0 = (float)SH_C0_0;
+  float col[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) col[c] = c0 * sh[c];
+  if (sh_dim > 1) {
+    float Y[16];
+    const float tw[3] = {twc[0], twc[1], twc[2]};
+    sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], tw, sh_dim, Y);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      col[c] = col[c] + ((Y[1] * sh[3 + c] + Y[2] * sh[6 + c]) + Y[3] * sh[9 + c]);
+    if (sh_dim > 4) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        col[c] = col[c] + ((((Y[4] * sh[12 + c] + Y[5] * sh[15 + c]) + Y[6] * sh[18 + c]) +
+                            Y[7] * sh[21 + c]) +
+                           Y[8] * sh[24 + c]);
+      if (sh_dim > 9) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          col[c] = col[c] + ((((((Y[9] * sh[27 + c] + Y[10] * sh[30 + c]) + Y[11] * sh[33 + c]) +
+                                Y[12] * sh[36 + c]) +
+                               Y[13] * sh[39 + c]) +
+                              Y[14] * sh[42 + c]) +
+                             Y[15] * sh[45 + c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) colors[(size_t)i * 3 + c] = col[c];
+}
+
+// Low-rank factor backward (kernel.cu:1019-1031, rank r).  The reference's
+// atomics hit rows private to the thread, so its sums run in loop order (k
+// outer, lm inner); this kernel keeps that order with plain read-modify-write.
+// ylm may be NULL: Y is then recomputed from pws and twc.
+__global__ __launch_bounds__(256) void k_lowrank_backward(
+    int n, int K, int P, int Q, int rank, const float* __restrict__ A, const float* __restrict__ B,
+    const float* __restrict__ ylm, const float* __restrict__ pws, const float* __restrict__ twc,
+    const float* __restrict__ dw, float* __restrict__ ddc, float* __restrict__ dA,
+    float* __restrict__ dB) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int L = P * Q / K;
+  float Y[16];
+#pragma unroll
+  for (int l = 0; l < 16; ++l) Y[l] = 0.f;
+  if (ylm) {
+    for (int l = 0; l < L; ++l) Y[l + 1] = ylm[(size_t)i * L + l];
+  } else if (L > 0) {
+    const float tw[3] = {twc[0], twc[1], twc[2]};
+    sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], tw, L + 1, Y);
+  }
+  const float* a = A + (size_t)i * P * rank;
+  const float* b = B + (size_t)i * rank * Q;
+  float* da = dA + (size_t)i * P * rank;
+  float* db = dB + (size_t)i * rank * Q;
+  for (int j = 0; j < P * rank; ++j) da[j] = 0.f;
+  for (int j = 0; j < rank * Q; ++j) db[j] = 0.f;
+  for (int k = 0; k < K; ++k) {
+    const float dwk = dw[(size_t)i * K + k];
+    if (ddc) ddc[(size_t)i * K + k] = dwk * (float)SH_C0_0;
+    int row = k / Q, col = k - (k / Q) * Q;
+    for (int lm = 0; lm < L; ++lm) {
+      const float y = Y[lm + 1];
+      for (int r = 0; r < rank; ++r) {
+        da[row * rank + r] = da[row * rank + r] + dwk * b[r * Q + col] * y;
+        db[r * Q + col] = db[r * Q + col] + dwk * a[row * rank + r] * y;
+      }
+      col += K;
+      while (col >= Q) { col -= Q; ++row; }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// records
+// ---------------------------------------------------------------------------
+template <int DIM>
+__device__ __forceinline__ void store_record(float4* __restrict__ rec, float ux, float uy,
+                                             const float* ci, float alpha, const float* f) {
+  constexpr int F4 = RecStride<DIM>::f4;
+  rec[0] = make_float4(ux, uy, ci[0], 2.f * ci[1]);
+  rec[1] = make_float4(ci[2], alpha, mcut_of(alpha), 0.f);
+#pragma unroll
+  for (int q = 0; q < F4 - 2; ++q) {
+    float v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = (4 * q + e < DIM) ? f[4 * q + e] : 0.f;
+    rec[2 + q] = make_float4(v[0], v[1], v[2], v[3]);
+  }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_pack(int n, const float* __restrict__ us,
+                                              const float* __restrict__ cinv,
+                                              const float* __restrict__ alphas,
+                                              const float* __restrict__ feat,
+                                              float4* __restrict__ recs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float f[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) f[k] = feat[(size_t)i * DIM + k];
+  const float ci[3] = {cinv[3 * i], cinv[3 * i + 1], cinv[3 * i + 2]};
+  store_record<DIM>(recs + (size_t)i * RecStride<DIM>::f4, us[2 * i], us[2 * i + 1], ci, alphas[i],
+                    f);
+}
+
+// getRects (kernel.cu:127-162): returns the tile count (0 = culled) and the
+// packed rect (x0 | y0 << 16, x1 | y1 << 16).
+__device__ __forceinline__ uint32_t rect_of(float ux, float uy, int ax, int ay, int gx, int gy,
+                                            uint2* rect) {
+  const float xs = (float)ax, ys = (float)ay;
+  auto cl = [](int32_t v, int g) -> uint32_t {
+    const int32_t m = v > 0 ? v : 0;
+    return (uint32_t)m < (uint32_t)g ? (uint32_t)m : (uint32_t)g;
+  };
+  const uint32_t x0 = cl(sat_i32((ux - xs) / 16.f), gx);
+  const uint32_t y0 = cl(sat_i32((uy - ys) / 16.f), gy);
+  const uint32_t x1 = cl(sat_i32((((ux + xs) + 16.f) - 1.f) / 16.f), gx);
+  const uint32_t y1 = cl(sat_i32((((uy + ys) + 16.f) - 1.f) / 16.f), gy);
+  rect->x = x0 | (y0 << 16);
+  rect->y = x1 | (y1 << 16);
+  return (y1 - y0) * (x1 - x0);
+}
+
+__global__ __launch_bounds__(256) void k_rects(int n, const float* __restrict__ us,
+                                               int32_t* __restrict__ areas,
+                                               float* __restrict__ depths, int gx, int gy,
+                                               uint2* __restrict__ rects,
+                                               uint32_t* __restrict__ counts,
+                                               uint32_t* __restrict__ dq_max) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t dq = 0;
+  if (i < n) {
+    uint32_t cnt = 0;
+    const float depth = depths[i];
+    if (!((double)depth < kMinDepth)) {
+      uint2 r;
+      cnt = rect_of(us[2 * i], us[2 * i + 1], areas[2 * i], areas[2 * i + 1], gx, gy, &r);
+      if (cnt == 0) {
+        depths[i] = kBadMarker;
+        areas[2 * i] = 0; areas[2 * i + 1] = 0;
+      } else {
+        rects[i] = r;
+        dq = sat_u32(depth * 1000.f);
+      }
+    }
+    counts[i] = cnt;
+  }
+  dq = wave_max_u32(dq);
+  if ((threadIdx.x & 63) == 0 && dq) atomicMax(dq_max, dq);
+}
+
+// ---------------------------------------------------------------------------
+// fused preprocess (WeightGSFunctionTest/Fast forward up to getRects)
+// ---------------------------------------------------------------------------
+template <int K, bool LOWRANK>
+__global__ __launch_bounds__(256) void k_preprocess(
+    int n, int sh_dim, int P, int Q, int rank, const rgp_camera* __restrict__ cam,
+    const float* __restrict__ pws, const float4* __restrict__ rots,
+    const float* __restrict__ scales, const float* __restrict__ alphas,
+    const float* __restrict__ shs, const float* __restrict__ dc, const float* __restrict__ A,
+    const float* __restrict__ B, float* __restrict__ depths, float4* __restrict__ recs,
+    uint2* __restrict__ rects, uint32_t* __restrict__ counts, uint32_t* __restrict__ dq_max,
+    int gx, int gy, float* __restrict__ us_out, float* __restrict__ cinv_out,
+    int32_t* __restrict__ areas_out, float* __restrict__ w_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t dqv = 0;
+  if (i < n) {
+    const float px = pws[3 * i], py = pws[3 * i + 1], pz = pws[3 * i + 2];
+    const Proj p = project_one(cam->Rcw, cam->tcw, px, py, pz, cam->fx, cam->fy, cam->cx, cam->cy);
+    // palette weights are evaluated for every Gaussian (kernel.cu:670-767
+    // has no depth test)
+    float Y[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) Y[l] = 0.f;
+    float w[K];
+    if (!LOWRANK) {
+      if (sh_dim > 1) sh_basis(px, py, pz, cam->twc, sh_dim, Y);
+      sh_weights_full<K>(shs + (size_t)i * sh_dim * K, Y, sh_dim, w);
+    } else {
+      const int L = P * Q / K;
+      if (L > 0) sh_basis(px, py, pz, cam->twc, L + 1, Y);
+      sh_weights_lowrank<K>(dc + (size_t)i * K, A + (size_t)i * P * rank,
+                            B + (size_t)i * rank * Q, Y, L, Q, rank, w);
+    }
+    if (w_out) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) w_out[(size_t)i * K + k] = w[k];
+    }
+    float depth = kBadMarker;
+    uint32_t cnt = 0;
+    if (p.ok) {
+      depth = p.z;
+      if (us_out) { us_out[2 * i] = p.ux; us_out[2 * i + 1] = p.uy; }
+      const float4 q = rots[i];
+      float c3[6], c2[3], ci[3];
+      int ar[2];
+      cov3d_one(q.x, q.y, q.z, q.w, scales[3 * i], scales[3 * i + 1], scales[3 * i + 2], c3);
+      cov2d_one(c3, p.x, p.y, p.z, cam->Rcw, cam->fx, cam->fy, cam->tan_fovx, cam->tan_fovy, c2);
+      if (!inverse_one(c2[0], c2[1], c2[2], ci, ar)) {
+        depth = kBadMarker;
+      } else {
+        if (cinv_out) { cinv_out[3 * i] = ci[0]; cinv_out[3 * i + 1] = ci[1]; cinv_out[3 * i + 2] = ci[2]; }
+        uint2 r;
+        cnt = rect_of(p.ux, p.uy, ar[0], ar[1], gx, gy, &r);
+        if (cnt == 0) {
+          depth = kBadMarker;
+          ar[0] = 0; ar[1] = 0;
+        } else {
+          rects[i] = r;
+          dqv = sat_u32(depth * 1000.f);
+          store_record<K>(recs + (size_t)i * RecStride<K>::f4, p.ux, p.uy, ci, alphas[i], w);
+        }
+        if (areas_out) { areas_out[2 * i] = ar[0]; areas_out[2 * i + 1] = ar[1]; }
+      }
+    }
+    depths[i] = depth;
+    counts[i] = cnt;
+  }
+  dqv = wave_max_u32(dqv);
+  if ((threadIdx.x & 63) == 0 && dqv) atomicMax(dq_max, dqv);
+}
+
+// ---------------------------------------------------------------------------
+// full-rank SH backward: dshs[n, lm*K + k] = dw[n,k] * Y_lm(n)
+// (gsmodel.py:140-143).  A block owns 128 Gaussians; their output rows are
+// one contiguous span written with coalesced stores from LDS-held factors.
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(256) void k_sh_backward_full(int n, int sh_dim,
+                                                          const rgp_camera* __restrict__ cam,
+                                                          const float* __restrict__ pws,
+                                                          const float* __restrict__ dw,
+                                                          float* __restrict__ dshs) {
+  constexpr int G = 128;
+  __shared__ float sY[G][17];
+  __shared__ float sW[G][K];
+  const int g0 = blockIdx.x * G;
+  const int t = threadIdx.x;
+  if (t < G) {
+    const int i = g0 + t;
+    float Y[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) Y[l] = 0.f;
+    if (i < n) {
+      if (sh_dim > 1) sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], cam->twc, sh_dim, Y);
+#pragma unroll
+      for (int k = 0; k < K; ++k) sW[t][k] = dw[(size_t)i * K + k];
+    }
+    Y[0] = (float)SH_C0_0;
+#pragma unroll
+    for (int l = 0; l < 16; ++l) sY[t][l] = Y[l];
+  }
+  __syncthreads();
+  const int cnt = min(G, n - g0);
+  const int row = sh_dim * K;
+  const int64_t total = (int64_t)cnt * row;
+  float* out = dshs + (size_t)g0 * row;
+  for (int64_t e = t; e < total; e += blockDim.x) {
+    const int gl = (int)(e / row);
+    const int r = (int)(e - (int64_t)gl * row);
+    const int lm = r / K, k = r - lm * K;
+    out[e] = sW[gl][k] * sY[gl][lm];
+  }
+}
+
+}  // namespace rgp
+
+using namespace rgp;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+static inline int grid1(int n) { return (n + 255) / 256; }
+
+extern "C" int rgp_project(int n, const float* pws, const float* Rcw, const float* tcw, float fx,
+                           float fy, float cx, float cy, float* us, float* pcs, float* depths,
+                           hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_project: n < 0");
+  if (n == 0) return RGP_OK;
+  hipLaunchKernelGGL(k_project, dim3(grid1(n)), dim3(256), 0, s, n, pws, Rcw, tcw, fx, fy, cx, cy,
+                     us, pcs, depths);
+  return check_launch("rgp_project");
+}
+
+extern "C" int rgp_compute_cov3d(int n, const float* rots, const float* scales,
+                                 const float* depths, float* cov3ds, hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_compute_cov3d: n < 0");
+  if (n == 0) return RGP_OK;
+  hipLaunchKernelGGL(k_cov3d, dim3(grid1(n)), dim3(256), 0, s, n, (const float4*)rots, scales,
+                     depths, cov3ds);
+  return check_launch("rgp_compute_cov3d");
+}
+
+extern "C" int rgp_compute_cov2d(int n, const float* cov3ds, const float* pcs, const float* Rcw,
+                                 const float* depths, float fx, float fy, float width,
+                                 float height, float* cov2ds, hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_compute_cov2d: n < 0");
+  if (n == 0) return RGP_OK;
+  // gausplat.cu:357-358, in float
+  const float tfx = width / (2.f * fx);
+  const float tfy = height / (2.f * fy);
+  hipLaunchKernelGGL(k_cov2d, dim3(grid1(n)), dim3(256), 0, s, n, cov3ds, pcs, Rcw, depths, fx,
+                     fy, tfx, tfy, cov2ds);
+  return check_launch("rgp_compute_cov2d");
+}
+
+extern "C" int rgp_inverse_cov2d(int n, const float* cov2ds, float* depths, float* cinv2ds,
+                                 int32_t* areas, hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_inverse_cov2d: n < 0");
+  if (n == 0) return RGP_OK;
+  hipLaunchKernelGGL(k_inverse, dim3(grid1(n)), dim3(256), 0, s, n, cov2ds, depths, cinv2ds,
+                     areas);
+  return check_launch("rgp_inverse_cov2d");
+}
+
+#define RGP_K_SWITCH(K, CALL)                    \
+  switch (K) {                                   \
+    case 1: { constexpr int KK = 1; CALL; } break; \
+    case 2: { constexpr int KK = 2; CALL; } break; \
+    case 3: { constexpr int KK = 3; CALL; } break; \
+    case 4: { constexpr int KK = 4; CALL; } break; \
+    case 5: { constexpr int KK = 5; CALL; } break; \
+    case 6: { constexpr int KK = 6; CALL; } break; \
+    case 7: { constexpr int KK = 7; CALL; } break; \
+    case 8: { constexpr int KK = 8; CALL; } break; \
+    default: break;                              \
+  }
+
+extern "C" int rgp_sh2weight(int n, int K, int sh_dim, const float* shs, const float* pws,
+                             const float* twc, float* weights, float* basis, hipStream_t s) {
+  if (n < 0 || K < 1 || K > RGP_MAX_DIM)
+    return set_error(RGP_ERR_ARG, "rgp_sh2weight: need n >= 0 and 1 <= K <= 8");
+  if (!(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
+    return set_error(RGP_ERR_ARG, "rgp_sh2weight: sh_dim must be 1, 4, 9 or 16");
+  if (n == 0) return RGP_OK;
+  RGP_K_SWITCH(K, hipLaunchKernelGGL(k_sh2weight<KK>, dim3(grid1(n)), dim3(256), 0, s, n, sh_dim,
+                                     shs, pws, twc, weights, basis));
+  return check_launch("rgp_sh2weight");
+}
+
+static int check_lowrank(const char* who, int K, int P, int Q, int rank) {
+  if (K < 1 || K > RGP_MAX_DIM) return set_error(RGP_ERR_ARG, "%s: need 1 <= K <= 8", who);
+  if (P < 1 || Q < 1 || rank < 1) return set_error(RGP_ERR_ARG, "%s: P, Q, rank must be >= 1", who);
+  if ((P * Q) % K != 0) return set_error(RGP_ERR_ARG, "%s: P*Q must be a multiple of K", who);
+  const int L = P * Q / K;
+  if (!(L == 0 || L == 3 || L == 8 || L == 15))
+    return set_error(RGP_ERR_ARG, "%s: P*Q/K must be 3, 8 or 15 (SH degree 1-3)", who);
+  return RGP_OK;
+}
+
+extern "C" int rgp_sh2weight_fast(int n, int K, int P, int Q, int rank, const float* dc,
+                                  const float* A, const float* B, const float* pws,
+                                  const float* twc, float* weights, float* dweight_ddc,
+                                  float* ylm, hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_sh2weight_fast: n < 0");
+  int rc = check_lowrank("rgp_sh2weight_fast", K, P, Q, rank);
+  if (rc) return rc;
+  if (n == 0) return RGP_OK;
+  RGP_K_SWITCH(K, hipLaunchKernelGGL(k_sh2weight_fast<KK>, dim3(grid1(n)), dim3(256), 0, s, n, P,
+                                     Q, rank, dc, A, B, pws, twc, weights, dweight_ddc, ylm));
+  return check_launch("rgp_sh2weight_fast");
+}
+
+extern "C" int rgp_sh2weight_fast_backward(int n, int K, int P, int Q, int rank, const float* A,
+                                           const float* B, const float* ylm,
+                                           const float* dloss_dw, float* dA, float* dB,
+                                           hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_sh2weight_fast_backward: n < 0");
+  int rc = check_lowrank("rgp_sh2weight_fast_backward", K, P, Q, rank);
+  if (rc) return rc;
+  if (!ylm) return set_error(RGP_ERR_ARG, "rgp_sh2weight_fast_backward: ylm is required");
+  if (n == 0) return RGP_OK;
+  hipLaunchKernelGGL(k_lowrank_backward, dim3(grid1(n)), dim3(256), 0, s, n, K, P, Q, rank, A, B,
+                     ylm, (const float*)nullptr, (const float*)nullptr, dloss_dw, (float*)nullptr,
+                     dA, dB);
+  return check_launch("rgp_sh2weight_fast_backward");
+}
+
+extern "C" int rgp_sh2color(int n, int sh_dim, const float* shs, const float* pws,
+                            const float* twc, float* colors, hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_sh2color: n < 0");
+  if (!(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
+    return set_error(RGP_ERR_ARG, "rgp_sh2color: sh_dim must be 1, 4, 9 or 16");
+  if (n == 0) return RGP_OK;
+  hipLaunchKernelGGL(k_sh2color, dim3(grid1(n)), dim3(256), 0, s, n, sh_dim, shs, pws, twc,
+                     colors);
+  return check_launch("rgp_sh2color");
+}
+
+extern "C" int rgp_record_floats(int dim) {
+  if (dim < 1 || dim > RGP_MAX_DIM) return 0;
+  return 8 + 4 * ((dim + 3) / 4);
+}
+
+extern "C" int rgp_pack_records(int n, int dim, const float* us, const float* cinv2ds,
+                                const float* alphas, const float* feat, float* records,
+                                hipStream_t s) {
+  if (n < 0 || dim < 1 || dim > RGP_MAX_DIM)
+    return set_error(RGP_ERR_ARG, "rgp_pack_records: need n >= 0 and 1 <= dim <= 8");
+  if (n == 0) return RGP_OK;
+  RGP_K_SWITCH(dim, hipLaunchKernelGGL(k_pack<KK>, dim3(grid1(n)), dim3(256), 0, s, n, us,
+                                       cinv2ds, alphas, feat, (float4*)records));
+  return check_launch("rgp_pack_records");
+}
+
+extern "C" int rgp_bin_rects(int n, const float* us, int32_t* areas, float* depths, int width,
+                             int height, void* ws, int64_t capacity, hipStream_t s) {
+  if (n < 0 || width <= 0 || height <= 0)
+    return set_error(RGP_ERR_ARG, "rgp_bin_rects: bad size");
+  BinLayout L = bin_layout(n, capacity, width, height);
+  char* base = (char*)ws;
+  int rc = bin_reset_header(ws, s);
+  if (rc) return rc;
+  if (n == 0) return RGP_OK;
+  hipLaunchKernelGGL(k_rects, dim3(grid1(n)), dim3(256), 0, s, n, us, areas, depths, L.gx, L.gy,
+                     (uint2*)(base + L.rects), (uint32_t*)(base + L.counts),
+                     (uint32_t*)(base + L.header) + kHdrDqMax);
+  return check_launch("rgp_bin_rects");
+}
+
+extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int rank,
+                                     const rgp_camera* cam, const float* pws, const float* rots,
+                                     const float* scales, const float* alphas, const float* shs,
+                                     const float* dc, const float* A, const float* B,
+                                     float* depths, float* records, void* ws, int64_t capacity,
+                                     int width, int height, float* us, float* cinv2ds,
+                                     int32_t* areas, float* weights, hipStream_t s) {
+  if (n < 0 || K < 1 || K > RGP_MAX_DIM || width <= 0 || height <= 0)
+    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight: bad size or K");
+  if (mode == 1) {
+    int rc = check_lowrank("rgp_preprocess_weight", K, P, Q, rank);
+    if (rc) return rc;
+  } else if (mode != 0) {
+    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight: mode must be 0 or 1");
+  }
+  // mode 0: P carries the number of SH coefficients per channel
+  const int sh_dim = mode == 0 ? P : 16;
+  if (mode == 0 && !(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
+    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight: sh_dim (P) must be 1, 4, 9 or 16");
+  BinLayout L = bin_layout(n, capacity, width, height);
+  char* base = (char*)ws;
+  int rc = bin_reset_header(ws, s);
+  if (rc) return rc;
+  if (n == 0) return RGP_OK;
+  if (mode == 0) {
+    RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess<KK, false>), dim3(grid1(n)), dim3(256), 0, s,
+                                       n, sh_dim, P, Q, rank, cam, pws, (const float4*)rots,
+                                       scales, alphas, shs, dc, A, B, depths, (float4*)records,
+                                       (uint2*)(base + L.rects), (uint32_t*)(base + L.counts),
+                                       (uint32_t*)(base + L.header) + kHdrDqMax, L.gx, L.gy, us,
+                                       cinv2ds, areas, weights));
+  } else {
+    RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess<KK, true>), dim3(grid1(n)), dim3(256), 0, s,
+                                       n, sh_dim, P, Q, rank, cam, pws, (const float4*)rots,
+                                       scales, alphas, shs, dc, A, B, depths, (float4*)records,
+                                       (uint2*)(base + L.rects), (uint32_t*)(base + L.counts),
+                                       (uint32_t*)(base + L.header) + kHdrDqMax, L.gx, L.gy, us,
+                                       cinv2ds, areas, weights));
+  }
+  return check_launch("rgp_preprocess_weight");
+}
+
+extern "C" int rgp_sh_backward_full(int n, int K, int sh_dim, const rgp_camera* cam,
+                                    const float* pws, const float* dw, float* dshs,
+                                    hipStream_t s) {
+  if (n < 0 || K < 1 || K > RGP_MAX_DIM)
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_full: bad n or K");
+  if (!(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_full: sh_dim must be 1, 4, 9 or 16");
+  if (n == 0) return RGP_OK;
+  const int blocks = (n + 127) / 128;
+  RGP_K_SWITCH(K, hipLaunchKernelGGL(k_sh_backward_full<KK>, dim3(blocks), dim3(256), 0, s, n,
+                                     sh_dim, cam, pws, dw, dshs));
+  return check_launch("rgp_sh_backward_full");
+}
+
+extern "C" int rgp_sh_backward_lowrank(int n, int K, int P, int Q, int rank,
+                                       const rgp_camera* cam, const float* pws, const float* A,
+                                       const float* B, const float* dw, float* ddc, float* dA,
+                                       float* dB, hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_sh_backward_lowrank: n < 0");
+  int rc = check_lowrank("rgp_sh_backward_lowrank", K, P, Q, rank);
+  if (rc) return rc;
+  if (n == 0) return RGP_OK;
+  // cam is a device pointer: pass the address of its twc field
+  const float* twc = (const float*)((const char*)cam + offsetof(rgp_camera, twc));
+  hipLaunchKernelGGL(k_lowrank_backward, dim3(grid1(n)), dim3(256), 0, s, n, K, P, Q, rank, A, B,
+                     (const float*)nullptr, pws, twc, dw, ddc, dA, dB);
+  return check_launch("rgp_sh_backward_lowrank");
+}

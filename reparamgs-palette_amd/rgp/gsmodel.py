@@ -1,0 +1,205 @@
+"""Autograd wrappers and model glue of the palette-weight finetune, on the
+fused gfx950 kernels.
+
+Same classes, signatures and semantics as the reference gsplat/gsmodel.py:
+`WeightGSFunctionTest.apply(pws, shs, alphas, scales, rots, cam, K)` and
+`WeightGSFunctionFast.apply(pws, dc, A, B, alphas, scales, rots, cam, K, Q)`
+return `(layer_W (K,H,W), depths > 0.2)` and their backward returns gradients
+for the SH coefficients only (gsmodel.py:104-200).  Instead of the reference's
+chain of six extension calls, the forward runs one fused preprocess kernel
+(project + covariances + SH weights + inverse + tile rects, writing packed
+splat records), the binning kernels and the compositing kernel; the backward
+runs the reverse compositing kernel and one SH-chain kernel that recomputes
+the basis from the positions rather than storing it.  The values are the ones
+of the reference chain.
+"""
+import math
+
+import torch
+import torch.nn.functional as F
+
+from . import ops
+from ._lib import call, f32c, I64, ptr, require_cuda, stream
+
+
+def _rec(n, K, dev):
+    return torch.empty((n, ops.record_floats(K)), dtype=torch.float32, device=dev)
+
+
+def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None, B=None, Q=None,
+                   sh_dim=None, return_state=False):
+    """Forward of the palette-weight rasterizer.  Full rank when `shs`
+    (N, sh_dim*K) is given, low rank (dc, A (N,P,r), B (N,r,Q)) otherwise."""
+    require_cuda(pws, alphas, scales, rots)
+    dev = pws.device
+    n = pws.shape[0]
+    W, H = int(cam.width), int(cam.height)
+    camd = ops.camera_struct(cam, dev)
+    pws_c, rots_c, scales_c, alphas_c = f32c(pws), f32c(rots), f32c(scales), f32c(alphas).reshape(-1)
+    depths = torch.empty((n,), dtype=torch.float32, device=dev)
+    records = _rec(n, K, dev)
+    if shs is not None:
+        shs_c = f32c(shs)
+        sd = shs_c.shape[1] // K if sh_dim is None else sh_dim
+        args = (0, sd, 0, 1, shs_c, None, None, None)
+    else:
+        dc_c, A_c, B_c = f32c(dc), f32c(A), f32c(B)
+        if A_c.dim() == 2:
+            P, r = A_c.shape[1], 1
+        else:
+            P, r = A_c.shape[1], A_c.shape[2]
+        args = (1, P, int(Q), r, None, dc_c, A_c, B_c)
+    mode, P_, Q_, r_, shs_c, dc_c, A_c, B_c = args
+
+    def fill(ws, cap):
+        call("rgp_preprocess_weight", n, K, mode, P_, Q_, r_, ptr(camd), ptr(pws_c), ptr(rots_c), ptr(scales_c),
+             ptr(alphas_c), ptr(shs_c), ptr(dc_c), ptr(A_c), ptr(B_c), ptr(depths), ptr(records), ptr(ws),
+             I64(int(cap)), W, H, None, None, None, None, stream(dev))
+
+    layer, contrib, final_tau, ranges, ids, m, _ = ops.splat_pipeline(n, W, H, K, records, depths, fill, 0.0, dev)
+    if return_state:
+        return layer, depths, dict(records=records, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids,
+                                   camd=camd, m=m, mode=mode, P=P_, Q=Q_, rank=r_)
+    return layer, depths
+
+
+class WeightGSFunctionTest(torch.autograd.Function):
+    """Full-rank palette weights (gsmodel.py:104-144)."""
+
+    @staticmethod
+    def forward(ctx, pws, shs, alphas, scales, rots, cam, num_palette):
+        K = int(num_palette)
+        layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, shs=shs, return_state=True)
+        ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"])
+        ctx.save_for_backward(f32c(pws), st["records"], st["contrib"], st["final_tau"], st["ranges"], st["ids"],
+                              st["camd"])
+        return layer, depths > 0.2
+
+    @staticmethod
+    def backward(ctx, dloss_dlayer, _):
+        W, H, K, n, sh_dim = ctx.meta
+        pws, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
+        dw = ops.draw_backward(W, H, K, ranges, ids, records, contrib, final_tau, dloss_dlayer.contiguous(), n)
+        dshs = torch.empty((n, sh_dim * K), dtype=torch.float32, device=pws.device)
+        call("rgp_sh_backward_full", n, K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(dshs), stream(pws.device))
+        return (None, dshs, None, None, None, None, None)
+
+
+class WeightGSFunctionFast(torch.autograd.Function):
+    """Low-rank palette weights (gsmodel.py:151-200), rank r factors
+    A (N,P,r) and B (N,r,Q) or the reference's rank-1 (N,P) and (N,Q)."""
+
+    @staticmethod
+    def forward(ctx, pws, shs_dc, shs_A, shs_B, alphas, scales, rots, cam, num_palette, Q):
+        K = int(num_palette)
+        layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, dc=shs_dc, A=shs_A, B=shs_B, Q=Q,
+                                           return_state=True)
+        ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"], st["Q"], st["rank"])
+        ctx.save_for_backward(f32c(pws), f32c(shs_A), f32c(shs_B), st["records"], st["contrib"], st["final_tau"],
+                              st["ranges"], st["ids"], st["camd"])
+        return layer, depths > 0.2
+
+    @staticmethod
+    def backward(ctx, dloss_dlayer, _):
+        W, H, K, n, P, Q, r = ctx.meta
+        pws, A, B, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
+        dw = ops.draw_backward(W, H, K, ranges, ids, records, contrib, final_tau, dloss_dlayer.contiguous(), n)
+        ddc = torch.empty((n, K), dtype=torch.float32, device=pws.device)
+        dA = torch.empty(A.shape, dtype=torch.float32, device=pws.device)
+        dB = torch.empty(B.shape, dtype=torch.float32, device=pws.device)
+        call("rgp_sh_backward_lowrank", n, K, P, Q, r, ptr(camd), ptr(pws), ptr(A), ptr(B), ptr(dw), ptr(ddc),
+             ptr(dA), ptr(dB), stream(pws.device))
+        return (None, ddc, dA, dB, None, None, None, None, None, None)
+
+
+# ---------------------------------------------------------------------------
+# model glue, restated from gsplat/gsmodel.py (torch code, off the kernel path)
+# ---------------------------------------------------------------------------
+def hsl_to_rgb(h, s, l):
+    """gsmodel.py:33-49."""
+    h6 = h / (2 * math.pi) * 6.0
+    C = (1.0 - torch.abs(2.0 * l - 1.0)) * s
+    X = C * (1.0 - torch.abs(h6 % 2.0 - 1.0))
+    m = l - C / 2.0
+    z = torch.zeros_like(C)
+    sectors = torch.stack([torch.stack([C, X, z], -1), torch.stack([X, C, z], -1), torch.stack([z, C, X], -1),
+                           torch.stack([z, X, C], -1), torch.stack([X, z, C], -1), torch.stack([C, z, X], -1)], 1)
+    idx = h6.long().clamp(0, 5)
+    rgb1 = sectors[torch.arange(len(h), device=h.device), idx]
+    return (rgb1 + m.unsqueeze(-1)).clamp(0.0, 1.0)
+
+
+def get_palette_rgb(hue_base, gap_raw, sat_raw, lit_raw):
+    """gsmodel.py:203-212."""
+    two_pi = 2 * math.pi
+    g = F.softplus(gap_raw)
+    gaps = two_pi * g / g.sum()
+    hues = ((hue_base % two_pi) + torch.cumsum(F.pad(gaps[:-1], (1, 0)), 0)) % two_pi
+    return hsl_to_rgb(hues, torch.sigmoid(sat_raw), torch.sigmoid(lit_raw))
+
+
+def reconstruct_high_shs(params):
+    """gsmodel.py:90-97, rank r: C = A @ B per Gaussian, flattened."""
+    if params["low_rank"]:
+        A, B = params["high_shs_A"], params["high_shs_B"]
+        n = A.shape[0]
+        if A.dim() == 2:
+            return (A.unsqueeze(2) * B.unsqueeze(1)).reshape(n, -1)
+        return torch.bmm(A, B.contiguous()).reshape(n, -1)
+    return params["high_shs_w"]
+
+
+def get_shs(low_shs, high_shs):
+    """gsplat/utils.py:157-158."""
+    return torch.cat((low_shs, high_shs), dim=1)
+
+
+class FinetuneModel(torch.nn.Module):
+    """gsmodel.py:292-329."""
+
+    def __init__(self, sh_order=3, num_palette=4, use_fast=True):
+        super().__init__()
+        self.sh_order = sh_order
+        self.num_palette = num_palette
+        self.splat_dim = num_palette + 2
+        self.use_fast = use_fast
+        self.mask = None
+
+    def forward(self, pws, params, alphas, scales, rots, cam):
+        if params["low_rank"] and self.use_fast:
+            W_tilde, self.mask = WeightGSFunctionFast.apply(pws, params["low_shs_w"], params["high_shs_A"],
+                                                            params["high_shs_B"], alphas, scales, rots, cam,
+                                                            self.splat_dim, params["reshape_Q"])
+        else:
+            shs_w = get_shs(params["low_shs_w"], reconstruct_high_shs(params))
+            W_tilde, self.mask = WeightGSFunctionTest.apply(pws, shs_w, alphas, scales, rots, cam, self.splat_dim)
+        palette_rgb = get_palette_rgb(params["hue_base"], params["gap_raw"], params["sat_raw"], params["lit_raw"])
+        return W_tilde, palette_rgb
+
+
+def bake_palette_edit(params, palette_rgb_new=None):
+    """gsmodel.py:488-509: per-Gaussian colour SH = weight SH @ P_full."""
+    with torch.no_grad():
+        P_chroma = palette_rgb_new if palette_rgb_new is not None else get_palette_rgb(
+            params["hue_base"], params["gap_raw"], params["sat_raw"], params["lit_raw"])
+        dev, dt = P_chroma.device, P_chroma.dtype
+        P_full = torch.cat([torch.zeros(1, 3, device=dev, dtype=dt), torch.ones(1, 3, device=dev, dtype=dt),
+                            P_chroma], 0)
+        shs_w = get_shs(params["low_shs_w"], reconstruct_high_shs(params))
+        n, K = shs_w.shape[0], P_full.shape[0]
+        num_sh = shs_w.shape[1] // K
+        baked = torch.einsum("nlk,kc->nlc", shs_w.reshape(n, num_sh, K), P_full)
+        return baked.reshape(n, num_sh * 3)
+
+
+@torch.no_grad()
+def render_vanilla(pws, shs, alphas, scales, rots, cam):
+    """gsmodel.py:13-26: the colour path through the gsplatcu-compatible ops."""
+    import gsplatcu as gsc
+    us, pcs, depths = gsc.project(pws, cam.Rcw, cam.tcw, cam.fx, cam.fy, cam.cx, cam.cy, False)
+    cov3ds, = gsc.computeCov3D(rots, scales, depths, False)
+    cov2ds, = gsc.computeCov2D(cov3ds, pcs, cam.Rcw, depths, cam.fx, cam.fy, cam.width, cam.height, False)
+    colors, = gsc.sh2Color(shs, pws, cam.twc, False)
+    cinv2ds, areas = gsc.inverseCov2D(cov2ds, depths, False)
+    image, _, _, _, _ = gsc.splat_color(cam.height, cam.width, us, cinv2ds, alphas, depths, colors, areas)
+    return image

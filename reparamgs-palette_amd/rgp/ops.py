@@ -1,0 +1,204 @@
+"""Tensor-level wrappers over the librgp C ABI.
+
+Outputs are allocated with torch's caching allocator on the input's device and
+the kernels run on torch's current stream.  Functions that must know the
+number of (tile, Gaussian) pairs read it back once, as the reference does
+(gsplatcu/gausplat.cu:201); the capacity-based variants used for graph capture
+never synchronise.
+"""
+import numpy as np
+import torch
+
+from ._lib import I64, call, f32c, i32c, lib, ptr, require_cuda, stream
+
+TILE = 16
+
+
+def grid_dims(width, height):
+    return (width + TILE - 1) // TILE, (height + TILE - 1) // TILE
+
+
+def record_floats(dim):
+    return lib().rgp_record_floats(int(dim))
+
+
+# --------------------------------------------------------------------------
+# camera
+# --------------------------------------------------------------------------
+def camera_struct_host(Rcw, tcw, twc, fx, fy, cx, cy, width, height):
+    """The rgp_camera struct of include/rgp.h as 32 float32 words."""
+    f32 = np.float32
+    buf = np.zeros(32, f32)
+    buf[0:9] = np.asarray(Rcw, f32).reshape(9)
+    buf[9:12] = np.asarray(tcw, f32).reshape(3)
+    buf[12:15] = np.asarray(twc, f32).reshape(3)
+    buf[15:19] = [f32(fx), f32(fy), f32(cx), f32(cy)]
+    # gausplat.cu:357-358: width / (2 * focal_x) in float
+    buf[19] = f32(f32(width) / (f32(2) * f32(fx)))
+    buf[20] = f32(f32(height) / (f32(2) * f32(fy)))
+    buf[21:23] = np.array([width, height], np.int32).view(f32)
+    return buf
+
+
+def camera_struct(cam, device):
+    """Device rgp_camera for a reference-style camera object (Rcw, tcw, twc
+    tensors or arrays; fx, fy, cx, cy floats; width, height ints).  Cached on
+    the object and rebuilt when any field changes."""
+    def val(x):
+        if isinstance(x, torch.Tensor):
+            return x.detach().float().cpu().numpy()
+        return np.asarray(x, np.float32)
+
+    key_parts = []
+    for name in ("Rcw", "tcw", "twc"):
+        x = getattr(cam, name)
+        if isinstance(x, torch.Tensor):
+            key_parts.append((id(x), x._version))
+        else:
+            key_parts.append(np.asarray(x, np.float32).tobytes())
+    key = (tuple(key_parts), float(cam.fx), float(cam.fy), float(cam.cx), float(cam.cy),
+           int(cam.width), int(cam.height), str(device))
+    cached = getattr(cam, "_rgp_cam", None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    host = camera_struct_host(val(cam.Rcw), val(cam.tcw), val(cam.twc), cam.fx, cam.fy, cam.cx,
+                              cam.cy, cam.width, cam.height)
+    dev = torch.from_numpy(host).to(device)
+    try:
+        cam._rgp_cam = (key, dev)
+    except AttributeError:
+        pass
+    return dev
+
+
+# --------------------------------------------------------------------------
+# binning
+# --------------------------------------------------------------------------
+def workspace_bytes(n, capacity, width, height):
+    return int(lib().rgp_bin_workspace_bytes(int(n), I64(int(capacity)), int(width), int(height)))
+
+
+def new_workspace(n, capacity, width, height, device):
+    return torch.empty(workspace_bytes(n, capacity, width, height), dtype=torch.uint8, device=device)
+
+
+def header(ws, words=4):
+    """Header words [M, overflow, dq_max, M_eff] as a device int32 view."""
+    return ws[: 4 * 64].view(torch.int32)[:words]
+
+
+def bin_sort(n, depths, width, height, ws, capacity, device, keys=False):
+    """createKeys + sort + getRanges into fresh ids (capacity,) / ranges."""
+    gx, gy = grid_dims(width, height)
+    ids = torch.empty(max(int(capacity), 1), dtype=torch.int32, device=device)
+    ranges = torch.empty((gx * gy, 2), dtype=torch.int32, device=device)
+    keys_t = torch.empty(max(int(capacity), 1), dtype=torch.int64, device=device) if keys else None
+    call("rgp_bin_sort", int(n), ptr(depths), int(width), int(height), ptr(ws), I64(int(capacity)),
+         ptr(ids), ptr(keys_t), ptr(ranges), stream(device))
+    return ids, ranges, keys_t
+
+
+def pack_records(us, cinv2ds, alphas, feat):
+    require_cuda(us, cinv2ds, alphas, feat)
+    n, dim = feat.shape[0], feat.shape[-1]
+    rec = torch.empty((n, record_floats(dim)), dtype=torch.float32, device=us.device)
+    call("rgp_pack_records", int(n), int(dim), ptr(f32c(us)), ptr(f32c(cinv2ds)),
+         ptr(f32c(alphas).reshape(-1)), ptr(f32c(feat).reshape(n, dim)), ptr(rec), stream(us.device))
+    return rec
+
+
+def draw_forward(width, height, dim, ranges, ids, records, empty_tau):
+    dev = records.device
+    layer = torch.empty((dim, height, width), dtype=torch.float32, device=dev)
+    contrib = torch.empty((height, width), dtype=torch.int32, device=dev)
+    final_tau = torch.empty((height, width), dtype=torch.float32, device=dev)
+    call("rgp_draw_forward", int(width), int(height), int(dim), ptr(ranges), ptr(ids), ptr(records),
+         float(empty_tau), ptr(layer), ptr(contrib), ptr(final_tau), stream(dev))
+    return layer, contrib, final_tau
+
+
+def draw_backward(width, height, dim, ranges, ids, records, contrib, final_tau, dlayer, n):
+    dev = records.device
+    dfeat = torch.zeros((n, dim), dtype=torch.float32, device=dev)
+    call("rgp_draw_backward", int(width), int(height), int(dim), ptr(i32c(ranges)), ptr(i32c(ids)),
+         ptr(records), ptr(i32c(contrib)), ptr(f32c(final_tau)), ptr(f32c(dlayer)), ptr(dfeat),
+         stream(dev))
+    return dfeat
+
+
+class CapacityCache:
+    """Pair capacity per (n, width, height): grows to 1.25x the largest pair
+    count seen, so repeated renders of a scene reuse one workspace size."""
+
+    def __init__(self):
+        self.cap = {}
+
+    def get(self, n, width, height):
+        return self.cap.get((n, width, height), max(1024, 4 * n))
+
+    def update(self, n, width, height, m):
+        key = (n, width, height)
+        if m > self.cap.get(key, 0) or key not in self.cap:
+            self.cap[key] = max(int(m * 1.25) + 1024, self.cap.get(key, 0))
+
+
+CAPACITY = CapacityCache()
+
+
+def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, device,
+                   exact=True):
+    """Scan -> (read M) -> sort -> ranges -> draw.  ws_fill(ws, cap) must
+    write rects/counts into a workspace (the getRects stage or the fused
+    preprocess).  With exact=True the pair count is read back and the pair
+    buffers are sized from it (one host sync, as gausplat.cu:201)."""
+    cap = CAPACITY.get(n, width, height)
+    m = None
+    while True:
+        ws = new_workspace(n, cap, width, height, device)
+        ws_fill(ws, cap)
+        call("rgp_bin_scan", int(n), int(width), int(height), ptr(ws), I64(int(cap)), stream(device))
+        if not exact:
+            break
+        m = int(header(ws, 1).item())
+        CAPACITY.update(n, width, height, m)
+        if m <= cap:
+            break
+        cap = CAPACITY.get(n, width, height)
+    ids, ranges, _ = bin_sort(n, depths, width, height, ws, cap, device)
+    layer, contrib, final_tau = draw_forward(width, height, dim, ranges, ids, records, empty_tau)
+    return layer, contrib, final_tau, ranges, ids, m, ws
+
+
+def mix(W, palette):
+    """RGB (3,H,W) = sum_k palette[k] W[k]."""
+    require_cuda(W, palette)
+    K = W.shape[0]
+    hw = W.shape[1] * W.shape[2]
+    rgb = torch.empty((3,) + tuple(W.shape[1:]), dtype=torch.float32, device=W.device)
+    call("rgp_mix", int(K), I64(hw), ptr(f32c(W)), ptr(f32c(palette)), ptr(rgb), stream(W.device))
+    return rgb
+
+
+def mix_backward(W, palette, drgb):
+    K = W.shape[0]
+    hw = W.shape[1] * W.shape[2]
+    dW = torch.empty_like(W)
+    dpal = torch.zeros((K, 3), dtype=torch.float32, device=W.device)
+    call("rgp_mix_backward", int(K), I64(hw), ptr(f32c(W)), ptr(f32c(palette)), ptr(f32c(drgb)),
+         ptr(dW), ptr(dpal), stream(W.device))
+    return dW, dpal
+
+
+class Mix(torch.autograd.Function):
+    """Palette mix with its backward on the GPU kernels."""
+
+    @staticmethod
+    def forward(ctx, W, palette):
+        ctx.save_for_backward(W, palette)
+        return mix(W, palette)
+
+    @staticmethod
+    def backward(ctx, drgb):
+        W, palette = ctx.saved_tensors
+        dW, dpal = mix_backward(W, palette, drgb.contiguous())
+        return dW, dpal

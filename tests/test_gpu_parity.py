@@ -1,0 +1,229 @@
+"""Parity of the gfx950 kernels against the CPU oracle, through the
+gsplatcu-compatible module and the fused autograd wrappers.
+
+Bar: bit-exact for every integer output (areas, tile lists, sort order,
+ranges, contrib) and for every FP32 output of the forward (the kernels and
+the FP32 oracle perform the same IEEE operations in the same order).  The
+backward sums per-pixel terms with atomics in an order that varies from run
+to run; each gradient must lie within 1e-5 * sum|terms| of the exact sum
+(the oracle sums the same FP32 terms in float64).
+"""
+import numpy as np
+import pytest
+import torch
+
+import gsplatcu as gsc
+from oracle import oracle as O
+from rgp import gsmodel, ops
+from rgp.scene import make_scene, orbit_cameras, workload_c0
+
+pytestmark = pytest.mark.gpu
+GRAD_TOL = 1e-5
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+def same(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, f"{what}: shape {a.shape} vs {b.shape}"
+    bad = ~((a == b) | (np.isnan(a) & np.isnan(b)) if a.dtype.kind == "f" else (a == b))
+    assert not bad.any(), f"{what}: {bad.sum()} of {a.size} differ; first at {np.argwhere(bad)[:3].tolist()}"
+
+
+def grad_close(g, ref, ref_abs, what):
+    err = np.abs(g.astype(np.float64) - ref)
+    lim = GRAD_TOL * ref_abs + 1e-30
+    assert (err <= lim).all(), f"{what}: max err/lim {np.max(err / lim):.3g}"
+
+
+@pytest.fixture(scope="module")
+def c0():
+    print("case: C0 20k Gaussians 256x256 K=5", flush=True)
+    return workload_c0()
+
+
+def gpu_chain(scene, cam, dev, fast=False):
+    """The reference chain of WeightGSFunctionTest/Fast.forward on gsplatcu."""
+    tc = cam.to_torch(dev)
+    pws = T(scene.pws, dev)
+    us, pcs, depths = gsc.project(pws, tc.Rcw, tc.tcw, cam.fx, cam.fy, cam.cx, cam.cy, False)
+    c3, = gsc.computeCov3D(T(scene.rots, dev), T(scene.scales, dev), depths, False)
+    c2, = gsc.computeCov2D(c3, pcs, tc.Rcw, depths, cam.fx, cam.fy, cam.width, cam.height, False)
+    if fast:
+        w, dwdc, ylm = gsc.sh2Weight_fast(T(scene.dc, dev), T(scene.A, dev), T(scene.B, dev), pws, tc.twc, True,
+                                          scene.K, scene.Q)
+        extra = dict(ylm=ylm, dwdc=dwdc)
+    else:
+        w, basis, _ = gsc.sh2Weight(T(scene.shs, dev), pws, tc.twc, True, scene.K)
+        extra = dict(basis=basis)
+    cinv, areas = gsc.inverseCov2D(c2, depths, False)
+    layer, contrib, final_tau, ranges, ids = gsc.splat_weight(cam.height, cam.width, scene.K, us, cinv,
+                                                              T(scene.alphas, dev), depths, w, areas)
+    return dict(us=us, pcs=pcs, depths=depths, cov3ds=c3, cov2ds=c2, weights=w, cinv=cinv, areas=areas,
+                layer=layer, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids, **extra)
+
+
+def check_forward(g, o, fast):
+    for k in ("us", "pcs", "cov3ds", "cov2ds", "weights", "cinv", "areas", "depths", "ranges", "ids", "layer",
+              "contrib", "final_tau"):
+        same(N(g[k]), o[k], k)
+    if fast:
+        same(N(g["ylm"]), o["ylm"], "ylm")
+    else:
+        same(N(g["basis"]).reshape(o["basis"].shape), o["basis"], "basis")
+
+
+def test_forward_chain_bitexact_c0(device, c0):
+    scene, cam = c0
+    o = O.forward_weight(scene, cam)
+    g = gpu_chain(scene, cam, device)
+    check_forward(g, o, fast=False)
+    assert len(o["ids"]) > 10000
+
+
+def test_backward_chain_c0(device, c0):
+    scene, cam = c0
+    o = O.forward_weight(scene, cam)
+    g = gpu_chain(scene, cam, device)
+    dl = np.random.default_rng(1).normal(size=o["layer"].shape).astype(np.float32)
+    ob = O.backward_weight(scene, o, dl)
+    dw, = gsc.splat_weight_B(cam.height, cam.width, scene.K, g["us"], g["cinv"], T(scene.alphas, device),
+                             g["depths"], g["weights"], g["contrib"], g["final_tau"], g["ranges"], g["ids"],
+                             T(dl, device))
+    assert dw.shape == (scene.n, 1, scene.K)
+    grad_close(N(dw).reshape(scene.n, scene.K), ob["dweights"], ob["dweights_abs"], "dweights")
+
+
+def test_fused_wrapper_full_rank_c0(device, c0):
+    scene, cam = c0
+    o = O.forward_weight(scene, cam)
+    tc = cam.to_torch(device)
+    shs = T(scene.shs, device).requires_grad_(True)
+    layer, mask = gsmodel.WeightGSFunctionTest.apply(T(scene.pws, device), shs, T(scene.alphas, device),
+                                                     T(scene.scales, device), T(scene.rots, device), tc, scene.K)
+    same(N(layer), o["layer"], "layer")
+    same(N(mask), o["mask"], "mask")
+    dl = np.random.default_rng(2).normal(size=o["layer"].shape).astype(np.float32)
+    layer.backward(T(dl, device))
+    ob = O.backward_weight(scene, o, dl)
+    grad_close(N(shs.grad), ob["dshs"], ob["dshs_abs"], "dshs")
+
+
+@pytest.mark.parametrize("rank", [1, 2])
+def test_fused_wrapper_low_rank(device, rank):
+    print(f"case: low rank r={rank} 4000 Gaussians 192x160", flush=True)
+    scene = make_scene(4000, seed=3 + rank, extent=1.0, K=5, lowrank=True, rank=rank, P=5, Q=15)
+    cam = orbit_cameras(3, 4.0, 192, 160)[1]
+    o = O.forward_weight(scene, cam)
+    g = gpu_chain(scene, cam, device, fast=True)
+    check_forward(g, o, fast=True)
+    tc = cam.to_torch(device)
+    dc = T(scene.dc, device).requires_grad_(True)
+    A = T(scene.A if rank > 1 else scene.A[:, :, 0], device).requires_grad_(True)
+    B = T(scene.B if rank > 1 else scene.B[:, 0, :], device).requires_grad_(True)
+    layer, mask = gsmodel.WeightGSFunctionFast.apply(T(scene.pws, device), dc, A, B, T(scene.alphas, device),
+                                                     T(scene.scales, device), T(scene.rots, device), tc,
+                                                     scene.K, scene.Q)
+    same(N(layer), o["layer"], "layer")
+    dl = np.random.default_rng(5).normal(size=o["layer"].shape).astype(np.float32)
+    layer.backward(T(dl, device))
+    ob = O.backward_weight(scene, o, dl)
+    grad_close(N(dc.grad), ob["ddc"], ob["ddc_abs"], "ddc")
+    grad_close(N(A.grad).reshape(ob["dA"].shape), ob["dA"], ob["dA_abs"] * 1.0001 + 1e-12, "dA")
+    grad_close(N(B.grad).reshape(ob["dB"].shape), ob["dB"], ob["dB_abs"] * 1.0001 + 1e-12, "dB")
+
+
+def test_lowrank_backward_op_bitexact(device):
+    """sh2Weight_fast_B sums in loop order, like the reference's thread-private
+    atomics: bit-exact against the FP32 oracle."""
+    scene = make_scene(3000, seed=9, K=5, lowrank=True, rank=2, P=5, Q=15)
+    cam = orbit_cameras(1, 4.0, 64, 64)[0]
+    w, ylm = O.sh2weight_fast(scene.dc, scene.A, scene.B, scene.pws, cam.twc, 5, 15)
+    dw = np.random.default_rng(0).normal(size=(scene.n, 5)).astype(np.float32)
+    oA, oB = O.sh2weight_fast_backward(scene.A, scene.B, ylm, dw, 5, 15)
+    gA, gB = gsc.sh2Weight_fast_B(T(scene.A, device), T(scene.B, device), T(ylm, device), T(dw, device), 5, 15)
+    same(N(gA), oA, "dA")
+    same(N(gB), oB, "dB")
+
+
+def test_colour_path_bitexact(device):
+    scene = make_scene(5000, seed=11, K=5)
+    cam = orbit_cameras(2, 4.0, 200, 120)[0]
+    shs = np.random.default_rng(3).normal(0, 0.3, (scene.n, 48)).astype(np.float32)
+    img = gsmodel.render_vanilla(T(scene.pws, device), T(shs, device), T(scene.alphas, device),
+                                 T(scene.scales, device), T(scene.rots, device), cam.to_torch(device))
+    us, pcs, depths = O.project(scene.pws, cam.Rcw, cam.tcw, cam.fx, cam.fy, cam.cx, cam.cy)
+    c3 = O.cov3d(scene.rots, scene.scales, depths)
+    c2 = O.cov2d(c3, pcs, cam.Rcw, depths, cam.fx, cam.fy, cam.width, cam.height)
+    col = O.sh2color(shs, scene.pws, cam.twc)
+    cinv, areas = O.inverse_cov2d(c2, depths)
+    rects, counts = O.get_rects(us, areas, depths, cam.width, cam.height)
+    keys, ids, ranges = O.bin_sort(depths, rects, counts, cam.width, cam.height)
+    ref, _, _, _, _ = O.draw(cam.width, cam.height, ranges, ids, us, cinv, scene.alphas, col, 1.0)
+    same(N(img), ref, "image")
+
+
+def test_mix_matches_bake(device, c0):
+    """mix(render(w), P) against the oracle mix, and the linear identity with
+    the bake-then-render colour path (gsmodel.py:488-509, 13-26)."""
+    scene, cam = c0
+    o = O.forward_weight(scene, cam)
+    pal = scene.palette
+    rgb = ops.mix(T(o["layer"], device), T(pal, device))
+    same(N(rgb), O.mix(pal, o["layer"]), "mix")
+    # bake: colour SH = weight SH @ P, rendered by the colour path
+    shs_w = scene.shs.reshape(scene.n, 16, scene.K)
+    baked = np.einsum("nlk,kc->nlc", shs_w.astype(np.float64), pal.astype(np.float64)).astype(np.float32)
+    img = gsmodel.render_vanilla(T(scene.pws, device), T(baked.reshape(scene.n, 48), device),
+                                 T(scene.alphas, device), T(scene.scales, device), T(scene.rots, device),
+                                 cam.to_torch(device))
+    # empty tiles: the colour path leaves final_tau = 1 but colour 0, the weight path 0 and 0
+    diff = np.abs(N(img) - N(rgb))
+    assert diff.max() < 2e-4, diff.max()
+
+
+def test_binning_many_tiles(device):
+    """1080p with a dense scene: two tile bytes and two depth bytes in the
+    sort, hundreds of sort blocks."""
+    print("case: 200k Gaussians 1920x1080", flush=True)
+    scene = make_scene(200_000, seed=21, extent=3.0, K=5)
+    cam = orbit_cameras(4, 6.0, 1920, 1080)[1]
+    o = O.forward_weight(scene, cam)
+    g = gpu_chain(scene, cam, device)
+    for k in ("depths", "areas", "ranges", "ids", "contrib", "final_tau", "layer"):
+        same(N(g[k]), o[k], k)
+    assert len(o["ids"]) > 4096 * 50
+
+
+def test_edge_cases(device):
+    """M = 0 (everything behind the camera), M = 1 (the reference leaves
+    every range (0,0)), N = 1, and a Gaussian over the whole image."""
+    cam = orbit_cameras(1, 4.0, 96, 80)[0]
+    tc = cam.to_torch(device)
+    for n, tweak in ((5, "behind"), (1, "single"), (3, "huge")):
+        print(f"case: edge {tweak}", flush=True)
+        sc = make_scene(n, seed=31, K=3)
+        if tweak == "behind":
+            sc.pws[:] = cam.twc - 2.0 * (cam.Rcw[2] * 1.0)  # 2 units behind the camera
+        if tweak == "single":
+            # a tiny splat in the middle of one tile
+            # pixel (40, 40): its 2-pixel box stays inside tile (2, 2)
+            pc = np.array([(40.0 - cam.cx) * 4.0 / cam.fx, (40.0 - cam.cy) * 4.0 / cam.fy, 4.0])
+            sc.pws[:] = (np.linalg.inv(cam.Rcw.astype(np.float64)) @ (pc - cam.tcw)).astype(np.float32)
+            sc.scales[:] = 1e-4
+        if tweak == "huge":
+            sc.scales[:] = 2.0
+        o = O.forward_weight(sc, cam)
+        g = gpu_chain(sc, cam, device)
+        for k in ("depths", "areas", "ranges", "ids", "contrib", "final_tau", "layer"):
+            same(N(g[k]), o[k], f"{tweak}:{k}")
+        if tweak == "behind":
+            assert len(o["ids"]) == 0
+        if tweak == "single":
+            assert len(o["ids"]) == 1 and (o["ranges"] == 0).all()

@@ -139,6 +139,15 @@ int rgp_draw_forward(int width, int height, int dim, const int32_t* ranges, cons
                      const float* records, float empty_tau, float* layer, int32_t* contrib,
                      float* final_tau, hipStream_t stream);
 
+/* rgp_draw_forward plus two counters added into counts (int64[2], caller
+ * zero-fills): list entries evaluated per pixel before it stops, summed, and
+ * contributing entries (alpha' >= 0.002), summed.  Instrumentation of the
+ * forward's algorithmic work; no counterpart in the reference. */
+int rgp_draw_forward_counts(int width, int height, int dim, const int32_t* ranges,
+                            const int32_t* ids, const float* records, float empty_tau,
+                            float* layer, int32_t* contrib, float* final_tau, int64_t* counts,
+                            hipStream_t stream);
+
 /* draw_weight_B (kernel.cu:1291-1379, gausplat.cu:252-292): reverse-order
  * backward of the channel compositing.  dfeat (N, dim) must be zero-filled;
  * gradients are added with wavefront reductions then one atomic per

@@ -297,12 +297,23 @@ def _recon(scene):
     return C.reshape(n, -1)
 
 
-def backward_weight(scene, fwd, dlayer):
+def backward_weight(scene, fwd, dlayer, parallel=False):
     """The backward of WeightGSFunctionTest / Fast: splat_weight_B, then the
     SH chain.  Returns a dict: dweights (N,K) float64 plus its abs-sum bound,
-    and dshs (full rank) or ddc, dA, dB (low rank), in float64."""
+    and dshs (full rank) or ddc, dA, dB (low rank), in float64.
+    parallel=True: the timing form (OpenMP over tiles, atomic adds in the
+    arithmetic type, reference order of the SH chain in that type)."""
     dt = fwd["dt"]
     K = fwd["K"]
+    if parallel:
+        g = draw_backward(fwd["width"], fwd["height"], K, fwd["ranges"], fwd["ids"], fwd["us"], fwd["cinv"],
+                          scene.alphas, fwd["contrib"], fwd["final_tau"], dlayer, dt, mode=1)
+        if fwd["fast"]:
+            dA, dB = sh2weight_fast_backward(scene.A, scene.B, fwd["ylm"], g, K, scene.Q, dt)
+            return dict(dweights=g, ddc=g * dt(0.28209479177387814), dA=dA, dB=dB)
+        basis = fwd["basis"]
+        n, L = basis.shape
+        return dict(dweights=g, dshs=(g[:, None, :] * basis[:, :, None]).reshape(n, L * K))
     g, gabs = draw_backward(fwd["width"], fwd["height"], K, fwd["ranges"], fwd["ids"], fwd["us"], fwd["cinv"],
                             scene.alphas, fwd["contrib"], fwd["final_tau"], dlayer, dt)
     out = dict(dweights=g, dweights_abs=gabs)

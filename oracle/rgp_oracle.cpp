@@ -466,6 +466,7 @@ int64_t bin_sort(int n, const T* depths, const uint32_t* rects, const uint32_t* 
     const T depth = depths[i];
     if ((double)depth < MIN_DEPTH) continue;
     uint64_t o = (i == 0) ? 0 : off[i - 1];
+    if (counts[i] == 0) continue;  // getRects leaves such Gaussians culled
     const uint32_t* r = rects + 4 * i;
     for (uint32_t y = r[1]; y < r[3]; ++y)
       for (uint32_t x = r[0]; x < r[2]; ++x) {

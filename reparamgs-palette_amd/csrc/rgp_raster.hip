@@ -17,14 +17,17 @@
 
 namespace rgp {
 
-template <int DIM>
+// COUNT: also accumulate, over the launch, the list entries each pixel
+// evaluates before it stops (counts[0]) and its contributions (counts[1]).
+template <int DIM, bool COUNT = false>
 __global__ __launch_bounds__(256) void k_draw_fwd(int W, int H, int gx, int tiles,
                                                   const int2* __restrict__ ranges,
                                                   const int32_t* __restrict__ ids,
                                                   const float4* __restrict__ recs, float empty_tau,
                                                   float* __restrict__ layer,
                                                   int32_t* __restrict__ contrib,
-                                                  float* __restrict__ final_tau) {
+                                                  float* __restrict__ final_tau,
+                                                  unsigned long long* __restrict__ counts = nullptr) {
   constexpr int F4 = RecStride<DIM>::f4;
   __shared__ float4 s_rec[256 * F4];
   const int tile = xcd_tile(blockIdx.x, tiles);
@@ -41,6 +44,7 @@ __global__ __launch_bounds__(256) void k_draw_fwd(int W, int H, int gx, int tile
   float tau = 1.f;
   int cont = 0;
   bool done = !inside;
+  int n_eval = 0, n_con = 0;  // COUNT only
   if (rg.y - rg.x <= 0) {
     tau = empty_tau;  // the reference kernel returns before writing
   } else {
@@ -75,9 +79,14 @@ __global__ __launch_bounds__(256) void k_draw_fwd(int W, int H, int gx, int tile
         }
         cont = b0 + j - rg.x + 1;
         tau = tau * (1.f - ap);
-        if (tau < 0.0001f) done = true;
+        if (COUNT) ++n_con;
+        if (tau < 0.0001f) {
+          done = true;
+          if (COUNT) n_eval = cont;
+        }
       }
     }
+    if (COUNT && inside && !done) n_eval = rg.y - rg.x;
   }
   if (inside) {
     const size_t HW = (size_t)W * H, pix = (size_t)py * W + px;
@@ -85,6 +94,18 @@ __global__ __launch_bounds__(256) void k_draw_fwd(int W, int H, int gx, int tile
     for (int k = 0; k < DIM; ++k) layer[k * HW + pix] = acc[k];
     contrib[pix] = cont;
     final_tau[pix] = tau;
+  }
+  if (COUNT) {
+    unsigned long long e = (unsigned long long)n_eval, c = (unsigned long long)n_con;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      e += __shfl_xor(e, o);
+      c += __shfl_xor(c, o);
+    }
+    if (lane == 0) {
+      atomicAdd(&counts[0], e);
+      atomicAdd(&counts[1], c);
+    }
   }
 }
 
@@ -294,11 +315,26 @@ extern "C" int rgp_draw_forward(int width, int height, int dim, const int32_t* r
     return set_error(RGP_ERR_ARG, "rgp_draw_forward: bad size or dim");
   const int gx = (width + kTile - 1) / kTile, gy = (height + kTile - 1) / kTile;
   const int tiles = gx * gy;
-  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL(k_draw_fwd<DD>, dim3(tiles), dim3(256), 0, s, width,
+  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false>), dim3(tiles), dim3(256), 0, s,
+                                         width, height, gx, tiles, (const int2*)ranges, ids,
+                                         (const float4*)records, empty_tau, layer, contrib,
+                                         final_tau, (unsigned long long*)nullptr));
+  return check_launch("rgp_draw_forward");
+}
+
+extern "C" int rgp_draw_forward_counts(int width, int height, int dim, const int32_t* ranges,
+                                       const int32_t* ids, const float* records, float empty_tau,
+                                       float* layer, int32_t* contrib, float* final_tau,
+                                       int64_t* counts, hipStream_t s) {
+  if (width <= 0 || height <= 0 || dim < 1 || dim > RGP_MAX_DIM)
+    return set_error(RGP_ERR_ARG, "rgp_draw_forward_counts: bad size or dim");
+  const int gx = (width + kTile - 1) / kTile, gy = (height + kTile - 1) / kTile;
+  const int tiles = gx * gy;
+  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, true>), dim3(tiles), dim3(256), 0, s, width,
                                          height, gx, tiles, (const int2*)ranges, ids,
                                          (const float4*)records, empty_tau, layer, contrib,
-                                         final_tau));
-  return check_launch("rgp_draw_forward");
+                                         final_tau, (unsigned long long*)counts));
+  return check_launch("rgp_draw_forward_counts");
 }
 
 extern "C" int rgp_draw_backward(int width, int height, int dim, const int32_t* ranges,

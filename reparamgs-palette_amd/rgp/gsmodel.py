@@ -20,6 +20,7 @@ import torch.nn.functional as F
 
 from . import ops
 from ._lib import call, f32c, I64, ptr, require_cuda, stream
+from .profiling import SPANS
 
 
 def _rec(n, K, dev):
@@ -81,7 +82,8 @@ class WeightGSFunctionTest(torch.autograd.Function):
         pws, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
         dw = ops.draw_backward(W, H, K, ranges, ids, records, contrib, final_tau, dloss_dlayer.contiguous(), n)
         dshs = torch.empty((n, sh_dim * K), dtype=torch.float32, device=pws.device)
-        call("rgp_sh_backward_full", n, K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(dshs), stream(pws.device))
+        with SPANS.span("sh_backward"):
+            call("rgp_sh_backward_full", n, K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(dshs), stream(pws.device))
         return (None, dshs, None, None, None, None, None)
 
 
@@ -107,8 +109,9 @@ class WeightGSFunctionFast(torch.autograd.Function):
         ddc = torch.empty((n, K), dtype=torch.float32, device=pws.device)
         dA = torch.empty(A.shape, dtype=torch.float32, device=pws.device)
         dB = torch.empty(B.shape, dtype=torch.float32, device=pws.device)
-        call("rgp_sh_backward_lowrank", n, K, P, Q, r, ptr(camd), ptr(pws), ptr(A), ptr(B), ptr(dw), ptr(ddc),
-             ptr(dA), ptr(dB), stream(pws.device))
+        with SPANS.span("sh_backward"):
+            call("rgp_sh_backward_lowrank", n, K, P, Q, r, ptr(camd), ptr(pws), ptr(A), ptr(B), ptr(dw), ptr(ddc),
+                 ptr(dA), ptr(dB), stream(pws.device))
         return (None, ddc, dA, dB, None, None, None, None, None, None)
 
 

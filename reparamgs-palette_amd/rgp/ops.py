@@ -10,6 +10,7 @@ import numpy as np
 import torch
 
 from ._lib import I64, call, f32c, i32c, lib, ptr, require_cuda, stream
+from .profiling import SPANS
 
 TILE = 16
 
@@ -112,17 +113,33 @@ def draw_forward(width, height, dim, ranges, ids, records, empty_tau):
     layer = torch.empty((dim, height, width), dtype=torch.float32, device=dev)
     contrib = torch.empty((height, width), dtype=torch.int32, device=dev)
     final_tau = torch.empty((height, width), dtype=torch.float32, device=dev)
-    call("rgp_draw_forward", int(width), int(height), int(dim), ptr(ranges), ptr(ids), ptr(records),
-         float(empty_tau), ptr(layer), ptr(contrib), ptr(final_tau), stream(dev))
+    with SPANS.span("draw_forward"):
+        call("rgp_draw_forward", int(width), int(height), int(dim), ptr(ranges), ptr(ids), ptr(records),
+             float(empty_tau), ptr(layer), ptr(contrib), ptr(final_tau), stream(dev))
     return layer, contrib, final_tau
+
+
+def draw_forward_counts(width, height, dim, ranges, ids, records, empty_tau):
+    """Forward compositing that also counts, over all pixels, the list entries
+    evaluated before each pixel stops (E) and the contributing ones (C)."""
+    dev = records.device
+    layer = torch.empty((dim, height, width), dtype=torch.float32, device=dev)
+    contrib = torch.empty((height, width), dtype=torch.int32, device=dev)
+    final_tau = torch.empty((height, width), dtype=torch.float32, device=dev)
+    counts = torch.zeros(2, dtype=torch.int64, device=dev)
+    call("rgp_draw_forward_counts", int(width), int(height), int(dim), ptr(ranges), ptr(ids), ptr(records),
+         float(empty_tau), ptr(layer), ptr(contrib), ptr(final_tau), ptr(counts), stream(dev))
+    e, c = counts.tolist()
+    return layer, contrib, final_tau, e, c
 
 
 def draw_backward(width, height, dim, ranges, ids, records, contrib, final_tau, dlayer, n):
     dev = records.device
     dfeat = torch.zeros((n, dim), dtype=torch.float32, device=dev)
-    call("rgp_draw_backward", int(width), int(height), int(dim), ptr(i32c(ranges)), ptr(i32c(ids)),
-         ptr(records), ptr(i32c(contrib)), ptr(f32c(final_tau)), ptr(f32c(dlayer)), ptr(dfeat),
-         stream(dev))
+    ranges, ids, contrib, final_tau, dlayer = i32c(ranges), i32c(ids), i32c(contrib), f32c(final_tau), f32c(dlayer)
+    with SPANS.span("draw_backward"):
+        call("rgp_draw_backward", int(width), int(height), int(dim), ptr(ranges), ptr(ids), ptr(records),
+             ptr(contrib), ptr(final_tau), ptr(dlayer), ptr(dfeat), stream(dev))
     return dfeat
 
 
@@ -155,8 +172,10 @@ def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, d
     m = None
     while True:
         ws = new_workspace(n, cap, width, height, device)
-        ws_fill(ws, cap)
-        call("rgp_bin_scan", int(n), int(width), int(height), ptr(ws), I64(int(cap)), stream(device))
+        with SPANS.span("preprocess"):
+            ws_fill(ws, cap)
+        with SPANS.span("scan"):
+            call("rgp_bin_scan", int(n), int(width), int(height), ptr(ws), I64(int(cap)), stream(device))
         if not exact:
             break
         m = int(header(ws, 1).item())
@@ -164,7 +183,8 @@ def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, d
         if m <= cap:
             break
         cap = CAPACITY.get(n, width, height)
-    ids, ranges, _ = bin_sort(n, depths, width, height, ws, cap, device)
+    with SPANS.span("sort"):
+        ids, ranges, _ = bin_sort(n, depths, width, height, ws, cap, device)
     layer, contrib, final_tau = draw_forward(width, height, dim, ranges, ids, records, empty_tau)
     return layer, contrib, final_tau, ranges, ids, m, ws
 

@@ -227,3 +227,17 @@ def test_edge_cases(device):
             assert len(o["ids"]) == 0
         if tweak == "single":
             assert len(o["ids"]) == 1 and (o["ranges"] == 0).all()
+
+
+def test_forward_counts_match_oracle(device, c0):
+    """The instrumented forward counts exactly the evaluations and
+    contributions of the oracle's per-pixel loop (the roofline's work)."""
+    scene, cam = c0
+    o = O.forward_weight(scene, cam)
+    layer, depths, st = gsmodel.render_weights(T(scene.pws, device), T(scene.alphas, device), T(scene.scales, device),
+                                               T(scene.rots, device), cam.to_torch(device), scene.K,
+                                               shs=T(scene.shs, device), return_state=True)
+    lay, contrib, ft, e, c = ops.draw_forward_counts(cam.width, cam.height, scene.K, st["ranges"], st["ids"],
+                                                     st["records"], 0.0)
+    same(N(lay), o["layer"], "layer")
+    assert (e, c) == (o["n_eval"], o["n_contrib"])

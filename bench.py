@@ -61,7 +61,7 @@ def main():
     import torch
 
     from rgp import gsmodel, ops
-    from rgp.losses import l1_ssim_loss
+    from rgp.losses import L1SSIMLoss
     from rgp.profiling import SPANS
     from rgp.scene import make_scene, orbit_cameras
 
@@ -105,7 +105,7 @@ def main():
         tc = tcams[i % len(tcams)]
         layer, _ = gsmodel.WeightGSFunctionTest.apply(pws, shs, alphas, scales, rots, tc, K)
         rgb = ops.Mix.apply(layer, torch.cat([bw, chroma], 0))
-        loss = l1_ssim_loss(rgb, targets[i % len(tcams)])
+        loss = L1SSIMLoss.apply(rgb, targets[i % len(tcams)], 0.2)
         opt.zero_grad(set_to_none=True)
         loss.backward()
         opt.step()

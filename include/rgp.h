@@ -203,6 +203,24 @@ int rgp_mix(int K, int64_t hw, const float* W, const float* palette, float* rgb,
 int rgp_mix_backward(int K, int64_t hw, const float* W, const float* palette, const float* drgb,
                      float* dW, float* dpalette, hipStream_t stream);
 
+/* ---- image loss of the finetune-step workload: (1 - lam) L1 + lam (1 - SSIM)
+ * with the 11x11 Gaussian window of gsplat/pytorch_ssim.py:17-66 and
+ * vanilla_3dgs/gsplat/pytorch_ssim.py:64-67 (no gsplatcu counterpart; the
+ * reference computes it with torch conv2d). ------------------------------ */
+
+/* img, target (C,H,W); window11 a host array of the 11 normalised taps.
+ * abc (3,C,H,W) receives the per-pixel SSIM partials the backward needs;
+ * sums (float64[2], caller zero-fills) the SSIM-map and |img - target| sums;
+ * loss (device float) the loss value. */
+int rgp_l1_ssim_forward(int channels, int height, int width, const float* img,
+                        const float* target, const float* window11, float C1, float C2, float lam,
+                        float* abc, double* sums, float* loss, hipStream_t stream);
+
+/* dimg (C,H,W) = grad_loss (device float) * dloss/dimg. */
+int rgp_l1_ssim_backward(int channels, int height, int width, const float* img,
+                         const float* target, const float* abc, const float* window11, float lam,
+                         const float* grad_loss, float* dimg, hipStream_t stream);
+
 /* Fill n 32-bit words with a value (graph-capture safe, no memset node). */
 int rgp_fill_u32(void* dst, uint32_t value, int64_t n, hipStream_t stream);
 

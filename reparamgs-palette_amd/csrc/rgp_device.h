@@ -283,17 +283,30 @@ __device__ __forceinline__ void sh_weights_lowrank(const float* __restrict__ dc,
 
 // ---------------------------------------------------------------------------
 // splat record: what the compositing kernels read per list entry.
-//   f4[0] = (u.x, u.y, cinv.x, 2*cinv.y)   f4[1] = (cinv.z, alpha, mcut, 0)
+//   f4[0] = (u.x, u.y, cinv.x, 2*cinv.y)   f4[1] = (cinv.z, alpha, mcut, ry)
 //   f4[2..] = the dim feature channels (weights or colours), zero padded.
 // 2*cinv.y is exact, so the Mahalanobis distance below is bit-identical to
 // the reference expression.  mcut bounds the distance beyond which
 // alpha * exp(-maha/2) < 0.002 with a margin far above rounding, so a skip on
 // maha > mcut never changes a result; NaN (alpha NaN) disables the skip.
+// ry bounds |dy| inside that region: min over dx of the quadratic form is
+// dy^2 / cov_yy, so |dy| > sqrt(mcut * cov_yy) implies maha > mcut.  A
+// wavefront whose pixel rows all lie farther than ry from u.y skips the entry.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float mcut_of(float alpha) {
   if (alpha != alpha) return alpha;  // NaN: never skip
   if (alpha < 0.002f) return -1.0f;  // alpha * e <= alpha < 0.002 for every e <= 1
   return 2.0f * logf(alpha / 0.002f) * 1.00001f + 1e-3f;
+}
+
+__device__ __forceinline__ float ry_of(float mcut, float cxx, float cxy, float cyy) {
+  const float inf = __uint_as_float(0x7f800000u);
+  if (!(mcut >= 0.f)) return mcut < 0.f ? -1.f : inf;  // no contribution / NaN
+  const double det = (double)cxx * cyy - (double)cxy * cxy;
+  if (!(cxx > 0.f) || !(det > 0.0)) return inf;  // not positive definite: no bound
+  const double cov_yy = (double)cxx / det;
+  const double r = sqrt((double)mcut * cov_yy) * 1.0001 + 1e-3;
+  return r < 3.0e38 ? (float)r : inf;
 }
 
 template <int DIM>

@@ -225,7 +225,8 @@ __device__ __forceinline__ void store_record(float4* __restrict__ rec, float ux,
                                              const float* ci, float alpha, const float* f) {
   constexpr int F4 = RecStride<DIM>::f4;
   rec[0] = make_float4(ux, uy, ci[0], 2.f * ci[1]);
-  rec[1] = make_float4(ci[2], alpha, mcut_of(alpha), 0.f);
+  const float mc = mcut_of(alpha);
+  rec[1] = make_float4(ci[2], alpha, mc, ry_of(mc, ci[0], ci[1], ci[2]));
 #pragma unroll
   for (int q = 0; q < F4 - 2; ++q) {
     float v[4];

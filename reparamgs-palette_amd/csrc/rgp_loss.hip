@@ -1,0 +1,195 @@
+// Fused (1 - lambda) L1 + lambda (1 - SSIM) image loss of the finetune step
+// for gfx950, forward and backward.  SSIM is the 11x11 Gaussian-window form
+// of gsplat/pytorch_ssim.py:17-66 with zero padding; the window is applied as
+// two separable passes over LDS tiles (it is an outer product).
+//
+// Forward, per 32x32 output tile and channel: stage the image and target with
+// a 5-pixel halo in LDS, blur the five maps x, y, x^2, y^2, xy horizontally
+// then vertically, evaluate the SSIM map and, per pixel, the partials of the
+// SSIM map with respect to the blurred maps that depend on the image:
+//   a = ds/dmu_x, b = ds/dE[x^2], c = ds/dE[xy]
+// (s = N/D, N = (2 mu_x mu_y + C1)(2 sxy + C2), D = (mu_x^2 + mu_y^2 + C1)
+// (sx^2 + sy^2 + C2)).  Sums of the SSIM map and of |x - y| go to two float64
+// accumulators with one atomic per block.
+// Backward: the adjoint of a zero-padded blur with a symmetric window is the
+// same blur, so dS/dx = G*a + 2 x (G*b) + y (G*c), evaluated with the same
+// LDS tiling; the L1 part adds sign(x - y).
+#include "rgp_internal.h"
+
+namespace rgp {
+
+constexpr int kST = 32;            // output tile edge
+constexpr int kSR = 5;             // window radius
+constexpr int kSI = kST + 2 * kSR;  // staged tile edge
+
+struct Win11 {
+  float g[11];
+};
+
+__device__ __forceinline__ float block_sum_f(float v, float* lds) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  v = wave_sum_to63(v);
+  if (lane == 63) lds[w] = v;
+  __syncthreads();
+  float s = 0.f;
+  if (threadIdx.x == 0)
+    for (int j = 0; j < (int)(blockDim.x >> 6); ++j) s += lds[j];
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __restrict__ img,
+                                                  const float* __restrict__ tgt, Win11 win,
+                                                  float C1, float C2, float* __restrict__ abc,
+                                                  double* __restrict__ sums) {
+  __shared__ float sx[kSI][kSI + 1], sy[kSI][kSI + 1];
+  __shared__ float hb[5][kSI][kST + 1];
+  __shared__ float red[8];
+  const int c = blockIdx.z;
+  const size_t HW = (size_t)H * W;
+  const float* X = img + c * HW;
+  const float* Y = tgt + c * HW;
+  const int x0 = blockIdx.x * kST - kSR, y0 = blockIdx.y * kST - kSR;
+  for (int i = threadIdx.x; i < kSI * kSI; i += blockDim.x) {
+    const int r = i / kSI, q = i - r * kSI;
+    const int gy = y0 + r, gx = x0 + q;
+    const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
+    sx[r][q] = in ? X[(size_t)gy * W + gx] : 0.f;
+    sy[r][q] = in ? Y[(size_t)gy * W + gx] : 0.f;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kSI * kST; i += blockDim.x) {
+    const int r = i / kST, q = i - r * kST;
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
+#pragma unroll
+    for (int j = 0; j < 11; ++j) {
+      const float a = sx[r][q + j], b = sy[r][q + j], g = win.g[j];
+      m0 += g * a; m1 += g * b; m2 += g * (a * a); m3 += g * (b * b); m4 += g * (a * b);
+    }
+    hb[0][r][q] = m0; hb[1][r][q] = m1; hb[2][r][q] = m2; hb[3][r][q] = m3; hb[4][r][q] = m4;
+  }
+  __syncthreads();
+  float s_sum = 0.f, l1_sum = 0.f;
+  for (int i = threadIdx.x; i < kST * kST; i += blockDim.x) {
+    const int r = i / kST, q = i - r * kST;
+    const int gy = y0 + kSR + r, gx = x0 + kSR + q;
+    if (gy >= H || gx >= W) continue;
+    float mx = 0.f, my = 0.f, exx = 0.f, eyy = 0.f, exy = 0.f;
+#pragma unroll
+    for (int j = 0; j < 11; ++j) {
+      const float g = win.g[j];
+      mx += g * hb[0][r + j][q]; my += g * hb[1][r + j][q]; exx += g * hb[2][r + j][q];
+      eyy += g * hb[3][r + j][q]; exy += g * hb[4][r + j][q];
+    }
+    const float A1 = 2.f * mx * my + C1, A2 = 2.f * (exy - mx * my) + C2;
+    const float B1 = mx * mx + my * my + C1, B2 = (exx - mx * mx) + (eyy - my * my) + C2;
+    const float D = B1 * B2, Dinv = 1.f / D;
+    const float s = A1 * A2 * Dinv;
+    const float dN = 2.f * my * (A2 - A1), dD = 2.f * mx * (B2 - B1);
+    const size_t o = c * HW + (size_t)gy * W + gx;
+    abc[o] = (dN - s * dD) * Dinv;             // ds/dmu_x
+    abc[3 * HW + o] = -s * B1 * Dinv;          // ds/dE[x^2]
+    abc[6 * HW + o] = 2.f * A1 * Dinv;         // ds/dE[xy]
+    s_sum += s;
+    l1_sum += fabsf(sx[r + kSR][q + kSR] - sy[r + kSR][q + kSR]);
+  }
+  const float bs = block_sum_f(s_sum, red);
+  __syncthreads();
+  const float bl = block_sum_f(l1_sum, red);
+  if (threadIdx.x == 0) {
+    unsafeAtomicAdd(&sums[0], (double)bs);
+    unsafeAtomicAdd(&sums[1], (double)bl);
+  }
+}
+
+__global__ void k_ssim_finish(const double* __restrict__ sums, double n, float lam,
+                              float* __restrict__ loss) {
+  if (threadIdx.x == 0)
+    *loss = (float)((1.0 - lam) * (sums[1] / n) + lam * (1.0 - sums[0] / n));
+}
+
+__global__ __launch_bounds__(256) void k_ssim_bwd(int H, int W, const float* __restrict__ img,
+                                                  const float* __restrict__ tgt,
+                                                  const float* __restrict__ abc, Win11 win,
+                                                  float lam, double n,
+                                                  const float* __restrict__ gout,
+                                                  float* __restrict__ dimg) {
+  __shared__ float sa[3][kSI][kSI + 1];
+  __shared__ float hb[3][kSI][kST + 1];
+  const int c = blockIdx.z;
+  const size_t HW = (size_t)H * W;
+  const int x0 = blockIdx.x * kST - kSR, y0 = blockIdx.y * kST - kSR;
+  for (int i = threadIdx.x; i < kSI * kSI; i += blockDim.x) {
+    const int r = i / kSI, q = i - r * kSI;
+    const int gy = y0 + r, gx = x0 + q;
+    const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
+    const size_t o = c * HW + (size_t)gy * W + gx;
+    sa[0][r][q] = in ? abc[o] : 0.f;
+    sa[1][r][q] = in ? abc[3 * HW + o] : 0.f;
+    sa[2][r][q] = in ? abc[6 * HW + o] : 0.f;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kSI * kST; i += blockDim.x) {
+    const int r = i / kST, q = i - r * kST;
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f;
+#pragma unroll
+    for (int j = 0; j < 11; ++j) {
+      const float g = win.g[j];
+      m0 += g * sa[0][r][q + j]; m1 += g * sa[1][r][q + j]; m2 += g * sa[2][r][q + j];
+    }
+    hb[0][r][q] = m0; hb[1][r][q] = m1; hb[2][r][q] = m2;
+  }
+  __syncthreads();
+  const float go = *gout;
+  const float kl1 = (float)((1.0 - lam) / n) * go, kss = (float)(-lam / n) * go;
+  for (int i = threadIdx.x; i < kST * kST; i += blockDim.x) {
+    const int r = i / kST, q = i - r * kST;
+    const int gy = y0 + kSR + r, gx = x0 + kSR + q;
+    if (gy >= H || gx >= W) continue;
+    float ga = 0.f, gb = 0.f, gc = 0.f;
+#pragma unroll
+    for (int j = 0; j < 11; ++j) {
+      const float g = win.g[j];
+      ga += g * hb[0][r + j][q]; gb += g * hb[1][r + j][q]; gc += g * hb[2][r + j][q];
+    }
+    const size_t o = c * HW + (size_t)gy * W + gx;
+    const float x = img[o], y = tgt[o];
+    const float sg = x > y ? 1.f : (x < y ? -1.f : 0.f);
+    dimg[o] = kl1 * sg + kss * (ga + 2.f * x * gb + y * gc);
+  }
+}
+
+}  // namespace rgp
+
+using namespace rgp;
+
+static Win11 make_win(const float* w) {
+  Win11 v;
+  for (int j = 0; j < 11; ++j) v.g[j] = w[j];
+  return v;
+}
+
+extern "C" int rgp_l1_ssim_forward(int channels, int height, int width, const float* img,
+                                   const float* target, const float* window11, float C1, float C2,
+                                   float lam, float* abc, double* sums, float* loss,
+                                   hipStream_t s) {
+  if (channels <= 0 || height <= 0 || width <= 0 || channels > 65535 || !window11)
+    return set_error(RGP_ERR_ARG, "rgp_l1_ssim_forward: bad size or window");
+  const dim3 grid((width + kST - 1) / kST, (height + kST - 1) / kST, channels);
+  hipLaunchKernelGGL(k_ssim_fwd, grid, dim3(256), 0, s, height, width, img, target,
+                     make_win(window11), C1, C2, abc, sums);
+  hipLaunchKernelGGL(k_ssim_finish, dim3(1), dim3(64), 0, s, (const double*)sums,
+                     (double)channels * height * width, lam, loss);
+  return check_launch("rgp_l1_ssim_forward");
+}
+
+extern "C" int rgp_l1_ssim_backward(int channels, int height, int width, const float* img,
+                                    const float* target, const float* abc, const float* window11,
+                                    float lam, const float* grad_loss, float* dimg,
+                                    hipStream_t s) {
+  if (channels <= 0 || height <= 0 || width <= 0 || channels > 65535 || !window11)
+    return set_error(RGP_ERR_ARG, "rgp_l1_ssim_backward: bad size or window");
+  const dim3 grid((width + kST - 1) / kST, (height + kST - 1) / kST, channels);
+  hipLaunchKernelGGL(k_ssim_bwd, grid, dim3(256), 0, s, height, width, img, target, abc,
+                     make_win(window11), lam, (double)channels * height * width, grad_loss, dimg);
+  return check_launch("rgp_l1_ssim_backward");
+}

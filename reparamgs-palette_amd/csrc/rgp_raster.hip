@@ -37,6 +37,7 @@ __global__ __launch_bounds__(256) void k_draw_fwd(int W, int H, int gx, int tile
   const bool inside = px < W && py < H;
   const int2 rg = ranges[tile];
   const float fpx = (float)px, fpy = (float)py;
+  const float ycen = (float)(ty * kTile + w * 4) + 1.5f;  // centre of this wavefront's rows
 
   float acc[DIM];
 #pragma unroll
@@ -60,9 +61,11 @@ __global__ __launch_bounds__(256) void k_draw_fwd(int W, int H, int gx, int tile
       const int cnt = min(256, rg.y - b0);
       for (int j = 0; j < cnt; ++j) {
         if (__all(done)) break;
-        if (done) continue;
         const float4 g0 = s_rec[j * F4];
         const float4 g1 = s_rec[j * F4 + 1];
+        // wavefront-uniform: the entry cannot reach any of this strip's rows
+        if ((g0.y - ycen > g1.w + 1.5f) || (ycen - g0.y > g1.w + 1.5f)) continue;
+        if (done) continue;
         const float dx = g0.x - fpx, dy = g0.y - fpy;
         const float m = maha_dist(g0, g1.x, dx, dy);
         if (m > g1.z) continue;
@@ -109,6 +112,13 @@ __global__ __launch_bounds__(256) void k_draw_fwd(int W, int H, int gx, int tile
   }
 }
 
+// Backward.  Entries are visited in reverse list order from the largest
+// contrib of the workgroup (each wavefront starts at its own largest contrib);
+// per pixel the transmittance is recovered by division as in the reference.
+// The K gradient terms of an entry are summed over the wavefront with DPP,
+// the four wavefront sums go to LDS, and after each batch one thread per
+// entry adds their total to the Gaussian's gradient: one atomic per
+// (workgroup, entry, channel) instead of one per (pixel, entry, channel).
 template <int DIM>
 __global__ __launch_bounds__(256) void k_draw_bwd(int W, int H, int gx, int tiles,
                                                   const int2* __restrict__ ranges,
@@ -120,8 +130,9 @@ __global__ __launch_bounds__(256) void k_draw_bwd(int W, int H, int gx, int tile
                                                   float* __restrict__ dfeat) {
   constexpr int F4 = RecStride<DIM>::f4;
   __shared__ float4 s_rec[256 * F4];
+  __shared__ float s_part[4][DIM][256];
   __shared__ int32_t s_id[256];
-  __shared__ int32_t s_max;
+  __shared__ int32_t s_max[4];
   const int tile = xcd_tile(blockIdx.x, tiles);
   const int ty = tile / gx, tx = tile - ty * gx;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
@@ -130,6 +141,7 @@ __global__ __launch_bounds__(256) void k_draw_bwd(int W, int H, int gx, int tile
   const int2 rg = ranges[tile];
   if (rg.y - rg.x <= 0) return;
   const float fpx = (float)px, fpy = (float)py;
+  const float ycen = (float)(ty * kTile + w * 4) + 1.5f;
 
   float dl[DIM];
   float tau = 0.f;
@@ -141,19 +153,15 @@ __global__ __launch_bounds__(256) void k_draw_bwd(int W, int H, int gx, int tile
     tau = final_tau[pix];
     cont = contrib[pix];
   }
-  if (t == 0) s_max = 0;
-  __syncthreads();
-  {
-    int c = cont;
+  int wmax = cont;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) c = max(c, __shfl_xor(c, o));
-    if (lane == 0) atomicMax(&s_max, c);
-  }
+  for (int o = 32; o > 0; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
+  if (lane == 0) s_max[w] = wmax;
   __syncthreads();
-  const int maxc = s_max;
+  const int maxc = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
   for (int top = rg.x + maxc; top > rg.x; top -= 256) {
     const int lo = max(rg.x, top - 256);
-    __syncthreads();
+    const int cnt = top - lo;
     const int e = top - 1 - t;
     if (e >= lo) {
       const int g = ids[e];
@@ -162,15 +170,21 @@ __global__ __launch_bounds__(256) void k_draw_bwd(int W, int H, int gx, int tile
 #pragma unroll
       for (int q = 0; q < F4; ++q) s_rec[t * F4 + q] = src[q];
     }
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) {
+      s_part[0][k][t] = 0.f; s_part[1][k][t] = 0.f; s_part[2][k][t] = 0.f; s_part[3][k][t] = 0.f;
+    }
     __syncthreads();
-    const int cnt = top - lo;
-    for (int j = 0; j < cnt; ++j) {
+    // entries with forward index >= wmax hold no contributor of this wavefront
+    const int jstart = max(0, top - rg.x - wmax);
+    for (int j = jstart; j < cnt; ++j) {
       const int fi = top - 1 - j - rg.x;  // index of the entry in forward order
+      const float4 g0 = s_rec[j * F4];
+      const float4 g1 = s_rec[j * F4 + 1];
+      if ((g0.y - ycen > g1.w + 1.5f) || (ycen - g0.y > g1.w + 1.5f)) continue;
       bool on = false;
       float gv = 0.f;
       if (fi < cont) {
-        const float4 g0 = s_rec[j * F4];
-        const float4 g1 = s_rec[j * F4 + 1];
         const float dx = g0.x - fpx, dy = g0.y - fpy;
         const float m = maha_dist(g0, g1.x, dx, dy);
         if (!(m > g1.z)) {
@@ -183,14 +197,23 @@ __global__ __launch_bounds__(256) void k_draw_bwd(int W, int H, int gx, int tile
         }
       }
       if (__any(on)) {
-        const int g = s_id[j];
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
-          const float s = wave_sum_to63(on ? dl[k] * gv : 0.f);
-          if (lane == 63) unsafeAtomicAdd(&dfeat[(size_t)g * DIM + k], s);
+          const float sum = wave_sum_to63(on ? dl[k] * gv : 0.f);
+          if (lane == 63) s_part[w][k][j] = sum;
         }
       }
     }
+    __syncthreads();
+    if (t < cnt) {
+      const int g = s_id[t];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) {
+        const float v = (s_part[0][k][t] + s_part[1][k][t]) + (s_part[2][k][t] + s_part[3][k][t]);
+        if (v != 0.f) unsafeAtomicAdd(&dfeat[(size_t)g * DIM + k], v);
+      }
+    }
+    __syncthreads();
   }
 }
 

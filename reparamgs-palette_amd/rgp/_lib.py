@@ -49,6 +49,8 @@ _SIGS = {
     "rgp_mix": (I, [I, I64, P, P, P, P]),
     "rgp_mix_backward": (I, [I, I64, P, P, P, P, P, P]),
     "rgp_fill_u32": (I, [P, U32, I64, P]),
+    "rgp_l1_ssim_forward": (I, [I, I, I, P, P, P, F, F, F, P, P, P, P]),
+    "rgp_l1_ssim_backward": (I, [I, I, I, P, P, P, P, F, P, P, P]),
 }
 
 

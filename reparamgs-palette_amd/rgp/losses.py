@@ -44,3 +44,56 @@ def ssim(img1, img2, range_=1.0, window_size=11):
 
 def l1_ssim_loss(image, target, lam=0.2):
     return (1.0 - lam) * torch.abs(image - target).mean() + lam * (1.0 - ssim(image, target))
+
+
+def window_taps(size=11, sigma=1.5):
+    """The normalised 1-D taps of create_window (pytorch_ssim.py:17-29), as
+    float32 values."""
+    import numpy as np
+    g = np.array([math.exp(-(x - size // 2) ** 2 / float(2 * sigma ** 2)) for x in range(size)], np.float32)
+    return (g / g.sum()).astype(np.float32)
+
+
+class L1SSIMLoss(torch.autograd.Function):
+    """(1 - lam) * mean|img - target| + lam * (1 - SSIM(img, target)) on the
+    fused gfx950 kernels; the target is a constant."""
+
+    @staticmethod
+    def forward(ctx, img, target, lam=0.2, range_=1.0):
+        import ctypes
+
+        from ._lib import call, f32c, ptr, require_cuda, stream
+        require_cuda(img, target)
+        x, y = f32c(img), f32c(target)
+        if x.dim() == 4:
+            assert x.shape[0] == 1
+            x, y = x[0], y[0]
+        C, H, W = x.shape
+        taps = window_taps()
+        ctx.taps = taps
+        abc = torch.empty((3, C, H, W), dtype=torch.float32, device=x.device)
+        sums = torch.zeros(2, dtype=torch.float64, device=x.device)
+        loss = torch.empty((), dtype=torch.float32, device=x.device)
+        call("rgp_l1_ssim_forward", C, H, W, ptr(x), ptr(y), taps.ctypes.data_as(ctypes.c_void_p),
+             (0.01 * range_) ** 2, (0.03 * range_) ** 2, float(lam), ptr(abc), ptr(sums), ptr(loss), stream(x.device))
+        ctx.lam = float(lam)
+        ctx.shape = img.shape
+        ctx.save_for_backward(x, y, abc)
+        return loss
+
+    @staticmethod
+    def backward(ctx, gout):
+        import ctypes
+
+        from ._lib import call, ptr, stream
+        x, y, abc = ctx.saved_tensors
+        C, H, W = x.shape
+        g = gout.detach().float().contiguous().reshape(1)
+        dimg = torch.empty_like(x)
+        call("rgp_l1_ssim_backward", C, H, W, ptr(x), ptr(y), ptr(abc), ctx.taps.ctypes.data_as(ctypes.c_void_p),
+             ctx.lam, ptr(g), ptr(dimg), stream(x.device))
+        return dimg.reshape(ctx.shape), None, None, None
+
+
+def l1_ssim_loss_fused(image, target, lam=0.2):
+    return L1SSIMLoss.apply(image, target, lam)

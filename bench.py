@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--cpu-views", type=int, default=1, help="cameras in the CPU-oracle sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (the known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters in a --pmc profile)")
     return ap.parse_args()
 
 
@@ -67,6 +70,13 @@ def main():
 
     dev = torch.device("cuda:0")
     K = 5
+    if args.copy_calibration:
+        a = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     t_setup = time.time()
     scene = make_scene(args.n, seed=args.seed, extent=3.0, K=K)
     cams = orbit_cameras(args.views, 6.0, args.width, args.height)

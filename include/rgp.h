@@ -156,6 +156,17 @@ int rgp_draw_backward(int width, int height, int dim, const int32_t* ranges, con
                       const float* records, const int32_t* contrib, const float* final_tau,
                       const float* dlayer, float* dfeat, hipStream_t stream);
 
+/* Replay form of draw_weight_B's result: the forward recurrence is replayed
+ * front to back and every contributing (pixel, entry) adds
+ * dlayer[k,p] * tau * alpha' to dfeat, tau being the transmittance in front
+ * of the entry (the factor the forward applied to the features).  Needs no
+ * contrib / final_tau; the contributing set is the forward's.  The values
+ * differ from rgp_draw_backward only by the rounding of the reference's
+ * division-recovered tau (kernel.cu:1371).  dfeat zero-filled by caller. */
+int rgp_draw_backward_replay(int width, int height, int dim, const int32_t* ranges,
+                             const int32_t* ids, const float* records, const float* dlayer,
+                             float* dfeat, hipStream_t stream);
+
 /* ---- fused paths (no reference counterpart as single calls; they produce
  * the same values as the reference chains they replace) ------------------ */
 
@@ -210,7 +221,8 @@ int rgp_mix_backward(int K, int64_t hw, const float* W, const float* palette, co
 
 /* img, target (C,H,W); window11 a host array of the 11 normalised taps.
  * abc (3,C,H,W) receives the per-pixel SSIM partials the backward needs;
- * sums (float64[2], caller zero-fills) the SSIM-map and |img - target| sums;
+ * sums (float64[128], caller zero-fills) 64 pairs of partial SSIM-map and
+ * |img - target| sums;
  * loss (device float) the loss value. */
 int rgp_l1_ssim_forward(int channels, int height, int width, const float* img,
                         const float* target, const float* window11, float C1, float C2, float lam,

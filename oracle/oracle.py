@@ -235,8 +235,10 @@ def draw(width, height, ranges, ids, us, cinv, alphas, feat, final_tau_init=0.0,
 
 
 def draw_backward(width, height, K, ranges, ids, us, cinv, alphas, contrib, final_tau, dlayer, dt=f32, mode=0):
-    """Returns (grad (N,K) float64, gabs (N,K) float64) for mode 0; for mode 1
-    the gradient accumulated with atomic adds in the arithmetic type."""
+    """Returns (grad (N,K) float64, gabs (N,K) float64) for mode 0 (the
+    reference's division-recovered tau) and mode 2 (tau replayed front to
+    back); for mode 1 the mode-0 gradient accumulated with atomic adds in the
+    arithmetic type."""
     n = us.shape[0]
     grad, gabs = np.zeros((n, K), f64), np.zeros((n, K), f64)
     grad_t = np.zeros((n, K), dt)
@@ -244,7 +246,7 @@ def draw_backward(width, height, K, ranges, ids, us, cinv, alphas, contrib, fina
         width, height, K, _p(_c(ranges, i32)), _p(_c(ids, i32)), _p(_c(us, dt)), _p(_c(cinv, dt)),
         _p(_c(alphas, dt).reshape(-1)), _p(_c(contrib, i32)), _p(_c(final_tau, dt)), _p(_c(dlayer, dt)), mode,
         _p(grad), _p(gabs), _p(grad_t))
-    return (grad, gabs) if mode == 0 else grad_t
+    return grad_t if mode == 1 else (grad, gabs)
 
 
 def mix(palette, W, dt=f32):
@@ -297,7 +299,7 @@ def _recon(scene):
     return C.reshape(n, -1)
 
 
-def backward_weight(scene, fwd, dlayer, parallel=False):
+def backward_weight(scene, fwd, dlayer, parallel=False, replay=False):
     """The backward of WeightGSFunctionTest / Fast: splat_weight_B, then the
     SH chain.  Returns a dict: dweights (N,K) float64 plus its abs-sum bound,
     and dshs (full rank) or ddc, dA, dB (low rank), in float64.
@@ -315,7 +317,7 @@ def backward_weight(scene, fwd, dlayer, parallel=False):
         n, L = basis.shape
         return dict(dweights=g, dshs=(g[:, None, :] * basis[:, :, None]).reshape(n, L * K))
     g, gabs = draw_backward(fwd["width"], fwd["height"], K, fwd["ranges"], fwd["ids"], fwd["us"], fwd["cinv"],
-                            scene.alphas, fwd["contrib"], fwd["final_tau"], dlayer, dt)
+                            scene.alphas, fwd["contrib"], fwd["final_tau"], dlayer, dt, mode=2 if replay else 0)
     out = dict(dweights=g, dweights_abs=gabs)
     if fwd["fast"]:
         C0 = f64(0.28209479177387814) if dt == f64 else f64(f32(0.28209479177387814))

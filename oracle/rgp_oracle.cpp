@@ -552,11 +552,53 @@ void draw(int width, int height, int dim, const int32_t* ranges, const int32_t* 
 // their absolute values into gabs (N,K), which bounds the reordering error of
 // any atomic summation.  mode 1: OpenMP over tiles, atomic adds in T.
 // ---------------------------------------------------------------------------
+// replay: the same terms with tau taken from a front-to-back replay of the
+// forward recurrence (tau before the entry, the value the forward multiplied
+// the weights by) instead of the reference's division from final_tau; the
+// replay stops where the forward stopped, so the contributing set is the same.
+template <typename T>
+void draw_backward_replay(int width, int height, int K, const int32_t* ranges, const int32_t* ids,
+                          const T* us, const T* cinv, const T* alphas, const T* dlayer,
+                          double* grad, double* gabs) {
+  const int gx = (width + BLOCK - 1) / BLOCK, gy = (height + BLOCK - 1) / BLOCK;
+  const size_t HW = (size_t)width * height;
+  for (int tile = 0; tile < gx * gy; ++tile) {
+    const int tx = tile % gx, ty = tile / gx;
+    const int r0 = ranges[2 * tile], r1 = ranges[2 * tile + 1];
+    if (r1 - r0 == 0) continue;
+    for (int py = ty * BLOCK; py < std::min(height, ty * BLOCK + BLOCK); ++py)
+      for (int px = tx * BLOCK; px < std::min(width, tx * BLOCK + BLOCK); ++px) {
+        const size_t pix = (size_t)py * width + px;
+        T tau = 1;
+        for (int j = r0; j < r1; ++j) {
+          const int g = ids[j];
+          const T dx = us[2 * g] - (T)px, dy = us[2 * g + 1] - (T)py;
+          const T cx = cinv[3 * g], cy = cinv[3 * g + 1], cz = cinv[3 * g + 2];
+          const T maha = fmx((T)0, cx * dx * dx + cz * dy * dy + (T)2 * cy * dx * dy);
+          const T ap = fmn((T)0.99f, alphas[g] * ex((T)-0.5f * maha));
+          if (ap < (T)0.002f) continue;
+          const T ta = tau * ap;
+          for (int k = 0; k < K; ++k) {
+            const T term = dlayer[k * HW + pix] * ta;
+            grad[(size_t)g * K + k] += (double)term;
+            gabs[(size_t)g * K + k] += std::fabs((double)term);
+          }
+          tau *= ((T)1 - ap);
+          if (tau < (T)0.0001f) break;
+        }
+      }
+  }
+}
+
 template <typename T>
 void draw_backward(int width, int height, int K, const int32_t* ranges, const int32_t* ids,
                    const T* us, const T* cinv, const T* alphas, const int32_t* contrib,
                    const T* final_tau, const T* dlayer, int mode, double* grad, double* gabs,
                    T* grad_t) {
+  if (mode == 2) {
+    draw_backward_replay(width, height, K, ranges, ids, us, cinv, alphas, dlayer, grad, gabs);
+    return;
+  }
   const int gx = (width + BLOCK - 1) / BLOCK, gy = (height + BLOCK - 1) / BLOCK;
   const size_t HW = (size_t)width * height;
 #pragma omp parallel for schedule(dynamic, 4) if (mode == 1)

@@ -65,6 +65,28 @@ __device__ __forceinline__ float exp_cw(float x) {
   return p * __uint_as_float((uint32_t)(ni + 127) << 23);
 }
 
+// exp_cw for x in [-85, 0]: the same operations without the range handling,
+// which never triggers there (n >= -123), hence bit-identical results.
+__device__ __forceinline__ float exp_cw_neg(float x) {
+  float n = __builtin_rintf(x * 1.44269504088896341f);
+  float r = __builtin_fmaf(n, -0.693145751953125f, x);
+  r = __builtin_fmaf(n, -1.42860676533018570e-06f, r);
+  float p = 1.98412698e-4f;
+  p = __builtin_fmaf(p, r, 1.38888889e-3f);
+  p = __builtin_fmaf(p, r, 8.33333377e-3f);
+  p = __builtin_fmaf(p, r, 4.16666679e-2f);
+  p = __builtin_fmaf(p, r, 1.66666672e-1f);
+  p = __builtin_fmaf(p, r, 0.5f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  return p * __uint_as_float((uint32_t)((int)n + 127) << 23);
+}
+
+// exp(-m/2) for a clamped Mahalanobis distance m >= 0.
+__device__ __forceinline__ float gauss_of(float m) {
+  return m <= 170.0f ? exp_cw_neg(-0.5f * m) : exp_cw(-0.5f * m);
+}
+
 // Float -> int conversions that saturate and send NaN to 0.
 __device__ __forceinline__ int32_t sat_i32(float x) {
   if (x != x) return 0;

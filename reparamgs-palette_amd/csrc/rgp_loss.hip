@@ -96,15 +96,20 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __r
   __syncthreads();
   const float bl = block_sum_f(l1_sum, red);
   if (threadIdx.x == 0) {
-    unsafeAtomicAdd(&sums[0], (double)bs);
-    unsafeAtomicAdd(&sums[1], (double)bl);
+    // 64 accumulator pairs so that blocks do not queue on one address
+    const int slot = (blockIdx.x + blockIdx.y * gridDim.x + blockIdx.z * gridDim.x * gridDim.y) & 63;
+    unsafeAtomicAdd(&sums[2 * slot], (double)bs);
+    unsafeAtomicAdd(&sums[2 * slot + 1], (double)bl);
   }
 }
 
 __global__ void k_ssim_finish(const double* __restrict__ sums, double n, float lam,
                               float* __restrict__ loss) {
-  if (threadIdx.x == 0)
-    *loss = (float)((1.0 - lam) * (sums[1] / n) + lam * (1.0 - sums[0] / n));
+  if (threadIdx.x == 0) {
+    double s = 0.0, l = 0.0;
+    for (int i = 0; i < 64; ++i) { s += sums[2 * i]; l += sums[2 * i + 1]; }
+    *loss = (float)((1.0 - lam) * (l / n) + lam * (1.0 - s / n));
+  }
 }
 
 __global__ __launch_bounds__(256) void k_ssim_bwd(int H, int W, const float* __restrict__ img,

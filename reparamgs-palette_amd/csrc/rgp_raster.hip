@@ -1,219 +1,386 @@
 // Tile compositing kernels for gfx950: the forward of draw_weight/draw_color
 // (gsplatcu/kernel.cu:194-372), the reverse-order backward of draw_weight_B
-// (kernel.cu:1291-1379) and the palette mix.
+// (kernel.cu:1291-1379), a front-to-back replay backward, and the palette mix.
 //
-// A 256-thread workgroup owns one 16x16 tile; wavefront w covers rows
-// 4w..4w+3 (64 pixels).  The tile's list is staged in LDS 256 entries at a
-// time as packed splat records; each wavefront then walks the batch on its
-// own, without the per-entry workgroup barrier of the reference.  A pixel
-// whose transmittance drops below 1e-4 stops; a wavefront whose pixels have
-// all stopped leaves the batch loop, and the workgroup leaves when all its
-// pixels have stopped (one barrier count per batch).  Per entry and pixel the
-// Mahalanobis distance is compared with the record's cutoff first, so exp()
-// runs only where alpha' can reach 0.002.  Per pixel the arithmetic is the
-// reference recurrence, operation for operation, so results do not depend on
-// any of this scheduling.
+// Layout.  One wavefront (a 64-thread workgroup) owns one 16x16 tile.  Lane l
+// holds the column px = x0 + (l & 15) in four row slots: slot j covers row
+// y0 + 4j + (l >> 4), so slot j across the wavefront is a 16x4 strip.  The
+// tile's list is staged in LDS 64 entries at a time (one per lane, the next
+// batch prefetched into registers while the current one is composited); no
+// barrier beyond the wavefront is needed.
+//
+// Work per entry.  The terms of the Mahalanobis distance that depend only on
+// dx, (cinv.x dx) dx and (2 cinv.y) dx, are formed once per lane and reused by
+// its four pixels; per pixel the operations and their order are those of
+// mahaSqDist, so every value is bit-identical to the reference recurrence.
+// While staging, each lane also works out which of the four strips its entry
+// can reach (|dy| within the record's cutoff radius ry); the wavefront reads
+// that 4-bit mask into scalar registers and skips unreachable strips with a
+// scalar branch.  A pixel then compares the distance with the record's cutoff
+// mcut before evaluating exp().  A pixel has stopped when its transmittance
+// is below 1e-4 (pixels outside the image start at 0).
 #include "rgp_internal.h"
 
 namespace rgp {
 
-// COUNT: also accumulate, over the launch, the list entries each pixel
-// evaluates before it stops (counts[0]) and its contributions (counts[1]).
+constexpr int kSlots = 4;
+
+// A splat record held in registers.
+template <int F4>
+struct RecRegs {
+  float4 v[F4];
+};
+
+// Loads the record of list entry e when begin <= e < end (else zeros).
+template <int F4>
+__device__ __forceinline__ RecRegs<F4> load_rec(const float4* __restrict__ recs,
+                                                const int32_t* __restrict__ ids, int e, int begin,
+                                                int end, int* id) {
+  RecRegs<F4> r;
+  const bool ok = e >= begin && e < end;
+  const int g = ok ? ids[e] : 0;
+  *id = g;
+  const float4* src = recs + (size_t)g * F4;
+#pragma unroll
+  for (int q = 0; q < F4; ++q) r.v[q] = ok ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+  return r;
+}
+
+// Stages one record in LDS, replacing its ry by the mask of strips of tile
+// row ty it can reach (bit j: rows ty*16 + 4j .. + 3).
+template <int F4>
+__device__ __forceinline__ void stage_rec(float4* s_rec, int slot, const RecRegs<F4>& r, int ty) {
+  uint32_t mask = 0;
+  const float uy = r.v[0].y, ry = r.v[1].w;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const float yc = (float)(ty * kTile + 4 * j) + 1.5f;
+    if (!((uy - yc > ry + 1.5f) || (yc - uy > ry + 1.5f))) mask |= 1u << j;
+  }
+  s_rec[slot * F4] = r.v[0];
+  s_rec[slot * F4 + 1] = make_float4(r.v[1].x, r.v[1].y, r.v[1].z, __uint_as_float(mask));
+#pragma unroll
+  for (int q = 2; q < F4; ++q) s_rec[slot * F4 + q] = r.v[q];
+}
+
+__device__ __forceinline__ uint32_t uniform_bits(float v) {
+  return __builtin_amdgcn_readfirstlane(__float_as_uint(v));
+}
+
+// Mask of slots in which some lane's pixel is still compositing.
+__device__ __forceinline__ uint32_t live_slots(const float* tau) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j)
+    if (__any(!(tau[j] < 0.0001f))) m |= 1u << j;
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// forward.  COUNT: also accumulate, over the launch, the list entries each
+// pixel evaluates before it stops (counts[0]) and its contributions
+// (counts[1]).
+// ---------------------------------------------------------------------------
 template <int DIM, bool COUNT = false>
-__global__ __launch_bounds__(256) void k_draw_fwd(int W, int H, int gx, int tiles,
-                                                  const int2* __restrict__ ranges,
-                                                  const int32_t* __restrict__ ids,
-                                                  const float4* __restrict__ recs, float empty_tau,
-                                                  float* __restrict__ layer,
-                                                  int32_t* __restrict__ contrib,
-                                                  float* __restrict__ final_tau,
-                                                  unsigned long long* __restrict__ counts = nullptr) {
+__global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles,
+                                                 const int2* __restrict__ ranges,
+                                                 const int32_t* __restrict__ ids,
+                                                 const float4* __restrict__ recs, float empty_tau,
+                                                 float* __restrict__ layer,
+                                                 int32_t* __restrict__ contrib,
+                                                 float* __restrict__ final_tau,
+                                                 unsigned long long* __restrict__ counts) {
   constexpr int F4 = RecStride<DIM>::f4;
-  __shared__ float4 s_rec[256 * F4];
+  __shared__ float4 s_rec[64 * F4];
   const int tile = xcd_tile(blockIdx.x, tiles);
   const int ty = tile / gx, tx = tile - ty * gx;
-  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int px = tx * kTile + (lane & 15), py = ty * kTile + w * 4 + (lane >> 4);
-  const bool inside = px < W && py < H;
+  const int lane = threadIdx.x;
+  const int px = tx * kTile + (lane & 15);
+  const int y0 = ty * kTile + (lane >> 4);
+  const float fpx = (float)px;
   const int2 rg = ranges[tile];
-  const float fpx = (float)px, fpy = (float)py;
-  const float ycen = (float)(ty * kTile + w * 4) + 1.5f;  // centre of this wavefront's rows
 
-  float acc[DIM];
+  float acc[kSlots][DIM];
+  float tau[kSlots], fpy[kSlots];
+  int cont[kSlots];
+  int n_eval[kSlots], n_con[kSlots];
 #pragma unroll
-  for (int k = 0; k < DIM; ++k) acc[k] = 0.f;
-  float tau = 1.f;
-  int cont = 0;
-  bool done = !inside;
-  int n_eval = 0, n_con = 0;  // COUNT only
-  if (rg.y - rg.x <= 0) {
-    tau = empty_tau;  // the reference kernel returns before writing
-  } else {
-    for (int b0 = rg.x; b0 < rg.y; b0 += 256) {
-      if (__syncthreads_count(done) == 256) break;
-      const int e = b0 + t;
-      if (e < rg.y) {
-        const float4* src = recs + (size_t)ids[e] * F4;
+  for (int j = 0; j < kSlots; ++j) {
 #pragma unroll
-        for (int q = 0; q < F4; ++q) s_rec[t * F4 + q] = src[q];
-      }
+    for (int k = 0; k < DIM; ++k) acc[j][k] = 0.f;
+    const bool in = px < W && y0 + 4 * j < H;
+    tau[j] = in ? 1.f : 0.f;
+    fpy[j] = (float)(y0 + 4 * j);
+    cont[j] = 0;
+    n_eval[j] = 0;
+    n_con[j] = 0;
+  }
+  if (rg.y - rg.x > 0) {
+    int nid = 0;
+    RecRegs<F4> nxt = load_rec<F4>(recs, ids, rg.x + lane, rg.x, rg.y, &nid);
+    for (int b0 = rg.x; b0 < rg.y; b0 += 64) {
+      const uint32_t live = live_slots(tau);
+      if (live == 0) break;
+      __syncthreads();  // previous batch fully read
+      stage_rec<F4>(s_rec, lane, nxt, ty);
       __syncthreads();
-      const int cnt = min(256, rg.y - b0);
-      for (int j = 0; j < cnt; ++j) {
-        if (__all(done)) break;
-        const float4 g0 = s_rec[j * F4];
-        const float4 g1 = s_rec[j * F4 + 1];
-        // wavefront-uniform: the entry cannot reach any of this strip's rows
-        if ((g0.y - ycen > g1.w + 1.5f) || (ycen - g0.y > g1.w + 1.5f)) continue;
-        if (done) continue;
-        const float dx = g0.x - fpx, dy = g0.y - fpy;
-        const float m = maha_dist(g0, g1.x, dx, dy);
-        if (m > g1.z) continue;
-        const float ap = fminf(0.99f, g1.y * exp_cw(-0.5f * m));
-        if (ap < 0.002f) continue;
-        const float ta = tau * ap;
+      nxt = load_rec<F4>(recs, ids, b0 + 64 + lane, rg.x, rg.y, &nid);
+      const int cnt = min(64, rg.y - b0);
+      for (int jj = 0; jj < cnt; ++jj) {
+        const float4 g0 = s_rec[jj * F4];
+        const float4 g1 = s_rec[jj * F4 + 1];
+        const uint32_t slots = uniform_bits(g1.w) & live;
+        if (slots == 0) continue;
+        const float dx = g0.x - fpx;
+        const float ta0 = g0.z * dx * dx;  // (cinv.x dx) dx
+        const float tc0 = g0.w * dx;       // (2 cinv.y) dx
 #pragma unroll
-        for (int q = 0; q < F4 - 2; ++q) {
-          const float4 f = s_rec[j * F4 + 2 + q];
-          if (4 * q + 0 < DIM) acc[4 * q + 0] = acc[4 * q + 0] + ta * f.x;
-          if (4 * q + 1 < DIM) acc[4 * q + 1] = acc[4 * q + 1] + ta * f.y;
-          if (4 * q + 2 < DIM) acc[4 * q + 2] = acc[4 * q + 2] + ta * f.z;
-          if (4 * q + 3 < DIM) acc[4 * q + 3] = acc[4 * q + 3] + ta * f.w;
-        }
-        cont = b0 + j - rg.x + 1;
-        tau = tau * (1.f - ap);
-        if (COUNT) ++n_con;
-        if (tau < 0.0001f) {
-          done = true;
-          if (COUNT) n_eval = cont;
+        for (int j = 0; j < kSlots; ++j) {
+          if (!(slots & (1u << j))) continue;
+          if (tau[j] < 0.0001f) continue;
+          const float dy = g0.y - fpy[j];
+          const float m = fmaxf(0.0f, (ta0 + g1.x * dy * dy) + tc0 * dy);
+          if (m > g1.z) continue;
+          const float ap = fminf(0.99f, g1.y * gauss_of(m));
+          if (ap < 0.002f) continue;
+          const float ta = tau[j] * ap;
+#pragma unroll
+          for (int q = 0; q < F4 - 2; ++q) {
+            const float4 f = s_rec[jj * F4 + 2 + q];
+            if (4 * q + 0 < DIM) acc[j][4 * q + 0] = acc[j][4 * q + 0] + ta * f.x;
+            if (4 * q + 1 < DIM) acc[j][4 * q + 1] = acc[j][4 * q + 1] + ta * f.y;
+            if (4 * q + 2 < DIM) acc[j][4 * q + 2] = acc[j][4 * q + 2] + ta * f.z;
+            if (4 * q + 3 < DIM) acc[j][4 * q + 3] = acc[j][4 * q + 3] + ta * f.w;
+          }
+          cont[j] = b0 + jj - rg.x + 1;
+          tau[j] = tau[j] * (1.f - ap);
+          if (COUNT) {
+            ++n_con[j];
+            if (tau[j] < 0.0001f) n_eval[j] = cont[j];
+          }
         }
       }
     }
-    if (COUNT && inside && !done) n_eval = rg.y - rg.x;
   }
-  if (inside) {
-    const size_t HW = (size_t)W * H, pix = (size_t)py * W + px;
+  const size_t HW = (size_t)W * H;
+  unsigned long long ce = 0, cc = 0;
 #pragma unroll
-    for (int k = 0; k < DIM; ++k) layer[k * HW + pix] = acc[k];
-    contrib[pix] = cont;
-    final_tau[pix] = tau;
+  for (int j = 0; j < kSlots; ++j) {
+    const int py = y0 + 4 * j;
+    if (px < W && py < H) {
+      const size_t pix = (size_t)py * W + px;
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) layer[k * HW + pix] = acc[j][k];
+      contrib[pix] = cont[j];
+      // the reference kernel returns before writing when the list is empty
+      final_tau[pix] = rg.y - rg.x > 0 ? tau[j] : empty_tau;
+      if (COUNT) {
+        ce += (rg.y - rg.x > 0 && !(tau[j] < 0.0001f)) ? (unsigned long long)(rg.y - rg.x)
+                                                       : (unsigned long long)n_eval[j];
+        cc += n_con[j];
+      }
+    }
   }
   if (COUNT) {
-    unsigned long long e = (unsigned long long)n_eval, c = (unsigned long long)n_con;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-      e += __shfl_xor(e, o);
-      c += __shfl_xor(c, o);
+      ce += __shfl_xor(ce, o);
+      cc += __shfl_xor(cc, o);
     }
     if (lane == 0) {
-      atomicAdd(&counts[0], e);
-      atomicAdd(&counts[1], c);
+      atomicAdd(&counts[0], ce);
+      atomicAdd(&counts[1], cc);
     }
   }
 }
 
-// Backward.  Entries are visited in reverse list order from the largest
-// contrib of the workgroup (each wavefront starts at its own largest contrib);
-// per pixel the transmittance is recovered by division as in the reference.
-// The K gradient terms of an entry are summed over the wavefront with DPP,
-// the four wavefront sums go to LDS, and after each batch one thread per
-// entry adds their total to the Gaussian's gradient: one atomic per
-// (workgroup, entry, channel) instead of one per (pixel, entry, channel).
+// Adds the four-slot lane sums of an entry's gradient terms into dfeat: DPP
+// reduction over the wavefront, one atomic per channel from lane 63.
 template <int DIM>
-__global__ __launch_bounds__(256) void k_draw_bwd(int W, int H, int gx, int tiles,
-                                                  const int2* __restrict__ ranges,
-                                                  const int32_t* __restrict__ ids,
-                                                  const float4* __restrict__ recs,
-                                                  const int32_t* __restrict__ contrib,
-                                                  const float* __restrict__ final_tau,
-                                                  const float* __restrict__ dlayer,
-                                                  float* __restrict__ dfeat) {
+__device__ __forceinline__ void flush_grad(const float* part, float* __restrict__ dfeat, int g,
+                                           int lane) {
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) {
+    const float sum = wave_sum_to63(part[k]);
+    if (lane == 63 && sum != 0.f) unsafeAtomicAdd(&dfeat[(size_t)g * DIM + k], sum);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// backward, reference form (draw_weight_B): entries in reverse list order
+// from the tile's largest contrib; a pixel joins at its own contrib and
+// recovers the transmittance by division from final_tau.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles,
+                                                 const int2* __restrict__ ranges,
+                                                 const int32_t* __restrict__ ids,
+                                                 const float4* __restrict__ recs,
+                                                 const int32_t* __restrict__ contrib,
+                                                 const float* __restrict__ final_tau,
+                                                 const float* __restrict__ dlayer,
+                                                 float* __restrict__ dfeat) {
   constexpr int F4 = RecStride<DIM>::f4;
-  __shared__ float4 s_rec[256 * F4];
-  __shared__ float s_part[4][DIM][256];
-  __shared__ int32_t s_id[256];
-  __shared__ int32_t s_max[4];
+  __shared__ float4 s_rec[64 * F4];
+  __shared__ int32_t s_id[64];
   const int tile = xcd_tile(blockIdx.x, tiles);
   const int ty = tile / gx, tx = tile - ty * gx;
-  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int px = tx * kTile + (lane & 15), py = ty * kTile + w * 4 + (lane >> 4);
-  const bool inside = px < W && py < H;
+  const int lane = threadIdx.x;
+  const int px = tx * kTile + (lane & 15);
+  const int y0 = ty * kTile + (lane >> 4);
+  const float fpx = (float)px;
   const int2 rg = ranges[tile];
   if (rg.y - rg.x <= 0) return;
-  const float fpx = (float)px, fpy = (float)py;
-  const float ycen = (float)(ty * kTile + w * 4) + 1.5f;
 
-  float dl[DIM];
-  float tau = 0.f;
-  int cont = 0;
-  const size_t HW = (size_t)W * H, pix = (size_t)py * W + px;
+  const size_t HW = (size_t)W * H;
+  float dl[kSlots][DIM];
+  float tau[kSlots], fpy[kSlots];
+  int cont[kSlots];
+  int maxc = 0;
+  uint32_t smax_bits[kSlots];
 #pragma unroll
-  for (int k = 0; k < DIM; ++k) dl[k] = inside ? dlayer[k * HW + pix] : 0.f;
-  if (inside) {
-    tau = final_tau[pix];
-    cont = contrib[pix];
+  for (int j = 0; j < kSlots; ++j) {
+    const int py = y0 + 4 * j;
+    const bool in = px < W && py < H;
+    const size_t pix = (size_t)py * W + px;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) dl[j][k] = in ? dlayer[k * HW + pix] : 0.f;
+    tau[j] = in ? final_tau[pix] : 0.f;
+    cont[j] = in ? contrib[pix] : 0;
+    fpy[j] = (float)py;
+    int c = cont[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c = max(c, __shfl_xor(c, o));
+    smax_bits[j] = (uint32_t)c;
+    maxc = max(maxc, c);
   }
-  int wmax = cont;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
-  if (lane == 0) s_max[w] = wmax;
-  __syncthreads();
-  const int maxc = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
-  for (int top = rg.x + maxc; top > rg.x; top -= 256) {
-    const int lo = max(rg.x, top - 256);
+  int nid = 0;
+  RecRegs<F4> nxt = load_rec<F4>(recs, ids, rg.x + maxc - 1 - lane, rg.x, rg.x + maxc, &nid);
+  for (int top = rg.x + maxc; top > rg.x; top -= 64) {
+    const int lo = max(rg.x, top - 64);
     const int cnt = top - lo;
-    const int e = top - 1 - t;
-    if (e >= lo) {
-      const int g = ids[e];
-      s_id[t] = g;
-      const float4* src = recs + (size_t)g * F4;
-#pragma unroll
-      for (int q = 0; q < F4; ++q) s_rec[t * F4 + q] = src[q];
-    }
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) {
-      s_part[0][k][t] = 0.f; s_part[1][k][t] = 0.f; s_part[2][k][t] = 0.f; s_part[3][k][t] = 0.f;
+    __syncthreads();
+    if (lane < cnt) {
+      stage_rec<F4>(s_rec, lane, nxt, ty);
+      s_id[lane] = nid;
     }
     __syncthreads();
-    // entries with forward index >= wmax hold no contributor of this wavefront
-    const int jstart = max(0, top - rg.x - wmax);
-    for (int j = jstart; j < cnt; ++j) {
-      const int fi = top - 1 - j - rg.x;  // index of the entry in forward order
-      const float4 g0 = s_rec[j * F4];
-      const float4 g1 = s_rec[j * F4 + 1];
-      if ((g0.y - ycen > g1.w + 1.5f) || (ycen - g0.y > g1.w + 1.5f)) continue;
+    nxt = load_rec<F4>(recs, ids, top - 65 - lane, rg.x, rg.y, &nid);
+    for (int jj = 0; jj < cnt; ++jj) {
+      const int fi = top - 1 - jj - rg.x;  // index of the entry in forward order
+      const float4 g0 = s_rec[jj * F4];
+      const float4 g1 = s_rec[jj * F4 + 1];
+      uint32_t slots = uniform_bits(g1.w);
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j)
+        if ((uint32_t)fi >= smax_bits[j]) slots &= ~(1u << j);
+      if (slots == 0) continue;
+      const float dx = g0.x - fpx;
+      const float ta0 = g0.z * dx * dx;
+      const float tc0 = g0.w * dx;
+      float part[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) part[k] = 0.f;
       bool on = false;
-      float gv = 0.f;
-      if (fi < cont) {
-        const float dx = g0.x - fpx, dy = g0.y - fpy;
-        const float m = maha_dist(g0, g1.x, dx, dy);
-        if (!(m > g1.z)) {
-          const float ap = fminf(0.99f, g1.y * exp_cw(-0.5f * m));
-          if (!(ap < 0.002f)) {
-            tau = tau / (1.f - ap);
-            gv = ap * tau;
-            on = true;
-          }
-        }
-      }
-      if (__any(on)) {
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-          const float sum = wave_sum_to63(on ? dl[k] * gv : 0.f);
-          if (lane == 63) s_part[w][k][j] = sum;
-        }
-      }
-    }
-    __syncthreads();
-    if (t < cnt) {
-      const int g = s_id[t];
+      for (int j = 0; j < kSlots; ++j) {
+        if (!(slots & (1u << j))) continue;
+        if (fi >= cont[j]) continue;
+        const float dy = g0.y - fpy[j];
+        const float m = fmaxf(0.0f, (ta0 + g1.x * dy * dy) + tc0 * dy);
+        if (m > g1.z) continue;
+        const float ap = fminf(0.99f, g1.y * gauss_of(m));
+        if (ap < 0.002f) continue;
+        tau[j] = tau[j] / (1.f - ap);
+        const float gv = ap * tau[j];
 #pragma unroll
-      for (int k = 0; k < DIM; ++k) {
-        const float v = (s_part[0][k][t] + s_part[1][k][t]) + (s_part[2][k][t] + s_part[3][k][t]);
-        if (v != 0.f) unsafeAtomicAdd(&dfeat[(size_t)g * DIM + k], v);
+        for (int k = 0; k < DIM; ++k) part[k] = part[k] + dl[j][k] * gv;
+        on = true;
       }
+      if (__any(on)) flush_grad<DIM>(part, dfeat, s_id[jj], lane);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// backward, replay form: the forward recurrence is run again front to back;
+// the term of a contributing entry is dL/dW_k * tau * alpha' with tau the
+// transmittance in front of it, the factor the forward applied to the
+// weights, so no per-pixel state from the forward is read.  The set of
+// contributing (pixel, entry) pairs is the forward's (same stop rule).
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, int tiles,
+                                                        const int2* __restrict__ ranges,
+                                                        const int32_t* __restrict__ ids,
+                                                        const float4* __restrict__ recs,
+                                                        const float* __restrict__ dlayer,
+                                                        float* __restrict__ dfeat) {
+  constexpr int F4 = RecStride<DIM>::f4;
+  __shared__ float4 s_rec[64 * F4];
+  __shared__ int32_t s_id[64];
+  const int tile = xcd_tile(blockIdx.x, tiles);
+  const int ty = tile / gx, tx = tile - ty * gx;
+  const int lane = threadIdx.x;
+  const int px = tx * kTile + (lane & 15);
+  const int y0 = ty * kTile + (lane >> 4);
+  const float fpx = (float)px;
+  const int2 rg = ranges[tile];
+  if (rg.y - rg.x <= 0) return;
+
+  const size_t HW = (size_t)W * H;
+  float dl[kSlots][DIM];
+  float tau[kSlots], fpy[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int py = y0 + 4 * j;
+    const bool in = px < W && py < H;
+    const size_t pix = (size_t)py * W + px;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) dl[j][k] = in ? dlayer[k * HW + pix] : 0.f;
+    tau[j] = in ? 1.f : 0.f;
+    fpy[j] = (float)py;
+  }
+  int nid = 0;
+  RecRegs<F4> nxt = load_rec<F4>(recs, ids, rg.x + lane, rg.x, rg.y, &nid);
+  for (int b0 = rg.x; b0 < rg.y; b0 += 64) {
+    const uint32_t live = live_slots(tau);
+    if (live == 0) break;
     __syncthreads();
+    stage_rec<F4>(s_rec, lane, nxt, ty);
+    s_id[lane] = nid;
+    __syncthreads();
+    nxt = load_rec<F4>(recs, ids, b0 + 64 + lane, rg.x, rg.y, &nid);
+    const int cnt = min(64, rg.y - b0);
+    for (int jj = 0; jj < cnt; ++jj) {
+      const float4 g0 = s_rec[jj * F4];
+      const float4 g1 = s_rec[jj * F4 + 1];
+      const uint32_t slots = uniform_bits(g1.w) & live;
+      if (slots == 0) continue;
+      const float dx = g0.x - fpx;
+      const float ta0 = g0.z * dx * dx;
+      const float tc0 = g0.w * dx;
+      float part[DIM];
+#pragma unroll
+      for (int k = 0; k < DIM; ++k) part[k] = 0.f;
+      bool on = false;
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        if (!(slots & (1u << j))) continue;
+        if (tau[j] < 0.0001f) continue;
+        const float dy = g0.y - fpy[j];
+        const float m = fmaxf(0.0f, (ta0 + g1.x * dy * dy) + tc0 * dy);
+        if (m > g1.z) continue;
+        const float ap = fminf(0.99f, g1.y * gauss_of(m));
+        if (ap < 0.002f) continue;
+        const float ta = tau[j] * ap;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) part[k] = part[k] + dl[j][k] * ta;
+        tau[j] = tau[j] * (1.f - ap);
+        on = true;
+      }
+      if (__any(on)) flush_grad<DIM>(part, dfeat, s_id[jj], lane);
+    }
   }
 }
 
@@ -330,15 +497,21 @@ using namespace rgp;
     default: break;                                \
   }
 
+static int draw_grid(const char* who, int width, int height, int dim, int* gx, int* tiles) {
+  if (width <= 0 || height <= 0 || dim < 1 || dim > RGP_MAX_DIM)
+    return set_error(RGP_ERR_ARG, "%s: bad size or dim", who);
+  *gx = (width + kTile - 1) / kTile;
+  *tiles = *gx * ((height + kTile - 1) / kTile);
+  return RGP_OK;
+}
+
 extern "C" int rgp_draw_forward(int width, int height, int dim, const int32_t* ranges,
                                 const int32_t* ids, const float* records, float empty_tau,
                                 float* layer, int32_t* contrib, float* final_tau,
                                 hipStream_t s) {
-  if (width <= 0 || height <= 0 || dim < 1 || dim > RGP_MAX_DIM)
-    return set_error(RGP_ERR_ARG, "rgp_draw_forward: bad size or dim");
-  const int gx = (width + kTile - 1) / kTile, gy = (height + kTile - 1) / kTile;
-  const int tiles = gx * gy;
-  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false>), dim3(tiles), dim3(256), 0, s,
+  int gx, tiles;
+  if (int rc = draw_grid("rgp_draw_forward", width, height, dim, &gx, &tiles)) return rc;
+  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false>), dim3(tiles), dim3(64), 0, s,
                                          width, height, gx, tiles, (const int2*)ranges, ids,
                                          (const float4*)records, empty_tau, layer, contrib,
                                          final_tau, (unsigned long long*)nullptr));
@@ -349,12 +522,10 @@ extern "C" int rgp_draw_forward_counts(int width, int height, int dim, const int
                                        const int32_t* ids, const float* records, float empty_tau,
                                        float* layer, int32_t* contrib, float* final_tau,
                                        int64_t* counts, hipStream_t s) {
-  if (width <= 0 || height <= 0 || dim < 1 || dim > RGP_MAX_DIM)
-    return set_error(RGP_ERR_ARG, "rgp_draw_forward_counts: bad size or dim");
-  const int gx = (width + kTile - 1) / kTile, gy = (height + kTile - 1) / kTile;
-  const int tiles = gx * gy;
-  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, true>), dim3(tiles), dim3(256), 0, s, width,
-                                         height, gx, tiles, (const int2*)ranges, ids,
+  int gx, tiles;
+  if (int rc = draw_grid("rgp_draw_forward_counts", width, height, dim, &gx, &tiles)) return rc;
+  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, true>), dim3(tiles), dim3(64), 0, s,
+                                         width, height, gx, tiles, (const int2*)ranges, ids,
                                          (const float4*)records, empty_tau, layer, contrib,
                                          final_tau, (unsigned long long*)counts));
   return check_launch("rgp_draw_forward_counts");
@@ -364,15 +535,24 @@ extern "C" int rgp_draw_backward(int width, int height, int dim, const int32_t* 
                                  const int32_t* ids, const float* records, const int32_t* contrib,
                                  const float* final_tau, const float* dlayer, float* dfeat,
                                  hipStream_t s) {
-  if (width <= 0 || height <= 0 || dim < 1 || dim > RGP_MAX_DIM)
-    return set_error(RGP_ERR_ARG, "rgp_draw_backward: bad size or dim");
-  const int gx = (width + kTile - 1) / kTile, gy = (height + kTile - 1) / kTile;
-  const int tiles = gx * gy;
-  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL(k_draw_bwd<DD>, dim3(tiles), dim3(256), 0, s, width,
+  int gx, tiles;
+  if (int rc = draw_grid("rgp_draw_backward", width, height, dim, &gx, &tiles)) return rc;
+  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL(k_draw_bwd<DD>, dim3(tiles), dim3(64), 0, s, width,
                                          height, gx, tiles, (const int2*)ranges, ids,
                                          (const float4*)records, contrib, final_tau, dlayer,
                                          dfeat));
   return check_launch("rgp_draw_backward");
+}
+
+extern "C" int rgp_draw_backward_replay(int width, int height, int dim, const int32_t* ranges,
+                                        const int32_t* ids, const float* records,
+                                        const float* dlayer, float* dfeat, hipStream_t s) {
+  int gx, tiles;
+  if (int rc = draw_grid("rgp_draw_backward_replay", width, height, dim, &gx, &tiles)) return rc;
+  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL(k_draw_bwd_replay<DD>, dim3(tiles), dim3(64), 0, s,
+                                         width, height, gx, tiles, (const int2*)ranges, ids,
+                                         (const float4*)records, dlayer, dfeat));
+  return check_launch("rgp_draw_backward_replay");
 }
 
 extern "C" int rgp_mix_batched(int count, int K, const float* const* W, float* const* rgb,

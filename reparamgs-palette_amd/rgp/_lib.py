@@ -41,6 +41,7 @@ _SIGS = {
     "rgp_draw_forward": (I, [I, I, I, P, P, P, F, P, P, P, P]),
     "rgp_draw_forward_counts": (I, [I, I, I, P, P, P, F, P, P, P, P, P]),
     "rgp_draw_backward": (I, [I, I, I, P, P, P, P, P, P, P, P]),
+    "rgp_draw_backward_replay": (I, [I, I, I, P, P, P, P, P, P]),
     "rgp_preprocess_weight": (I, [I, I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I64, I, I,
                                   P, P, P, P, P]),
     "rgp_sh_backward_full": (I, [I, I, I, P, P, P, P, P]),

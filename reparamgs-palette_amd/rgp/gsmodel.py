@@ -27,6 +27,21 @@ def _rec(n, K, dev):
     return torch.empty((n, ops.record_floats(K)), dtype=torch.float32, device=dev)
 
 
+# How the wrappers' backward forms the per-Gaussian weight gradient:
+#  "replay"    : front-to-back replay of the forward (rgp_draw_backward_replay);
+#                the terms use the forward's own transmittance, the exact
+#                derivative of the computed forward.
+#  "reference" : the reference's reverse traversal with the transmittance
+#                recovered by division from final_tau (kernel.cu:1342-1378).
+BACKWARD_MODE = "replay"
+
+
+def _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dlayer):
+    if BACKWARD_MODE == "reference":
+        return ops.draw_backward(W, H, K, ranges, ids, records, contrib, final_tau, dlayer, n)
+    return ops.draw_backward_replay(W, H, K, ranges, ids, records, dlayer, n)
+
+
 def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None, B=None, Q=None,
                    sh_dim=None, return_state=False):
     """Forward of the palette-weight rasterizer.  Full rank when `shs`
@@ -80,7 +95,7 @@ class WeightGSFunctionTest(torch.autograd.Function):
     def backward(ctx, dloss_dlayer, _):
         W, H, K, n, sh_dim = ctx.meta
         pws, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
-        dw = ops.draw_backward(W, H, K, ranges, ids, records, contrib, final_tau, dloss_dlayer.contiguous(), n)
+        dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())
         dshs = torch.empty((n, sh_dim * K), dtype=torch.float32, device=pws.device)
         with SPANS.span("sh_backward"):
             call("rgp_sh_backward_full", n, K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(dshs), stream(pws.device))
@@ -105,7 +120,7 @@ class WeightGSFunctionFast(torch.autograd.Function):
     def backward(ctx, dloss_dlayer, _):
         W, H, K, n, P, Q, r = ctx.meta
         pws, A, B, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
-        dw = ops.draw_backward(W, H, K, ranges, ids, records, contrib, final_tau, dloss_dlayer.contiguous(), n)
+        dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())
         ddc = torch.empty((n, K), dtype=torch.float32, device=pws.device)
         dA = torch.empty(A.shape, dtype=torch.float32, device=pws.device)
         dB = torch.empty(B.shape, dtype=torch.float32, device=pws.device)

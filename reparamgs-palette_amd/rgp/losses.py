@@ -72,7 +72,7 @@ class L1SSIMLoss(torch.autograd.Function):
         taps = window_taps()
         ctx.taps = taps
         abc = torch.empty((3, C, H, W), dtype=torch.float32, device=x.device)
-        sums = torch.zeros(2, dtype=torch.float64, device=x.device)
+        sums = torch.zeros(128, dtype=torch.float64, device=x.device)
         loss = torch.empty((), dtype=torch.float32, device=x.device)
         call("rgp_l1_ssim_forward", C, H, W, ptr(x), ptr(y), taps.ctypes.data_as(ctypes.c_void_p),
              (0.01 * range_) ** 2, (0.03 * range_) ** 2, float(lam), ptr(abc), ptr(sums), ptr(loss), stream(x.device))

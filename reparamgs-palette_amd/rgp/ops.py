@@ -143,6 +143,17 @@ def draw_backward(width, height, dim, ranges, ids, records, contrib, final_tau, 
     return dfeat
 
 
+def draw_backward_replay(width, height, dim, ranges, ids, records, dlayer, n):
+    """Gradient of the features by a front-to-back replay of the forward."""
+    dev = records.device
+    dfeat = torch.zeros((n, dim), dtype=torch.float32, device=dev)
+    ranges, ids, dlayer = i32c(ranges), i32c(ids), f32c(dlayer)
+    with SPANS.span("draw_backward"):
+        call("rgp_draw_backward_replay", int(width), int(height), int(dim), ptr(ranges), ptr(ids), ptr(records),
+             ptr(dlayer), ptr(dfeat), stream(dev))
+    return dfeat
+
+
 class CapacityCache:
     """Pair capacity per (n, width, height): grows to 1.25x the largest pair
     count seen, so repeated renders of a scene reuse one workspace size."""

@@ -100,8 +100,10 @@ def test_backward_chain_c0(device, c0):
     grad_close(N(dw).reshape(scene.n, scene.K), ob["dweights"], ob["dweights_abs"], "dweights")
 
 
-def test_fused_wrapper_full_rank_c0(device, c0):
+@pytest.mark.parametrize("mode", ["replay", "reference"])
+def test_fused_wrapper_full_rank_c0(device, c0, mode):
     scene, cam = c0
+    gsmodel.BACKWARD_MODE = mode
     o = O.forward_weight(scene, cam)
     tc = cam.to_torch(device)
     shs = T(scene.shs, device).requires_grad_(True)
@@ -111,7 +113,8 @@ def test_fused_wrapper_full_rank_c0(device, c0):
     same(N(mask), o["mask"], "mask")
     dl = np.random.default_rng(2).normal(size=o["layer"].shape).astype(np.float32)
     layer.backward(T(dl, device))
-    ob = O.backward_weight(scene, o, dl)
+    gsmodel.BACKWARD_MODE = "replay"
+    ob = O.backward_weight(scene, o, dl, replay=(mode == "replay"))
     grad_close(N(shs.grad), ob["dshs"], ob["dshs_abs"], "dshs")
 
 
@@ -133,7 +136,7 @@ def test_fused_wrapper_low_rank(device, rank):
     same(N(layer), o["layer"], "layer")
     dl = np.random.default_rng(5).normal(size=o["layer"].shape).astype(np.float32)
     layer.backward(T(dl, device))
-    ob = O.backward_weight(scene, o, dl)
+    ob = O.backward_weight(scene, o, dl, replay=True)
     grad_close(N(dc.grad), ob["ddc"], ob["ddc_abs"], "ddc")
     grad_close(N(A.grad).reshape(ob["dA"].shape), ob["dA"], ob["dA_abs"] * 1.0001 + 1e-12, "dA")
     grad_close(N(B.grad).reshape(ob["dB"].shape), ob["dB"], ob["dB_abs"] * 1.0001 + 1e-12, "dB")

@@ -359,3 +359,24 @@ def test_mix_linear_identity(small):
     baked = O.draw(cam.width, cam.height, o["ranges"], o["ids"], o["us"], o["cinv"], scene.alphas,
                    o["weights"] @ P, 0.0, f64)[0]
     assert np.allclose(rgb, baked, rtol=1e-10, atol=1e-12)
+
+
+def test_backward_replay_vs_division(small):
+    """The replayed transmittance (tau in front of each entry, as the forward
+    used it) and the reference's division-recovered one give the same
+    gradient up to FP32 rounding; both agree with float64."""
+    scene, cam = small
+    o32 = O.forward_weight(scene, cam)
+    o64 = O.forward_weight(scene, cam, dt=f64)
+    same = (o32["contrib"] == o64["contrib"]).all()
+    dl = np.random.default_rng(8).normal(size=o32["layer"].shape).astype(f32)
+    div = O.backward_weight(scene, o32, dl)
+    rep = O.backward_weight(scene, o32, dl, replay=True)
+    ref = O.backward_weight(scene, o64, dl.astype(f64))
+    scale = div["dweights_abs"] + 1e-30
+    assert (np.abs(div["dweights"] - rep["dweights"]) <= 1e-5 * scale).all()
+    if same:
+        # FP32 vs FP64 differ mostly through alpha' itself (~1e-5 of sum|terms|)
+        assert (np.abs(rep["dweights"] - ref["dweights"]) <= 1e-4 * scale).all()
+        assert (np.abs(div["dweights"] - ref["dweights"]) <= 1e-4 * scale).all()
+        assert np.abs(rep["dweights"] - ref["dweights"]).sum() <= np.abs(div["dweights"] - ref["dweights"]).sum()

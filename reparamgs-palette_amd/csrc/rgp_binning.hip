@@ -342,10 +342,14 @@ __global__ __launch_bounds__(256) void k_sort_finish(const uint32_t* __restrict_
                                                      int32_t* __restrict__ ids_out,
                                                      uint64_t* __restrict__ keys_out) {
   const uint32_t M = hdr[kHdrMEff];
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= M) return;
-  if ((int)hdr[kHdrLast] < 0) ids_out[e] = v0[e];
-  if (keys_out) keys_out[e] = hdr[kHdrCur + 8] ? k1[e] : k0[e];
+  const bool copy_ids = (int)hdr[kHdrLast] < 0;
+  if (!copy_ids && !keys_out) return;
+  const bool from1 = hdr[kHdrCur + 8] != 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < M;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (copy_ids) ids_out[e] = v0[e];
+    if (keys_out) keys_out[e] = from1 ? k1[e] : k0[e];
+  }
 }
 
 // getRanges (kernel.cu:165-187) by binary search per tile; (0,0) for empty
@@ -434,7 +438,8 @@ extern "C" int rgp_bin_sort(int n, const float* depths, int width, int height, v
       hipLaunchKernelGGL(k_sort_scatter, dim3(L.nblk_cap), dim3(256), 0, s, p, hdr, k0, k1, v0,
                          v1, ids_out, hist, totals, L.nblk_cap);
     }
-    hipLaunchKernelGGL(k_sort_finish, dim3((unsigned)((L.cap + 255) / 256)), dim3(256), 0, s, hdr,
+    const int64_t fin = (L.cap + 255) / 256;
+    hipLaunchKernelGGL(k_sort_finish, dim3((unsigned)(fin < 1024 ? fin : 1024)), dim3(256), 0, s, hdr,
                        k0, k1, v0, ids_out, keys_out);
   }
   hipLaunchKernelGGL(k_ranges, dim3((L.tiles + 255) / 256), dim3(256), 0, s, L.tiles, hdr, k0, k1,

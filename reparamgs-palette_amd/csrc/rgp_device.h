@@ -303,6 +303,37 @@ __device__ __forceinline__ void sh_weights_lowrank(const float* __restrict__ dc,
   }
 }
 
+// Rank-R low-rank weights for degree-3 SH (L = 15) with Q = 15, so P = K:
+// the factors live in registers and every index is a compile-time constant.
+// Same operations in the same order as sh_weights_lowrank.
+template <int K, int R>
+__device__ __forceinline__ void sh_weights_lowrank_q15(const float* __restrict__ dc,
+                                                       const float* __restrict__ a,
+                                                       const float* __restrict__ b,
+                                                       const float* Y, float* w) {
+  constexpr int Q = 15, P = K;
+  float A[P * R], B[R * Q];
+#pragma unroll
+  for (int i = 0; i < P * R; ++i) A[i] = a[i];
+#pragma unroll
+  for (int i = 0; i < R * Q; ++i) B[i] = b[i];
+#pragma unroll
+  for (int k = 0; k < K; ++k) w[k] = dmul(SH_C0_0, dc[k]);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    float acc = 0.f;
+#pragma unroll
+    for (int lm = 0; lm < 15; ++lm) {
+      const int g = lm * K + k, row = g / Q, col = g % Q;
+      float c = A[row * R] * B[col];
+#pragma unroll
+      for (int r = 1; r < R; ++r) c = c + A[row * R + r] * B[r * Q + col];
+      acc = acc + c * Y[lm + 1];
+    }
+    w[k] = w[k] + acc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // splat record: what the compositing kernels read per list entry.
 //   f4[0] = (u.x, u.y, cinv.x, 2*cinv.y)   f4[1] = (cinv.z, alpha, mcut, ry)

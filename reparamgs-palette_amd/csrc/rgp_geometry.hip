@@ -217,6 +217,58 @@ __global__ __launch_bounds__(256) void k_lowrank_backward(
   }
 }
 
+// The same with Q = 15, P = K and rank R known at compile time: factors and
+// gradient accumulators in registers, one store per output.  Same operations
+// in the same order.
+template <int K, int R>
+__global__ __launch_bounds__(256) void k_lowrank_backward_q15(
+    int n, const float* __restrict__ A, const float* __restrict__ B,
+    const float* __restrict__ ylm, const float* __restrict__ pws, const float* __restrict__ twc,
+    const float* __restrict__ dw, float* __restrict__ ddc, float* __restrict__ dA,
+    float* __restrict__ dB) {
+  constexpr int Q = 15, P = K;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float Y[16];
+  if (ylm) {
+#pragma unroll
+    for (int l = 0; l < 15; ++l) Y[l + 1] = ylm[(size_t)i * 15 + l];
+  } else {
+    const float tw[3] = {twc[0], twc[1], twc[2]};
+    sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], tw, 16, Y);
+  }
+  float a[P * R], b[R * Q], da[P * R], db[R * Q];
+#pragma unroll
+  for (int j = 0; j < P * R; ++j) { a[j] = A[(size_t)i * P * R + j]; da[j] = 0.f; }
+#pragma unroll
+  for (int j = 0; j < R * Q; ++j) { b[j] = B[(size_t)i * R * Q + j]; db[j] = 0.f; }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float dwk = dw[(size_t)i * K + k];
+    if (ddc) ddc[(size_t)i * K + k] = dwk * (float)SH_C0_0;
+#pragma unroll
+    for (int lm = 0; lm < 15; ++lm) {
+      const int g = lm * K + k, row = g / Q, col = g % Q;
+      const float y = Y[lm + 1];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        da[row * R + r] = da[row * R + r] + dwk * b[r * Q + col] * y;
+        db[r * Q + col] = db[r * Q + col] + dwk * a[row * R + r] * y;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < P * R; ++j) dA[(size_t)i * P * R + j] = da[j];
+#pragma unroll
+  for (int j = 0; j < R * Q; ++j) dB[(size_t)i * R * Q + j] = db[j];
+}
+
+// Launch the low-rank backward, picking the register form when it applies.
+static void launch_lowrank_backward(int n, int K, int P, int Q, int rank, const float* A,
+                                    const float* B, const float* ylm, const float* pws,
+                                    const float* twc, const float* dw, float* ddc, float* dA,
+                                    float* dB, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // records
 // ---------------------------------------------------------------------------
@@ -301,7 +353,9 @@ __global__ __launch_bounds__(256) void k_rects(int n, const float* __restrict__ 
 // ---------------------------------------------------------------------------
 // fused preprocess (WeightGSFunctionTest/Fast forward up to getRects)
 // ---------------------------------------------------------------------------
-template <int K, bool LOWRANK>
+// WMODE: 0 full rank; 1 low rank, generic P, Q, rank; 2 / 3 low rank with
+// Q = 15, P = K and rank 1 / 2 (factors in registers).
+template <int K, int WMODE>
 __global__ __launch_bounds__(256) void k_preprocess(
     int n, int sh_dim, int P, int Q, int rank, const rgp_camera* __restrict__ cam,
     const float* __restrict__ pws, const float4* __restrict__ rots,
@@ -322,14 +376,19 @@ __global__ __launch_bounds__(256) void k_preprocess(
 #pragma unroll
     for (int l = 0; l < 16; ++l) Y[l] = 0.f;
     float w[K];
-    if (!LOWRANK) {
+    if (WMODE == 0) {
       if (sh_dim > 1) sh_basis(px, py, pz, cam->twc, sh_dim, Y);
       sh_weights_full<K>(shs + (size_t)i * sh_dim * K, Y, sh_dim, w);
-    } else {
+    } else if (WMODE == 1) {
       const int L = P * Q / K;
       if (L > 0) sh_basis(px, py, pz, cam->twc, L + 1, Y);
       sh_weights_lowrank<K>(dc + (size_t)i * K, A + (size_t)i * P * rank,
                             B + (size_t)i * rank * Q, Y, L, Q, rank, w);
+    } else {
+      constexpr int R = WMODE == 2 ? 1 : 2;
+      sh_basis(px, py, pz, cam->twc, 16, Y);
+      sh_weights_lowrank_q15<K, R>(dc + (size_t)i * K, A + (size_t)i * K * R, B + (size_t)i * R * 15, Y,
+                                   w);
     }
     if (w_out) {
 #pragma unroll
@@ -475,6 +534,26 @@ extern "C" int rgp_inverse_cov2d(int n, const float* cov2ds, float* depths, floa
     default: break;                              \
   }
 
+namespace rgp {
+static void launch_lowrank_backward(int n, int K, int P, int Q, int rank, const float* A,
+                                    const float* B, const float* ylm, const float* pws,
+                                    const float* twc, const float* dw, float* ddc, float* dA,
+                                    float* dB, hipStream_t s) {
+  if (Q == 15 && P == K && P * Q / K == 15 && (rank == 1 || rank == 2)) {
+    if (rank == 1) {
+      RGP_K_SWITCH(K, hipLaunchKernelGGL((k_lowrank_backward_q15<KK, 1>), dim3(grid1(n)), dim3(256),
+                                         0, s, n, A, B, ylm, pws, twc, dw, ddc, dA, dB));
+    } else {
+      RGP_K_SWITCH(K, hipLaunchKernelGGL((k_lowrank_backward_q15<KK, 2>), dim3(grid1(n)), dim3(256),
+                                         0, s, n, A, B, ylm, pws, twc, dw, ddc, dA, dB));
+    }
+    return;
+  }
+  hipLaunchKernelGGL(k_lowrank_backward, dim3(grid1(n)), dim3(256), 0, s, n, K, P, Q, rank, A, B,
+                     ylm, pws, twc, dw, ddc, dA, dB);
+}
+}  // namespace rgp
+
 extern "C" int rgp_sh2weight(int n, int K, int sh_dim, const float* shs, const float* pws,
                              const float* twc, float* weights, float* basis, hipStream_t s) {
   if (n < 0 || K < 1 || K > RGP_MAX_DIM)
@@ -519,9 +598,8 @@ extern "C" int rgp_sh2weight_fast_backward(int n, int K, int P, int Q, int rank,
   if (rc) return rc;
   if (!ylm) return set_error(RGP_ERR_ARG, "rgp_sh2weight_fast_backward: ylm is required");
   if (n == 0) return RGP_OK;
-  hipLaunchKernelGGL(k_lowrank_backward, dim3(grid1(n)), dim3(256), 0, s, n, K, P, Q, rank, A, B,
-                     ylm, (const float*)nullptr, (const float*)nullptr, dloss_dw, (float*)nullptr,
-                     dA, dB);
+  rgp::launch_lowrank_backward(n, K, P, Q, rank, A, B, ylm, nullptr, nullptr, dloss_dw, nullptr, dA,
+                               dB, s);
   return check_launch("rgp_sh2weight_fast_backward");
 }
 
@@ -591,21 +669,22 @@ extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int r
   int rc = bin_reset_header(ws, s);
   if (rc) return rc;
   if (n == 0) return RGP_OK;
-  if (mode == 0) {
-    RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess<KK, false>), dim3(grid1(n)), dim3(256), 0, s,
-                                       n, sh_dim, P, Q, rank, cam, pws, (const float4*)rots,
-                                       scales, alphas, shs, dc, A, B, depths, (float4*)records,
-                                       (uint2*)(base + L.rects), (uint32_t*)(base + L.counts),
-                                       (uint32_t*)(base + L.header) + kHdrDqMax, L.gx, L.gy, us,
-                                       cinv2ds, areas, weights));
-  } else {
-    RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess<KK, true>), dim3(grid1(n)), dim3(256), 0, s,
-                                       n, sh_dim, P, Q, rank, cam, pws, (const float4*)rots,
-                                       scales, alphas, shs, dc, A, B, depths, (float4*)records,
-                                       (uint2*)(base + L.rects), (uint32_t*)(base + L.counts),
-                                       (uint32_t*)(base + L.header) + kHdrDqMax, L.gx, L.gy, us,
-                                       cinv2ds, areas, weights));
+  int wmode = mode;
+  if (mode == 1 && Q == 15 && P == K && (rank == 1 || rank == 2)) wmode = rank + 1;
+#define RGP_PRE(WM)                                                                               \
+  RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess<KK, WM>), dim3(grid1(n)), dim3(256), 0, s, n,  \
+                                     sh_dim, P, Q, rank, cam, pws, (const float4*)rots, scales,   \
+                                     alphas, shs, dc, A, B, depths, (float4*)records,             \
+                                     (uint2*)(base + L.rects), (uint32_t*)(base + L.counts),      \
+                                     (uint32_t*)(base + L.header) + kHdrDqMax, L.gx, L.gy, us,    \
+                                     cinv2ds, areas, weights))
+  switch (wmode) {
+    case 0: RGP_PRE(0); break;
+    case 1: RGP_PRE(1); break;
+    case 2: RGP_PRE(2); break;
+    default: RGP_PRE(3); break;
   }
+#undef RGP_PRE
   return check_launch("rgp_preprocess_weight");
 }
 
@@ -633,7 +712,6 @@ extern "C" int rgp_sh_backward_lowrank(int n, int K, int P, int Q, int rank,
   if (n == 0) return RGP_OK;
   // cam is a device pointer: pass the address of its twc field
   const float* twc = (const float*)((const char*)cam + offsetof(rgp_camera, twc));
-  hipLaunchKernelGGL(k_lowrank_backward, dim3(grid1(n)), dim3(256), 0, s, n, K, P, Q, rank, A, B,
-                     (const float*)nullptr, pws, twc, dw, ddc, dA, dB);
+  rgp::launch_lowrank_backward(n, K, P, Q, rank, A, B, nullptr, pws, twc, dw, ddc, dA, dB, s);
   return check_launch("rgp_sh_backward_lowrank");
 }

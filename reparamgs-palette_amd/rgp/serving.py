@@ -1,0 +1,432 @@
+"""Serving loop for palette-weight scenes resident in HBM.
+
+There is no serving code in the reference.  Its nearest operation is the
+palette edit by bake + re-render (finetune.py:472-478, gsplat/gsmodel.py:488-509).
+Here a palette-only edit re-mixes the cached K-channel weight image of the
+scene's current view (rgp_mix_batched), which by linearity equals the bake.
+
+* Scenes are admitted against an HBM budget: their parameters plus the render
+  buffers of every resolution they will be served at.
+* Each (scene, resolution) owns a render slot with static buffers and a HIP
+  graph of the capacity-bounded pipeline: preprocess -> scan -> sort ->
+  composite.  A replay takes a new pose through the slot's device camera
+  buffer and never synchronises the host.  A pair-count overflow is detected
+  from the workspace header after the fact and the request is re-rendered
+  with a larger capacity.
+* Continuous batching: each loop iteration admits every request that has
+  arrived.  Pose requests replay their slot's graph on the scene's render
+  stream, followed by the request's own mix.  Palette edits pending at once
+  go out in one batched mix launch per K on the edit stream, after waiting
+  for the renders their weight images come from.  Later renders of a scene
+  wait for the edits still reading its weight image.  Work on different
+  streams overlaps on the device.
+* Latency is the host time from a request's arrival to the moment the loop
+  sees its completion event; the report gives p50/p99 overall and for
+  palette-only edits, and frames (requests) per second.
+
+Luminance edit (the reference has none): the edited palette is
+clamp((P + dP) * s, 0, 1) on the chromatic rows; black and white stay fixed.
+"""
+import collections
+import dataclasses
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import ops
+from ._lib import I64, call, f32c, ptr, stream
+from .scene import look_at_camera, make_scene
+
+RESOLUTIONS = ((1280, 720), (1920, 1080), (2560, 1440))
+
+
+class AdmissionError(RuntimeError):
+    pass
+
+
+@dataclasses.dataclass
+class SceneSpec:
+    name: str
+    n: int
+    K: int
+    lowrank: bool
+    rank: int = 2
+    seed: int = 0
+
+    def param_bytes(self):
+        geo = self.n * (12 + 16 + 12 + 4)
+        if self.lowrank:
+            P = self.K  # P x Q = K x 15
+            return geo + self.n * 4 * (self.K + self.rank * (P + 15))
+        return geo + self.n * 4 * 16 * self.K
+
+    def slot_bytes(self, width, height, cap_per_gaussian=6.0):
+        """Render buffers of one resolution: records, depths, workspace at the
+        planning capacity, ids, ranges, weight image, contrib, final_tau."""
+        rec = self.n * 4 * (8 + 4 * math.ceil(self.K / 4))
+        cap = int(cap_per_gaussian * self.n)
+        tiles = ((width + 15) // 16) * ((height + 15) // 16)
+        ws = 256 + self.n * 16 + cap * 24 + 256 * 4 * (cap // 4096 + 1)
+        img = width * height * 4 * (self.K + 2)
+        return rec + self.n * 4 + ws + cap * 4 + tiles * 8 + img
+
+    def footprint(self, resolutions=RESOLUTIONS):
+        return self.param_bytes() + sum(self.slot_bytes(w, h) for w, h in resolutions)
+
+
+def default_specs():
+    """The six resident scenes of the serving workload."""
+    return [SceneSpec("s200k", 200_000, 5, False, seed=1), SceneSpec("s500k", 500_000, 7, True, seed=2),
+            SceneSpec("s1m_a", 1_000_000, 5, False, seed=3), SceneSpec("s1m_b", 1_000_000, 7, True, seed=4),
+            SceneSpec("s2m", 2_000_000, 5, True, seed=5), SceneSpec("s3m", 3_000_000, 7, True, seed=6)]
+
+
+class SceneStore:
+    """Admission of scenes against an HBM budget (bytes)."""
+
+    def __init__(self, budget_bytes, resolutions=RESOLUTIONS):
+        self.budget = int(budget_bytes)
+        self.resolutions = resolutions
+        self.used = 0
+        self.specs = {}
+
+    def admit(self, spec):
+        need = spec.footprint(self.resolutions)
+        if self.used + need > self.budget:
+            raise AdmissionError(f"scene {spec.name} needs {need / 2**30:.2f} GiB; "
+                                 f"{(self.budget - self.used) / 2**30:.2f} GiB of the budget left")
+        self.used += need
+        self.specs[spec.name] = spec
+        return need
+
+    def release(self, name):
+        spec = self.specs.pop(name)
+        self.used -= spec.footprint(self.resolutions)
+
+
+class ResidentScene:
+    """Device copy of one synthetic finetuned scene."""
+
+    def __init__(self, spec, device):
+        self.spec = spec
+        sc = make_scene(spec.n, seed=spec.seed, extent=3.0, K=spec.K, lowrank=spec.lowrank, rank=spec.rank,
+                        P=spec.K if spec.lowrank else None, Q=15)
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
+        self.n, self.K = spec.n, spec.K
+        self.pws, self.rots, self.scales, self.alphas = T(sc.pws), T(sc.rots), T(sc.scales), T(sc.alphas)
+        self.palette = sc.palette.copy()
+        if spec.lowrank:
+            self.mode, self.P, self.Q, self.rank = 1, sc.P, sc.Q, sc.rank
+            self.shs, self.dc, self.A, self.B = None, T(sc.dc), T(sc.A), T(sc.B)
+        else:
+            self.mode, self.P, self.Q, self.rank = 0, 16, 0, 1
+            self.shs, self.dc, self.A, self.B = T(sc.shs), None, None, None
+
+
+class RenderSlot:
+    """Static buffers and the captured render graph of one (scene, resolution)."""
+
+    def __init__(self, scene, width, height, capacity, device, use_graph=True):
+        self.scene, self.W, self.H, self.cap = scene, width, height, int(capacity)
+        n, K = scene.n, scene.K
+        f = dict(dtype=torch.float32, device=device)
+        self.camd = torch.zeros(32, **f)
+        self.depths = torch.empty(n, **f)
+        self.records = torch.empty((n, ops.record_floats(K)), **f)
+        self.ws = ops.new_workspace(n, self.cap, width, height, device)
+        gx, gy = ops.grid_dims(width, height)
+        self.ids = torch.empty(max(self.cap, 1), dtype=torch.int32, device=device)
+        self.ranges = torch.empty((gx * gy, 2), dtype=torch.int32, device=device)
+        self.layer = torch.empty((K, height, width), **f)
+        self.contrib = torch.empty((height, width), dtype=torch.int32, device=device)
+        self.final_tau = torch.empty((height, width), **f)
+        self.graph = None
+        if use_graph:
+            s = torch.cuda.Stream(device)
+            s.wait_stream(torch.cuda.current_stream(device))
+            with torch.cuda.stream(s):
+                self.launch()
+            torch.cuda.current_stream(device).wait_stream(s)
+            torch.cuda.synchronize(device)
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self.launch()
+            torch.cuda.synchronize(device)
+
+    def launch(self):
+        sc, W, H = self.scene, self.W, self.H
+        st = stream(self.camd.device)
+        call("rgp_preprocess_weight", sc.n, sc.K, sc.mode, sc.P, sc.Q, sc.rank, ptr(self.camd), ptr(sc.pws),
+             ptr(sc.rots), ptr(sc.scales), ptr(sc.alphas), ptr(sc.shs), ptr(sc.dc), ptr(sc.A), ptr(sc.B),
+             ptr(self.depths), ptr(self.records), ptr(self.ws), I64(self.cap), W, H, None, None, None, None, st)
+        call("rgp_bin_scan", sc.n, W, H, ptr(self.ws), I64(self.cap), st)
+        call("rgp_bin_sort", sc.n, ptr(self.depths), W, H, ptr(self.ws), I64(self.cap), ptr(self.ids), None,
+             ptr(self.ranges), st)
+        call("rgp_draw_forward", W, H, sc.K, ptr(self.ranges), ptr(self.ids), ptr(self.records), 0.0,
+             ptr(self.layer), ptr(self.contrib), ptr(self.final_tau), st)
+
+    def render(self, cam_host):
+        """Enqueue a render on the current stream with the pose in cam_host
+        (a pinned float32 tensor holding the rgp_camera struct)."""
+        self.camd.copy_(cam_host, non_blocking=True)
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self.launch()
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float
+    scene: str
+    kind: str  # "pose" | "edit"
+    pose: int = 0
+    res: tuple = (1920, 1080)
+    dpal: np.ndarray = None
+    lum: float = 1.0
+    done: float = None
+    out: torch.Tensor = None
+
+
+def make_requests(specs, n_requests, rate, seed=0, pose_frac=0.4, n_poses=64):
+    """Poisson arrivals; every scene's first request is a pose request."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests)) if rate > 0 else np.zeros(n_requests)
+    seen = set()
+    reqs = []
+    for i in range(n_requests):
+        sp = specs[rng.integers(len(specs))]
+        kind = "pose" if (sp.name not in seen or rng.random() < pose_frac) else "edit"
+        seen.add(sp.name)
+        res = RESOLUTIONS[rng.integers(len(RESOLUTIONS))]
+        dpal = rng.normal(0.0, 0.1, (sp.K, 3)).astype(np.float32)
+        reqs.append(Request(i, float(t[i]), sp.name, kind, int(rng.integers(n_poses)), res, dpal,
+                            float(rng.uniform(0.7, 1.3))))
+    return reqs
+
+
+def edited_palette(base, dpal, lum):
+    P = base.astype(np.float32).copy()
+    P[2:] = np.clip((P[2:] + dpal[2:]) * np.float32(lum), 0.0, 1.0)
+    return P
+
+
+class Engine:
+    """Continuous-batching server over resident scenes."""
+
+    def __init__(self, specs, device, budget_bytes=200 * 2**30, n_streams=3, use_graph=True, cap_margin=1.5,
+                 radius=6.0, n_poses=64):
+        self.device = device
+        self.store = SceneStore(budget_bytes)
+        self.scenes = {}
+        for sp in specs:
+            self.store.admit(sp)
+            self.scenes[sp.name] = ResidentScene(sp, device)
+        self.use_graph = use_graph
+        self.cap_margin = cap_margin
+        self.n_poses = n_poses
+        self.radius = radius
+        self.render_streams = [torch.cuda.Stream(device) for _ in range(n_streams)]
+        self.edit_stream = torch.cuda.Stream(device)
+        self.scene_stream = {name: self.render_streams[i % n_streams] for i, name in enumerate(self.scenes)}
+        self.slots = {}
+        self.session = {}  # scene -> (slot, render event)
+        self.last_edit = {}  # scene -> event of the last edit batch reading its weight image
+        self.cams = {}
+        self.base_pal = {}
+
+    # -- cameras -----------------------------------------------------------
+    def camera(self, pose, width, height):
+        key = (pose, width, height)
+        if key not in self.cams:
+            th = 2.0 * math.pi * pose / self.n_poses
+            el = 0.3 * math.sin(3.0 * th)
+            pos = (self.radius * math.cos(el) * math.sin(th), self.radius * math.sin(el),
+                   -self.radius * math.cos(el) * math.cos(th))
+            c = look_at_camera(pos, width, height)
+            host = ops.camera_struct_host(c.Rcw, c.tcw, c.twc, c.fx, c.fy, c.cx, c.cy, width, height)
+            self.cams[key] = torch.from_numpy(host).pin_memory()
+        return self.cams[key]
+
+    def slot(self, name, res, min_cap=None):
+        key = (name, res)
+        sl = self.slots.get(key)
+        if sl is None or (min_cap is not None and sl.cap < min_cap):
+            sc = self.scenes[name]
+            cap = min_cap or self._plan_capacity(sc, res)
+            sl = RenderSlot(sc, res[0], res[1], cap, self.device, self.use_graph)
+            self.slots[key] = sl
+        return sl
+
+    def _plan_capacity(self, sc, res, samples=4):
+        """Capacity from exact pair counts of a few poses, times a margin."""
+        m = 0
+        for p in range(0, self.n_poses, max(1, self.n_poses // samples)):
+            probe = RenderSlot(sc, res[0], res[1], 1, self.device, use_graph=False)
+            probe.camd.copy_(self.camera(p, res[0], res[1]))
+            call("rgp_preprocess_weight", sc.n, sc.K, sc.mode, sc.P, sc.Q, sc.rank, ptr(probe.camd), ptr(sc.pws),
+                 ptr(sc.rots), ptr(sc.scales), ptr(sc.alphas), ptr(sc.shs), ptr(sc.dc), ptr(sc.A), ptr(sc.B),
+                 ptr(probe.depths), ptr(probe.records), ptr(probe.ws), I64(1), res[0], res[1], None, None, None,
+                 None, stream(self.device))
+            call("rgp_bin_scan", sc.n, res[0], res[1], ptr(probe.ws), I64(1), stream(self.device))
+            m = max(m, int(ops.header(probe.ws, 1).item()))
+            del probe
+        return int(m * self.cap_margin) + 4096
+
+    def prepare(self, requests):
+        """Build every render slot the requests need (outside any timing)."""
+        for r in requests:
+            if r.kind == "pose":
+                self.slot(r.scene, r.res)
+        torch.cuda.synchronize(self.device)
+
+    # -- launching ---------------------------------------------------------
+    def _launch_pose(self, r, pose=None, res=None, palette=None):
+        """Render the request's view (or the given one) and mix it with the
+        scene palette (or the given edited palette)."""
+        sc = self.scenes[r.scene]
+        st = self.scene_stream[r.scene]
+        pose = r.pose if pose is None else pose
+        res = r.res if res is None else res
+        sl = self.slot(r.scene, res)
+        with torch.cuda.stream(st):
+            ev = self.last_edit.get(r.scene)
+            if ev is not None:
+                st.wait_event(ev)  # edits still reading the old weight image
+            sl.render(self.camera(pose, *res))
+            if palette is None:
+                if r.scene not in self.base_pal:
+                    self.base_pal[r.scene] = torch.from_numpy(sc.palette.astype(np.float32)).to(self.device)
+                pal = self.base_pal[r.scene]
+            else:
+                pal = torch.from_numpy(palette).pin_memory().to(self.device, non_blocking=True)
+            r.out = ops.mix(sl.layer, pal)
+            hdr = torch.empty(4, dtype=torch.int32).pin_memory()
+            hdr.copy_(ops.header(sl.ws), non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(st)
+        self.session[r.scene] = (sl, done, pose, res)
+        return done, hdr
+
+    def _launch_edits(self, edits):
+        """One batched mix launch per K for the given palette edits."""
+        st = self.edit_stream
+        events = []
+        by_k = collections.defaultdict(list)
+        for r in edits:
+            by_k[self.scenes[r.scene].K].append(r)
+        with torch.cuda.stream(st):
+            for K, group in by_k.items():
+                waits = set()
+                Wp, Rp, hw, pals = [], [], [], []
+                for r in group:
+                    sl, rev, _, res = self.session[r.scene]
+                    if id(rev) not in waits:
+                        st.wait_event(rev)
+                        waits.add(id(rev))
+                    r.out = torch.empty((3, sl.H, sl.W), dtype=torch.float32, device=self.device)
+                    Wp.append(sl.layer.data_ptr())
+                    Rp.append(r.out.data_ptr())
+                    hw.append(sl.H * sl.W)
+                    pals.append(edited_palette(self.scenes[r.scene].palette, r.dpal, r.lum))
+                # one upload: [W pointers | output pointers | pixel counts | palettes]
+                n = len(group)
+                ints = np.array(Wp + Rp + hw, dtype=np.int64)
+                palf = np.concatenate(pals).astype(np.float32).reshape(-1)
+                pin = torch.from_numpy(np.concatenate([ints.view(np.uint8), palf.view(np.uint8)])).pin_memory()
+                dev = pin.to(self.device, non_blocking=True)
+                base = dev.data_ptr()
+                call("rgp_mix_batched", n, K, base, base + 8 * n, base + 16 * n, base + 24 * n, I64(max(hw)),
+                     stream(self.device))
+            ev = torch.cuda.Event()
+            ev.record(st)
+        for r in edits:
+            self.last_edit[r.scene] = ev
+        return ev
+
+    # -- running -----------------------------------------------------------
+    def run(self, requests, mode="batched", realtime=True, max_batch=256, rerender_edits=False):
+        """Serve the requests.  mode "batched": continuous batching;
+        "serial": one request at a time, each synchronised before the next.
+        realtime: honour arrival times (else everything arrives at t=0).
+        rerender_edits (serial only): a palette edit re-renders the scene's
+        current view, the reference's way (bake + render, finetune.py:472-478),
+        instead of re-mixing the cached weight image."""
+        for r in requests:
+            r.done, r.out = None, None
+        pending = collections.deque(sorted(requests, key=lambda q: q.arrival))
+        inflight = []  # (event, [requests], hdr or None)
+        redo = []
+        torch.cuda.synchronize(self.device)
+        t0 = time.perf_counter()
+        arrive = (lambda r: r.arrival) if realtime else (lambda r: 0.0)
+        while pending or inflight or redo:
+            now = time.perf_counter() - t0
+            batch = []
+            while pending and arrive(pending[0]) <= now and len(batch) < max_batch:
+                batch.append(pending.popleft())
+            batch = redo + batch
+            redo = []
+            if mode == "serial":
+                for r in batch:
+                    if r.kind == "pose":
+                        ev, hdr = self._launch_pose(r)
+                    else:
+                        if rerender_edits:
+                            _, _, pose, res = self.session[r.scene]
+                            ev, hdr = self._launch_pose(
+                                r, pose, res, edited_palette(self.scenes[r.scene].palette, r.dpal, r.lum))
+                        else:
+                            ev, hdr = self._launch_edits([r]), None
+                    ev.synchronize()
+                    if hdr is not None and int(hdr[1]) != 0:
+                        self.slot(r.scene, r.res, min_cap=int(int(hdr[0]) * self.cap_margin) + 4096)
+                        redo.append(r)
+                        continue
+                    r.done = time.perf_counter() - t0
+            else:
+                edits = []
+                for r in batch:
+                    if r.kind == "pose":
+                        if any(e.scene == r.scene for e in edits):
+                            # earlier edits of this scene use the view before this pose
+                            inflight.append((self._launch_edits(edits), edits, None))
+                            edits = []
+                        ev, hdr = self._launch_pose(r)
+                        inflight.append((ev, [r], hdr))
+                    else:
+                        edits.append(r)
+                if edits:
+                    inflight.append((self._launch_edits(edits), edits, None))
+            still = []
+            for ev, rs, hdr in inflight:
+                if ev.query():
+                    t = time.perf_counter() - t0
+                    if hdr is not None and int(hdr[1]) != 0:
+                        r = rs[0]
+                        self.slot(r.scene, r.res, min_cap=int(int(hdr[0]) * self.cap_margin) + 4096)
+                        redo.append(r)
+                        continue
+                    for r in rs:
+                        r.done = t
+                else:
+                    still.append((ev, rs, hdr))
+            inflight = still
+            if not batch and inflight and not redo:
+                time.sleep(0)
+            elif not batch and not inflight and pending and realtime:
+                time.sleep(max(0.0, min(2e-4, arrive(pending[0]) - (time.perf_counter() - t0))))
+        wall = time.perf_counter() - t0
+        return report(requests, wall, realtime)
+
+
+def report(requests, wall, realtime=True):
+    lat = np.array([(r.done - (r.arrival if realtime else 0.0)) * 1e3 for r in requests])
+    edit = np.array([(r.done - (r.arrival if realtime else 0.0)) * 1e3 for r in requests if r.kind == "edit"])
+    pct = lambda a, q: float(np.percentile(a, q)) if len(a) else None
+    return {"requests": len(requests), "wall_s": wall, "frames_per_s": len(requests) / wall,
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "edit_p50_ms": pct(edit, 50),
+            "edit_p99_ms": pct(edit, 99), "edits": int(len(edit))}

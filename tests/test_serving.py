@@ -1,0 +1,95 @@
+"""Serving loop: admission by memory budget and request generation on the
+CPU; on the GPU, every result of a continuously batched run (and of the
+one-at-a-time run) equals the same request rendered alone with the
+exact-size pipeline, bit for bit."""
+import numpy as np
+import pytest
+
+from rgp.serving import AdmissionError, SceneSpec, SceneStore, edited_palette, make_requests
+
+SMALL = [SceneSpec("a", 20_000, 5, False, seed=1), SceneSpec("b", 30_000, 7, True, seed=2),
+         SceneSpec("c", 40_000, 5, True, seed=3)]
+SMALL_RES = ((320, 192), (480, 272))
+
+
+def test_admission_budget():
+    specs = [SceneSpec("x", 1_000_000, 5, False), SceneSpec("y", 3_000_000, 7, True)]
+    need = [s.footprint() for s in specs]
+    store = SceneStore(need[0] + need[1] - 1)
+    store.admit(specs[0])
+    with pytest.raises(AdmissionError):
+        store.admit(specs[1])
+    store.release("x")
+    store.admit(specs[1])
+    assert store.used == need[1]
+
+
+def test_default_workload_fits_200gb():
+    from rgp.serving import default_specs
+    store = SceneStore(200 * 2**30)
+    for s in default_specs():
+        store.admit(s)
+    assert store.used < 200 * 2**30
+
+
+def test_request_stream():
+    specs = SMALL
+    reqs = make_requests(specs, 4000, rate=500.0, seed=3)
+    first = {}
+    for r in reqs:
+        first.setdefault(r.scene, r)
+    assert all(r.kind == "pose" for r in first.values())
+    frac_pose = np.mean([r.kind == "pose" for r in reqs])
+    assert 0.35 < frac_pose < 0.45
+    gaps = np.diff([r.arrival for r in reqs])
+    assert abs(gaps.mean() - 1 / 500.0) < 2e-4
+
+
+def test_edited_palette():
+    P = np.array([[0, 0, 0], [1, 1, 1], [0.5, 0.2, 0.9]], np.float32)
+    out = edited_palette(P, np.full((3, 3), 0.6, np.float32), 1.2)
+    assert (out[:2] == P[:2]).all()
+    assert np.allclose(out[2], np.clip((P[2] + 0.6) * 1.2, 0, 1))
+
+
+def expected_outputs(engine, reqs):
+    """Each request alone: exact-size pipeline of rgp.gsmodel + mix."""
+    import torch
+    from rgp import gsmodel, ops
+    view = {}
+    out = {}
+    for r in sorted(reqs, key=lambda q: q.arrival):
+        sc = engine.scenes[r.scene]
+        if r.kind == "pose":
+            view[r.scene] = (r.pose, r.res)
+        pose, res = view[r.scene]
+
+        class Cam:
+            pass
+        cam = Cam()
+        host = engine.camera(pose, *res).numpy()
+        cam.Rcw, cam.tcw, cam.twc = host[0:9].reshape(3, 3), host[9:12], host[12:15]
+        cam.fx, cam.fy, cam.cx, cam.cy = (float(v) for v in host[15:19])
+        cam.width, cam.height = res
+        kw = dict(shs=sc.shs) if sc.mode == 0 else dict(dc=sc.dc, A=sc.A, B=sc.B, Q=sc.Q)
+        layer, _ = gsmodel.render_weights(sc.pws, sc.alphas, sc.scales, sc.rots, cam, sc.K, **kw)
+        P = sc.palette if r.kind == "pose" else edited_palette(sc.palette, r.dpal, r.lum)
+        out[r.rid] = ops.mix(layer, torch.from_numpy(P.astype(np.float32)).to(layer.device)).cpu().numpy()
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["batched", "serial", "serial_rerender"])
+def test_results_alone_equal_in_mix(device, mode, monkeypatch):
+    import rgp.serving as S
+    monkeypatch.setattr(S, "RESOLUTIONS", SMALL_RES)
+    print(f"case: serving {mode}", flush=True)
+    eng = S.Engine(SMALL, device, n_poses=8)
+    reqs = S.make_requests(SMALL, 120, rate=0.0, seed=5, n_poses=8)
+    eng.prepare(reqs)
+    rep = eng.run(reqs, mode=mode.split("_")[0], realtime=False, rerender_edits=mode.endswith("rerender"))
+    assert rep["requests"] == 120 and rep["edits"] > 40
+    exp = expected_outputs(eng, reqs)
+    for r in reqs:
+        got = r.out.cpu().numpy()
+        assert np.array_equal(got, exp[r.rid]), (r.rid, r.kind, r.scene)

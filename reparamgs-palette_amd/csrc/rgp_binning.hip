@@ -57,8 +57,13 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* dst, uint32_t v, int
 // ---------------------------------------------------------------------------
 // scan
 // ---------------------------------------------------------------------------
+// Sums the counts of one chunk and takes the maximum of its per-64-Gaussian
+// quantised-depth maxima (written by the rect kernels without atomics).
 __global__ __launch_bounds__(256) void k_scan_reduce(int n, const uint32_t* __restrict__ counts,
-                                                     uint32_t* __restrict__ partials) {
+                                                     const uint32_t* __restrict__ dqw,
+                                                     uint32_t* __restrict__ partials,
+                                                     uint32_t* __restrict__ dqpart) {
+  static_assert(kScanChunk == 64 * 64, "one wavefront reads the chunk's 64 group maxima");
   __shared__ uint32_t lds[16];
   const int64_t base = (int64_t)blockIdx.x * kScanChunk;
   uint32_t s = 0;
@@ -67,6 +72,11 @@ __global__ __launch_bounds__(256) void k_scan_reduce(int n, const uint32_t* __re
     const int64_t i = base + j * 256 + threadIdx.x;
     if (i < n) s += counts[i];
   }
+  if (threadIdx.x < 64) {
+    const int64_t g = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const uint32_t dq = wave_max_u32(g < ((int64_t)n + 63) / 64 ? dqw[g] : 0u);
+    if (threadIdx.x == 0) dqpart[blockIdx.x] = dq;
+  }
   uint32_t tot;
   block_excl_scan(s, &tot, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
@@ -74,26 +84,34 @@ __global__ __launch_bounds__(256) void k_scan_reduce(int n, const uint32_t* __re
 
 // Scans the block sums, publishes M, the overflow flag and the sort plan.
 __global__ __launch_bounds__(1024) void k_scan_top(int nscan, uint32_t* __restrict__ partials,
+                                                   const uint32_t* __restrict__ dqpart,
                                                    uint32_t* __restrict__ hdr, int64_t cap,
                                                    int tile_bytes) {
   __shared__ uint32_t lds[16];
-  uint32_t carry = 0;
+  __shared__ uint32_t s_dq[16];
+  uint32_t carry = 0, dqm = 0;
   for (int base = 0; base < nscan; base += blockDim.x) {
     const int i = base + threadIdx.x;
     const uint32_t v = i < nscan ? partials[i] : 0u;
+    if (i < nscan) dqm = max(dqm, dqpart[i]);
     uint32_t tot;
     const uint32_t ex = block_excl_scan(v, &tot, lds);
     if (i < nscan) partials[i] = carry + ex;
     carry += tot;
   }
+  dqm = wave_max_u32(dqm);
+  if ((threadIdx.x & 63) == 0) s_dq[threadIdx.x >> 6] = dqm;
+  __syncthreads();
   if (threadIdx.x == 0) {
+    uint32_t dq = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) dq = max(dq, s_dq[w]);
+    hdr[kHdrDqMax] = dq;
     const uint32_t M = carry;
     const bool ovf = (int64_t)M > cap;
     const uint32_t meff = ovf ? 0u : M;
     hdr[kHdrM] = M;
     hdr[kHdrOverflow] = ovf ? 1u : 0u;
     hdr[kHdrMEff] = meff;
-    const uint32_t dq = hdr[kHdrDqMax];
     int depth_bytes = 0;
     while (depth_bytes < 4 && (dq >> (8 * depth_bytes)) != 0) ++depth_bytes;
     uint32_t cur = 0;
@@ -401,9 +419,11 @@ extern "C" int rgp_bin_scan(int n, int width, int height, void* ws, int64_t capa
   while (tile_bytes < 4 && ((uint32_t)(L.tiles - 1) >> (8 * tile_bytes)) != 0) ++tile_bytes;
   if (L.nscan > 0)
     hipLaunchKernelGGL(k_scan_reduce, dim3(L.nscan), dim3(256), 0, s, n,
-                       (const uint32_t*)(base + L.counts), (uint32_t*)(base + L.partials));
+                       (const uint32_t*)(base + L.counts), (const uint32_t*)(base + L.dqw),
+                       (uint32_t*)(base + L.partials), (uint32_t*)(base + L.dqpart));
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, L.nscan,
-                     (uint32_t*)(base + L.partials), hdr, L.cap, tile_bytes);
+                     (uint32_t*)(base + L.partials), (const uint32_t*)(base + L.dqpart), hdr, L.cap,
+                     tile_bytes);
   if (L.nscan > 0)
     hipLaunchKernelGGL(k_scan_down, dim3(L.nscan), dim3(256), 0, s, n,
                        (uint32_t*)(base + L.counts), (const uint32_t*)(base + L.partials));

@@ -327,7 +327,7 @@ __global__ __launch_bounds__(256) void k_rects(int n, const float* __restrict__ 
                                                float* __restrict__ depths, int gx, int gy,
                                                uint2* __restrict__ rects,
                                                uint32_t* __restrict__ counts,
-                                               uint32_t* __restrict__ dq_max) {
+                                               uint32_t* __restrict__ dqw) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t dq = 0;
   if (i < n) {
@@ -346,26 +346,33 @@ __global__ __launch_bounds__(256) void k_rects(int n, const float* __restrict__ 
     }
     counts[i] = cnt;
   }
+  // one plain store per group of 64 Gaussians; the scan reduces them
   dq = wave_max_u32(dq);
-  if ((threadIdx.x & 63) == 0 && dq) atomicMax(dq_max, dq);
+  if ((threadIdx.x & 63) == 0 && i < n) dqw[i >> 6] = dq;
 }
 
 // ---------------------------------------------------------------------------
 // fused preprocess (WeightGSFunctionTest/Fast forward up to getRects)
 // ---------------------------------------------------------------------------
 // WMODE: 0 full rank; 1 low rank, generic P, Q, rank; 2 / 3 low rank with
-// Q = 15, P = K and rank 1 / 2 (factors in registers).
+// Q = 15, P = K and rank 1 / 2 (factors in registers).  One 64-thread block
+// per 64 Gaussians; the block's quantised-depth maximum goes to dqw[block]
+// with a plain store (a same-address atomic per wavefront cost more than the
+// rest of the kernel).  Coefficient rows are read straight from global
+// memory: staging them through LDS measured slower.
 template <int K, int WMODE>
-__global__ __launch_bounds__(256) void k_preprocess(
+__global__ __launch_bounds__(64) void k_preprocess(
     int n, int sh_dim, int P, int Q, int rank, const rgp_camera* __restrict__ cam,
     const float* __restrict__ pws, const float4* __restrict__ rots,
     const float* __restrict__ scales, const float* __restrict__ alphas,
     const float* __restrict__ shs, const float* __restrict__ dc, const float* __restrict__ A,
     const float* __restrict__ B, float* __restrict__ depths, float4* __restrict__ recs,
-    uint2* __restrict__ rects, uint32_t* __restrict__ counts, uint32_t* __restrict__ dq_max,
+    uint2* __restrict__ rects, uint32_t* __restrict__ counts, uint32_t* __restrict__ dqw,
     int gx, int gy, float* __restrict__ us_out, float* __restrict__ cinv_out,
     int32_t* __restrict__ areas_out, float* __restrict__ w_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int R = WMODE == 2 ? 1 : 2;
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * 64 + t;
   uint32_t dqv = 0;
   if (i < n) {
     const float px = pws[3 * i], py = pws[3 * i + 1], pz = pws[3 * i + 2];
@@ -385,7 +392,6 @@ __global__ __launch_bounds__(256) void k_preprocess(
       sh_weights_lowrank<K>(dc + (size_t)i * K, A + (size_t)i * P * rank,
                             B + (size_t)i * rank * Q, Y, L, Q, rank, w);
     } else {
-      constexpr int R = WMODE == 2 ? 1 : 2;
       sh_basis(px, py, pz, cam->twc, 16, Y);
       sh_weights_lowrank_q15<K, R>(dc + (size_t)i * K, A + (size_t)i * K * R, B + (size_t)i * R * 15, Y,
                                    w);
@@ -425,7 +431,7 @@ __global__ __launch_bounds__(256) void k_preprocess(
     counts[i] = cnt;
   }
   dqv = wave_max_u32(dqv);
-  if ((threadIdx.x & 63) == 0 && dqv) atomicMax(dq_max, dqv);
+  if (t == 0) dqw[blockIdx.x] = dqv;
 }
 
 // ---------------------------------------------------------------------------
@@ -641,7 +647,7 @@ extern "C" int rgp_bin_rects(int n, const float* us, int32_t* areas, float* dept
   if (n == 0) return RGP_OK;
   hipLaunchKernelGGL(k_rects, dim3(grid1(n)), dim3(256), 0, s, n, us, areas, depths, L.gx, L.gy,
                      (uint2*)(base + L.rects), (uint32_t*)(base + L.counts),
-                     (uint32_t*)(base + L.header) + kHdrDqMax);
+                     (uint32_t*)(base + L.dqw));
   return check_launch("rgp_bin_rects");
 }
 
@@ -672,11 +678,11 @@ extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int r
   int wmode = mode;
   if (mode == 1 && Q == 15 && P == K && (rank == 1 || rank == 2)) wmode = rank + 1;
 #define RGP_PRE(WM)                                                                               \
-  RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess<KK, WM>), dim3(grid1(n)), dim3(256), 0, s, n,  \
+  RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess<KK, WM>), dim3((n + 63) / 64), dim3(64), 0, s, n, \
                                      sh_dim, P, Q, rank, cam, pws, (const float4*)rots, scales,   \
                                      alphas, shs, dc, A, B, depths, (float4*)records,             \
                                      (uint2*)(base + L.rects), (uint32_t*)(base + L.counts),      \
-                                     (uint32_t*)(base + L.header) + kHdrDqMax, L.gx, L.gy, us,    \
+                                     (uint32_t*)(base + L.dqw), L.gx, L.gy, us,                   \
                                      cinv2ds, areas, weights))
   switch (wmode) {
     case 0: RGP_PRE(0); break;

@@ -37,7 +37,7 @@ constexpr int kSortItems = 16;    // keys per thread per sort block
 constexpr int kSortTile = 256 * kSortItems;
 
 struct BinLayout {
-  size_t header, counts, rects, partials, keys0, keys1, vals0, vals1, hist, totals, total;
+  size_t header, counts, rects, dqw, partials, dqpart, keys0, keys1, vals0, vals1, hist, totals, total;
   int gx, gy, tiles, nscan, nblk_cap;
   int64_t cap;
 };
@@ -56,7 +56,9 @@ inline BinLayout bin_layout(int n, int64_t cap, int width, int height) {
   L.header = o; o = align256(o + kHdrWords * 4);
   L.counts = o; o = align256(o + (size_t)n * 4);
   L.rects = o; o = align256(o + (size_t)n * 8);
+  L.dqw = o; o = align256(o + (size_t)((n + 63) / 64) * 4);
   L.partials = o; o = align256(o + (size_t)(L.nscan + 1) * 4);
+  L.dqpart = o; o = align256(o + (size_t)(L.nscan + 1) * 4);
   L.keys0 = o; o = align256(o + (size_t)L.cap * 8);
   L.keys1 = o; o = align256(o + (size_t)L.cap * 8);
   L.vals0 = o; o = align256(o + (size_t)L.cap * 4);

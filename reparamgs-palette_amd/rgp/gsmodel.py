@@ -43,9 +43,11 @@ def _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dlayer):
 
 
 def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None, B=None, Q=None,
-                   sh_dim=None, return_state=False):
+                   sh_dim=None, return_state=False, exact=False):
     """Forward of the palette-weight rasterizer.  Full rank when `shs`
-    (N, sh_dim*K) is given, low rank (dc, A (N,P,r), B (N,r,Q)) otherwise."""
+    (N, sh_dim*K) is given, low rank (dc, A (N,P,r), B (N,r,Q)) otherwise.
+    The pair buffers keep the cached capacity unless exact=True (see
+    ops.splat_pipeline); the results do not depend on it."""
     require_cuda(pws, alphas, scales, rots)
     dev = pws.device
     n = pws.shape[0]
@@ -72,7 +74,8 @@ def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None,
              ptr(alphas_c), ptr(shs_c), ptr(dc_c), ptr(A_c), ptr(B_c), ptr(depths), ptr(records), ptr(ws),
              I64(int(cap)), W, H, None, None, None, None, stream(dev))
 
-    layer, contrib, final_tau, ranges, ids, m, _ = ops.splat_pipeline(n, W, H, K, records, depths, fill, 0.0, dev)
+    layer, contrib, final_tau, ranges, ids, m, _ = ops.splat_pipeline(n, W, H, K, records, depths, fill, 0.0, dev,
+                                                                      exact=exact)
     if return_state:
         return layer, depths, dict(records=records, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids,
                                    camd=camd, m=m, mode=mode, P=P_, Q=Q_, rank=r_)

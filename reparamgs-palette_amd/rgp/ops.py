@@ -175,29 +175,46 @@ CAPACITY = CapacityCache()
 
 def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, device,
                    exact=True):
-    """Scan -> (read M) -> sort -> ranges -> draw.  ws_fill(ws, cap) must
-    write rects/counts into a workspace (the getRects stage or the fused
-    preprocess).  With exact=True the pair count is read back and the pair
-    buffers are sized from it (one host sync, as gausplat.cu:201)."""
+    """Scan -> sort -> ranges -> draw.  ws_fill(ws, cap) must write
+    rects/counts into a workspace (the getRects stage or the fused
+    preprocess).
+
+    exact=True: the pair count M is read back right after the scan and the
+    pair buffers are sized from it (one host sync, as gausplat.cu:201).
+    exact=False: the pair buffers take the cached capacity, the sort and draw
+    are launched at once, and only then is the overflow flag read (the host
+    waits for the scan while the GPU runs the sort and draw); on overflow the
+    pipeline runs again with the grown capacity.  Both give the same M,
+    ranges, ids[:M] and images."""
     cap = CAPACITY.get(n, width, height)
-    m = None
     while True:
         ws = new_workspace(n, cap, width, height, device)
         with SPANS.span("preprocess"):
             ws_fill(ws, cap)
         with SPANS.span("scan"):
             call("rgp_bin_scan", int(n), int(width), int(height), ptr(ws), I64(int(cap)), stream(device))
+        if exact:
+            m = int(header(ws, 1).item())
+            CAPACITY.update(n, width, height, m)
+            if m > cap:
+                cap = CAPACITY.get(n, width, height)
+                continue
+        else:
+            hdr = torch.empty(2, dtype=torch.int32, pin_memory=True)
+            hdr.copy_(header(ws, 2), non_blocking=True)
+            scanned = torch.cuda.Event()
+            scanned.record(torch.cuda.current_stream(device))
+        with SPANS.span("sort"):
+            ids, ranges, _ = bin_sort(n, depths, width, height, ws, cap, device)
+        layer, contrib, final_tau = draw_forward(width, height, dim, ranges, ids, records, empty_tau)
         if not exact:
-            break
-        m = int(header(ws, 1).item())
-        CAPACITY.update(n, width, height, m)
-        if m <= cap:
-            break
-        cap = CAPACITY.get(n, width, height)
-    with SPANS.span("sort"):
-        ids, ranges, _ = bin_sort(n, depths, width, height, ws, cap, device)
-    layer, contrib, final_tau = draw_forward(width, height, dim, ranges, ids, records, empty_tau)
-    return layer, contrib, final_tau, ranges, ids, m, ws
+            scanned.synchronize()
+            m = int(hdr[0])
+            CAPACITY.update(n, width, height, m)
+            if int(hdr[1]) != 0:
+                cap = CAPACITY.get(n, width, height)
+                continue
+        return layer, contrib, final_tau, ranges, ids, m, ws
 
 
 def mix(W, palette):

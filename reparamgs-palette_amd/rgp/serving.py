@@ -348,13 +348,16 @@ class Engine:
         return ev
 
     # -- running -----------------------------------------------------------
-    def run(self, requests, mode="batched", realtime=True, max_batch=256, rerender_edits=False):
+    def run(self, requests, mode="batched", realtime=True, max_batch=256, rerender_edits=False,
+            keep_outputs=True):
         """Serve the requests.  mode "batched": continuous batching;
         "serial": one request at a time, each synchronised before the next.
         realtime: honour arrival times (else everything arrives at t=0).
         rerender_edits (serial only): a palette edit re-renders the scene's
         current view, the reference's way (bake + render, finetune.py:472-478),
-        instead of re-mixing the cached weight image."""
+        instead of re-mixing the cached weight image.
+        keep_outputs=False hands each frame off when it completes (r.out is
+        dropped), so frame memory is recycled as a server would."""
         for r in requests:
             r.done, r.out = None, None
         pending = collections.deque(sorted(requests, key=lambda q: q.arrival))
@@ -387,6 +390,8 @@ class Engine:
                         redo.append(r)
                         continue
                     r.done = time.perf_counter() - t0
+                    if not keep_outputs:
+                        r.out = None
             else:
                 edits = []
                 for r in batch:
@@ -412,6 +417,8 @@ class Engine:
                         continue
                     for r in rs:
                         r.done = t
+                        if not keep_outputs:
+                            r.out = None
                 else:
                     still.append((ev, rs, hdr))
             inflight = still

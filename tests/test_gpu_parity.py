@@ -244,3 +244,24 @@ def test_forward_counts_match_oracle(device, c0):
                                                      st["records"], 0.0)
     same(N(lay), o["layer"], "layer")
     assert (e, c) == (o["n_eval"], o["n_contrib"])
+
+
+def test_deferred_overflow_check_matches_exact(device, c0):
+    """exact=False launches sort and draw before reading the overflow flag;
+    a capacity far below M must overflow, grow and give the exact-mode
+    results (same M, ids[:M], ranges and images)."""
+    scene, cam = c0
+    args = (T(scene.pws, device), T(scene.alphas, device), T(scene.scales, device), T(scene.rots, device),
+            cam.to_torch(device), scene.K)
+    kw = dict(shs=T(scene.shs, device), return_state=True)
+    la, da, sa = gsmodel.render_weights(*args, exact=True, **kw)
+    key = (scene.n, cam.width, cam.height)
+    ops.CAPACITY.cap[key] = 1024
+    lb, db, sb = gsmodel.render_weights(*args, exact=False, **kw)
+    m = sa["m"]
+    assert m > 4096 and sb["m"] == m and ops.CAPACITY.cap[key] > m
+    same(N(lb), N(la), "layer")
+    same(N(db), N(da), "depths")
+    for k in ("contrib", "final_tau", "ranges"):
+        same(N(sb[k]), N(sa[k]), k)
+    same(N(sb["ids"][:m]), N(sa["ids"][:m]), "ids")

@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving measurement (profiling)")
+    ap.add_argument("--no-lowrank", action="store_true", help="skip the rank-2 1M / 3M finetune steps")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--serve-rate", type=float, default=500.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -96,6 +97,53 @@ def measure_serving(args, dev):
         f"(cached {out['one_at_a_time_cached_frames_per_s']}) frames/s")
     del eng
     return out
+
+
+def measure_lowrank(args, dev, sizes=(1_000_000, 3_000_000), steps=16, warmup=2):
+    """Rank-2 low-rank finetune step (WeightGSFunctionFast, K=5, A (N,5,2),
+    B (N,2,15)) at 1080p over 8 orbit views, for 1M and 3M Gaussians."""
+    import numpy as np
+    import torch
+
+    from rgp import gsmodel, ops
+    from rgp.losses import L1SSIMLoss
+    from rgp.scene import make_scene, orbit_cameras
+
+    out = {}
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    for n in sizes:
+        sc = make_scene(n, seed=args.seed, extent=3.0, K=5, lowrank=True, rank=2, P=5, Q=15)
+        tcams = [c.to_torch(dev) for c in orbit_cameras(8, 6.0, args.width, args.height)]
+        pws, rots, scales, alphas = T(sc.pws), T(sc.rots), T(sc.scales), T(sc.alphas)
+        dc, A, B = (T(x).requires_grad_(True) for x in (sc.dc, sc.A, sc.B))
+        pal = T(sc.palette)
+        with torch.no_grad():
+            targets = [ops.mix(gsmodel.render_weights(pws, alphas, scales, rots, tc, 5, dc=dc, A=A, B=B, Q=15)[0],
+                               torch.clamp(pal + 0.05, 0, 1)) for tc in tcams]
+        opt = torch.optim.Adam([dc, A, B], lr=1e-3)
+
+        def step(i):
+            tc = tcams[i % len(tcams)]
+            layer, _ = gsmodel.WeightGSFunctionFast.apply(pws, dc, A, B, alphas, scales, rots, tc, 5, 15)
+            loss = L1SSIMLoss.apply(ops.Mix.apply(layer, pal), targets[i % len(tcams)], 0.2)
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            opt.step()
+
+        for i in range(warmup):
+            step(i)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            step(i)
+        torch.cuda.synchronize()
+        sps = steps / (time.perf_counter() - t0)
+        out[f"{n // 1_000_000}M"] = round(sps, 2)
+        log(f"rank-2 low-rank fwd+bwd, {n} Gaussians: {sps:.2f} steps/s")
+        del sc, pws, rots, scales, alphas, dc, A, B, targets, opt
+        torch.cuda.empty_cache()
+    return {"steps_per_s": out, "workload": "rank-2 factors A (N,5,2), B (N,2,15), K=5, 1920x1080, 8 orbit views, "
+                                            "L1+SSIM, Adam on dc/A/B"}
 
 
 def main():
@@ -223,6 +271,7 @@ def main():
                 "traffic": None, "kernel": name, "mean_launch_ms": round(mean_ms, 4)}
 
     serving = None if args.no_serve else measure_serving(args, dev)
+    lowrank = None if args.no_lowrank else measure_lowrank(args, dev)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -236,11 +285,20 @@ def main():
             O.backward_weight(scene, o, dl, parallel=True)
             sample.append(v)
         c1 = time.perf_counter()
+        # the same sample on one thread
+        O.set_threads(1)
+        s0 = time.perf_counter()
+        o = O.forward_weight(scene, cams[0])
+        O.backward_weight(scene, o, np.random.default_rng(0).normal(size=o["layer"].shape).astype(np.float32),
+                          parallel=True)
+        s1 = time.perf_counter()
+        O.set_threads(threads)
         cpu = {"value": round(len(sample) / (c1 - c0), 5), "unit": "steps/s", "cores": threads, "kind": "port",
                "sample": f"CPU oracle (FP32, OpenMP over tiles) forward + backward rasterization of "
                          f"{len(sample)} view(s) of the same 1M-Gaussian 1080p K=5 scene, "
-                         f"{c1 - c0:.1f}s; loss and optimizer not included"}
-        log(f"cpu oracle: {cpu['value']} steps/s on {threads} threads")
+                         f"{c1 - c0:.1f}s; loss and optimizer not included",
+               "value_1_thread": round(1.0 / (s1 - s0), 5)}
+        log(f"cpu oracle: {cpu['value']} steps/s on {threads} threads, {cpu['value_1_thread']} on 1 thread")
 
     out = {
         "metric": METRIC, "value": round(args.steps / (t1 - t0), 3), "unit": "steps/s", "n_gpus": 1,
@@ -252,7 +310,7 @@ def main():
         "frames_per_s": round(fps, 2), "ms_per_frame": round(1e3 / fps, 3),
         "pairs_per_view": int(np.mean([w["pairs"] for w in work])),
         "p99_palette_edit_ms": serving["edit_p99_ms"] if serving else None,
-        "serving": serving,
+        "serving": serving, "lowrank_rank2": lowrank,
         "kernels": breakdown, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

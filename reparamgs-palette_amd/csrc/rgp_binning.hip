@@ -57,10 +57,11 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* dst, uint32_t v, int
 // ---------------------------------------------------------------------------
 // scan
 // ---------------------------------------------------------------------------
-// Sums the counts of one chunk and takes the maximum of its per-64-Gaussian
-// quantised-depth maxima (written by the rect kernels without atomics).
+// Sums the counts of one chunk and, when dqpart is given, takes the maximum
+// of its per-64-Gaussian quantised-depth maxima (written by the rect kernels
+// without atomics; ndq of them).
 __global__ __launch_bounds__(256) void k_scan_reduce(int n, const uint32_t* __restrict__ counts,
-                                                     const uint32_t* __restrict__ dqw,
+                                                     int ndq, const uint32_t* __restrict__ dqw,
                                                      uint32_t* __restrict__ partials,
                                                      uint32_t* __restrict__ dqpart) {
   static_assert(kScanChunk == 64 * 64, "one wavefront reads the chunk's 64 group maxima");
@@ -72,9 +73,9 @@ __global__ __launch_bounds__(256) void k_scan_reduce(int n, const uint32_t* __re
     const int64_t i = base + j * 256 + threadIdx.x;
     if (i < n) s += counts[i];
   }
-  if (threadIdx.x < 64) {
+  if (dqpart && threadIdx.x < 64) {
     const int64_t g = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    const uint32_t dq = wave_max_u32(g < ((int64_t)n + 63) / 64 ? dqw[g] : 0u);
+    const uint32_t dq = wave_max_u32(g < ndq ? dqw[g] : 0u);
     if (threadIdx.x == 0) dqpart[blockIdx.x] = dq;
   }
   uint32_t tot;
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(1024) void k_scan_top(int nscan, uint32_t* __restri
   for (int base = 0; base < nscan; base += blockDim.x) {
     const int i = base + threadIdx.x;
     const uint32_t v = i < nscan ? partials[i] : 0u;
-    if (i < nscan) dqm = max(dqm, dqpart[i]);
+    if (dqpart && i < nscan) dqm = max(dqm, dqpart[i]);
     uint32_t tot;
     const uint32_t ex = block_excl_scan(v, &tot, lds);
     if (i < nscan) partials[i] = carry + ex;
@@ -103,6 +104,7 @@ __global__ __launch_bounds__(1024) void k_scan_top(int nscan, uint32_t* __restri
   if ((threadIdx.x & 63) == 0) s_dq[threadIdx.x >> 6] = dqm;
   __syncthreads();
   if (threadIdx.x == 0) {
+    hdr[kHdrBig] = 0;
     uint32_t dq = 0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) dq = max(dq, s_dq[w]);
     hdr[kHdrDqMax] = dq;
@@ -393,6 +395,264 @@ __global__ __launch_bounds__(256) void k_ranges(int tiles, const uint32_t* __res
   ranges[t] = e > a ? make_int2((int)a, (int)e) : make_int2(0, 0);
 }
 
+// ---------------------------------------------------------------------------
+// Tile-bucket path.  The order the reference's stable (tile, depth) sort
+// produces inside one tile is ascending (uint32(depth*1000), Gaussian id):
+// createKeys writes the pairs of Gaussian i before those of i + 1
+// (kernel.cu:96-124) and the radix sort keeps that order among equal keys.
+// So: count pairs per (tile, Gaussian block) in LDS, scan those counts in
+// tile-major order (which yields M, every tile's range and every block's
+// slots), scatter the Gaussian ids into their tile's slots with LDS atomics
+// (any order inside a tile), then sort each tile by the key
+// (depth_q << 32 | id), whose values are all distinct, so the result is the
+// reference's order.
+// ---------------------------------------------------------------------------
+
+// Calls fn(tile, gaussian) for every pair of the Gaussians [g0, g1): a thread
+// walks the rect of its own Gaussian when it covers at most kTbBig tiles;
+// larger rects are walked by the whole wavefront, 64 tiles a step.
+template <class F>
+__device__ __forceinline__ void tb_for_pairs(int g0, int g1, const uint32_t* __restrict__ counts,
+                                             const uint2* __restrict__ rects, uint32_t gx, F fn) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (int b0 = g0; b0 < g1; b0 += (int)blockDim.x) {
+    const int i = b0 + (int)threadIdx.x;
+    uint32_t c = 0;
+    uint2 r = make_uint2(0u, 0u);
+    if (i < g1) {
+      c = counts[i];
+      if (c) r = rects[i];
+    }
+    const uint32_t x0 = r.x & 0xffffu, y0 = r.x >> 16, x1 = r.y & 0xffffu;
+    if (c != 0 && c <= (uint32_t)kTbBig) {
+      uint32_t tx = x0, row = y0 * gx;
+      for (uint32_t k = 0; k < c; ++k) {
+        fn(row + tx, i);
+        if (++tx == x1) {
+          tx = x0;
+          row += gx;
+        }
+      }
+    }
+    uint64_t big = __ballot(c > (uint32_t)kTbBig);
+    while (big) {
+      const int src = __ffsll((unsigned long long)big) - 1;
+      big &= big - 1;
+      const uint32_t bc = __shfl(c, src);
+      const uint32_t bx0 = __shfl(x0, src), by0 = __shfl(y0, src), bx1 = __shfl(x1, src);
+      const int bi = __shfl(i, src);
+      const uint32_t w = bx1 - bx0;
+      for (uint32_t k = lane; k < bc; k += 64) {
+        const uint32_t yy = k / w, xx = k - yy * w;
+        fn((by0 + yy) * gx + bx0 + xx, bi);
+      }
+    }
+  }
+}
+
+// Block b counts the pairs of its Gaussians per tile; tbpart[t * nb + b].
+__global__ __launch_bounds__(kTbThreads) void k_tb_count(int n, int chunk, int tiles, int nb, int gx,
+                                                         const uint32_t* __restrict__ counts,
+                                                         const uint2* __restrict__ rects,
+                                                         uint32_t* __restrict__ tbpart) {
+  extern __shared__ uint32_t s_hist[];  // tiles
+  const int b = blockIdx.x;
+  for (int q = threadIdx.x; q < tiles; q += kTbThreads) s_hist[q] = 0;
+  __syncthreads();
+  const int g0 = b * chunk, g1 = min(n, g0 + chunk);
+  tb_for_pairs(g0, g1, counts, rects, (uint32_t)gx,
+               [&](uint32_t tile, int) { atomicAdd(&s_hist[tile], 1u); });
+  __syncthreads();
+  for (int q = threadIdx.x; q < tiles; q += kTbThreads) tbpart[(size_t)q * nb + b] = s_hist[q];
+}
+
+// Block b writes the ids of its pairs into its slots of every tile: tbpart
+// holds inclusive offsets, so block b owns
+// [tbpart[t*nb+b] - count(t, b), tbpart[t*nb+b]).
+__global__ __launch_bounds__(kTbThreads) void k_tb_scatter(int n, int chunk, int tiles, int nb, int gx,
+                                                           const uint32_t* __restrict__ counts,
+                                                           const uint2* __restrict__ rects,
+                                                           const uint32_t* __restrict__ tbpart,
+                                                           const uint32_t* __restrict__ hdr,
+                                                           int32_t* __restrict__ ids) {
+  if (hdr[kHdrMEff] == 0) return;
+  extern __shared__ uint32_t s_cur[];  // tiles
+  const int b = blockIdx.x;
+  for (int q = threadIdx.x; q < tiles; q += kTbThreads) s_cur[q] = tbpart[(size_t)q * nb + b];
+  __syncthreads();
+  const int g0 = b * chunk, g1 = min(n, g0 + chunk);
+  tb_for_pairs(g0, g1, counts, rects, (uint32_t)gx,
+               [&](uint32_t tile, int g) { ids[atomicSub(&s_cur[tile], 1u) - 1u] = g; });
+}
+
+__device__ __forceinline__ uint64_t tb_key(const float* __restrict__ depths, int32_t g) {
+  return ((uint64_t)sat_u32(depths[g] * 1000.f) << 32) | (uint32_t)g;
+}
+
+// Bitonic network whose comparators all put the smaller key at the lower
+// index, over P = 2^p slots of which the first n hold keys: the slots >= n
+// act as +infinity, so comparators reaching them are no-ops and no padding
+// is stored.  Comparator q of a stage has its lower index
+// i = ((q & ~(h - 1)) << 1) | (q & (h - 1)), h = k/2 for the "flip" of block
+// size k (partner i ^ (k - 1)) and h = j for a half-cleaner (partner i + j).
+__device__ __forceinline__ void tb_cx(uint64_t* a, uint32_t i, uint32_t l, uint32_t n) {
+  if (l < n) {
+    const uint64_t x = a[i], y = a[l];
+    if (y < x) {
+      a[i] = y;
+      a[l] = x;
+    }
+  }
+}
+
+__device__ __forceinline__ void tb_flip(uint64_t* a, uint32_t n, uint32_t P, uint32_t k) {
+  const uint32_t h = k >> 1;
+  for (uint32_t q = threadIdx.x; q < (P >> 1); q += blockDim.x) {
+    const uint32_t i = ((q & ~(h - 1)) << 1) | (q & (h - 1));
+    tb_cx(a, i, i ^ (k - 1), n);
+  }
+}
+
+__device__ __forceinline__ void tb_half(uint64_t* a, uint32_t n, uint32_t P, uint32_t j) {
+  for (uint32_t q = threadIdx.x; q < (P >> 1); q += blockDim.x) {
+    const uint32_t i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+    tb_cx(a, i, i + j, n);
+  }
+}
+
+// Complete sort of a[0, n), P = 2^p >= n, by all threads of the block.
+__device__ __forceinline__ void tb_bitonic(uint64_t* a, uint32_t n, uint32_t P) {
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    tb_flip(a, n, P, k);
+    __syncthreads();
+    for (uint32_t j = k >> 2; j > 0; j >>= 1) {
+      tb_half(a, n, P, j);
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t n) {
+  uint32_t P = 1;
+  while (P < n) P <<= 1;
+  return P;
+}
+
+__device__ __forceinline__ void tb_range(int tile, int nb, const uint32_t* __restrict__ tbpart,
+                                         uint32_t* start, uint32_t* end) {
+  *start = tile == 0 ? 0u : tbpart[(size_t)tile * nb - 1];
+  *end = tbpart[(size_t)tile * nb + nb - 1];
+}
+
+// One block per tile: its range, and the sort of its ids when the list has
+// at most kTbSortSmall entries; longer lists go to the queue of k_tb_sort_big.
+__global__ __launch_bounds__(256) void k_tb_sort(int tiles, int nb, uint32_t* __restrict__ hdr,
+                                                 const uint32_t* __restrict__ tbpart,
+                                                 const float* __restrict__ depths,
+                                                 int32_t* __restrict__ ids,
+                                                 uint64_t* __restrict__ keys_out,
+                                                 int2* __restrict__ ranges,
+                                                 uint32_t* __restrict__ bigq) {
+  __shared__ uint64_t s[kTbSortSmall];
+  const int tile = blockIdx.x;
+  const uint32_t M = hdr[kHdrMEff];
+  uint32_t start, end;
+  tb_range(tile, nb, tbpart, &start, &end);
+  // empty tiles keep (0,0), and so does every tile when M == 1 (kernel.cu:165-187)
+  if (threadIdx.x == 0)
+    ranges[tile] = (M <= 1 || end == start) ? make_int2(0, 0) : make_int2((int)start, (int)end);
+  if (M == 0 || end == start) return;
+  const uint32_t n = end - start;
+  if (n > (uint32_t)kTbSortSmall) {
+    if (threadIdx.x == 0) bigq[atomicAdd(&hdr[kHdrBig], 1u)] = (uint32_t)tile;
+    return;
+  }
+  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) s[e] = tb_key(depths, ids[start + e]);
+  __syncthreads();
+  tb_bitonic(s, n, pow2_at_least(n));
+  const uint64_t thi = (uint64_t)tile << 32;
+  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const uint64_t k = s[e];
+    ids[start + e] = (int32_t)(uint32_t)k;
+    if (keys_out) keys_out[start + e] = thi | (k >> 32);
+  }
+}
+
+// Lists longer than kTbSortSmall, taken from the queue by a fixed grid.  Up
+// to kTbSortLds keys are sorted in LDS; longer lists are sorted in chunks of
+// kTbSortLds in LDS, then merged by the network's stages that cross chunks
+// in global memory (tkeys) and the rest of each merge in LDS.
+__global__ __launch_bounds__(256) void k_tb_sort_big(int nb, const uint32_t* __restrict__ hdr,
+                                                     const uint32_t* __restrict__ tbpart,
+                                                     const uint32_t* __restrict__ bigq,
+                                                     const float* __restrict__ depths,
+                                                     uint64_t* __restrict__ tkeys,
+                                                     int32_t* __restrict__ ids,
+                                                     uint64_t* __restrict__ keys_out) {
+  constexpr uint32_t C = kTbSortLds;
+  __shared__ uint64_t s[C];
+  const uint32_t nq = hdr[kHdrBig];
+  for (uint32_t w = blockIdx.x; w < nq; w += gridDim.x) {
+    const int tile = (int)bigq[w];
+    uint32_t start, end;
+    tb_range(tile, nb, tbpart, &start, &end);
+    const uint32_t n = end - start;
+    const uint64_t thi = (uint64_t)tile << 32;
+    if (n <= C) {
+      for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) s[e] = tb_key(depths, ids[start + e]);
+      __syncthreads();
+      tb_bitonic(s, n, pow2_at_least(n));
+      for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+        const uint64_t k = s[e];
+        ids[start + e] = (int32_t)(uint32_t)k;
+        if (keys_out) keys_out[start + e] = thi | (k >> 32);
+      }
+      __syncthreads();
+      continue;
+    }
+    uint64_t* seg = tkeys + start;
+    const uint32_t P = pow2_at_least(n);
+    // chunks sorted in LDS (the network's blocks of size <= C)
+    for (uint32_t c0 = 0; c0 < n; c0 += C) {
+      const uint32_t nc = min(C, n - c0);
+      for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) s[e] = tb_key(depths, ids[start + c0 + e]);
+      __syncthreads();
+      tb_bitonic(s, nc, C);
+      for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) seg[c0 + e] = s[e];
+      __threadfence_block();
+      __syncthreads();
+    }
+    for (uint32_t k = 2 * C; k <= P; k <<= 1) {
+      tb_flip(seg, n, P, k);
+      __threadfence_block();
+      __syncthreads();
+      for (uint32_t j = k >> 2; j >= C; j >>= 1) {
+        tb_half(seg, n, P, j);
+        __threadfence_block();
+        __syncthreads();
+      }
+      for (uint32_t c0 = 0; c0 < n; c0 += C) {
+        const uint32_t nc = min(C, n - c0);
+        for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) s[e] = seg[c0 + e];
+        __syncthreads();
+        for (uint32_t j = C >> 1; j > 0; j >>= 1) {
+          tb_half(s, nc, C, j);
+          __syncthreads();
+        }
+        for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) seg[c0 + e] = s[e];
+        __threadfence_block();
+        __syncthreads();
+      }
+    }
+    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+      const uint64_t k = seg[e];
+      ids[start + e] = (int32_t)(uint32_t)k;
+      if (keys_out) keys_out[start + e] = thi | (k >> 32);
+    }
+    __syncthreads();
+  }
+}
+
 int bin_reset_header(void* ws, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, (uint32_t*)ws, 0u, (int64_t)kHdrWords);
   return check_launch("bin_reset_header");
@@ -409,6 +669,21 @@ extern "C" size_t rgp_bin_workspace_bytes(int n, int64_t capacity, int width, in
 
 extern "C" const uint32_t* rgp_bin_header(const void* ws) { return (const uint32_t*)ws; }
 
+// Dynamic LDS of the count / scatter kernels beyond the default 64 KiB,
+// requested once per process.
+static int tb_lds_attr() {
+  static int rc = [] {
+    const int bytes = kTbMaxTiles * 4;
+    if (hipFuncSetAttribute((const void*)k_tb_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes) != hipSuccess ||
+        hipFuncSetAttribute((const void*)k_tb_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            bytes) != hipSuccess)
+      return set_error(RGP_ERR_LAUNCH, "hipFuncSetAttribute(max dynamic LDS) failed");
+    return (int)RGP_OK;
+  }();
+  return rc;
+}
+
 extern "C" int rgp_bin_scan(int n, int width, int height, void* ws, int64_t capacity,
                             hipStream_t s) {
   if (n < 0 || width <= 0 || height <= 0) return set_error(RGP_ERR_ARG, "rgp_bin_scan: bad size");
@@ -417,10 +692,27 @@ extern "C" int rgp_bin_scan(int n, int width, int height, void* ws, int64_t capa
   uint32_t* hdr = (uint32_t*)(base + L.header);
   int tile_bytes = 0;
   while (tile_bytes < 4 && ((uint32_t)(L.tiles - 1) >> (8 * tile_bytes)) != 0) ++tile_bytes;
+  if (L.tb) {
+    int rc = tb_lds_attr();
+    if (rc) return rc;
+    uint32_t* part = (uint32_t*)(base + L.tbpart);
+    hipLaunchKernelGGL(k_tb_count, dim3(L.nb), dim3(kTbThreads), (size_t)L.tiles * 4, s, n, L.chunk,
+                       L.tiles, L.nb, L.gx, (const uint32_t*)(base + L.counts),
+                       (const uint2*)(base + L.rects), part);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(L.nscan), dim3(256), 0, s, (int)L.npart,
+                       (const uint32_t*)part, 0, (const uint32_t*)nullptr,
+                       (uint32_t*)(base + L.partials), (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, L.nscan, (uint32_t*)(base + L.partials),
+                       (const uint32_t*)nullptr, hdr, L.cap, tile_bytes);
+    hipLaunchKernelGGL(k_scan_down, dim3(L.nscan), dim3(256), 0, s, (int)L.npart, part,
+                       (const uint32_t*)(base + L.partials));
+    return check_launch("rgp_bin_scan");
+  }
   if (L.nscan > 0)
     hipLaunchKernelGGL(k_scan_reduce, dim3(L.nscan), dim3(256), 0, s, n,
-                       (const uint32_t*)(base + L.counts), (const uint32_t*)(base + L.dqw),
-                       (uint32_t*)(base + L.partials), (uint32_t*)(base + L.dqpart));
+                       (const uint32_t*)(base + L.counts), (n + 63) / 64,
+                       (const uint32_t*)(base + L.dqw), (uint32_t*)(base + L.partials),
+                       (uint32_t*)(base + L.dqpart));
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, L.nscan,
                      (uint32_t*)(base + L.partials), (const uint32_t*)(base + L.dqpart), hdr, L.cap,
                      tile_bytes);
@@ -439,6 +731,23 @@ extern "C" int rgp_bin_sort(int n, const float* depths, int width, int height, v
   BinLayout L = bin_layout(n, capacity, width, height);
   char* base = (char*)ws;
   const uint32_t* hdr = (const uint32_t*)(base + L.header);
+  if (L.tb) {
+    uint32_t* hdr_w = (uint32_t*)(base + L.header);
+    int rc = tb_lds_attr();
+    if (rc) return rc;
+    const uint32_t* part = (const uint32_t*)(base + L.tbpart);
+    uint64_t* tkeys = (uint64_t*)(base + L.tkeys);
+    if (n > 0 && L.cap > 0)
+      hipLaunchKernelGGL(k_tb_scatter, dim3(L.nb), dim3(kTbThreads), (size_t)L.tiles * 4, s, n, L.chunk,
+                         L.tiles, L.nb, L.gx, (const uint32_t*)(base + L.counts),
+                         (const uint2*)(base + L.rects), part, hdr, ids_out);
+    uint32_t* bigq = (uint32_t*)(base + L.bigq);
+    hipLaunchKernelGGL(k_tb_sort, dim3(L.tiles), dim3(256), 0, s, L.tiles, L.nb, hdr_w, part, depths,
+                       ids_out, keys_out, (int2*)ranges_out, bigq);
+    hipLaunchKernelGGL(k_tb_sort_big, dim3(kTbBigBlocks), dim3(256), 0, s, L.nb, (const uint32_t*)hdr, part,
+                       (const uint32_t*)bigq, depths, tkeys, ids_out, keys_out);
+    return check_launch("rgp_bin_sort");
+  }
   uint64_t* k0 = (uint64_t*)(base + L.keys0);
   uint64_t* k1 = (uint64_t*)(base + L.keys1);
   int32_t* v0 = (int32_t*)(base + L.vals0);

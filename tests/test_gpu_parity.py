@@ -265,3 +265,47 @@ def test_deferred_overflow_check_matches_exact(device, c0):
     for k in ("contrib", "final_tau", "ranges"):
         same(N(sb[k]), N(sa[k]), k)
     same(N(sb["ids"][:m]), N(sa["ids"][:m]), "ids")
+
+
+def gpu_bin(us, areas, depths, W, H, device, cap):
+    """getRects -> scan -> sort on the GPU from given us / areas / depths."""
+    from rgp._lib import I64, call, ptr, stream
+    n = len(depths)
+    ws = ops.new_workspace(n, cap, W, H, device)
+    a, d = T(areas, device), T(depths, device)
+    call("rgp_bin_rects", n, ptr(T(us, device)), ptr(a), ptr(d), W, H, ptr(ws), I64(cap), stream(device))
+    call("rgp_bin_scan", n, W, H, ptr(ws), I64(cap), stream(device))
+    ids, ranges, keys = ops.bin_sort(n, d, W, H, ws, cap, device, keys=True)
+    m = int(ops.header(ws, 1).item())
+    return N(keys[:m]), N(ids[:m]), N(ranges), m
+
+
+@pytest.mark.parametrize("case", ["long_lists_ties", "4k", "8k_radix"])
+def test_binning_sort_paths(device, case):
+    """Keys, ids and ranges against the oracle's stable sort: tile lists far
+    longer than one LDS sort (merged through global memory) with many equal
+    quantised depths, a 3840x2160 grid (32,400 tiles, the largest grid of
+    the tile-bucket path) and a 7680x4320 grid (the LSD radix path)."""
+    print(f"case: binning {case}", flush=True)
+    rng = np.random.default_rng(51)
+    if case == "long_lists_ties":
+        W, H, n = 64, 48, 12_000
+        us = rng.uniform(-10, 74, (n, 2)).astype(np.float32)
+        areas = rng.integers(30, 60, (n, 2)).astype(np.int32)
+        depths = (1.0 + 0.5 * rng.integers(0, 5, n)).astype(np.float32)
+    else:
+        W, H = (3840, 2160) if case == "4k" else (7680, 4320)
+        n = 100_000
+        us = np.stack([rng.uniform(-50, W + 50, n), rng.uniform(-50, H + 50, n)], 1).astype(np.float32)
+        areas = rng.integers(1, 48, (n, 2)).astype(np.int32)
+        depths = rng.uniform(0.1, 80.0, n).astype(np.float32)
+    a_o, d_o = areas.copy(), depths.copy()
+    rects, counts = O.get_rects(us, a_o, d_o, W, H)
+    keys, ids, ranges = O.bin_sort(d_o, rects, counts, W, H)
+    gk, gi, gr, m = gpu_bin(us, areas, depths, W, H, device, int(counts.sum()) + 16)
+    assert m == len(ids)
+    same(gk, keys, "keys")
+    same(gi, ids, "ids")
+    same(gr, ranges, "ranges")
+    if case == "long_lists_ties":
+        assert (ranges[:, 1] - ranges[:, 0]).max() > 2 * 4096

@@ -157,6 +157,7 @@ def main():
 
     from rgp import gsmodel, ops
     from rgp.losses import L1SSIMLoss
+    from rgp.optim import PaletteAdam
     from rgp.profiling import SPANS
     from rgp.scene import make_scene, orbit_cameras
 
@@ -201,10 +202,9 @@ def main():
     torch.cuda.synchronize()
     log(f"setup {time.time() - t_setup:.1f}s; pairs/view mean {np.mean([w['pairs'] for w in work]):.0f}")
 
-    try:
-        opt = torch.optim.Adam([shs, chroma], lr=1e-3, fused=True)
-    except (RuntimeError, TypeError):
-        opt = torch.optim.Adam([shs, chroma], lr=1e-3)
+    # Adam on the SH coefficients and the chromatic palette; the SH update runs
+    # fused with the SH backward (no (N, 80) gradient is stored)
+    opt = PaletteAdam([shs, chroma], lr=1e-3, fused_sh=[shs])
 
     def step(i):
         tc = tcams[i % len(tcams)]

@@ -190,6 +190,18 @@ int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int rank,
 int rgp_sh_backward_full(int n, int K, int sh_dim, const rgp_camera* cam, const float* pws,
                          const float* dw, float* dshs, hipStream_t stream);
 
+/* No counterpart in the reference: the SH backward above (gsmodel.py:140-143)
+ * fused with the update torch.optim.Adam (no amsgrad) applies to the full-rank
+ * coefficients in the finetune step (finetune.py:346-399), so dL/dshs is never
+ * stored.  shs, exp_avg and exp_avg_sq are (N, sh_dim*K) float32, 16-byte
+ * aligned, updated in place; step is the Adam step count after this update
+ * (>= 1); 1 - beta, the bias corrections and lr / bc1 are formed in double
+ * from the double hyper-parameters, as torch does, then rounded to float. */
+int rgp_sh_backward_adam_full(int n, int K, int sh_dim, const rgp_camera* cam, const float* pws,
+                              const float* dw, float* shs, float* exp_avg, float* exp_avg_sq,
+                              int64_t step, double lr, double beta1, double beta2, double eps,
+                              double weight_decay, hipStream_t stream);
+
 /* Low-rank SH backward of gsmodel.py:194-198 (dL/ddc = dw * SH_C0 and
  * sh2Weight_fast_B), Y recomputed from pws and twc. */
 int rgp_sh_backward_lowrank(int n, int K, int P, int Q, int rank, const rgp_camera* cam,

@@ -4,6 +4,8 @@
 // fused preprocess that replaces their chain in WeightGSFunctionTest/Fast
 // (gsplat/gsmodel.py:107-118, 155-170), record packing, and the SH backward.
 // One thread per Gaussian; every stage is HBM-bound.
+#include <cmath>
+
 #include "rgp_internal.h"
 
 namespace rgp {
@@ -477,6 +479,78 @@ __global__ __launch_bounds__(256) void k_sh_backward_full(int n, int sh_dim,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Full-rank SH gradient fused with the Adam update of the coefficients: every
+// coefficient's gradient g = dw[n,k] * Y_lm(n) (the product gsmodel.py:135-144
+// forms) is used where it is made, so dL/dshs is never stored.  The update is
+// torch.optim.Adam's (no amsgrad), element by element:
+//   g += wd * p;  m += (1 - b1) * (g - m);  v = v * b2 + ((1 - b2) * g) * g;
+//   p += -(lr / bc1) * (m / (sqrt(v) / sqrt(bc2) + eps)).
+// A block owns 64 Gaussians: their basis and dw in LDS, then p, m, v as
+// float4 runs (a row is SH*K floats, the block's span 64 rows).
+// ---------------------------------------------------------------------------
+struct AdamArgs {
+  float neg_step, omb1, b2, omb2, bc2_sqrt, eps, wd;
+};
+
+__device__ __forceinline__ void adam_elem(float& p, float& m, float& v, float g, const AdamArgs& a) {
+  if (a.wd != 0.f) g = g + a.wd * p;
+  m = m + a.omb1 * (g - m);
+  v = v * a.b2 + (a.omb2 * g) * g;
+  const float denom = sqrtf(v) / a.bc2_sqrt + a.eps;
+  p = p + a.neg_step * (m / denom);
+}
+
+template <int K, int SH>
+__global__ __launch_bounds__(256) void k_sh_adam_full(int n, const rgp_camera* __restrict__ cam,
+                                                      const float* __restrict__ pws,
+                                                      const float* __restrict__ dw, float* __restrict__ p,
+                                                      float* __restrict__ m, float* __restrict__ v,
+                                                      AdamArgs a) {
+  constexpr int G = 64, ROW = SH * K;
+  __shared__ float sY[G][SH + 1];
+  __shared__ float sW[G][K];
+  const int g0 = blockIdx.x * G, t = threadIdx.x;
+  if (t < G) {
+    const int i = g0 + t;
+    float Y[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) Y[l] = 0.f;
+    if (i < n) {
+      if (SH > 1) sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], cam->twc, SH, Y);
+#pragma unroll
+      for (int k = 0; k < K; ++k) sW[t][k] = dw[(size_t)i * K + k];
+    }
+    Y[0] = (float)SH_C0_0;
+#pragma unroll
+    for (int l = 0; l < SH; ++l) sY[t][l] = Y[l];
+  }
+  __syncthreads();
+  const int cnt = min(G, n - g0);
+  const int total = cnt * ROW;
+  const size_t base = (size_t)g0 * ROW;
+  auto grad = [&](int e) {
+    const int gl = e / ROW, r = e - gl * ROW, lm = r / K, k = r - lm * K;
+    return sW[gl][k] * sY[gl][lm];
+  };
+  float4* p4 = (float4*)(p + base);
+  float4* m4 = (float4*)(m + base);
+  float4* v4 = (float4*)(v + base);
+  const int n4 = total >> 2;
+  for (int q = t; q < n4; q += blockDim.x) {
+    float4 pp = p4[q], mm = m4[q], vv = v4[q];
+    adam_elem(pp.x, mm.x, vv.x, grad(4 * q), a);
+    adam_elem(pp.y, mm.y, vv.y, grad(4 * q + 1), a);
+    adam_elem(pp.z, mm.z, vv.z, grad(4 * q + 2), a);
+    adam_elem(pp.w, mm.w, vv.w, grad(4 * q + 3), a);
+    p4[q] = pp;
+    m4[q] = mm;
+    v4[q] = vv;
+  }
+  for (int e = 4 * n4 + t; e < total; e += blockDim.x)
+    adam_elem(p[base + e], m[base + e], v[base + e], grad(e), a);
+}
+
 }  // namespace rgp
 
 using namespace rgp;
@@ -706,6 +780,44 @@ extern "C" int rgp_sh_backward_full(int n, int K, int sh_dim, const rgp_camera* 
   RGP_K_SWITCH(K, hipLaunchKernelGGL(k_sh_backward_full<KK>, dim3(blocks), dim3(256), 0, s, n,
                                      sh_dim, cam, pws, dw, dshs));
   return check_launch("rgp_sh_backward_full");
+}
+
+extern "C" int rgp_sh_backward_adam_full(int n, int K, int sh_dim, const rgp_camera* cam,
+                                         const float* pws, const float* dw, float* shs, float* exp_avg,
+                                         float* exp_avg_sq, int64_t step, double lr, double beta1,
+                                         double beta2, double eps, double weight_decay, hipStream_t s) {
+  if (n < 0 || K < 1 || K > RGP_MAX_DIM)
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_full: bad n or K");
+  if (!(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_full: sh_dim must be 1, 4, 9 or 16");
+  if (step < 1) return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_full: step must be >= 1");
+  if ((((uintptr_t)shs) | ((uintptr_t)exp_avg) | ((uintptr_t)exp_avg_sq)) & 15)
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_full: shs and moments must be 16-byte aligned");
+  if (n == 0) return RGP_OK;
+  // scalars as torch.optim.Adam forms them: in double from the double
+  // hyper-parameters, then rounded once to float
+  const double bc1 = 1.0 - std::pow(beta1, (double)step);
+  const double bc2 = 1.0 - std::pow(beta2, (double)step);
+  AdamArgs a;
+  a.neg_step = (float)(-lr / bc1);
+  a.omb1 = (float)(1.0 - beta1);
+  a.b2 = (float)beta2;
+  a.omb2 = (float)(1.0 - beta2);
+  a.bc2_sqrt = (float)std::sqrt(bc2);
+  a.eps = (float)eps;
+  a.wd = (float)weight_decay;
+  const int blocks = (n + 63) / 64;
+#define RGP_ADAM(SH)                                                                              \
+  RGP_K_SWITCH(K, hipLaunchKernelGGL((k_sh_adam_full<KK, SH>), dim3(blocks), dim3(256), 0, s, n, cam, \
+                                     pws, dw, shs, exp_avg, exp_avg_sq, a))
+  switch (sh_dim) {
+    case 1: RGP_ADAM(1); break;
+    case 4: RGP_ADAM(4); break;
+    case 9: RGP_ADAM(9); break;
+    default: RGP_ADAM(16); break;
+  }
+#undef RGP_ADAM
+  return check_launch("rgp_sh_backward_adam_full");
 }
 
 extern "C" int rgp_sh_backward_lowrank(int n, int K, int P, int Q, int rank,

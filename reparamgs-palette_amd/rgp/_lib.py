@@ -17,6 +17,7 @@ P = ctypes.c_void_p
 I = ctypes.c_int
 I64 = ctypes.c_int64
 F = ctypes.c_float
+D = ctypes.c_double
 U32 = ctypes.c_uint32
 SZ = ctypes.c_size_t
 
@@ -45,6 +46,7 @@ _SIGS = {
     "rgp_preprocess_weight": (I, [I, I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I64, I, I,
                                   P, P, P, P, P]),
     "rgp_sh_backward_full": (I, [I, I, I, P, P, P, P, P]),
+    "rgp_sh_backward_adam_full": (I, [I, I, I, P, P, P, P, P, P, I64, D, D, D, D, D, P]),
     "rgp_sh_backward_lowrank": (I, [I, I, I, I, I, P, P, P, P, P, P, P, P, P]),
     "rgp_mix_batched": (I, [I, I, P, P, P, P, I64, P]),
     "rgp_mix": (I, [I, I64, P, P, P, P]),

@@ -90,6 +90,9 @@ class WeightGSFunctionTest(torch.autograd.Function):
         K = int(num_palette)
         layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, shs=shs, return_state=True)
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"])
+        # an optimizer that fuses the SH update (rgp.optim.PaletteAdam) takes dL/dw instead of dL/dshs
+        fused = getattr(shs, "_rgp_fused_adam", None)
+        ctx.fused = (fused, shs) if fused is not None else None
         ctx.save_for_backward(f32c(pws), st["records"], st["contrib"], st["final_tau"], st["ranges"], st["ids"],
                               st["camd"])
         return layer, depths > 0.2
@@ -99,6 +102,10 @@ class WeightGSFunctionTest(torch.autograd.Function):
         W, H, K, n, sh_dim = ctx.meta
         pws, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
         dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())
+        if ctx.fused is not None and ctx.fused[0].take(ctx.fused[1], dw, camd, pws, K, sh_dim):
+            ctx.fused = None
+            return (None, None, None, None, None, None, None)
+        ctx.fused = None
         dshs = torch.empty((n, sh_dim * K), dtype=torch.float32, device=pws.device)
         with SPANS.span("sh_backward"):
             call("rgp_sh_backward_full", n, K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(dshs), stream(pws.device))

@@ -1,0 +1,99 @@
+"""Adam for the palette finetune with the full-rank SH update fused into the
+backward.
+
+The finetune step (gsplat/finetune.py:346-399) runs torch.optim.Adam over the
+SH coefficients after WeightGSFunctionTest.backward has formed
+dL/dshs[n, lm*K+k] = dL/dw[n,k] * Y_lm(n) (gsmodel.py:135-144), an (N, 16K)
+tensor written once and read back by the optimizer.  PaletteAdam takes dL/dw
+(N, K) from that backward instead and applies the update with
+rgp_sh_backward_adam_full, which forms each gradient element where it is used.
+The arithmetic of the update is torch.optim.Adam's (no amsgrad, no maximize),
+element by element:
+
+    g += weight_decay * p
+    m += (1 - beta1) * (g - m)
+    v  = v * beta2 + ((1 - beta2) * g) * g
+    p += -(lr / (1 - beta1^t)) * (m / (sqrt(v) / sqrt(1 - beta2^t) + eps))
+
+Parameters listed in `fused_sh` must be the tensor passed as `shs` to
+WeightGSFunctionTest (a float32, contiguous (N, sh_dim*K) CUDA tensor).  Every
+other parameter, and a fused one whose gradient also arrives the usual way
+(several backward passes before one step), is updated by the same formula
+with torch operations.
+"""
+import math
+
+import torch
+
+from ._lib import I64, call, ptr, stream
+from .profiling import SPANS
+
+
+class PaletteAdam(torch.optim.Optimizer):
+    def __init__(self, params, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, weight_decay=0.0, fused_sh=()):
+        defaults = dict(lr=lr, betas=betas, eps=eps, weight_decay=weight_decay)
+        super().__init__(params, defaults)
+        self._pending = {}
+        known = {id(p) for g in self.param_groups for p in g["params"]}
+        for p in fused_sh:
+            if id(p) not in known:
+                raise ValueError("PaletteAdam: a fused_sh tensor is not among the optimised parameters")
+            if not (p.is_cuda and p.dtype == torch.float32 and p.dim() == 2 and p.is_contiguous()):
+                raise ValueError("PaletteAdam: fused_sh tensors must be contiguous float32 (N, sh_dim*K) on the GPU")
+            p._rgp_fused_adam = self
+
+    # -- called by WeightGSFunctionTest.backward ---------------------------
+    def take(self, p, dw, camd, pws, K, sh_dim):
+        """Keep dL/dw for p's update; False when p's gradient must be
+        materialised instead (a second backward before the step)."""
+        if id(p) in self._pending:
+            return False
+        self._pending[id(p)] = (dw, camd, pws, int(K), int(sh_dim))
+        return True
+
+    def zero_grad(self, set_to_none=True):
+        self._pending.clear()
+        super().zero_grad(set_to_none=set_to_none)
+
+    @staticmethod
+    def _materialise(p, pend):
+        dw, camd, pws, K, sh_dim = pend
+        g = torch.empty_like(p)
+        call("rgp_sh_backward_full", p.shape[0], K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(g), stream(p.device))
+        return g
+
+    @torch.no_grad()
+    def step(self, closure=None):
+        loss = None
+        if closure is not None:
+            with torch.enable_grad():
+                loss = closure()
+        for group in self.param_groups:
+            lr, (b1, b2), eps, wd = group["lr"], group["betas"], group["eps"], group["weight_decay"]
+            for p in group["params"]:
+                pend = self._pending.pop(id(p), None)
+                if pend is None and p.grad is None:
+                    continue
+                st = self.state[p]
+                if not st:
+                    st["step"] = 0
+                    st["exp_avg"] = torch.zeros_like(p, memory_format=torch.preserve_format)
+                    st["exp_avg_sq"] = torch.zeros_like(p, memory_format=torch.preserve_format)
+                st["step"] += 1
+                t = st["step"]
+                m, v = st["exp_avg"], st["exp_avg_sq"]
+                if pend is not None and p.grad is None:
+                    dw, camd, pws, K, sh_dim = pend
+                    with SPANS.span("sh_backward_adam"):
+                        call("rgp_sh_backward_adam_full", p.shape[0], K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(p),
+                             ptr(m), ptr(v), I64(t), float(lr), float(b1), float(b2), float(eps), float(wd),
+                             stream(p.device))
+                    continue
+                g = p.grad if pend is None else p.grad + self._materialise(p, pend)
+                if wd != 0:
+                    g = g.add(p, alpha=wd)
+                m.lerp_(g, 1 - b1)
+                v.mul_(b2).addcmul_(g, g, value=1 - b2)
+                denom = (v.sqrt() / math.sqrt(1 - b2 ** t)).add_(eps)
+                p.addcdiv_(m, denom, value=-lr / (1 - b1 ** t))
+        return loss

@@ -1,0 +1,106 @@
+"""The full-rank SH backward fused with Adam (rgp_sh_backward_adam_full,
+rgp.optim.PaletteAdam) against torch.optim.Adam applied to the materialised
+gradient dL/dshs = dL/dw * Y (rgp_sh_backward_full, itself bit-exact with the
+oracle): same dL/dw in, parameters and both moments out.
+
+torch's elementwise kernels may contract a multiply-add into one rounding
+where the fused kernel rounds twice: a few float32 ulp of the operands, which
+for a value near 0 (a parameter crossing zero, p - update cancelling) is
+large relative to the value itself.  The bar is 1e-6 * (|value| + the RMS of
+the tensor)."""
+import numpy as np
+import pytest
+import torch
+
+from rgp import gsmodel
+from rgp._lib import I64, call, ptr, stream
+from rgp.optim import PaletteAdam
+from rgp.scene import workload_c0
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def close(a, b, what, rtol=1e-6):
+    a, b = a.detach().cpu().numpy().astype(np.float64), b.detach().cpu().numpy().astype(np.float64)
+    err = np.abs(a - b)
+    lim = rtol * (np.abs(b) + np.sqrt(np.mean(b * b))) + 1e-30
+    assert (err <= lim).all(), f"{what}: max err/lim {float(np.max(err / lim)):.3g}"
+
+
+@pytest.mark.parametrize("sh_dim,K,wd", [(16, 5, 0.0), (16, 7, 1e-2), (9, 3, 0.0)])
+def test_fused_kernel_matches_torch_adam(device, sh_dim, K, wd):
+    print(f"case: fused adam sh_dim={sh_dim} K={K} wd={wd}", flush=True)
+    from rgp import ops
+    scene, cam = workload_c0()
+    n = 5000 + 37  # a partial last block
+    rng = np.random.default_rng(3)
+    pws = T(scene.pws[:n], device)
+    camd = ops.camera_struct(cam.to_torch(device), device)
+    p_f = T(rng.normal(0, 0.1, (n, sh_dim * K)).astype(np.float32), device)
+    m_f, v_f = torch.zeros_like(p_f), torch.zeros_like(p_f)
+    p_t = p_f.clone().requires_grad_(True)
+    ref = torch.optim.Adam([p_t], lr=3e-3, betas=(0.9, 0.999), eps=1e-8, weight_decay=wd, foreach=False)
+    for step in range(1, 4):
+        dw = T(rng.normal(0, 1e-3, (n, K)).astype(np.float32), device)
+        call("rgp_sh_backward_adam_full", n, K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(p_f), ptr(m_f), ptr(v_f),
+             I64(step), 3e-3, 0.9, 0.999, 1e-8, wd, stream(device))
+        g = torch.empty_like(p_f)
+        call("rgp_sh_backward_full", n, K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(g), stream(device))
+        p_t.grad = g
+        ref.step()
+    st = ref.state[p_t]
+    # with weight decay g = dw*Y + wd*p can cancel to near 0, where Adam's
+    # normalised step m / sqrt(v) turns an ulp of g into a visible change
+    rtol = 1e-6 if wd == 0 else 1e-5
+    close(p_f, p_t, "param", rtol)
+    close(m_f, st["exp_avg"], "exp_avg", rtol)
+    close(v_f, st["exp_avg_sq"], "exp_avg_sq", rtol)
+
+
+def test_palette_adam_in_the_finetune_step(device, monkeypatch):
+    """WeightGSFunctionTest hands dL/dw to PaletteAdam (no dL/dshs is
+    stored, shs.grad stays None); two steps equal torch.optim.Adam fed the
+    materialised gradient of the same dL/dw."""
+    from rgp import ops
+    scene, cam = workload_c0()
+    tc = cam.to_torch(device)
+    seen = []
+    take = PaletteAdam.take
+
+    def spy(self, p, dw, camd, pws, K, sh_dim):
+        seen.append((dw.clone(), camd, pws))
+        return take(self, p, dw, camd, pws, K, sh_dim)
+
+    monkeypatch.setattr(PaletteAdam, "take", spy)
+    args = [T(x, device) for x in (scene.alphas, scene.scales, scene.rots)]
+    pws = T(scene.pws, device)
+    shs = T(scene.shs, device).requires_grad_(True)
+    opt = PaletteAdam([shs], lr=1e-2, fused_sh=[shs])
+    shs_ref = shs.detach().clone().requires_grad_(True)
+    ref = torch.optim.Adam([shs_ref], lr=1e-2, foreach=False)
+    G = T(np.random.default_rng(1).normal(size=(scene.K, cam.height, cam.width)).astype(np.float32), device)
+    for step in range(2):
+        layer, _ = gsmodel.WeightGSFunctionTest.apply(pws, shs, args[0], args[1], args[2], tc, scene.K)
+        (layer * G).sum().backward()
+        assert shs.grad is None and len(seen) == step + 1
+        opt.step()
+        opt.zero_grad()
+        dw, camd, pws_s = seen[-1]
+        g = torch.empty_like(shs_ref)
+        call("rgp_sh_backward_full", scene.n, scene.K, 16, ptr(camd), ptr(pws_s), ptr(dw), ptr(g), stream(device))
+        shs_ref.grad = g
+        ref.step()
+    close(shs, shs_ref, "shs")
+    close(opt.state[shs]["exp_avg"], ref.state[shs_ref]["exp_avg"], "exp_avg")
+    close(opt.state[shs]["exp_avg_sq"], ref.state[shs_ref]["exp_avg_sq"], "exp_avg_sq")
+    # a second backward before the step falls back to the materialised gradient
+    for _ in range(2):
+        layer, _ = gsmodel.WeightGSFunctionTest.apply(pws, shs, args[0], args[1], args[2], tc, scene.K)
+        (layer * G).sum().backward()
+    assert shs.grad is not None
+    opt.step()
+    assert torch.isfinite(shs).all()

@@ -280,16 +280,17 @@ def gpu_bin(us, areas, depths, W, H, device, cap):
     return N(keys[:m]), N(ids[:m]), N(ranges), m
 
 
-@pytest.mark.parametrize("case", ["long_lists_ties", "4k", "8k_radix"])
+@pytest.mark.parametrize("case", ["long_lists_ties", "mid_lists", "4k", "8k_radix"])
 def test_binning_sort_paths(device, case):
     """Keys, ids and ranges against the oracle's stable sort: tile lists far
     longer than one LDS sort (merged through global memory) with many equal
-    quantised depths, a 3840x2160 grid (32,400 tiles, the largest grid of
+    quantised depths, lists between the per-tile and the long-list LDS
+    limits, a 3840x2160 grid (32,400 tiles, the largest grid of
     the tile-bucket path) and a 7680x4320 grid (the LSD radix path)."""
     print(f"case: binning {case}", flush=True)
     rng = np.random.default_rng(51)
-    if case == "long_lists_ties":
-        W, H, n = 64, 48, 12_000
+    if case in ("long_lists_ties", "mid_lists"):
+        W, H, n = 64, 48, (12_000 if case == "long_lists_ties" else 3_300)
         us = rng.uniform(-10, 74, (n, 2)).astype(np.float32)
         areas = rng.integers(30, 60, (n, 2)).astype(np.int32)
         depths = (1.0 + 0.5 * rng.integers(0, 5, n)).astype(np.float32)
@@ -307,5 +308,8 @@ def test_binning_sort_paths(device, case):
     same(gk, keys, "keys")
     same(gi, ids, "ids")
     same(gr, ranges, "ranges")
+    lens = ranges[:, 1] - ranges[:, 0]
     if case == "long_lists_ties":
-        assert (ranges[:, 1] - ranges[:, 0]).max() > 2 * 4096
+        assert lens.max() > 2 * 4096
+    if case == "mid_lists":
+        assert 2048 < lens.max() <= 4096

@@ -493,6 +493,17 @@ int64_t bin_sort(int n, const T* depths, const uint32_t* rects, const uint32_t* 
 }
 
 // ---------------------------------------------------------------------------
+// mahaSqDist (common.cuh:86-89): max(0, cx dx^2 + cz dy^2 + 2 cy dx dy) with
+// the two multiply-adds contracted, as nvcc contracts them by default:
+// fma((2 cy) dx, dy, fma(cz dy, dy, (cx dx) dx)).  The HIP kernels form the
+// same two fused operations.
+// ---------------------------------------------------------------------------
+template <typename T>
+static inline T maha_sq(T cx, T cy, T cz, T dx, T dy) {
+  return fmx((T)0, std::fma((T)2 * cy * dx, dy, std::fma(cz * dy, dy, cx * dx * dx)));
+}
+
+// ---------------------------------------------------------------------------
 // A12 draw_weight (kernel.cu:283-372) and draw_color (kernel.cu:194-276): one
 // pixel at a time over its tile list.  `dim` channels of per-Gaussian feature
 // (K weights, or 3 colours).  Outputs are written for every pixel of a tile
@@ -526,11 +537,13 @@ void draw(int width, int height, int dim, const int32_t* ranges, const int32_t* 
           const T cx = cinv[3 * g], cy = cinv[3 * g + 1], cz = cinv[3 * g + 2];
           ++cont_tmp;
           ++tot_e;
-          const T maha = fmx((T)0, cx * dx * dx + cz * dy * dy + (T)2 * cy * dx * dy);
+          const T maha = maha_sq(cx, cy, cz, dx, dy);
           const T ap = fmn((T)0.99f, alphas[g] * ex((T)-0.5f * maha));
           if (ap < (T)0.002f) continue;
           ++tot_c;
-          for (int k = 0; k < dim; ++k) acc[k] += tau * ap * feat[(size_t)g * dim + k];
+          // W_k += T alpha' w_k, the multiply-add contracted (one rounding)
+          const T ta = tau * ap;
+          for (int k = 0; k < dim; ++k) acc[k] = std::fma(ta, feat[(size_t)g * dim + k], acc[k]);
           cont = cont_tmp;
           tau *= ((T)1 - ap);
           if (tau < (T)0.0001f) break;
@@ -574,7 +587,7 @@ void draw_backward_replay(int width, int height, int K, const int32_t* ranges, c
           const int g = ids[j];
           const T dx = us[2 * g] - (T)px, dy = us[2 * g + 1] - (T)py;
           const T cx = cinv[3 * g], cy = cinv[3 * g + 1], cz = cinv[3 * g + 2];
-          const T maha = fmx((T)0, cx * dx * dx + cz * dy * dy + (T)2 * cy * dx * dy);
+          const T maha = maha_sq(cx, cy, cz, dx, dy);
           const T ap = fmn((T)0.99f, alphas[g] * ex((T)-0.5f * maha));
           if (ap < (T)0.002f) continue;
           const T ta = tau * ap;
@@ -619,7 +632,7 @@ void draw_backward(int width, int height, int K, const int32_t* ranges, const in
           const int g = ids[r1 - i - 1];
           const T dx = us[2 * g] - (T)px, dy = us[2 * g + 1] - (T)py;
           const T cx = cinv[3 * g], cy = cinv[3 * g + 1], cz = cinv[3 * g + 2];
-          const T maha = fmx((T)0, cx * dx * dx + cz * dy * dy + (T)2 * cy * dx * dy);
+          const T maha = maha_sq(cx, cy, cz, dx, dy);
           const T gg = ex((T)-0.5f * maha);
           const T ap = fmn((T)0.99f, alphas[g] * gg);
           if (ap < (T)0.002f) continue;

@@ -367,12 +367,6 @@ struct RecStride {
   static constexpr int f4 = 2 + (DIM + 3) / 4;
 };
 
-// Mahalanobis distance of mahaSqDist (common.cuh:86-89), clamped at 0.
-__device__ __forceinline__ float maha_dist(float4 g0, float cz, float dx, float dy) {
-  const float m = (g0.z * dx * dx + cz * dy * dy) + g0.w * dx * dy;
-  return fmaxf(0.0f, m);
-}
-
 // Blocks of one launch go round-robin to the 8 XCDs; give each XCD a
 // contiguous run of tiles so neighbouring tiles share its L2.
 __device__ __forceinline__ int xcd_tile(int b, int nblocks) {

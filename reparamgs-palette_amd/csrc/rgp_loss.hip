@@ -37,6 +37,13 @@ __device__ __forceinline__ float block_sum_f(float v, float* lds) {
   return s;
 }
 
+// Horizontal pass items: a row r of the staged tile and a group of four
+// output columns; vertical pass: a group of four output rows and a column.
+// Each item loads the 14 inputs its four outputs share once, so products and
+// LDS reads are not repeated per tap; sums use fmaf (the loss is compared
+// with a float64 reference within a tolerance, not bit for bit).
+constexpr int kSG = kST / 4;  // four-output groups per tile edge
+
 __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __restrict__ img,
                                                   const float* __restrict__ tgt, Win11 win,
                                                   float C1, float C2, float* __restrict__ abc,
@@ -57,40 +64,64 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __r
     sy[r][q] = in ? Y[(size_t)gy * W + gx] : 0.f;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kSI * kST; i += blockDim.x) {
-    const int r = i / kST, q = i - r * kST;
-    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
+  for (int i = threadIdx.x; i < kSI * kSG; i += blockDim.x) {
+    const int r = i % kSI, q0 = 4 * (i / kSI);
+    float a[14], b[14];
 #pragma unroll
-    for (int j = 0; j < 11; ++j) {
-      const float a = sx[r][q + j], b = sy[r][q + j], g = win.g[j];
-      m0 += g * a; m1 += g * b; m2 += g * (a * a); m3 += g * (b * b); m4 += g * (a * b);
+    for (int j = 0; j < 14; ++j) {
+      a[j] = sx[r][q0 + j];
+      b[j] = sy[r][q0 + j];
     }
-    hb[0][r][q] = m0; hb[1][r][q] = m1; hb[2][r][q] = m2; hb[3][r][q] = m3; hb[4][r][q] = m4;
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
+#pragma unroll
+      for (int j = 0; j < 11; ++j) {
+        const float g = win.g[j], u = a[o + j], v = b[o + j];
+        m0 = fmaf(g, u, m0);
+        m1 = fmaf(g, v, m1);
+        m2 = fmaf(g, u * u, m2);
+        m3 = fmaf(g, v * v, m3);
+        m4 = fmaf(g, u * v, m4);
+      }
+      hb[0][r][q0 + o] = m0; hb[1][r][q0 + o] = m1; hb[2][r][q0 + o] = m2;
+      hb[3][r][q0 + o] = m3; hb[4][r][q0 + o] = m4;
+    }
   }
   __syncthreads();
   float s_sum = 0.f, l1_sum = 0.f;
-  for (int i = threadIdx.x; i < kST * kST; i += blockDim.x) {
-    const int r = i / kST, q = i - r * kST;
-    const int gy = y0 + kSR + r, gx = x0 + kSR + q;
-    if (gy >= H || gx >= W) continue;
-    float mx = 0.f, my = 0.f, exx = 0.f, eyy = 0.f, exy = 0.f;
+  for (int i = threadIdx.x; i < kSG * kST; i += blockDim.x) {
+    const int q = i % kST, r0 = 4 * (i / kST);
+    float v[5][14];
 #pragma unroll
-    for (int j = 0; j < 11; ++j) {
-      const float g = win.g[j];
-      mx += g * hb[0][r + j][q]; my += g * hb[1][r + j][q]; exx += g * hb[2][r + j][q];
-      eyy += g * hb[3][r + j][q]; exy += g * hb[4][r + j][q];
+    for (int m = 0; m < 5; ++m)
+#pragma unroll
+      for (int j = 0; j < 14; ++j) v[m][j] = hb[m][r0 + j][q];
+    const int gx = x0 + kSR + q;
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      const int r = r0 + o, gy = y0 + kSR + r;
+      if (gy >= H || gx >= W) continue;
+      float mu[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < 11; ++j) {
+        const float g = win.g[j];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) mu[m] = fmaf(g, v[m][o + j], mu[m]);
+      }
+      const float mx = mu[0], my = mu[1], exx = mu[2], eyy = mu[3], exy = mu[4];
+      const float A1 = 2.f * mx * my + C1, A2 = 2.f * (exy - mx * my) + C2;
+      const float B1 = mx * mx + my * my + C1, B2 = (exx - mx * mx) + (eyy - my * my) + C2;
+      const float D = B1 * B2, Dinv = 1.f / D;
+      const float sv = A1 * A2 * Dinv;
+      const float dN = 2.f * my * (A2 - A1), dD = 2.f * mx * (B2 - B1);
+      const size_t o_ = c * HW + (size_t)gy * W + gx;
+      abc[o_] = (dN - sv * dD) * Dinv;     // ds/dmu_x
+      abc[3 * HW + o_] = -sv * B1 * Dinv;  // ds/dE[x^2]
+      abc[6 * HW + o_] = 2.f * A1 * Dinv;  // ds/dE[xy]
+      s_sum += sv;
+      l1_sum += fabsf(sx[r + kSR][q + kSR] - sy[r + kSR][q + kSR]);
     }
-    const float A1 = 2.f * mx * my + C1, A2 = 2.f * (exy - mx * my) + C2;
-    const float B1 = mx * mx + my * my + C1, B2 = (exx - mx * mx) + (eyy - my * my) + C2;
-    const float D = B1 * B2, Dinv = 1.f / D;
-    const float s = A1 * A2 * Dinv;
-    const float dN = 2.f * my * (A2 - A1), dD = 2.f * mx * (B2 - B1);
-    const size_t o = c * HW + (size_t)gy * W + gx;
-    abc[o] = (dN - s * dD) * Dinv;             // ds/dmu_x
-    abc[3 * HW + o] = -s * B1 * Dinv;          // ds/dE[x^2]
-    abc[6 * HW + o] = 2.f * A1 * Dinv;         // ds/dE[xy]
-    s_sum += s;
-    l1_sum += fabsf(sx[r + kSR][q + kSR] - sy[r + kSR][q + kSR]);
   }
   const float bs = block_sum_f(s_sum, red);
   __syncthreads();
@@ -133,33 +164,53 @@ __global__ __launch_bounds__(256) void k_ssim_bwd(int H, int W, const float* __r
     sa[2][r][q] = in ? abc[6 * HW + o] : 0.f;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kSI * kST; i += blockDim.x) {
-    const int r = i / kST, q = i - r * kST;
-    float m0 = 0.f, m1 = 0.f, m2 = 0.f;
+  for (int i = threadIdx.x; i < kSI * kSG; i += blockDim.x) {
+    const int r = i % kSI, q0 = 4 * (i / kSI);
+    float v[3][14];
 #pragma unroll
-    for (int j = 0; j < 11; ++j) {
-      const float g = win.g[j];
-      m0 += g * sa[0][r][q + j]; m1 += g * sa[1][r][q + j]; m2 += g * sa[2][r][q + j];
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int j = 0; j < 14; ++j) v[m][j] = sa[m][r][q0 + j];
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < 11; ++j) {
+        const float g = win.g[j];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) acc[m] = fmaf(g, v[m][o + j], acc[m]);
+      }
+#pragma unroll
+      for (int m = 0; m < 3; ++m) hb[m][r][q0 + o] = acc[m];
     }
-    hb[0][r][q] = m0; hb[1][r][q] = m1; hb[2][r][q] = m2;
   }
   __syncthreads();
   const float go = *gout;
   const float kl1 = (float)((1.0 - lam) / n) * go, kss = (float)(-lam / n) * go;
-  for (int i = threadIdx.x; i < kST * kST; i += blockDim.x) {
-    const int r = i / kST, q = i - r * kST;
-    const int gy = y0 + kSR + r, gx = x0 + kSR + q;
-    if (gy >= H || gx >= W) continue;
-    float ga = 0.f, gb = 0.f, gc = 0.f;
+  for (int i = threadIdx.x; i < kSG * kST; i += blockDim.x) {
+    const int q = i % kST, r0 = 4 * (i / kST);
+    float v[3][14];
 #pragma unroll
-    for (int j = 0; j < 11; ++j) {
-      const float g = win.g[j];
-      ga += g * hb[0][r + j][q]; gb += g * hb[1][r + j][q]; gc += g * hb[2][r + j][q];
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int j = 0; j < 14; ++j) v[m][j] = hb[m][r0 + j][q];
+    const int gx = x0 + kSR + q;
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      const int gy = y0 + kSR + r0 + o;
+      if (gy >= H || gx >= W) continue;
+      float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < 11; ++j) {
+        const float g = win.g[j];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) acc[m] = fmaf(g, v[m][o + j], acc[m]);
+      }
+      const size_t o_ = c * HW + (size_t)gy * W + gx;
+      const float x = img[o_], y = tgt[o_];
+      const float sg = x > y ? 1.f : (x < y ? -1.f : 0.f);
+      dimg[o_] = kl1 * sg + kss * (acc[0] + 2.f * x * acc[1] + y * acc[2]);
     }
-    const size_t o = c * HW + (size_t)gy * W + gx;
-    const float x = img[o], y = tgt[o];
-    const float sg = x > y ? 1.f : (x < y ? -1.f : 0.f);
-    dimg[o] = kl1 * sg + kss * (ga + 2.f * x * gb + y * gc);
   }
 }
 

@@ -11,8 +11,12 @@
 //
 // Work per entry.  The terms of the Mahalanobis distance that depend only on
 // dx, (cinv.x dx) dx and (2 cinv.y) dx, are formed once per lane and reused by
-// its four pixels; per pixel the operations and their order are those of
-// mahaSqDist, so every value is bit-identical to the reference recurrence.
+// its four pixels; per pixel the distance is
+// fma((2 cinv.y) dx, dy, fma(cinv.z dy, dy, (cinv.x dx) dx)) and the weight
+// update W_k = fma(T alpha', w_k, W_k): mahaSqDist and the accumulation of
+// kernel.cu:351-361 with their multiply-adds contracted (nvcc's default),
+// the same operations the CPU oracle performs, so every forward value is
+// bit-identical to it.
 // While staging, each lane also works out which of the four strips its entry
 // can reach (|dy| within the record's cutoff radius ry); the wavefront reads
 // that 4-bit mask into scalar registers and skips unreachable strips with a
@@ -139,7 +143,7 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
           if (!(slots & (1u << j))) continue;
           if (tau[j] < 0.0001f) continue;
           const float dy = g0.y - fpy[j];
-          const float m = fmaxf(0.0f, (ta0 + g1.x * dy * dy) + tc0 * dy);
+          const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
           if (m > g1.z) continue;
           const float ap = fminf(0.99f, g1.y * gauss_of(m));
           if (ap < 0.002f) continue;
@@ -147,10 +151,10 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
 #pragma unroll
           for (int q = 0; q < F4 - 2; ++q) {
             const float4 f = s_rec[jj * F4 + 2 + q];
-            if (4 * q + 0 < DIM) acc[j][4 * q + 0] = acc[j][4 * q + 0] + ta * f.x;
-            if (4 * q + 1 < DIM) acc[j][4 * q + 1] = acc[j][4 * q + 1] + ta * f.y;
-            if (4 * q + 2 < DIM) acc[j][4 * q + 2] = acc[j][4 * q + 2] + ta * f.z;
-            if (4 * q + 3 < DIM) acc[j][4 * q + 3] = acc[j][4 * q + 3] + ta * f.w;
+            if (4 * q + 0 < DIM) acc[j][4 * q + 0] = fmaf(ta, f.x, acc[j][4 * q + 0]);
+            if (4 * q + 1 < DIM) acc[j][4 * q + 1] = fmaf(ta, f.y, acc[j][4 * q + 1]);
+            if (4 * q + 2 < DIM) acc[j][4 * q + 2] = fmaf(ta, f.z, acc[j][4 * q + 2]);
+            if (4 * q + 3 < DIM) acc[j][4 * q + 3] = fmaf(ta, f.w, acc[j][4 * q + 3]);
           }
           cont[j] = b0 + jj - rg.x + 1;
           tau[j] = tau[j] * (1.f - ap);
@@ -287,14 +291,14 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
         if (!(slots & (1u << j))) continue;
         if (fi >= cont[j]) continue;
         const float dy = g0.y - fpy[j];
-        const float m = fmaxf(0.0f, (ta0 + g1.x * dy * dy) + tc0 * dy);
+        const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
         if (m > g1.z) continue;
         const float ap = fminf(0.99f, g1.y * gauss_of(m));
         if (ap < 0.002f) continue;
         tau[j] = tau[j] / (1.f - ap);
         const float gv = ap * tau[j];
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) part[k] = part[k] + dl[j][k] * gv;
+        for (int k = 0; k < DIM; ++k) part[k] = fmaf(dl[j][k], gv, part[k]);
         on = true;
       }
       if (__any(on)) flush_grad<DIM>(part, dfeat, s_id[jj], lane);
@@ -369,13 +373,13 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
         if (!(slots & (1u << j))) continue;
         if (tau[j] < 0.0001f) continue;
         const float dy = g0.y - fpy[j];
-        const float m = fmaxf(0.0f, (ta0 + g1.x * dy * dy) + tc0 * dy);
+        const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
         if (m > g1.z) continue;
         const float ap = fminf(0.99f, g1.y * gauss_of(m));
         if (ap < 0.002f) continue;
         const float ta = tau[j] * ap;
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) part[k] = part[k] + dl[j][k] * ta;
+        for (int k = 0; k < DIM; ++k) part[k] = fmaf(dl[j][k], ta, part[k]);
         tau[j] = tau[j] * (1.f - ap);
         on = true;
       }

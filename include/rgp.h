@@ -48,34 +48,51 @@ const char* rgp_last_error(void);
 int rgp_version(void);
 
 /* ---- per-Gaussian geometry (palette path calls them with calc_J = False) -- */
+/* The Jacobian outputs (calc_J = True) are nullable; when given they must be
+ * zero-filled by the caller like the reference's torch::full(..., 0.0)
+ * allocations (gausplat.cu:312-316, 350-354, 392-393, 442-446, 601-602):
+ * entries the reference does not write (culled Gaussians, the structural
+ * zeros of du/dpc) keep that fill. */
 
 /* gsc.project, ext.cpp:94-101 / gausplat.cu:381-419 / kernel.cu:618-663.
  * pws (N,3); Rcw (3,3), tcw (3) device.  Outputs us (N,2), pcs (N,3),
  * depths (N) must be zero-filled by the caller (the reference allocates them
- * with torch::full 0); culled Gaussians get depth -1 and keep zeros. */
+ * with torch::full 0); culled Gaussians get depth -1 and keep zeros.
+ * du_dpcs (N,2,3), kernel.cu:656-660: entries 0, 2, 4, 5. */
 int rgp_project(int n, const float* pws, const float* Rcw, const float* tcw, float fx, float fy,
-                float cx, float cy, float* us, float* pcs, float* depths, hipStream_t stream);
+                float cx, float cy, float* us, float* pcs, float* depths, float* du_dpcs,
+                hipStream_t stream);
 
 /* gsc.computeCov3D, ext.cpp:78-81 / gausplat.cu:298-331 / kernel.cu:422-501.
- * rots (N,4) (w,x,y,z), scales (N,3) linear -> cov3ds (N,6) (zero-filled). */
+ * rots (N,4) (w,x,y,z), scales (N,3) linear -> cov3ds (N,6) (zero-filled).
+ * dcov3d_drots (N,6,4), dcov3d_dscales (N,6,3), kernel.cu:463-500: both or
+ * neither. */
 int rgp_compute_cov3d(int n, const float* rots, const float* scales, const float* depths,
-                      float* cov3ds, hipStream_t stream);
+                      float* cov3ds, float* dcov3d_drots, float* dcov3d_dscales,
+                      hipStream_t stream);
 
-/* gsc.computeCov2D, ext.cpp:83-91 / gausplat.cu:333-379 / kernel.cu:508-611. */
+/* gsc.computeCov2D, ext.cpp:83-91 / gausplat.cu:333-379 / kernel.cu:508-611.
+ * dcov2d_dcov3ds (N,3,6), dcov2d_dpcs (N,3,3), kernel.cu:557-607: both or
+ * neither (the +-1.3 tan_fov clamp is not differentiated, as there). */
 int rgp_compute_cov2d(int n, const float* cov3ds, const float* pcs, const float* Rcw,
                       const float* depths, float fx, float fy, float width, float height,
-                      float* cov2ds, hipStream_t stream);
+                      float* cov2ds, float* dcov2d_dcov3ds, float* dcov2d_dpcs,
+                      hipStream_t stream);
 
 /* gsc.inverseCov2D, ext.cpp:73-75 / gausplat.cu:590-617 / kernel.cu:379-415.
- * depths is updated in place (NaN determinant -> -1). */
+ * depths is updated in place (NaN determinant -> -1).  dcinv2d_dcov2ds
+ * (N,3,3), kernel.cu:402-414, for the Gaussians that survive. */
 int rgp_inverse_cov2d(int n, const float* cov2ds, float* depths, float* cinv2ds, int32_t* areas,
-                      hipStream_t stream);
+                      float* dcinv2d_dcov2ds, hipStream_t stream);
 
 /* gsc.sh2Weight, ext.cpp:103-107 / gausplat.cu:425-462 / kernel.cu:670-870.
  * shs (N, sh_dim*K) band-major; twc (3) device.  basis (N, sh_dim) nullable
- * (the calc_J output dweight_dshs); the discarded dW/dpw is not produced. */
+ * (the calc_J output dweight_dshs); dweight_dpws (N,K,3) nullable, the
+ * reference's dW/dpw (kernel.cu:769-869), including its dY_9/dy term
+ * C30 (3x^2 - 2y^2) (kernel.cu:840; the exact derivative is
+ * C30 (3x^2 - 3y^2)). */
 int rgp_sh2weight(int n, int K, int sh_dim, const float* shs, const float* pws, const float* twc,
-                  float* weights, float* basis, hipStream_t stream);
+                  float* weights, float* basis, float* dweight_dpws, hipStream_t stream);
 
 /* gsc.sh2Weight_fast, ext.cpp:116-124 / gausplat.cu:469-509 / kernel.cu:884-987,
  * generalised from rank 1 to rank r: A (N,P,r), B (N,r,Q), P*Q = 15*K.

@@ -190,6 +190,59 @@ def sh2color(shs, pws, twc, dt=f32):
     return out
 
 
+# --------------------------------------------------------------------------
+# calc_J outputs (gausplat.cu:298-462): zero-filled like the reference's
+# torch::full(..., 0.0) allocations, then the written entries
+# --------------------------------------------------------------------------
+def project_jac(pws, Rcw, tcw, fx, fy, dt=f32):
+    """du/dpc (N,2,3)."""
+    pws, R, t = _c(pws, dt), _c(Rcw, dt), _c(tcw, dt)
+    n = pws.shape[0]
+    du = np.zeros((n, 2, 3), dt)
+    getattr(lib(), "orc_project_jac_" + _sfx(dt))(n, _p(pws), _p(R), _p(t), _scal(dt, fx), _scal(dt, fy), _p(du))
+    return du
+
+
+def cov3d_jac(rots, scales, depths, dt=f32):
+    """dcov3d/drot (N,6,4), dcov3d/dscale (N,6,3)."""
+    rots, scales, depths = _c(rots, dt), _c(scales, dt), _c(depths, dt)
+    n = rots.shape[0]
+    dr, ds = np.zeros((n, 6, 4), dt), np.zeros((n, 6, 3), dt)
+    getattr(lib(), "orc_cov3d_jac_" + _sfx(dt))(n, _p(rots), _p(scales), _p(depths), _p(dr), _p(ds))
+    return dr, ds
+
+
+def cov2d_jac(cov3ds, pcs, Rcw, depths, fx, fy, width, height, dt=f32):
+    """dcov2d/dcov3d (N,3,6), dcov2d/dpc (N,3,3)."""
+    cov3ds, pcs, R, depths = _c(cov3ds, dt), _c(pcs, dt), _c(Rcw, dt), _c(depths, dt)
+    n = pcs.shape[0]
+    d3, dp = np.zeros((n, 3, 6), dt), np.zeros((n, 3, 3), dt)
+    tfx, tfy = tan_fov(width, height, fx, fy, dt)
+    getattr(lib(), "orc_cov2d_jac_" + _sfx(dt))(n, _p(cov3ds), _p(pcs), _p(R), _p(depths), _scal(dt, fx),
+                                                _scal(dt, fy), _scal(dt, float(tfx)), _scal(dt, float(tfy)), _p(d3),
+                                                _p(dp))
+    return d3, dp
+
+
+def inverse_jac(cov2ds, depths, dt=f32):
+    """dcinv/dcov2d (N,3,3); depths as returned by inverse_cov2d."""
+    cov2ds, depths = _c(cov2ds, dt), _c(depths, dt)
+    n = cov2ds.shape[0]
+    out = np.zeros((n, 3, 3), dt)
+    getattr(lib(), "orc_inverse_jac_" + _sfx(dt))(n, _p(cov2ds), _p(depths), _p(out))
+    return out
+
+
+def sh2weight_jac(shs, pws, twc, K, dt=f32):
+    """dW/dpw (N,K,3)."""
+    shs, pws, twc = _c(shs, dt), _c(pws, dt), _c(twc, dt)
+    n = pws.shape[0]
+    sh_dim = shs.shape[1] // K
+    out = np.zeros((n, K, 3), dt)
+    getattr(lib(), "orc_sh2weight_jac_" + _sfx(dt))(n, K, sh_dim, _p(shs), _p(pws), _p(twc), _p(out))
+    return out
+
+
 def grid_dims(width, height):
     return (width + BLOCK - 1) // BLOCK, (height + BLOCK - 1) // BLOCK
 

@@ -8,8 +8,13 @@
 //
 // Every function is a template over the arithmetic type T:
 //   T = float  : the reference's FP32 arithmetic, operation by operation, in
-//                source order, without contraction (built with
-//                -ffp-contract=off).  Where the reference multiplies a float by
+//                source order, without implicit contraction (built with
+//                -ffp-contract=off).  The compositing recurrence is the one
+//                place with fused multiply-adds, written out as std::fma:
+//                mahaSqDist's two multiply-adds and the weight update
+//                W += (T alpha') w (see maha_sq and draw), the operations nvcc
+//                contracts by default and the HIP kernels perform as fmaf.
+//                Where the reference multiplies a float by
 //                a `double` SH constant (common.cuh:29-44 are double literals),
 //                the product is formed in double and rounded to float, exactly
 //                as the C++ usual arithmetic conversions prescribe.
@@ -668,6 +673,252 @@ void mix(int K, int64_t HW, const T* pal /*(K,3)*/, const T* W, T* rgb) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// calc_J outputs of A1-A5 (the Jacobians gsplatcu returns with calc_J=True).
+// Products of the small matrices accumulate from 0 over k in ascending order
+// (Matrix::operator*, matrix.cuh:53-70); the other expressions are evaluated
+// left to right as written in the reference, every operation rounded on its
+// own.  Entries the reference leaves unwritten stay as the caller filled them
+// (zeros, gausplat.cu:312-316, 350-354, 392-393, 442-446).
+// ---------------------------------------------------------------------------
+template <typename T>
+static void matmul(const T* A, const T* B, T* C, int I, int K, int J) {
+  for (int i = 0; i < I; ++i)
+    for (int j = 0; j < J; ++j) {
+      T acc = 0;
+      for (int k = 0; k < K; ++k) acc += A[i * K + k] * B[k * J + j];
+      C[i * J + j] = acc;
+    }
+}
+
+// du/dpc (N,2,3) of project (kernel.cu:648-660): entries 0, 2, 4, 5.
+template <typename T>
+void project_jac(int n, const T* pws, const T* R, const T* t, T fx, T fy, T* du) {
+  for (int i = 0; i < n; ++i) {
+    T pc[3];
+    for (int r = 0; r < 3; ++r) {
+      T s = 0;
+      for (int k = 0; k < 3; ++k) s += R[r * 3 + k] * pws[i * 3 + k];
+      pc[r] = s + t[r];
+    }
+    if ((double)pc[2] < MIN_DEPTH) continue;
+    const T z_inv = (T)1 / pc[2];
+    const T z2_inv = z_inv * z_inv;
+    T* o = du + (size_t)i * 6;
+    o[0] = fx * z_inv;
+    o[2] = -(pc[0] * fx) * z2_inv;
+    o[4] = fy * z_inv;
+    o[5] = -(pc[1] * fy) * z2_inv;
+  }
+}
+
+// dcov3d/drot (N,6,4) and dcov3d/dscale (N,6,3) of computeCov3D
+// (kernel.cu:463-500): dSigma/dM (6x9) times dM/dq (9x4) and dM/ds (9x3).
+template <typename T>
+void cov3d_jac(int n, const T* rots, const T* scales, const T* depths, T* drot, T* dscale) {
+  for (int i = 0; i < n; ++i) {
+    if ((double)depths[i] < MIN_DEPTH) continue;
+    const T w = rots[i * 4 + 0], x = rots[i * 4 + 1], y = rots[i * 4 + 2], z = rots[i * 4 + 3];
+    const T s0 = scales[i * 3 + 0], s1 = scales[i * 3 + 1], s2 = scales[i * 3 + 2];
+    const T xx = x * x, yy = y * y, zz = z * z;
+    const T xy = x * y, xz = x * z, yz = y * z;
+    const T xw = x * w, yw = y * w, zw = z * w;
+    const T one = 1, two = 2;
+    const T R[9] = {one - two * (yy + zz), two * (xy - zw), two * (xz + yw),
+                    two * (xy + zw), one - two * (xx + zz), two * (yz - xw),
+                    two * (xz - yw), two * (yz + xw), one - two * (xx + yy)};
+    const T s[3] = {s0, s1, s2};
+    T M[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) M[r * 3 + c] = R[r * 3 + c] * s[c];
+    // d(S00, S01, S02, S11, S12, S22) / d(M row-major)
+    T dS[54] = {0};
+    for (int c = 0; c < 3; ++c) {
+      dS[0 * 9 + c] = two * M[c];                 // S00 = M0.M0
+      dS[1 * 9 + c] = M[3 + c];                   // S01 = M0.M1
+      dS[1 * 9 + 3 + c] = M[c];
+      dS[2 * 9 + c] = M[6 + c];                   // S02 = M0.M2
+      dS[2 * 9 + 6 + c] = M[c];
+      dS[3 * 9 + 3 + c] = two * M[3 + c];         // S11
+      dS[4 * 9 + 3 + c] = M[6 + c];               // S12 = M1.M2
+      dS[4 * 9 + 6 + c] = M[3 + c];
+      dS[5 * 9 + 6 + c] = two * M[6 + c];         // S22
+    }
+    // dM[r][c] / d(w, x, y, z): M[r][c] = R[r][c] s_c
+    const T m4 = -4, m2 = -2;
+    const T dMq[36] = {
+        0, 0, m4 * s0 * y, m4 * s0 * z,                              // M00
+        m2 * s1 * z, two * s1 * y, two * s1 * x, m2 * s1 * w,        // M01
+        two * s2 * y, two * s2 * z, two * s2 * w, two * s2 * x,      // M02
+        two * s0 * z, two * s0 * y, two * s0 * x, two * s0 * w,      // M10
+        0, m4 * s1 * x, 0, m4 * s1 * z,                              // M11
+        m2 * s2 * x, m2 * s2 * w, two * s2 * z, two * s2 * y,        // M12
+        m2 * s0 * y, two * s0 * z, m2 * s0 * w, two * s0 * x,        // M20
+        two * s1 * x, two * s1 * w, two * s1 * z, two * s1 * y,      // M21
+        0, m4 * s2 * x, m4 * s2 * y, 0};                             // M22
+    T dMs[27] = {0};
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) dMs[(r * 3 + c) * 3 + c] = R[r * 3 + c];
+    matmul(dS, dMq, drot + (size_t)i * 24, 6, 9, 4);
+    matmul(dS, dMs, dscale + (size_t)i * 18, 6, 9, 3);
+  }
+}
+
+// dcov2d/dcov3d (N,3,6) and dcov2d/dpc (N,3,3) of computeCov2D
+// (kernel.cu:557-607).  The clamped x, y enter J and dM/dpc; the clamp
+// itself is not differentiated (as in the reference).
+template <typename T>
+void cov2d_jac(int n, const T* cov3ds, const T* pcs, const T* Rcw, const T* depths, T fx, T fy,
+               T tan_fovx, T tan_fovy, T* dcov3d, T* dpc) {
+  for (int i = 0; i < n; ++i) {
+    if ((double)depths[i] < MIN_DEPTH) continue;
+    T x = pcs[i * 3 + 0], y = pcs[i * 3 + 1];
+    const T z = pcs[i * 3 + 2];
+    const T* c3 = cov3ds + i * 6;
+    const T a = c3[0], b = c3[1], c = c3[2], d = c3[3], e = c3[4], f = c3[5];
+    const T limx = (T)1.3f * tan_fovx;
+    const T limy = (T)1.3f * tan_fovy;
+    x = fmn(limx, fmx(-limx, x / z)) * z;
+    y = fmn(limy, fmx(-limy, y / z)) * z;
+    const T z2 = z * z;
+    const T J[6] = {fx / z, (T)0, -(fx * x) / z2, (T)0, fy / z, -(fy * y) / z2};
+    T M[6];
+    matmul(J, Rcw, M, 2, 3, 3);
+    const T two = 2;
+    T* o = dcov3d + (size_t)i * 18;
+    // row r of the 2x2 result pairs rows (p, q) of M: (0,0), (0,1), (1,1)
+    const int pr[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+    for (int r = 0; r < 3; ++r) {
+      const T* P = M + 3 * pr[r][0];
+      const T* Q = M + 3 * pr[r][1];
+      if (pr[r][0] == pr[r][1]) {
+        o[r * 6 + 0] = P[0] * P[0];
+        o[r * 6 + 1] = two * P[0] * P[1];
+        o[r * 6 + 2] = two * P[0] * P[2];
+        o[r * 6 + 3] = P[1] * P[1];
+        o[r * 6 + 4] = two * P[1] * P[2];
+        o[r * 6 + 5] = P[2] * P[2];
+      } else {
+        o[r * 6 + 0] = P[0] * Q[0];
+        o[r * 6 + 1] = P[0] * Q[1] + P[1] * Q[0];
+        o[r * 6 + 2] = P[0] * Q[2] + P[2] * Q[0];
+        o[r * 6 + 3] = P[1] * Q[1];
+        o[r * 6 + 4] = P[1] * Q[2] + P[2] * Q[1];
+        o[r * 6 + 5] = P[2] * Q[2];
+      }
+    }
+    // dcov2d / dM (3x6): rows of Sigma times rows of M
+    const T S[9] = {a, b, c, b, d, e, c, e, f};
+    T dM[18] = {0};
+    for (int cc = 0; cc < 3; ++cc) {
+      const T* Sc = S + 3 * cc;
+      dM[0 * 6 + cc] = two * Sc[0] * M[0] + two * Sc[1] * M[1] + two * Sc[2] * M[2];
+      dM[1 * 6 + cc] = Sc[0] * M[3] + Sc[1] * M[4] + Sc[2] * M[5];
+      dM[1 * 6 + 3 + cc] = Sc[0] * M[0] + Sc[1] * M[1] + Sc[2] * M[2];
+      dM[2 * 6 + 3 + cc] = two * Sc[0] * M[3] + two * Sc[1] * M[4] + two * Sc[2] * M[5];
+    }
+    const T z2_inv = (T)1 / (z * z);
+    const T z3_inv = z2_inv / z;
+    T dMp[18] = {0};
+    for (int cc = 0; cc < 3; ++cc) {
+      const T R0 = Rcw[0 * 3 + cc], R1 = Rcw[1 * 3 + cc], R2 = Rcw[2 * 3 + cc];
+      dMp[cc * 3 + 0] = -fx * R2 * z2_inv;
+      dMp[cc * 3 + 2] = -fx * R0 * z2_inv + two * fx * R2 * x * z3_inv;
+      dMp[(3 + cc) * 3 + 1] = -fy * R2 * z2_inv;
+      dMp[(3 + cc) * 3 + 2] = -fy * R1 * z2_inv + two * fy * R2 * y * z3_inv;
+    }
+    matmul(dM, dMp, dpc + (size_t)i * 9, 3, 6, 3);
+  }
+}
+
+// dcinv/dcov2d (N,3,3) of inverseCov2D (kernel.cu:402-414).
+template <typename T>
+void inverse_jac(int n, const T* cov2ds, const T* depths, T* dcinv) {
+  for (int i = 0; i < n; ++i) {
+    if ((double)depths[i] < MIN_DEPTH) continue;
+    const T a = cov2ds[i * 3 + 0], b = cov2ds[i * 3 + 1], c = cov2ds[i * 3 + 2];
+    const T di = (T)1 / (a * c - b * b);
+    if (std::isnan(di)) continue;
+    const T d2 = di * di, two = 2;
+    T* o = dcinv + (size_t)i * 9;
+    o[0] = -c * c * d2;
+    o[1] = two * b * c * d2;
+    o[2] = -a * c * d2 + di;
+    o[3] = b * c * d2;
+    o[4] = -two * b * b * d2 - di;
+    o[5] = a * b * d2;
+    o[6] = -a * c * d2 + di;
+    o[7] = two * a * b * d2;
+    o[8] = -a * a * d2;
+  }
+}
+
+// dW/dpw (N,K,3) of sh2Weight (kernel.cu:769-869): dW/ddir (from the SH
+// polynomials, SH constants in double as in the reference, each sum added
+// to its float accumulator) times ddir/dpw = (I - dir dir^T) / |d|.
+template <typename T>
+void sh2weight_jac(int n, int K, int sh_dim, const T* shs, const T* pws, const T* twc, T* dwdpw) {
+  typedef double D;
+  for (int i = 0; i < n; ++i) {
+    const T* sh = shs + (size_t)i * sh_dim * K;
+    T* out = dwdpw + (size_t)i * K * 3;
+    if (sh_dim <= 1) {
+      for (int q = 0; q < K * 3; ++q) out[q] = 0;
+      continue;
+    }
+    const T d0 = pws[i * 3 + 0] - twc[0], d1 = pws[i * 3 + 1] - twc[1], d2 = pws[i * 3 + 2] - twc[2];
+    const T ni = (T)1 / std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const T x = d0 * ni, y = d1 * ni, z = d2 * ni;
+    const T ni3 = ni * ni * ni;
+    const T r00 = -d0 * d0 * ni3 + ni, r11 = -d1 * d1 * ni3 + ni, r22 = -d2 * d2 * ni3 + ni;
+    const T r01 = -d0 * d1 * ni3, r02 = -d0 * d2 * ni3, r12 = -d1 * d2 * ni3;
+    const T dr[9] = {r00, r01, r02, r01, r11, r12, r02, r12, r22};
+    const T xx = x * x, yy = y * y, zz = z * z;
+    for (int k = 0; k < K; ++k) {
+      auto S = [&](int l) { return (D)sh[l * K + k]; };
+      T g0 = (T)((D)(T)0 + SH_C1_2 * S(3));
+      T g1 = (T)((D)(T)0 + SH_C1_0 * S(1));
+      T g2 = (T)((D)(T)0 + SH_C1_1 * S(2));
+      if (sh_dim > 4) {
+        g0 = (T)((D)g0 + ((((SH_C2_0 * (D)y) * S(4) - ((SH_C2_2 * 2) * (D)x) * S(6)) +
+                          (SH_C2_3 * (D)z) * S(7)) + ((SH_C2_4 * 2) * (D)x) * S(8)));
+        g1 = (T)((D)g1 + ((((SH_C2_0 * (D)x) * S(4) + (SH_C2_1 * (D)z) * S(5)) -
+                          ((SH_C2_2 * 2) * (D)y) * S(6)) - ((SH_C2_4 * 2) * (D)y) * S(8)));
+        g2 = (T)((D)g2 + (((SH_C2_1 * (D)y) * S(5) + ((SH_C2_2 * 4) * (D)z) * S(6)) +
+                          (SH_C2_3 * (D)x) * S(7)));
+        if (sh_dim > 9) {
+          const T p0 = (T)4 * zz - (T)3 * xx - yy, p1 = (T)3 * xx - (T)3 * yy;
+          const T q0 = (T)3 * xx - (T)2 * yy, q1 = (T)4 * zz - xx - (T)3 * yy;
+          const T u0 = (T)6 * zz - (T)3 * xx - (T)3 * yy, u1 = xx - yy;
+          D a0 = (((6.0 * SH_C3_0) * (D)x) * (D)y) * S(9);
+          a0 = a0 + ((SH_C3_1 * (D)y) * (D)z) * S(10);
+          a0 = a0 - (((2.0 * SH_C3_2) * (D)x) * (D)y) * S(11);
+          a0 = a0 - (((6.0 * SH_C3_3) * (D)x) * (D)z) * S(12);
+          a0 = a0 + (SH_C3_4 * S(13)) * (D)p0;
+          a0 = a0 + (((2.0 * SH_C3_5) * (D)x) * (D)z) * S(14);
+          a0 = a0 + (SH_C3_6 * S(15)) * (D)p1;
+          D a1 = (SH_C3_0 * S(9)) * (D)q0;
+          a1 = a1 + ((SH_C3_1 * (D)x) * (D)z) * S(10);
+          a1 = a1 + (SH_C3_2 * S(11)) * (D)q1;
+          a1 = a1 - (((6.0 * SH_C3_3) * (D)y) * (D)z) * S(12);
+          a1 = a1 - (((2.0 * SH_C3_4) * (D)x) * (D)y) * S(13);
+          a1 = a1 - (((2.0 * SH_C3_5) * (D)y) * (D)z) * S(14);
+          a1 = a1 - (((6.0 * SH_C3_6) * (D)x) * (D)y) * S(15);
+          D a2 = ((SH_C3_1 * (D)x) * (D)y) * S(10);
+          a2 = a2 + (((8.0 * SH_C3_2) * (D)y) * (D)z) * S(11);
+          a2 = a2 + (SH_C3_3 * S(12)) * (D)u0;
+          a2 = a2 + (((8.0 * SH_C3_4) * (D)x) * (D)z) * S(13);
+          a2 = a2 + (SH_C3_5 * S(14)) * (D)u1;
+          g0 = (T)((D)g0 + a0);
+          g1 = (T)((D)g1 + a1);
+          g2 = (T)((D)g2 + a2);
+        }
+      }
+      for (int j = 0; j < 3; ++j) out[k * 3 + j] = g0 * dr[0 * 3 + j] + g1 * dr[1 * 3 + j] + g2 * dr[2 * 3 + j];
+    }
+  }
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------
@@ -727,6 +978,24 @@ void mix(int K, int64_t HW, const T* pal /*(K,3)*/, const T* W, T* rgb) {
   }                                                                                               \
   void orc_mix_##S(int K, int64_t HW, const T* pal, const T* W, T* rgb) {                         \
     orc::mix<T>(K, HW, pal, W, rgb);                                                              \
+  }                                                                                               \
+  void orc_project_jac_##S(int n, const T* pws, const T* R, const T* t, T fx, T fy, T* du) {      \
+    orc::project_jac<T>(n, pws, R, t, fx, fy, du);                                                \
+  }                                                                                               \
+  void orc_cov3d_jac_##S(int n, const T* rots, const T* scales, const T* depths, T* drot,         \
+                         T* dscale) {                                                             \
+    orc::cov3d_jac<T>(n, rots, scales, depths, drot, dscale);                                     \
+  }                                                                                               \
+  void orc_cov2d_jac_##S(int n, const T* cov3ds, const T* pcs, const T* Rcw, const T* depths,     \
+                         T fx, T fy, T tfx, T tfy, T* dcov3d, T* dpc) {                           \
+    orc::cov2d_jac<T>(n, cov3ds, pcs, Rcw, depths, fx, fy, tfx, tfy, dcov3d, dpc);                \
+  }                                                                                               \
+  void orc_inverse_jac_##S(int n, const T* cov2ds, const T* depths, T* dcinv) {                   \
+    orc::inverse_jac<T>(n, cov2ds, depths, dcinv);                                                \
+  }                                                                                               \
+  void orc_sh2weight_jac_##S(int n, int K, int sh_dim, const T* shs, const T* pws, const T* twc,  \
+                             T* dwdpw) {                                                          \
+    orc::sh2weight_jac<T>(n, K, sh_dim, shs, pws, twc, dwdpw);                                    \
   }
 
 extern "C" {

@@ -480,6 +480,289 @@ __global__ __launch_bounds__(256) void k_sh_backward_full(int n, int sh_dim,
 }
 
 // ---------------------------------------------------------------------------
+// calc_J outputs of the unfused operators (kernel.cu:402-414, 463-500,
+// 557-607, 648-660, 769-869).  Small matrix products accumulate from 0 over
+// k ascending with every term (Matrix::operator*, matrix.cuh:53-70); other
+// expressions are evaluated left to right as the reference writes them.
+// Entries the reference does not write are left to the caller's zero fill.
+// ---------------------------------------------------------------------------
+template <int I, int K, int J>
+__device__ __forceinline__ void small_matmul(const float* A, const float* B, float* C) {
+#pragma unroll
+  for (int i = 0; i < I; ++i)
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc = acc + A[i * K + k] * B[k * J + j];
+      C[i * J + j] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_project_jac(int n, const float* __restrict__ pws,
+                                                     const float* __restrict__ Rcw,
+                                                     const float* __restrict__ tcw, float fx,
+                                                     float fy, float* __restrict__ du) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float R[9], t[3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = Rcw[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) t[k] = tcw[k];
+  const Proj p = project_one(R, t, pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], fx, fy, 0.f, 0.f);
+  if (!p.ok) return;
+  const float z_inv = 1.f / p.z;
+  const float z2_inv = z_inv * z_inv;
+  float* o = du + (size_t)i * 6;
+  o[0] = fx * z_inv;
+  o[2] = -(p.x * fx) * z2_inv;
+  o[4] = fy * z_inv;
+  o[5] = -(p.y * fy) * z2_inv;
+}
+
+// dcov3d/drot (N,6,4), dcov3d/dscale (N,6,3): dSigma/dM (6x9) times dM/dq
+// and dM/ds, M = R(q) diag(s) with the unnormalised quaternion (w, x, y, z).
+__global__ __launch_bounds__(256) void k_cov3d_jac(int n, const float4* __restrict__ rots,
+                                                   const float* __restrict__ scales,
+                                                   const float* __restrict__ depths,
+                                                   float* __restrict__ drot,
+                                                   float* __restrict__ dscale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if ((double)depths[i] < kMinDepth) return;
+  const float4 q = rots[i];
+  const float w = q.x, x = q.y, y = q.z, z = q.w;
+  const float s0 = scales[3 * i], s1 = scales[3 * i + 1], s2 = scales[3 * i + 2];
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float xy = x * y, xz = x * z, yz = y * z;
+  const float xw = x * w, yw = y * w, zw = z * w;
+  const float R[9] = {1.f - 2.f * (yy + zz), 2.f * (xy - zw), 2.f * (xz + yw),
+                      2.f * (xy + zw), 1.f - 2.f * (xx + zz), 2.f * (yz - xw),
+                      2.f * (xz - yw), 2.f * (yz + xw), 1.f - 2.f * (xx + yy)};
+  const float s[3] = {s0, s1, s2};
+  float M[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) M[r * 3 + c] = R[r * 3 + c] * s[c];
+  float dS[54];
+#pragma unroll
+  for (int k = 0; k < 54; ++k) dS[k] = 0.f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    dS[0 * 9 + c] = 2.f * M[c];
+    dS[1 * 9 + c] = M[3 + c];
+    dS[1 * 9 + 3 + c] = M[c];
+    dS[2 * 9 + c] = M[6 + c];
+    dS[2 * 9 + 6 + c] = M[c];
+    dS[3 * 9 + 3 + c] = 2.f * M[3 + c];
+    dS[4 * 9 + 3 + c] = M[6 + c];
+    dS[4 * 9 + 6 + c] = M[3 + c];
+    dS[5 * 9 + 6 + c] = 2.f * M[6 + c];
+  }
+  const float dMq[36] = {
+      0.f, 0.f, -4.f * s0 * y, -4.f * s0 * z,
+      -2.f * s1 * z, 2.f * s1 * y, 2.f * s1 * x, -2.f * s1 * w,
+      2.f * s2 * y, 2.f * s2 * z, 2.f * s2 * w, 2.f * s2 * x,
+      2.f * s0 * z, 2.f * s0 * y, 2.f * s0 * x, 2.f * s0 * w,
+      0.f, -4.f * s1 * x, 0.f, -4.f * s1 * z,
+      -2.f * s2 * x, -2.f * s2 * w, 2.f * s2 * z, 2.f * s2 * y,
+      -2.f * s0 * y, 2.f * s0 * z, -2.f * s0 * w, 2.f * s0 * x,
+      2.f * s1 * x, 2.f * s1 * w, 2.f * s1 * z, 2.f * s1 * y,
+      0.f, -4.f * s2 * x, -4.f * s2 * y, 0.f};
+  float dMs[27];
+#pragma unroll
+  for (int k = 0; k < 27; ++k) dMs[k] = 0.f;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dMs[(r * 3 + c) * 3 + c] = R[r * 3 + c];
+  float out_r[24], out_s[18];
+  small_matmul<6, 9, 4>(dS, dMq, out_r);
+  small_matmul<6, 9, 3>(dS, dMs, out_s);
+#pragma unroll
+  for (int k = 0; k < 24; ++k) drot[(size_t)i * 24 + k] = out_r[k];
+#pragma unroll
+  for (int k = 0; k < 18; ++k) dscale[(size_t)i * 18 + k] = out_s[k];
+}
+
+// dcov2d/dcov3d (N,3,6), dcov2d/dpc (N,3,3); the clamped x, y enter J and
+// dM/dpc (the clamp is not differentiated, as in the reference).
+__global__ __launch_bounds__(256) void k_cov2d_jac(int n, const float* __restrict__ cov3ds,
+                                                   const float* __restrict__ pcs,
+                                                   const float* __restrict__ Rcw,
+                                                   const float* __restrict__ depths, float fx,
+                                                   float fy, float tfx, float tfy,
+                                                   float* __restrict__ d3, float* __restrict__ dp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if ((double)depths[i] < kMinDepth) return;
+  float R[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = Rcw[k];
+  float x = pcs[3 * i], y = pcs[3 * i + 1];
+  const float z = pcs[3 * i + 2];
+  const float* c3 = cov3ds + (size_t)i * 6;
+  const float a = c3[0], b = c3[1], c = c3[2], d = c3[3], e = c3[4], f = c3[5];
+  const float limx = 1.3f * tfx;
+  const float limy = 1.3f * tfy;
+  x = fminf(limx, fmaxf(-limx, x / z)) * z;
+  y = fminf(limy, fmaxf(-limy, y / z)) * z;
+  const float z2 = z * z;
+  const float J[6] = {fx / z, 0.0f, -(fx * x) / z2, 0.0f, fy / z, -(fy * y) / z2};
+  float M[6];
+  small_matmul<2, 3, 3>(J, R, M);
+  float o[18];
+  // rows: (M0, M0), (M0, M1), (M1, M1)
+  o[0] = M[0] * M[0];
+  o[1] = 2.f * M[0] * M[1];
+  o[2] = 2.f * M[0] * M[2];
+  o[3] = M[1] * M[1];
+  o[4] = 2.f * M[1] * M[2];
+  o[5] = M[2] * M[2];
+  o[6] = M[0] * M[3];
+  o[7] = M[0] * M[4] + M[1] * M[3];
+  o[8] = M[0] * M[5] + M[2] * M[3];
+  o[9] = M[1] * M[4];
+  o[10] = M[1] * M[5] + M[2] * M[4];
+  o[11] = M[2] * M[5];
+  o[12] = M[3] * M[3];
+  o[13] = 2.f * M[3] * M[4];
+  o[14] = 2.f * M[3] * M[5];
+  o[15] = M[4] * M[4];
+  o[16] = 2.f * M[4] * M[5];
+  o[17] = M[5] * M[5];
+#pragma unroll
+  for (int k = 0; k < 18; ++k) d3[(size_t)i * 18 + k] = o[k];
+  const float S[9] = {a, b, c, b, d, e, c, e, f};
+  float dM[18];
+#pragma unroll
+  for (int k = 0; k < 18; ++k) dM[k] = 0.f;
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    const float* Sc = S + 3 * cc;
+    dM[cc] = 2.f * Sc[0] * M[0] + 2.f * Sc[1] * M[1] + 2.f * Sc[2] * M[2];
+    dM[6 + cc] = Sc[0] * M[3] + Sc[1] * M[4] + Sc[2] * M[5];
+    dM[6 + 3 + cc] = Sc[0] * M[0] + Sc[1] * M[1] + Sc[2] * M[2];
+    dM[12 + 3 + cc] = 2.f * Sc[0] * M[3] + 2.f * Sc[1] * M[4] + 2.f * Sc[2] * M[5];
+  }
+  const float z2_inv = 1.f / (z * z);
+  const float z3_inv = z2_inv / z;
+  float dMp[18];
+#pragma unroll
+  for (int k = 0; k < 18; ++k) dMp[k] = 0.f;
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    const float R0 = R[cc], R1 = R[3 + cc], R2 = R[6 + cc];
+    dMp[cc * 3 + 0] = -fx * R2 * z2_inv;
+    dMp[cc * 3 + 2] = -fx * R0 * z2_inv + 2.f * fx * R2 * x * z3_inv;
+    dMp[(3 + cc) * 3 + 1] = -fy * R2 * z2_inv;
+    dMp[(3 + cc) * 3 + 2] = -fy * R1 * z2_inv + 2.f * fy * R2 * y * z3_inv;
+  }
+  float out[9];
+  small_matmul<3, 6, 3>(dM, dMp, out);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dp[(size_t)i * 9 + k] = out[k];
+}
+
+__global__ __launch_bounds__(256) void k_inverse_jac(int n, const float* __restrict__ cov2ds,
+                                                     const float* __restrict__ depths,
+                                                     float* __restrict__ dcinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if ((double)depths[i] < kMinDepth) return;
+  const float a = cov2ds[3 * i], b = cov2ds[3 * i + 1], c = cov2ds[3 * i + 2];
+  const float di = 1.f / (a * c - b * b);
+  if (di != di) return;
+  const float d2 = di * di;
+  float* o = dcinv + (size_t)i * 9;
+  o[0] = -c * c * d2;
+  o[1] = 2.f * b * c * d2;
+  o[2] = -a * c * d2 + di;
+  o[3] = b * c * d2;
+  o[4] = -2.f * b * b * d2 - di;
+  o[5] = a * b * d2;
+  o[6] = -a * c * d2 + di;
+  o[7] = 2.f * a * b * d2;
+  o[8] = -a * a * d2;
+}
+
+// dW/dpw (N,K,3): dW/ddir from the SH polynomials (double constants, each
+// band's sum added to the float accumulator, as in the reference; including
+// its dY_9/dy = C30 (3x^2 - 2y^2), kernel.cu:840) times
+// ddir/dpw = (I - dir dir^T) / |d|.
+template <int K>
+__global__ __launch_bounds__(256) void k_sh2weight_jac(int n, int sh_dim,
+                                                       const float* __restrict__ shs,
+                                                       const float* __restrict__ pws,
+                                                       const float* __restrict__ twc,
+                                                       float* __restrict__ out_all) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* out = out_all + (size_t)i * K * 3;
+  if (sh_dim <= 1) {
+#pragma unroll
+    for (int q = 0; q < K * 3; ++q) out[q] = 0.f;
+    return;
+  }
+  const float* sh = shs + (size_t)i * sh_dim * K;
+  const float d0 = pws[3 * i] - twc[0], d1 = pws[3 * i + 1] - twc[1], d2 = pws[3 * i + 2] - twc[2];
+  const float ni = 1.f / sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+  const float x = d0 * ni, y = d1 * ni, z = d2 * ni;
+  const float ni3 = ni * ni * ni;
+  const float r00 = -d0 * d0 * ni3 + ni, r11 = -d1 * d1 * ni3 + ni, r22 = -d2 * d2 * ni3 + ni;
+  const float r01 = -d0 * d1 * ni3, r02 = -d0 * d2 * ni3, r12 = -d1 * d2 * ni3;
+  const float dr[9] = {r00, r01, r02, r01, r11, r12, r02, r12, r22};
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const double X = x, Yd = y, Z = z;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    auto S = [&](int l) { return (double)sh[l * K + k]; };
+    float g0 = (float)(0.0 + SH_C1_2 * S(3));
+    float g1 = (float)(0.0 + SH_C1_0 * S(1));
+    float g2 = (float)(0.0 + SH_C1_1 * S(2));
+    if (sh_dim > 4) {
+      g0 = (float)((double)g0 + ((((SH_C2_0 * Yd) * S(4) - ((SH_C2_2 * 2) * X) * S(6)) +
+                                 (SH_C2_3 * Z) * S(7)) + ((SH_C2_4 * 2) * X) * S(8)));
+      g1 = (float)((double)g1 + ((((SH_C2_0 * X) * S(4) + (SH_C2_1 * Z) * S(5)) -
+                                 ((SH_C2_2 * 2) * Yd) * S(6)) - ((SH_C2_4 * 2) * Yd) * S(8)));
+      g2 = (float)((double)g2 + (((SH_C2_1 * Yd) * S(5) + ((SH_C2_2 * 4) * Z) * S(6)) +
+                                 (SH_C2_3 * X) * S(7)));
+      if (sh_dim > 9) {
+        const float p0 = 4.f * zz - 3.f * xx - yy, p1 = 3.f * xx - 3.f * yy;
+        const float q0 = 3.f * xx - 2.f * yy, q1 = 4.f * zz - xx - 3.f * yy;
+        const float u0 = 6.f * zz - 3.f * xx - 3.f * yy, u1 = xx - yy;
+        double a0 = (((6.0 * SH_C3_0) * X) * Yd) * S(9);
+        a0 = a0 + ((SH_C3_1 * Yd) * Z) * S(10);
+        a0 = a0 - (((2.0 * SH_C3_2) * X) * Yd) * S(11);
+        a0 = a0 - (((6.0 * SH_C3_3) * X) * Z) * S(12);
+        a0 = a0 + (SH_C3_4 * S(13)) * (double)p0;
+        a0 = a0 + (((2.0 * SH_C3_5) * X) * Z) * S(14);
+        a0 = a0 + (SH_C3_6 * S(15)) * (double)p1;
+        double a1 = (SH_C3_0 * S(9)) * (double)q0;
+        a1 = a1 + ((SH_C3_1 * X) * Z) * S(10);
+        a1 = a1 + (SH_C3_2 * S(11)) * (double)q1;
+        a1 = a1 - (((6.0 * SH_C3_3) * Yd) * Z) * S(12);
+        a1 = a1 - (((2.0 * SH_C3_4) * X) * Yd) * S(13);
+        a1 = a1 - (((2.0 * SH_C3_5) * Yd) * Z) * S(14);
+        a1 = a1 - (((6.0 * SH_C3_6) * X) * Yd) * S(15);
+        double a2 = ((SH_C3_1 * X) * Yd) * S(10);
+        a2 = a2 + (((8.0 * SH_C3_2) * Yd) * Z) * S(11);
+        a2 = a2 + (SH_C3_3 * S(12)) * (double)u0;
+        a2 = a2 + (((8.0 * SH_C3_4) * X) * Z) * S(13);
+        a2 = a2 + (SH_C3_5 * S(14)) * (double)u1;
+        g0 = (float)((double)g0 + a0);
+        g1 = (float)((double)g1 + a1);
+        g2 = (float)((double)g2 + a2);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) out[k * 3 + j] = g0 * dr[j] + g1 * dr[3 + j] + g2 * dr[6 + j];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Full-rank SH gradient fused with the Adam update of the coefficients: every
 // coefficient's gradient g = dw[n,k] * Y_lm(n) (the product gsmodel.py:135-144
 // forms) is used where it is made, so dL/dshs is never stored.  The update is
@@ -562,42 +845,61 @@ static inline int grid1(int n) { return (n + 255) / 256; }
 
 extern "C" int rgp_project(int n, const float* pws, const float* Rcw, const float* tcw, float fx,
                            float fy, float cx, float cy, float* us, float* pcs, float* depths,
-                           hipStream_t s) {
+                           float* du_dpcs, hipStream_t s) {
   if (n < 0) return set_error(RGP_ERR_ARG, "rgp_project: n < 0");
   if (n == 0) return RGP_OK;
   hipLaunchKernelGGL(k_project, dim3(grid1(n)), dim3(256), 0, s, n, pws, Rcw, tcw, fx, fy, cx, cy,
                      us, pcs, depths);
+  if (du_dpcs)
+    hipLaunchKernelGGL(k_project_jac, dim3(grid1(n)), dim3(256), 0, s, n, pws, Rcw, tcw, fx, fy,
+                       du_dpcs);
   return check_launch("rgp_project");
 }
 
 extern "C" int rgp_compute_cov3d(int n, const float* rots, const float* scales,
-                                 const float* depths, float* cov3ds, hipStream_t s) {
+                                 const float* depths, float* cov3ds, float* dcov3d_drots,
+                                 float* dcov3d_dscales, hipStream_t s) {
   if (n < 0) return set_error(RGP_ERR_ARG, "rgp_compute_cov3d: n < 0");
+  if (!dcov3d_drots != !dcov3d_dscales)
+    return set_error(RGP_ERR_ARG, "rgp_compute_cov3d: give both Jacobians or neither");
   if (n == 0) return RGP_OK;
   hipLaunchKernelGGL(k_cov3d, dim3(grid1(n)), dim3(256), 0, s, n, (const float4*)rots, scales,
                      depths, cov3ds);
+  if (dcov3d_drots)
+    hipLaunchKernelGGL(k_cov3d_jac, dim3(grid1(n)), dim3(256), 0, s, n, (const float4*)rots, scales,
+                       depths, dcov3d_drots, dcov3d_dscales);
   return check_launch("rgp_compute_cov3d");
 }
 
 extern "C" int rgp_compute_cov2d(int n, const float* cov3ds, const float* pcs, const float* Rcw,
                                  const float* depths, float fx, float fy, float width,
-                                 float height, float* cov2ds, hipStream_t s) {
+                                 float height, float* cov2ds, float* dcov2d_dcov3ds,
+                                 float* dcov2d_dpcs, hipStream_t s) {
   if (n < 0) return set_error(RGP_ERR_ARG, "rgp_compute_cov2d: n < 0");
+  if (!dcov2d_dcov3ds != !dcov2d_dpcs)
+    return set_error(RGP_ERR_ARG, "rgp_compute_cov2d: give both Jacobians or neither");
   if (n == 0) return RGP_OK;
   // gausplat.cu:357-358, in float
   const float tfx = width / (2.f * fx);
   const float tfy = height / (2.f * fy);
   hipLaunchKernelGGL(k_cov2d, dim3(grid1(n)), dim3(256), 0, s, n, cov3ds, pcs, Rcw, depths, fx,
                      fy, tfx, tfy, cov2ds);
+  if (dcov2d_dcov3ds)
+    hipLaunchKernelGGL(k_cov2d_jac, dim3(grid1(n)), dim3(256), 0, s, n, cov3ds, pcs, Rcw, depths, fx,
+                       fy, tfx, tfy, dcov2d_dcov3ds, dcov2d_dpcs);
   return check_launch("rgp_compute_cov2d");
 }
 
 extern "C" int rgp_inverse_cov2d(int n, const float* cov2ds, float* depths, float* cinv2ds,
-                                 int32_t* areas, hipStream_t s) {
+                                 int32_t* areas, float* dcinv2d_dcov2ds, hipStream_t s) {
   if (n < 0) return set_error(RGP_ERR_ARG, "rgp_inverse_cov2d: n < 0");
   if (n == 0) return RGP_OK;
   hipLaunchKernelGGL(k_inverse, dim3(grid1(n)), dim3(256), 0, s, n, cov2ds, depths, cinv2ds,
                      areas);
+  // after the cull: depths already holds -1 for a NaN determinant
+  if (dcinv2d_dcov2ds)
+    hipLaunchKernelGGL(k_inverse_jac, dim3(grid1(n)), dim3(256), 0, s, n, cov2ds, depths,
+                       dcinv2d_dcov2ds);
   return check_launch("rgp_inverse_cov2d");
 }
 
@@ -635,7 +937,8 @@ static void launch_lowrank_backward(int n, int K, int P, int Q, int rank, const 
 }  // namespace rgp
 
 extern "C" int rgp_sh2weight(int n, int K, int sh_dim, const float* shs, const float* pws,
-                             const float* twc, float* weights, float* basis, hipStream_t s) {
+                             const float* twc, float* weights, float* basis, float* dweight_dpws,
+                             hipStream_t s) {
   if (n < 0 || K < 1 || K > RGP_MAX_DIM)
     return set_error(RGP_ERR_ARG, "rgp_sh2weight: need n >= 0 and 1 <= K <= 8");
   if (!(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
@@ -643,6 +946,9 @@ extern "C" int rgp_sh2weight(int n, int K, int sh_dim, const float* shs, const f
   if (n == 0) return RGP_OK;
   RGP_K_SWITCH(K, hipLaunchKernelGGL(k_sh2weight<KK>, dim3(grid1(n)), dim3(256), 0, s, n, sh_dim,
                                      shs, pws, twc, weights, basis));
+  if (dweight_dpws)
+    RGP_K_SWITCH(K, hipLaunchKernelGGL(k_sh2weight_jac<KK>, dim3(grid1(n)), dim3(256), 0, s, n,
+                                       sh_dim, shs, pws, twc, dweight_dpws));
   return check_launch("rgp_sh2weight");
 }
 

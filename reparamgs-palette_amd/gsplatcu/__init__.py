@@ -15,8 +15,11 @@ Differences from the reference, all deliberate:
   reference's rank-1 (N,P), (N,Q), and calc_J=False returns the weights
   instead of dereferencing null (kernel.cu:911, 961);
 * the calc_J Jacobians of project / computeCov3D / computeCov2D /
-  inverseCov2D / sh2Color are not provided (the palette path never asks for
-  them: gsplat/gsmodel.py:107-115, 155-166); asking raises.
+  inverseCov2D / sh2Weight are computed by kernels that follow the
+  reference's formulas (kernel.cu:402-414, 463-500, 557-607, 656-660,
+  769-869), including sh2Weight's dY_9/dy = C30 (3x^2 - 2y^2)
+  (kernel.cu:840); sh2Color's (colour path, kernel.cu:1114-1167) is not
+  provided and asking raises.
 """
 import torch
 
@@ -48,56 +51,59 @@ def _inplace_i32(t):
 
 
 def project(pws, Rcw, tcw, focal_x, focal_y, center_x, center_y, calc_J):
-    """gausplat.cu:381-419."""
-    if calc_J:
-        _no_jacobian("project")
+    """gausplat.cu:381-419; calc_J adds du/dpc (N,2,3)."""
     require_cuda(pws, Rcw, tcw)
     n = pws.shape[0]
     dev = pws.device
     us = torch.zeros((n, 2), dtype=torch.float32, device=dev)
     pcs = torch.zeros((n, 3), dtype=torch.float32, device=dev)
     depths = torch.zeros((n,), dtype=torch.float32, device=dev)
+    du = torch.zeros((n, 2, 3), dtype=torch.float32, device=dev) if calc_J else None
     call("rgp_project", n, ptr(f32c(pws)), ptr(f32c(Rcw)), ptr(f32c(tcw)), float(focal_x), float(focal_y),
-         float(center_x), float(center_y), ptr(us), ptr(pcs), ptr(depths), stream(dev))
-    return [us, pcs, depths]
+         float(center_x), float(center_y), ptr(us), ptr(pcs), ptr(depths), ptr(du), stream(dev))
+    return [us, pcs, depths, du] if calc_J else [us, pcs, depths]
 
 
 def computeCov3D(rots, scales, depths, calc_J):
-    """gausplat.cu:298-331."""
-    if calc_J:
-        _no_jacobian("computeCov3D")
+    """gausplat.cu:298-331; calc_J adds dcov3d/drot (N,6,4) and
+    dcov3d/dscale (N,6,3)."""
     require_cuda(rots, scales, depths)
     n = rots.shape[0]
-    cov3ds = torch.zeros((n, 6), dtype=torch.float32, device=rots.device)
-    call("rgp_compute_cov3d", n, ptr(f32c(rots)), ptr(f32c(scales)), ptr(f32c(depths)), ptr(cov3ds),
-         stream(rots.device))
-    return [cov3ds]
+    dev = rots.device
+    cov3ds = torch.zeros((n, 6), dtype=torch.float32, device=dev)
+    dr = torch.zeros((n, 6, 4), dtype=torch.float32, device=dev) if calc_J else None
+    ds = torch.zeros((n, 6, 3), dtype=torch.float32, device=dev) if calc_J else None
+    call("rgp_compute_cov3d", n, ptr(f32c(rots)), ptr(f32c(scales)), ptr(f32c(depths)), ptr(cov3ds), ptr(dr),
+         ptr(ds), stream(dev))
+    return [cov3ds, dr, ds] if calc_J else [cov3ds]
 
 
 def computeCov2D(cov3ds, pcs, Rcw, depths, focal_x, focal_y, width, height, calc_J):
-    """gausplat.cu:333-379 (width/height arrive as floats)."""
-    if calc_J:
-        _no_jacobian("computeCov2D")
+    """gausplat.cu:333-379 (width/height arrive as floats); calc_J adds
+    dcov2d/dcov3d (N,3,6) and dcov2d/dpc (N,3,3)."""
     require_cuda(cov3ds, pcs, Rcw, depths)
     n = pcs.shape[0]
-    cov2ds = torch.zeros((n, 3), dtype=torch.float32, device=pcs.device)
+    dev = pcs.device
+    cov2ds = torch.zeros((n, 3), dtype=torch.float32, device=dev)
+    d3 = torch.zeros((n, 3, 6), dtype=torch.float32, device=dev) if calc_J else None
+    dp = torch.zeros((n, 3, 3), dtype=torch.float32, device=dev) if calc_J else None
     call("rgp_compute_cov2d", n, ptr(f32c(cov3ds)), ptr(f32c(pcs)), ptr(f32c(Rcw)), ptr(f32c(depths)),
-         float(focal_x), float(focal_y), float(width), float(height), ptr(cov2ds), stream(pcs.device))
-    return [cov2ds]
+         float(focal_x), float(focal_y), float(width), float(height), ptr(cov2ds), ptr(d3), ptr(dp), stream(dev))
+    return [cov2ds, d3, dp] if calc_J else [cov2ds]
 
 
 def inverseCov2D(cov2ds, depths, calc_J):
-    """gausplat.cu:590-617; depths updated in place."""
-    if calc_J:
-        _no_jacobian("inverseCov2D")
+    """gausplat.cu:590-617; depths updated in place; calc_J adds
+    dcinv/dcov2d (N,3,3)."""
     require_cuda(cov2ds, depths)
     n = cov2ds.shape[0]
     dev = cov2ds.device
     cinv2ds = torch.zeros((n, 3), dtype=torch.float32, device=dev)
     areas = torch.zeros((n, 2), dtype=torch.int32, device=dev)
+    dci = torch.zeros((n, 3, 3), dtype=torch.float32, device=dev) if calc_J else None
     d = _inplace_f32(depths)
-    call("rgp_inverse_cov2d", n, ptr(f32c(cov2ds)), ptr(d), ptr(cinv2ds), ptr(areas), stream(dev))
-    return [cinv2ds, areas]
+    call("rgp_inverse_cov2d", n, ptr(f32c(cov2ds)), ptr(d), ptr(cinv2ds), ptr(areas), ptr(dci), stream(dev))
+    return [cinv2ds, areas, dci] if calc_J else [cinv2ds, areas]
 
 
 def sh2Weight(shs, pws, twc, calc_J, num_palette):
@@ -110,45 +116,10 @@ def sh2Weight(shs, pws, twc, calc_J, num_palette):
     dev = pws.device
     weights = torch.zeros((n, K), dtype=torch.float32, device=dev)
     basis = torch.zeros((n, 1, sh_dim), dtype=torch.float32, device=dev) if calc_J else None
+    dwdp = torch.zeros((n, K, 3), dtype=torch.float32, device=dev) if calc_J else None
     call("rgp_sh2weight", n, K, sh_dim, ptr(f32c(shs)), ptr(f32c(pws)), ptr(f32c(twc)), ptr(weights),
-         ptr(basis), stream(dev))
-    if not calc_J:
-        return [weights]
-    return [weights, basis, _dweight_dpws(shs, pws, twc, K, sh_dim)]
-
-
-def _dweight_dpws(shs, pws, twc, K, sh_dim):
-    """dW/dpw (N,K,3) of kernel.cu:769-869: the Jacobian of the weights with
-    respect to the Gaussian position through the view direction.  Computed
-    with torch autograd over the same basis in float32 (the palette path
-    discards it, gsmodel.py:113)."""
-    with torch.enable_grad():
-        p = pws.detach().float().clone().requires_grad_(True)
-        d = p - twc.float().reshape(1, 3)
-        dirs = d / torch.sqrt((d * d).sum(1, keepdim=True))
-        Y = _sh_basis_torch(dirs, sh_dim)
-        coef = shs.detach().float().reshape(-1, sh_dim, K)
-        w = (Y[:, :, None] * coef).sum(1)
-        cols = []
-        for k in range(K):
-            g, = torch.autograd.grad(w[:, k].sum(), p, retain_graph=k < K - 1)
-            cols.append(g)
-    return torch.stack(cols, dim=1).contiguous()
-
-
-def _sh_basis_torch(dirs, sh_dim):
-    x, y, z = dirs[:, 0], dirs[:, 1], dirs[:, 2]
-    C = [0.28209479177387814, -0.4886025119029199, 0.4886025119029199, -0.4886025119029199,
-         1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792,
-         0.5462742152960396, -0.5900435899266435, 2.890611442640554, -0.4570457994644658,
-         0.3731763325901154, -0.4570457994644658, 1.445305721320277, -0.5900435899266435]
-    xx, yy, zz = x * x, y * y, z * z
-    cols = [torch.full_like(x, C[0]), C[1] * y, C[2] * z, C[3] * x,
-            C[4] * x * y, C[5] * y * z, C[6] * (2 * zz - xx - yy), C[7] * x * z, C[8] * (xx - yy),
-            C[9] * y * (3 * xx - yy), C[10] * x * y * z, C[11] * y * (4 * zz - xx - yy),
-            C[12] * z * (2 * zz - 3 * xx - 3 * yy), C[13] * x * (4 * zz - xx - yy), C[14] * z * (xx - yy),
-            C[15] * x * (xx - 3 * yy)]
-    return torch.stack(cols[:sh_dim], dim=1)
+         ptr(basis), ptr(dwdp), stream(dev))
+    return [weights, basis, dwdp] if calc_J else [weights]
 
 
 def _factors(shs_A, shs_B, K, Q):

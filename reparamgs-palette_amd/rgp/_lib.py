@@ -24,11 +24,11 @@ SZ = ctypes.c_size_t
 _SIGS = {
     "rgp_last_error": (ctypes.c_char_p, []),
     "rgp_version": (I, []),
-    "rgp_project": (I, [I, P, P, P, F, F, F, F, P, P, P, P]),
-    "rgp_compute_cov3d": (I, [I, P, P, P, P, P]),
-    "rgp_compute_cov2d": (I, [I, P, P, P, P, F, F, F, F, P, P]),
-    "rgp_inverse_cov2d": (I, [I, P, P, P, P, P]),
-    "rgp_sh2weight": (I, [I, I, I, P, P, P, P, P, P]),
+    "rgp_project": (I, [I, P, P, P, F, F, F, F, P, P, P, P, P]),
+    "rgp_compute_cov3d": (I, [I, P, P, P, P, P, P, P]),
+    "rgp_compute_cov2d": (I, [I, P, P, P, P, F, F, F, F, P, P, P, P]),
+    "rgp_inverse_cov2d": (I, [I, P, P, P, P, P, P]),
+    "rgp_sh2weight": (I, [I, I, I, P, P, P, P, P, P, P]),
     "rgp_sh2weight_fast": (I, [I, I, I, I, I, P, P, P, P, P, P, P, P, P]),
     "rgp_sh2weight_fast_backward": (I, [I, I, I, I, I, P, P, P, P, P, P, P]),
     "rgp_sh2color": (I, [I, I, P, P, P, P, P]),

@@ -313,3 +313,40 @@ def test_binning_sort_paths(device, case):
         assert lens.max() > 2 * 4096
     if case == "mid_lists":
         assert 2048 < lens.max() <= 4096
+
+
+@pytest.mark.parametrize("sh_dim", [16, 9, 4, 1])
+def test_calc_j_jacobians_bitexact(device, sh_dim):
+    """calc_J=True outputs of project, computeCov3D, computeCov2D,
+    inverseCov2D and sh2Weight against the oracle, bit for bit, on a scene
+    with Gaussians behind the camera and outside the clamp (their entries
+    keep the zero fill); the oracle's Jacobians are pinned by float64 finite
+    differences."""
+    print(f"case: calc_J sh_dim={sh_dim}", flush=True)
+    sc = make_scene(3000, seed=23, extent=4.0, K=5)  # ~370 behind the camera, ~1600 clamped
+    cam = orbit_cameras(3, 3.0, 320, 240)[1]
+    tc = cam.to_torch(device)
+    K = sc.K
+    shs = np.ascontiguousarray(sc.shs[:, : sh_dim * K])
+    pws = T(sc.pws, device)
+    us, pcs, depths, du = gsc.project(pws, tc.Rcw, tc.tcw, cam.fx, cam.fy, cam.cx, cam.cy, True)
+    c3, dr, ds = gsc.computeCov3D(T(sc.rots, device), T(sc.scales, device), depths, True)
+    c2, d3, dp = gsc.computeCov2D(c3, pcs, tc.Rcw, depths, cam.fx, cam.fy, cam.width, cam.height, True)
+    w, basis, dwdp = gsc.sh2Weight(T(shs, device), pws, tc.twc, True, K)
+    cinv, areas, dci = gsc.inverseCov2D(c2, depths, True)
+    o_us, o_pcs, o_dep = O.project(sc.pws, cam.Rcw, cam.tcw, cam.fx, cam.fy, cam.cx, cam.cy)
+    assert (o_dep < 0.2).sum() > 100
+    same(N(du), O.project_jac(sc.pws, cam.Rcw, cam.tcw, cam.fx, cam.fy), "du_dpcs")
+    o_c3 = O.cov3d(sc.rots, sc.scales, o_dep)
+    o_dr, o_ds = O.cov3d_jac(sc.rots, sc.scales, o_dep)
+    same(N(dr), o_dr, "dcov3d_drots")
+    same(N(ds), o_ds, "dcov3d_dscales")
+    o_c2 = O.cov2d(o_c3, o_pcs, cam.Rcw, o_dep, cam.fx, cam.fy, cam.width, cam.height)
+    o_d3, o_dp = O.cov2d_jac(o_c3, o_pcs, cam.Rcw, o_dep, cam.fx, cam.fy, cam.width, cam.height)
+    same(N(d3), o_d3, "dcov2d_dcov3ds")
+    same(N(dp), o_dp, "dcov2d_dpcs")
+    O.inverse_cov2d(o_c2, o_dep)
+    same(N(depths), o_dep, "depths")
+    same(N(dci), O.inverse_jac(o_c2, o_dep), "dcinv_dcov2ds")
+    same(N(dwdp), O.sh2weight_jac(shs, sc.pws, cam.twc, K), "dweight_dpws")
+    same(N(basis).reshape(sc.n, sh_dim), O.sh2weight(shs, sc.pws, cam.twc, K)[1], "basis")

@@ -410,3 +410,105 @@ def test_backward_replay_vs_division(small):
         # both errors come mostly from the FP32 alpha' they share: the replay is
         # not less accurate than the division (within 1% of its total error)
         assert np.abs(rep["dweights"] - ref["dweights"]).sum() <= 1.01 * np.abs(div["dweights"] - ref["dweights"]).sum()
+
+
+# --------------------------------------------------------------------------
+# calc_J outputs: float64 central differences of the forward functions
+# --------------------------------------------------------------------------
+def _fd(fn, x, eps):
+    """d fn(x) / dx by central differences: fn maps (n, d) -> (n, ...)."""
+    cols = []
+    for j in range(x.shape[1]):
+        xp, xm = x.copy(), x.copy()
+        xp[:, j] += eps
+        xm[:, j] -= eps
+        cols.append((fn(xp) - fn(xm)) / (2 * eps))
+    return np.stack(cols, axis=-1)
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / (np.abs(b).max() + 1e-300))
+
+
+def test_jacobians_fd64():
+    """Every calc_J output of project, computeCov3D, computeCov2D,
+    inverseCov2D and sh2Weight against central differences of the float64
+    forward it differentiates (computeCov2D on Gaussians inside the +-1.3
+    tan_fov clamp, whose derivative the reference leaves out)."""
+    sc = make_scene(300, seed=17, extent=1.0, K=5)
+    cam = orbit_cameras(1, 3.5, 160, 120)[0]
+    n, d = sc.n, f64
+    pws = sc.pws.astype(d)
+    R = cam.Rcw.astype(d)
+    us, pcs, dep = O.project(pws, R, cam.tcw, cam.fx, cam.fy, cam.cx, cam.cy, d)
+    ok = dep > 0.2
+    assert ok.sum() > 200
+    # project: d us / d pws = du/dpc R
+    du = O.project_jac(pws, R, cam.tcw, cam.fx, cam.fy, d)
+    fd = _fd(lambda p: O.project(p, R, cam.tcw, cam.fx, cam.fy, cam.cx, cam.cy, d)[0], pws, 1e-6)
+    assert _rel((du @ R)[ok], fd[ok]) < 1e-7
+    assert (du[:, 0, 1] == 0).all() and (du[:, 1, 0] == 0).all()
+    # computeCov3D
+    rots, scales = sc.rots.astype(d), sc.scales.astype(d)
+    dr, ds = O.cov3d_jac(rots, scales, dep, d)
+    assert _rel(dr[ok], _fd(lambda q: O.cov3d(q, scales, dep, d), rots, 1e-6)[ok]) < 1e-7
+    assert _rel(ds[ok], _fd(lambda s: O.cov3d(rots, s, dep, d), scales, 1e-8)[ok]) < 1e-6
+    # computeCov2D
+    c3 = O.cov3d(rots, scales, dep, d)
+    W, H = cam.width, cam.height
+    tfx, tfy = O.tan_fov(W, H, cam.fx, cam.fy, d)
+    inside = ok & (np.abs(pcs[:, 0] / pcs[:, 2]) < 1.29 * tfx) & (np.abs(pcs[:, 1] / pcs[:, 2]) < 1.29 * tfy)
+    assert inside.sum() > 100
+    d3, dp = O.cov2d_jac(c3, pcs, R, dep, cam.fx, cam.fy, W, H, d)
+    c2fn = lambda c, p: O.cov2d(c, p, R, dep, cam.fx, cam.fy, W, H, d)
+    assert _rel(d3[inside], _fd(lambda c: c2fn(c, pcs), c3, 1e-9)[inside]) < 1e-6
+    assert _rel(dp[inside], _fd(lambda p: c2fn(c3, p), pcs, 1e-7)[inside]) < 1e-6
+    # inverseCov2D
+    c2 = c2fn(c3, pcs)
+    dep2 = dep.copy()
+    O.inverse_cov2d(c2, dep2, d)
+    di = O.inverse_jac(c2, dep2, d)
+    ok2 = dep2 > 0.2
+
+    def inv(c):
+        dd = dep.copy()
+        return O.inverse_cov2d(c, dd, d)[0]
+    assert _rel(di[ok2], _fd(inv, c2, 1e-6)[ok2]) < 1e-6
+    # sh2Weight: dW/dpw through the view direction.  Quirk kept from the
+    # reference (kernel.cu:840): dY_9/dy, Y_9 = C30 y (3x^2 - y^2), is taken
+    # as C30 (3x^2 - 2y^2) instead of C30 (3x^2 - 3y^2), so the result
+    # exceeds the true derivative by C30 sh_9 y^2 ddir_y/dpw.
+    shs = sc.shs.astype(d)
+    K = sc.K
+    dw = O.sh2weight_jac(shs, pws, cam.twc, K, d)
+    fdw = _fd(lambda p: O.sh2weight(shs, p, cam.twc, K, d)[0], pws, 1e-6)
+    dv = pws - cam.twc.astype(d)
+    nrm = np.linalg.norm(dv, axis=1)
+    dirv = dv / nrm[:, None]
+    ddir_y = (np.eye(3)[1][None, :] - dirv[:, 1:2] * dirv) / nrm[:, None]  # d dir_y / d pw
+    quirk = (-0.5900435899266435 * shs[:, 9 * K:10 * K] * (dirv[:, 1:2] ** 2))[:, :, None] * ddir_y[:, None, :]
+    assert _rel(dw, fdw + quirk) < 1e-7
+    assert np.abs(quirk).max() > 1e-3 * np.abs(fdw).max()  # the quirk is visible here
+    for sh_dim in (1, 4, 9):
+        sub = np.ascontiguousarray(shs[:, : sh_dim * sc.K])
+        dws = O.sh2weight_jac(sub, pws, cam.twc, sc.K, d)
+        fds = _fd(lambda p: O.sh2weight(sub, p, cam.twc, sc.K, d)[0], pws, 1e-6)
+        assert np.abs(dws - fds).max() <= 1e-7 * (np.abs(fdw).max() + 1e-12), sh_dim
+
+
+def test_jacobians_fp32_vs_fp64():
+    """The FP32 Jacobians follow their float64 values within FP32 rounding
+    (cancellation aside: the bar is relative to the largest entry)."""
+    sc = make_scene(300, seed=18, extent=1.0, K=5)
+    cam = orbit_cameras(1, 3.5, 160, 120)[0]
+    out = {}
+    for dt in (f32, f64):
+        us, pcs, dep = O.project(sc.pws, cam.Rcw, cam.tcw, cam.fx, cam.fy, cam.cx, cam.cy, dt)
+        c3 = O.cov3d(sc.rots, sc.scales, dep, dt)
+        c2 = O.cov2d(c3, pcs, cam.Rcw, dep, cam.fx, cam.fy, cam.width, cam.height, dt)
+        out[dt] = [O.project_jac(sc.pws, cam.Rcw, cam.tcw, cam.fx, cam.fy, dt),
+                   *O.cov3d_jac(sc.rots, sc.scales, dep, dt),
+                   *O.cov2d_jac(c3, pcs, cam.Rcw, dep, cam.fx, cam.fy, cam.width, cam.height, dt),
+                   O.inverse_jac(c2, dep, dt), O.sh2weight_jac(sc.shs, sc.pws, cam.twc, sc.K, dt)]
+    for a, b in zip(out[f32], out[f64]):
+        assert _rel(a.astype(f64), b) < 2e-5

@@ -99,6 +99,31 @@ def measure_serving(args, dev):
     return out
 
 
+# bench.py's span names -> the kernel each one times in the default workload
+SPAN_KERNEL = {"draw_backward": "rgp::k_draw_bwd_replay<5>", "draw_forward": "rgp::k_draw_fwd<5, false>",
+               "sh_backward_adam": "rgp::k_sh_adam_full<5, 16>", "preprocess": "rgp::k_preprocess<5, 0>"}
+PMC_PROFILE = os.path.join(ROOT, "profiles", "pmc_finetune_step_1M_1080p_k5.json")
+
+
+def pmc_profile(args, span):
+    """HBM bytes per launch (FETCH_SIZE / WRITE_SIZE, copy-calibrated) and
+    VALU instructions of the kernel behind `span`, from the committed
+    rocprofv3 counter profile of this workload; None for other workloads."""
+    if (args.n, args.width, args.height) != (1_000_000, 1920, 1080) or span not in SPAN_KERNEL:
+        return None
+    try:
+        with open(PMC_PROFILE) as f:
+            prof = json.load(f)
+    except OSError:
+        return None
+    k = prof["kernels"].get(SPAN_KERNEL[span])
+    if k is None:
+        return None
+    return {"traffic_bytes": k["traffic_bytes"], "valu_insts": k["valu_insts"] * 1.0,
+            "source": "profiles/pmc_finetune_step_1M_1080p_k5.json (FETCH_SIZE x %.3f + WRITE_SIZE x %.3f)"
+                      % (prof["fetch_scale"], prof["write_scale"])}
+
+
 def measure_lowrank(args, dev, sizes=(1_000_000, 3_000_000), steps=16, warmup=2):
     """Rank-2 low-rank finetune step (WeightGSFunctionFast, K=5, A (N,5,2),
     B (N,2,15)) at 1080p over 8 orbit views, for 1M and 3M Gaussians."""
@@ -258,14 +283,21 @@ def main():
         flops = None
     breakdown = {k: {"launches": v[0], "mean_ms": round(v[2], 4), "share": round(v[1] / (1e3 * (t1 - t0)), 4)}
                  for k, v in spans.items()}
+    prof = pmc_profile(args, name)
     if flops is not None:
         achieved = flops / (total_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": name,
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                "traffic": prof["traffic_bytes"] if prof else None, "kernel": name,
                 "mean_launch_ms": round(mean_ms, 4),
                 "note": "FP32 VALU-bound compositing (no MFMA; 157.3 TFLOP/s is the FP32 vector peak); "
                         "algorithmic flops = 17 per evaluated (pixel, entry) + (4+2K) forward / (3+2K) backward "
                         "per contributing one, counted on the GPU per view"}
+        if prof:
+            # wave64 VALU instructions issue over 4 cycles on a 16-lane SIMD: 1024 SIMDs at 2.4 GHz
+            busy_ms = prof["valu_insts"] * 4 / (1024 * 2.4e9) * 1e3
+            roof["valu_issue_frac"] = round(busy_ms / mean_ms, 3)
+            roof["traffic_source"] = prof["source"]
     else:
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": name, "mean_launch_ms": round(mean_ms, 4)}

@@ -2,7 +2,8 @@
 // rasterizer.  All code here is compiled with -ffp-contract=off and correctly
 // rounded division and square root, so every FP32 operation is rounded once,
 // in source order: the arithmetic of the reference kernels
-// (gsplatcu/kernel.cu) written out without contraction.
+// (gsplatcu/kernel.cu) written out without implicit contraction; the fused
+// multiply-adds that remain are explicit fmaf calls.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -82,9 +83,31 @@ __device__ __forceinline__ float exp_cw_neg(float x) {
   return p * __uint_as_float((uint32_t)((int)n + 127) << 23);
 }
 
+// exp_cw_neg(-0.5f * m) for m in [0, 170], formed from m without the
+// scaling: n is the same (m (-0.5 log2e) and (-0.5 m) log2e are the same
+// real product), the reduced argument is -2r, and the Horner partials with
+// the coefficients scaled by (-1/2)^k are the original ones scaled by
+// (-1/2)^k.  Scaling by a power of two commutes with rounding for these
+// normal values, so the result is bit-identical (checked on every float of
+// [0, 170]); the final 2^n scaling is an ldexp.
+__device__ __forceinline__ float gauss_half(float m) {
+  const float n = __builtin_rintf(m * (-0.5f * 1.44269504088896341f));
+  float r = __builtin_fmaf(n, 2.0f * 0.693145751953125f, m);
+  r = __builtin_fmaf(n, 2.0f * 1.42860676533018570e-06f, r);
+  float t = 1.98412698e-4f * -0.0078125f;
+  t = __builtin_fmaf(t, r, 1.38888889e-3f * 0.015625f);
+  t = __builtin_fmaf(t, r, 8.33333377e-3f * -0.03125f);
+  t = __builtin_fmaf(t, r, 4.16666679e-2f * 0.0625f);
+  t = __builtin_fmaf(t, r, 1.66666672e-1f * -0.125f);
+  t = __builtin_fmaf(t, r, 0.5f * 0.25f);
+  t = __builtin_fmaf(t, r, -0.5f);
+  t = __builtin_fmaf(t, r, 1.0f);
+  return __builtin_ldexpf(t, (int)n);
+}
+
 // exp(-m/2) for a clamped Mahalanobis distance m >= 0.
 __device__ __forceinline__ float gauss_of(float m) {
-  return m <= 170.0f ? exp_cw_neg(-0.5f * m) : exp_cw(-0.5f * m);
+  return m <= 170.0f ? gauss_half(m) : exp_cw(-0.5f * m);
 }
 
 // Float -> int conversions that saturate and send NaN to 0.
@@ -248,10 +271,10 @@ __device__ __forceinline__ void sh_basis(float px, float py, float pz, const flo
 }
 
 // Full-rank weights (kernel.cu:712-767).  sh: band-major coefficients of one
-// Gaussian, sh[lm * K + k].
-template <int K>
-__device__ __forceinline__ void sh_weights_full(const float* __restrict__ sh, const float* Y,
-                                                int sh_dim, float* w) {
+// Gaussian, sh[lm * K + k], read through an accessor (global memory or a
+// staged LDS row); the expression tree is the same for both.
+template <int K, class Sh>
+__device__ __forceinline__ void sh_weights_full(const Sh& sh, const float* Y, int sh_dim, float* w) {
 #pragma unroll
   for (int k = 0; k < K; ++k) w[k] = (float)(0.0 + SH_C0_0 * (double)sh[k]);
   if (sh_dim > 1) {
@@ -387,6 +410,25 @@ __device__ __forceinline__ float wave_sum_to63(float v) {
   v = v + RGP_DPP(v, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1, 3
   v = v + RGP_DPP(v, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2, 3
   return v;
+}
+
+// Sum over the 16 lanes of each row; every lane of the row receives it.
+__device__ __forceinline__ float row_sum16(float v) {
+  v = v + RGP_DPP(v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+  v = v + RGP_DPP(v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+  v = v + RGP_DPP(v, 0x124, 0xF, 0xF, false);  // row_ror:4
+  v = v + RGP_DPP(v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  return v;
+}
+
+// Sum of lanes l, l^16, l^32 and l^48 into every one of them: the gfx950
+// v_permlane16_swap exchanges rows 0<->1 and 2<->3 between its two copies,
+// v_permlane32_swap the two halves of the wavefront.
+__device__ __forceinline__ float across_rows(float v) {
+  auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  v = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+  auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(q[0]) + __uint_as_float(q[1]);
 }
 
 }  // namespace rgp

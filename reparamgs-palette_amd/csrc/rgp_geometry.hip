@@ -357,11 +357,20 @@ __global__ __launch_bounds__(256) void k_rects(int n, const float* __restrict__ 
 // fused preprocess (WeightGSFunctionTest/Fast forward up to getRects)
 // ---------------------------------------------------------------------------
 // WMODE: 0 full rank; 1 low rank, generic P, Q, rank; 2 / 3 low rank with
-// Q = 15, P = K and rank 1 / 2 (factors in registers).  One 64-thread block
-// per 64 Gaussians; the block's quantised-depth maximum goes to dqw[block]
-// with a plain store (a same-address atomic per wavefront cost more than the
-// rest of the kernel).  Coefficient rows are read straight from global
-// memory: staging them through LDS measured slower.
+// Q = 15, P = K and rank 1 / 2 (factors in registers); 4 full rank with 16
+// coefficients per channel and 16-byte aligned rows, staged through LDS.
+// One 64-thread block per 64 Gaussians; the block's quantised-depth maximum
+// goes to dqw[block] with a plain store (a same-address atomic per wavefront
+// cost more than the rest of the kernel).
+//
+// Staging (WMODE 4): the block's 64 rows of 4K float4 are one contiguous
+// span, read with 4K coalesced 16-byte loads per lane (lanes reading their
+// own rows, 64*K bytes apart, touch 64 cache lines per load instead of 8).  Float4
+// q of row g is kept at LDS slot g*4K + (q ^ sw(g)); the swizzle permutes
+// aligned groups of four slots, so the stores stay contiguous per 16 lanes
+// and, for odd K, the row reads of 8 or 16 lanes hit distinct banks.
+__device__ __forceinline__ int sh_swizzle(int g) { return ((g >> 1) ^ (g >> 3)) & 3; }
+
 template <int K, int WMODE>
 __global__ __launch_bounds__(64) void k_preprocess(
     int n, int sh_dim, int P, int Q, int rank, const rgp_camera* __restrict__ cam,
@@ -373,8 +382,27 @@ __global__ __launch_bounds__(64) void k_preprocess(
     int gx, int gy, float* __restrict__ us_out, float* __restrict__ cinv_out,
     int32_t* __restrict__ areas_out, float* __restrict__ w_out) {
   constexpr int R = WMODE == 2 ? 1 : 2;
+  constexpr int R4 = 4 * K;  // float4 per staged row
+  __shared__ float4 s_sh[WMODE == 4 ? 64 * R4 : 1];
   const int t = threadIdx.x;
   const int i = blockIdx.x * 64 + t;
+  if constexpr (WMODE == 4) {
+    const int tot = min(64, n - (int)blockIdx.x * 64) * R4;
+    const float4* src = reinterpret_cast<const float4*>(shs) + (size_t)blockIdx.x * 64 * R4;
+    float4 v[R4];
+    if (tot == 64 * R4) {
+#pragma clang loop unroll(full)
+      for (int m = 0; m < R4; ++m) v[m] = src[t + 64 * m];
+#pragma clang loop unroll(full)
+      for (int m = 0; m < R4; ++m) {
+        const int e = t + 64 * m;
+        s_sh[e ^ sh_swizzle(e / R4)] = v[m];
+      }
+    } else {
+      for (int e = t; e < tot; e += 64) s_sh[e ^ sh_swizzle(e / R4)] = src[e];
+    }
+    __syncthreads();
+  }
   uint32_t dqv = 0;
   if (i < n) {
     const float px = pws[3 * i], py = pws[3 * i + 1], pz = pws[3 * i + 2];
@@ -388,6 +416,16 @@ __global__ __launch_bounds__(64) void k_preprocess(
     if (WMODE == 0) {
       if (sh_dim > 1) sh_basis(px, py, pz, cam->twc, sh_dim, Y);
       sh_weights_full<K>(shs + (size_t)i * sh_dim * K, Y, sh_dim, w);
+    } else if (WMODE == 4) {
+      sh_basis(px, py, pz, cam->twc, 16, Y);
+      float row[16 * K];
+      const int sw = sh_swizzle(t);
+#pragma unroll
+      for (int q = 0; q < R4; ++q) {
+        const float4 c = s_sh[t * R4 + (q ^ sw)];
+        row[4 * q] = c.x; row[4 * q + 1] = c.y; row[4 * q + 2] = c.z; row[4 * q + 3] = c.w;
+      }
+      sh_weights_full<K>(row, Y, 16, w);
     } else if (WMODE == 1) {
       const int L = P * Q / K;
       if (L > 0) sh_basis(px, py, pz, cam->twc, L + 1, Y);
@@ -819,6 +857,33 @@ __global__ __launch_bounds__(256) void k_sh_adam_full(int n, const rgp_camera* _
   float4* p4 = (float4*)(p + base);
   float4* m4 = (float4*)(m + base);
   float4* v4 = (float4*)(v + base);
+  if constexpr ((G * ROW / 4) % 256 == 0) {
+    // a full block is PER float4 of each array per thread: all 3 PER loads
+    // are issued before the first update
+    constexpr int PER = G * ROW / 4 / 256;
+    if (cnt == G) {
+      float4 pp[PER], mm[PER], vv[PER];
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        pp[j] = p4[t + 256 * j];
+        mm[j] = m4[t + 256 * j];
+        vv[j] = v4[t + 256 * j];
+      }
+      asm volatile("" ::: "memory");  // keep the loads ahead of the updates
+#pragma unroll
+      for (int j = 0; j < PER; ++j) {
+        const int q = t + 256 * j;
+        adam_elem(pp[j].x, mm[j].x, vv[j].x, grad(4 * q), a);
+        adam_elem(pp[j].y, mm[j].y, vv[j].y, grad(4 * q + 1), a);
+        adam_elem(pp[j].z, mm[j].z, vv[j].z, grad(4 * q + 2), a);
+        adam_elem(pp[j].w, mm[j].w, vv[j].w, grad(4 * q + 3), a);
+        p4[q] = pp[j];
+        m4[q] = mm[j];
+        v4[q] = vv[j];
+      }
+      return;
+    }
+  }
   const int n4 = total >> 2;
   for (int q = t; q < n4; q += blockDim.x) {
     float4 pp = p4[q], mm = m4[q], vv = v4[q];
@@ -1057,6 +1122,7 @@ extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int r
   if (n == 0) return RGP_OK;
   int wmode = mode;
   if (mode == 1 && Q == 15 && P == K && (rank == 1 || rank == 2)) wmode = rank + 1;
+  if (mode == 0 && sh_dim == 16 && ((uintptr_t)shs & 15) == 0) wmode = 4;
 #define RGP_PRE(WM)                                                                               \
   RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess<KK, WM>), dim3((n + 63) / 64), dim3(64), 0, s, n, \
                                      sh_dim, P, Q, rank, cam, pws, (const float4*)rots, scales,   \
@@ -1068,7 +1134,8 @@ extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int r
     case 0: RGP_PRE(0); break;
     case 1: RGP_PRE(1); break;
     case 2: RGP_PRE(2); break;
-    default: RGP_PRE(3); break;
+    case 3: RGP_PRE(3); break;
+    default: RGP_PRE(4); break;
   }
 #undef RGP_PRE
   return check_launch("rgp_preprocess_weight");

@@ -51,7 +51,9 @@ __device__ __forceinline__ RecRegs<F4> load_rec(const float4* __restrict__ recs,
 }
 
 // Stages one record in LDS, replacing its ry by the mask of strips of tile
-// row ty it can reach (bit j: rows ty*16 + 4j .. + 3).
+// row ty it can reach (bit j: rows ty*16 + 4j .. + 3).  Float4 q of slot s
+// sits at s_rec[q * 64 + s]: the wave's 16-byte stores cover the banks once
+// per quarter-wave, and a read of one slot is a broadcast.
 template <int F4>
 __device__ __forceinline__ void stage_rec(float4* s_rec, int slot, const RecRegs<F4>& r, int ty) {
   uint32_t mask = 0;
@@ -61,10 +63,10 @@ __device__ __forceinline__ void stage_rec(float4* s_rec, int slot, const RecRegs
     const float yc = (float)(ty * kTile + 4 * j) + 1.5f;
     if (!((uy - yc > ry + 1.5f) || (yc - uy > ry + 1.5f))) mask |= 1u << j;
   }
-  s_rec[slot * F4] = r.v[0];
-  s_rec[slot * F4 + 1] = make_float4(r.v[1].x, r.v[1].y, r.v[1].z, __uint_as_float(mask));
+  s_rec[slot] = r.v[0];
+  s_rec[64 + slot] = make_float4(r.v[1].x, r.v[1].y, r.v[1].z, __uint_as_float(mask));
 #pragma unroll
-  for (int q = 2; q < F4; ++q) s_rec[slot * F4 + q] = r.v[q];
+  for (int q = 2; q < F4; ++q) s_rec[q * 64 + slot] = r.v[q];
 }
 
 __device__ __forceinline__ uint32_t uniform_bits(float v) {
@@ -131,8 +133,8 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
       nxt = load_rec<F4>(recs, ids, b0 + 64 + lane, rg.x, rg.y, &nid);
       const int cnt = min(64, rg.y - b0);
       for (int jj = 0; jj < cnt; ++jj) {
-        const float4 g0 = s_rec[jj * F4];
-        const float4 g1 = s_rec[jj * F4 + 1];
+        const float4 g0 = s_rec[jj];
+        const float4 g1 = s_rec[64 + jj];
         const uint32_t slots = uniform_bits(g1.w) & live;
         if (slots == 0) continue;
         const float dx = g0.x - fpx;
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
           const float ta = tau[j] * ap;
 #pragma unroll
           for (int q = 0; q < F4 - 2; ++q) {
-            const float4 f = s_rec[jj * F4 + 2 + q];
+            const float4 f = s_rec[(2 + q) * 64 + jj];
             if (4 * q + 0 < DIM) acc[j][4 * q + 0] = fmaf(ta, f.x, acc[j][4 * q + 0]);
             if (4 * q + 1 < DIM) acc[j][4 * q + 1] = fmaf(ta, f.y, acc[j][4 * q + 1]);
             if (4 * q + 2 < DIM) acc[j][4 * q + 2] = fmaf(ta, f.z, acc[j][4 * q + 2]);
@@ -198,16 +200,25 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
   }
 }
 
-// Adds the four-slot lane sums of an entry's gradient terms into dfeat: DPP
-// reduction over the wavefront, one atomic per channel from lane 63.
+// Adds the four-slot lane sums of an entry's gradient terms into dfeat.  Each
+// channel is summed within the 16-lane rows; lane l then keeps the row sum of
+// channel l & 15, the four rows are added across, and lanes 0..DIM-1 issue
+// one atomic instruction over the entry's DIM contiguous floats.  (A separate
+// single-lane atomic per channel costs five instructions and, through the
+// compiler's atomic scan for a lane mask it cannot prove, some twenty VALU
+// operations more per entry.)
 template <int DIM>
 __device__ __forceinline__ void flush_grad(const float* part, float* __restrict__ dfeat, int g,
                                            int lane) {
+  static_assert(DIM <= 16, "one channel per lane of a row");
+  float v = 0.f;
 #pragma unroll
   for (int k = 0; k < DIM; ++k) {
-    const float sum = wave_sum_to63(part[k]);
-    if (lane == 63 && sum != 0.f) unsafeAtomicAdd(&dfeat[(size_t)g * DIM + k], sum);
+    const float r = row_sum16(part[k]);
+    v = (lane & 15) == k ? r : v;
   }
+  v = across_rows(v);
+  if (lane < DIM && v != 0.f) unsafeAtomicAdd(&dfeat[(size_t)g * DIM + lane], v);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,8 +283,8 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
     nxt = load_rec<F4>(recs, ids, top - 65 - lane, rg.x, rg.y, &nid);
     for (int jj = 0; jj < cnt; ++jj) {
       const int fi = top - 1 - jj - rg.x;  // index of the entry in forward order
-      const float4 g0 = s_rec[jj * F4];
-      const float4 g1 = s_rec[jj * F4 + 1];
+      const float4 g0 = s_rec[jj];
+      const float4 g1 = s_rec[64 + jj];
       uint32_t slots = uniform_bits(g1.w);
 #pragma unroll
       for (int j = 0; j < kSlots; ++j)
@@ -357,8 +368,8 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
     nxt = load_rec<F4>(recs, ids, b0 + 64 + lane, rg.x, rg.y, &nid);
     const int cnt = min(64, rg.y - b0);
     for (int jj = 0; jj < cnt; ++jj) {
-      const float4 g0 = s_rec[jj * F4];
-      const float4 g1 = s_rec[jj * F4 + 1];
+      const float4 g0 = s_rec[jj];
+      const float4 g1 = s_rec[64 + jj];
       const uint32_t slots = uniform_bits(g1.w) & live;
       if (slots == 0) continue;
       const float dx = g0.x - fpx;

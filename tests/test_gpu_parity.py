@@ -118,6 +118,22 @@ def test_fused_wrapper_full_rank_c0(device, c0, mode):
     grad_close(N(shs.grad), ob["dshs"], ob["dshs_abs"], "dshs")
 
 
+def test_fused_forward_unaligned_coefficients(device, c0):
+    """Coefficient rows that are not 16-byte aligned are read per lane from
+    global memory instead of through the LDS staging; same bits."""
+    scene, cam = c0
+    o = O.forward_weight(scene, cam)
+    flat = torch.empty(scene.shs.size + 1, dtype=torch.float32, device=device)
+    shs = flat[1:].view(scene.shs.shape)
+    shs.copy_(T(scene.shs, device))
+    assert shs.is_contiguous() and shs.data_ptr() % 16 != 0
+    layer, mask = gsmodel.WeightGSFunctionTest.apply(T(scene.pws, device), shs, T(scene.alphas, device),
+                                                     T(scene.scales, device), T(scene.rots, device),
+                                                     cam.to_torch(device), scene.K)
+    same(N(layer), o["layer"], "layer")
+    same(N(mask), o["mask"], "mask")
+
+
 @pytest.mark.parametrize("rank", [1, 2])
 def test_fused_wrapper_low_rank(device, rank):
     print(f"case: low rank r={rank} 4000 Gaussians 192x160", flush=True)

@@ -26,16 +26,20 @@ scene's current view (rgp_mix_batched), which by linearity equals the bake.
 
 Luminance edit (the reference has none): the edited palette is
 clamp((P + dP) * s, 0, 1) on the chromatic rows; black and white stay fixed.
+
+Scenes are synthetic (rgp.scene.make_scene) or read from an exported
+.pply + .gswp pair (SceneSpec.from_files, rgp.formats).
 """
 import collections
 import dataclasses
 import math
 import time
+import types
 
 import numpy as np
 import torch
 
-from . import ops
+from . import formats, ops
 from ._lib import I64, call, f32c, ptr, stream
 from .scene import look_at_camera, make_scene
 
@@ -54,12 +58,25 @@ class SceneSpec:
     lowrank: bool
     rank: int = 2
     seed: int = 0
+    files: tuple = None  # (pply, gswp): read the scene instead of generating it
+    P: int = 0           # low-rank factor shape P x Q (0: P = K, Q = 15)
+    Q: int = 15
+
+    @classmethod
+    def from_files(cls, name, pply_path, sidecar_path=None):
+        """A scene exported as .pply + .gswp; sizes come from the sidecar header."""
+        sidecar_path = sidecar_path or formats.sidecar_path_for(pply_path)
+        h = formats.read_gswp(sidecar_path)
+        if h["num_sh"] != 16:
+            raise ValueError(f"{sidecar_path}: num_sh = {h['num_sh']}; the serving loop renders 16 coefficients")
+        return cls(name, h["N"], h["splat_dim"], h["low_rank"], rank=1, files=(pply_path, sidecar_path),
+                   P=h["P"], Q=h["Q"] or 15)
 
     def param_bytes(self):
         geo = self.n * (12 + 16 + 12 + 4)
         if self.lowrank:
-            P = self.K  # P x Q = K x 15
-            return geo + self.n * 4 * (self.K + self.rank * (P + 15))
+            P = self.P or self.K  # P x Q = K x 15
+            return geo + self.n * 4 * (self.K + self.rank * (P + self.Q))
         return geo + self.n * 4 * 16 * self.K
 
     def slot_bytes(self, width, height, cap_per_gaussian=6.0):
@@ -107,16 +124,19 @@ class SceneStore:
 
 
 class ResidentScene:
-    """Device copy of one synthetic finetuned scene."""
+    """Device copy of one finetuned scene (synthetic, or read from files)."""
 
     def __init__(self, spec, device):
         self.spec = spec
-        sc = make_scene(spec.n, seed=spec.seed, extent=3.0, K=spec.K, lowrank=spec.lowrank, rank=spec.rank,
-                        P=spec.K if spec.lowrank else None, Q=15)
+        if spec.files:
+            sc = types.SimpleNamespace(rank=1, **formats.load_scene_files(*spec.files))
+        else:
+            sc = make_scene(spec.n, seed=spec.seed, extent=3.0, K=spec.K, lowrank=spec.lowrank, rank=spec.rank,
+                            P=(spec.P or spec.K) if spec.lowrank else None, Q=spec.Q)
         T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
         self.n, self.K = spec.n, spec.K
         self.pws, self.rots, self.scales, self.alphas = T(sc.pws), T(sc.rots), T(sc.scales), T(sc.alphas)
-        self.palette = sc.palette.copy()
+        self.palette = np.array(sc.palette, dtype=np.float32)
         if spec.lowrank:
             self.mode, self.P, self.Q, self.rank = 1, sc.P, sc.Q, sc.rank
             self.shs, self.dc, self.A, self.B = None, T(sc.dc), T(sc.A), T(sc.B)

@@ -93,3 +93,38 @@ def test_results_alone_equal_in_mix(device, mode, monkeypatch):
     for r in reqs:
         got = r.out.cpu().numpy()
         assert np.array_equal(got, exp[r.rid]), (r.rid, r.kind, r.scene)
+
+
+@pytest.mark.gpu
+def test_scenes_from_files(device, monkeypatch, tmp_path):
+    """Scenes exported as .pply + .gswp (dense and rank-1) are admitted from
+    their sidecar headers, load the arrays the files hold, and serve results
+    equal to the same requests rendered alone."""
+    import rgp.serving as S
+    from rgp import formats
+    from rgp.scene import make_scene
+    monkeypatch.setattr(S, "RESOLUTIONS", SMALL_RES)
+    print("case: serving scenes from .pply + .gswp", flush=True)
+    specs = []
+    for name, lowrank in (("file_dense", False), ("file_rank1", True)):
+        sc = make_scene(20_000, seed=7, extent=3.0, K=5, lowrank=lowrank, rank=1, P=5 if lowrank else None, Q=15)
+        kw = dict(dc=sc.dc, A=sc.A, B=sc.B) if lowrank else dict(shs=sc.shs)
+        pply, _ = formats.export(formats.checkpoint(sc.pws, sc.rots, sc.scales, sc.alphas, sc.palette, **kw),
+                                 tmp_path / f"{name}.pply")
+        spec = S.SceneSpec.from_files(name, pply)
+        assert (spec.n, spec.K, spec.lowrank) == (20_000, 5, lowrank)
+        specs.append(spec)
+    eng = S.Engine(specs, device, n_poses=8)
+    for spec in specs:
+        want = formats.load_scene_files(*spec.files)
+        got = eng.scenes[spec.name]
+        assert np.array_equal(got.scales.cpu().numpy(), want["scales"])
+        assert np.array_equal(got.alphas.cpu().numpy(), want["alphas"])
+        assert np.array_equal(got.palette, want["palette"])
+    reqs = S.make_requests(specs, 60, rate=0.0, seed=6, n_poses=8)
+    eng.prepare(reqs)
+    rep = eng.run(reqs, mode="batched", realtime=False)
+    assert rep["requests"] == 60
+    exp = expected_outputs(eng, reqs)
+    for r in reqs:
+        assert np.array_equal(r.out.cpu().numpy(), exp[r.rid]), (r.rid, r.kind, r.scene)

@@ -262,6 +262,37 @@ int rgp_l1_ssim_backward(int channels, int height, int width, const float* img,
                          const float* target, const float* abc, const float* window11, float lam,
                          const float* grad_loss, float* dimg, hipStream_t stream);
 
+/* ---- barycentric palette loss of the finetune step (no gsplatcu
+ * counterpart: the reference builds it from torch operations,
+ * barycentric_coords_3d at gsplat/gsmodel.py:219-285 and the loss terms at
+ * finetune.py:353-392).  K_c chromatic palette colours (1 <= Kc <= 6), the
+ * weight images have Kc + 2 channels (black, white, the chromatic ones). -- */
+
+/* Float64 accumulators the two entry points below take as workspace. */
+size_t rgp_bary_workspace_bytes(int Kc);
+
+/* image (3,H,W) the frozen render C; w_tilde (Kc+2,H,W) the rendered weights;
+ * gt_alpha (H,W) or NULL (with it: C / max(a, 1e-3) clamped to [0,1],
+ * W_bary * a, and lambda_alpha * mean((sum_k W_tilde - a)^2)); palette (Kc,3);
+ * clamp != 0 applies W_bary <- max(W_bary,0) / max(sum, 1e-8).
+ * Loss L = mean((W_tilde - W_bary)^2) + lambda_pos mean(relu(-W_tilde))
+ * [+ lambda_alpha L_alpha].  Writes dw_tilde = dL/dW_tilde (Kc+2,H,W),
+ * dpalette = dL/dpalette (Kc,3), loss[4] = (L, L_geo, L_pos, L_alpha) and,
+ * if w_bary is not NULL, the target weights W_bary (Kc+2,H,W). */
+int rgp_bary_loss(int width, int height, int Kc, const float* image, const float* w_tilde,
+                  const float* gt_alpha, const float* palette, float tau, int clamp,
+                  float lambda_pos, float lambda_alpha, float* w_bary, float* dw_tilde,
+                  float* dpalette, float* loss, void* workspace, hipStream_t stream);
+
+/* barycentric_coords_3d(C, P, tau) alone: w_bary (Kc+2,H,W). */
+int rgp_bary_coords(int width, int height, int Kc, const float* image, const float* palette,
+                    float tau, float* w_bary, hipStream_t stream);
+
+/* Its backward: dpalette (Kc,3) = sum over pixels of dw_bary . dW_bary/dP. */
+int rgp_bary_coords_backward(int width, int height, int Kc, const float* image,
+                             const float* palette, float tau, const float* dw_bary,
+                             float* dpalette, void* workspace, hipStream_t stream);
+
 /* Fill n 32-bit words with a value (graph-capture safe, no memset node). */
 int rgp_fill_u32(void* dst, uint32_t value, int64_t n, hipStream_t stream);
 

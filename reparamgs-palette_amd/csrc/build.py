@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT_DIR = os.path.join(PKG, "lib")
 OUT = os.path.join(OUT_DIR, "librgp.so")
-SOURCES = ["rgp_capi.hip", "rgp_geometry.hip", "rgp_binning.hip", "rgp_raster.hip", "rgp_loss.hip"]
+SOURCES = ["rgp_capi.hip", "rgp_geometry.hip", "rgp_binning.hip", "rgp_raster.hip", "rgp_loss.hip", "rgp_bary.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -105,6 +105,34 @@ __device__ __forceinline__ float gauss_half(float m) {
   return __builtin_ldexpf(t, (int)n);
 }
 
+// log(1 + y) for y >= 0 (an exp output).  u = fl(1 + y) with its exact
+// rounding error e (Fast2Sum); log u = n ln2 + log m with m in [sqrt(1/2),
+// sqrt(2)) and log m = 2 atanh(s) = 2 (s + s^3/3 + s^5/5 + s^7/7 + s^9/9),
+// s = (m - 1)/(m + 1), |s| < 0.172; log(1 + y) = log u + e / u.  Within
+// 4 ulp of log1p over [0, e^20].
+__device__ __forceinline__ float log1p_pos(float y) {
+  if (y != y) return y;
+  if (y == __uint_as_float(0x7f800000u)) return y;
+  const float u = 1.0f + y;
+  const float e = y >= 1.0f ? 1.0f - (u - y) : y - (u - 1.0f);
+  const uint32_t b = __float_as_uint(u);
+  int n = (int)(b >> 23) - 127;
+  float m = __uint_as_float((b & 0x7fffffu) | 0x3f800000u);
+  if (m > 1.41421356f) { m = 0.5f * m; n += 1; }
+  const float f = m - 1.0f;
+  const float s = f / (2.0f + f);
+  const float z = s * s;
+  float p = 1.0f / 9.0f;
+  p = __builtin_fmaf(p, z, 1.0f / 7.0f);
+  p = __builtin_fmaf(p, z, 0.2f);
+  p = __builtin_fmaf(p, z, 1.0f / 3.0f);
+  const float t = 2.0f * s;
+  const float lm = __builtin_fmaf(t * z, p, t);
+  const float fn = (float)n;
+  const float lu = __builtin_fmaf(fn, 0.693145751953125f, __builtin_fmaf(fn, 1.42860676533018570e-06f, lm));
+  return lu + e / u;
+}
+
 // exp(-m/2) for a clamped Mahalanobis distance m >= 0.
 __device__ __forceinline__ float gauss_of(float m) {
   return m <= 170.0f ? gauss_half(m) : exp_cw(-0.5f * m);

@@ -54,6 +54,10 @@ _SIGS = {
     "rgp_fill_u32": (I, [P, U32, I64, P]),
     "rgp_l1_ssim_forward": (I, [I, I, I, P, P, P, F, F, F, P, P, P, P]),
     "rgp_l1_ssim_backward": (I, [I, I, I, P, P, P, P, F, P, P, P]),
+    "rgp_bary_workspace_bytes": (ctypes.c_size_t, [I]),
+    "rgp_bary_loss": (I, [I, I, I, P, P, P, P, F, I, F, F, P, P, P, P, P, P]),
+    "rgp_bary_coords": (I, [I, I, I, P, P, F, P, P]),
+    "rgp_bary_coords_backward": (I, [I, I, I, P, P, F, P, P, P, P]),
 }
 
 

@@ -97,3 +97,112 @@ class L1SSIMLoss(torch.autograd.Function):
 
 def l1_ssim_loss_fused(image, target, lam=0.2):
     return L1SSIMLoss.apply(image, target, lam)
+
+
+# ---------------------------------------------------------------------------
+# barycentric palette loss of the finetune step (gsplat/gsmodel.py:219-285,
+# finetune.py:42-76 and 353-392)
+# ---------------------------------------------------------------------------
+def _bary_ws(Kc, device):
+    from ._lib import lib
+    return torch.empty(int(lib().rgp_bary_workspace_bytes(int(Kc))), dtype=torch.uint8, device=device)
+
+
+class BaryLoss(torch.autograd.Function):
+    """L_geo + lambda_pos L_pos [+ lambda_alpha L_alpha] against the barycentric
+    target weights of the frozen render C (finetune.py:353-392): one kernel
+    pass (rgp_bary_loss) forms the loss, dL/dW_tilde and dL/dpalette; the
+    backward only scales them.  Returns (loss, parts) with parts = (L_geo,
+    L_pos, L_alpha), not differentiable."""
+
+    @staticmethod
+    def forward(ctx, W_tilde, palette_rgb, C, gt_alpha=None, clamp=False, lambda_pos=0.5, lambda_alpha=1.0,
+                tau=1e-9):
+        from ._lib import call, f32c, ptr, require_cuda, stream
+        require_cuda(W_tilde, palette_rgb, C, gt_alpha)
+        Wt, P, Ci = f32c(W_tilde), f32c(palette_rgb), f32c(C)
+        Kc = P.shape[0]
+        Kt, H, W = Wt.shape
+        if Kt != Kc + 2 or Ci.shape != (3, H, W):
+            raise ValueError(f"W_tilde {tuple(Wt.shape)}, C {tuple(Ci.shape)} and palette {tuple(P.shape)} disagree")
+        a = None if gt_alpha is None else f32c(gt_alpha).reshape(H, W)
+        dW, dP = torch.empty_like(Wt), torch.empty_like(P)
+        out = torch.empty(4, dtype=torch.float32, device=Wt.device)
+        call("rgp_bary_loss", W, H, Kc, ptr(Ci), ptr(Wt), ptr(a), ptr(P), float(tau), int(bool(clamp)),
+             float(lambda_pos), float(lambda_alpha), None, ptr(dW), ptr(dP), ptr(out), ptr(_bary_ws(Kc, Wt.device)),
+             stream(Wt.device))
+        ctx.save_for_backward(dW, dP)
+        ctx.shapes = (W_tilde.shape, palette_rgb.shape)
+        parts = out[1:]
+        ctx.mark_non_differentiable(parts)
+        return out[0], parts
+
+    @staticmethod
+    def backward(ctx, gloss, _):
+        dW, dP = ctx.saved_tensors
+        g = gloss.to(torch.float32)
+        return ((dW * g).reshape(ctx.shapes[0]), (dP * g).reshape(ctx.shapes[1]), None, None, None, None, None,
+                None)
+
+
+def bary_loss(W_tilde, palette_rgb, C, gt_alpha=None, clamp=False, lambda_pos=0.5, lambda_alpha=1.0, tau=1e-9):
+    return BaryLoss.apply(W_tilde, palette_rgb, C, gt_alpha, clamp, lambda_pos, lambda_alpha, tau)
+
+
+class BaryCoords(torch.autograd.Function):
+    """barycentric_coords_3d (gsmodel.py:219-285) on the gfx950 kernels: the
+    (K+2, H, W) weights, differentiable in the palette (C is a constant)."""
+
+    @staticmethod
+    def forward(ctx, C, P, tau=1e-9):
+        from ._lib import call, f32c, ptr, require_cuda, stream
+        require_cuda(C, P)
+        Ci, Pc = f32c(C), f32c(P)
+        Kc = Pc.shape[0]
+        _, H, W = Ci.shape
+        out = torch.empty((Kc + 2, H, W), dtype=torch.float32, device=Ci.device)
+        call("rgp_bary_coords", W, H, Kc, ptr(Ci), ptr(Pc), float(tau), ptr(out), stream(Ci.device))
+        ctx.save_for_backward(Ci, Pc)
+        ctx.tau = float(tau)
+        return out
+
+    @staticmethod
+    def backward(ctx, dWb):
+        from ._lib import call, f32c, ptr, stream
+        Ci, Pc = ctx.saved_tensors
+        Kc = Pc.shape[0]
+        _, H, W = Ci.shape
+        dP = torch.empty_like(Pc)
+        call("rgp_bary_coords_backward", W, H, Kc, ptr(Ci), ptr(Pc), ctx.tau, ptr(f32c(dWb)), ptr(dP),
+             ptr(_bary_ws(Kc, Ci.device)), stream(Ci.device))
+        return None, dP, None
+
+
+def barycentric_coords_3d(C, P, tau=1e-9, eps=1e-12):
+    """Drop-in for gsmodel.py:219: the kernels use eps = 1e-12, the reference default."""
+    if eps != 1e-12:
+        raise ValueError("barycentric_coords_3d: only eps = 1e-12 is implemented")
+    return BaryCoords.apply(C, P, tau)
+
+
+def palette_compactness_loss(palette_rgb):
+    """finetune.py:42-63: the hull volume of black, white and the K chromatic
+    colours as K tetrahedra (black, white, c_k, c_{k+1}) fanned round the grey
+    axis; |det| / 6 each."""
+    K = palette_rgb.shape[0]
+    if K < 2:
+        raise ValueError("palette_compactness_loss needs at least 2 chromatic colours")
+    nxt = torch.roll(palette_rgb, -1, 0)
+    white = torch.ones_like(palette_rgb)
+    # det of the rows (white - black, c_k - black, c_{k+1} - black) as a triple product
+    det = (white * torch.linalg.cross(palette_rgb, nxt, dim=1)).sum(1)
+    return (torch.abs(det) / 6.0).sum()
+
+
+def hue_separation_loss(gap_raw, min_gap_fraction=0.5):
+    """finetune.py:66-76: squared shortfall of each hue gap below
+    min_gap_fraction of the even spacing, averaged."""
+    K = gap_raw.shape[0]
+    g = F.softplus(gap_raw)
+    gaps = (2 * math.pi) * g / g.sum()
+    return (F.relu(min_gap_fraction * (2 * math.pi) / K - gaps) ** 2).mean()

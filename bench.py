@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving measurement (profiling)")
     ap.add_argument("--no-lowrank", action="store_true", help="skip the rank-2 1M / 3M finetune steps")
+    ap.add_argument("--no-bary", action="store_true", help="skip the barycentric-loss finetune step")
+    ap.add_argument("--bary-only", action="store_true", help="run only the barycentric-loss finetune step (profiling)")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--serve-rate", type=float, default=500.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -101,7 +103,7 @@ def measure_serving(args, dev):
 
 # bench.py's span names -> the kernel each one times in the default workload
 SPAN_KERNEL = {"draw_backward": "rgp::k_draw_bwd_replay<5>", "draw_forward": "rgp::k_draw_fwd<5, false>",
-               "sh_backward_adam": "rgp::k_sh_adam_full<5, 16>", "preprocess": "rgp::k_preprocess<5, 0>"}
+               "sh_backward_adam": "rgp::k_sh_adam_full<5, 16>", "preprocess": "rgp::k_preprocess<5, 4>"}
 PMC_PROFILE = os.path.join(ROOT, "profiles", "pmc_finetune_step_1M_1080p_k5.json")
 
 
@@ -171,6 +173,136 @@ def measure_lowrank(args, dev, sizes=(1_000_000, 3_000_000), steps=16, warmup=2)
                                             "L1+SSIM, Adam on dc/A/B"}
 
 
+def torch_bary_loss(Wt, P, C, tau=1e-9, eps=1e-12, lam_pos=0.5):
+    """The finetune step's loss as the reference builds it, with torch
+    operations on (HW, K_c, K_c+2) tensors (barycentric_coords_3d,
+    gsplat/gsmodel.py:219-285, then finetune.py:368-385): the baseline the
+    fused kernel is timed against."""
+    import torch
+    import torch.nn.functional as F
+    Kc = P.shape[0]
+    H, W = C.shape[1:]
+    dev = P.device
+    i, j = torch.arange(Kc, device=dev), (torch.arange(Kc, device=dev) + 1) % Kc
+    A = torch.stack([torch.ones(Kc, 3, device=dev), P[i], P[j]], dim=2)
+    det = torch.linalg.det(A)
+    inv = torch.linalg.inv(A) * (det / (det + eps))[:, None, None]
+    rhs = C.permute(1, 2, 0).reshape(-1, 3)
+    bgd = torch.einsum("kij,nj->nki", inv, rhs)
+    alpha = 1.0 - bgd.sum(-1)
+    coords = torch.cat([alpha[..., None], bgd], -1)
+    n = rhs.shape[0]
+    rows = torch.arange(Kc, device=dev)
+    raw = torch.zeros(n, Kc, Kc + 2, device=dev)
+    raw[:, :, 0], raw[:, :, 1] = coords[..., 0], coords[..., 1]
+    raw[:, rows, i + 2] = coords[..., 2]
+    raw[:, rows, j + 2] = coords[..., 3]
+    sp = F.softplus(coords)
+    q = sp / (sp.sum(-1, keepdim=True) + eps)
+    proj = torch.zeros_like(raw)
+    proj[:, :, 0], proj[:, :, 1] = q[..., 0], q[..., 1]
+    proj[:, rows, i + 2] = q[..., 2]
+    proj[:, rows, j + 2] = q[..., 3]
+    Paug = torch.cat([torch.zeros(1, 3, device=dev), torch.ones(1, 3, device=dev), P], 0)
+    err = ((proj @ Paug - rhs[:, None, :]) ** 2).sum(-1)
+    p = F.softmax(-err / tau, dim=1)
+    Wb = (p[..., None] * raw).sum(1).T.reshape(Kc + 2, H, W)
+    return F.mse_loss(Wt, Wb) + lam_pos * torch.mean(F.relu(-Wt))
+
+
+def measure_bary(args, dev, n=1_000_000, views=8, steps=24, warmup=3):
+    """The reference's own finetune step (finetune.py:346-399): palette-weight
+    render, barycentric target of the frozen render, L_geo + L_pos +
+    compactness + hue separation, backward, Adam on the SH weights and the
+    palette parameters.  Also times the fused loss kernel alone (HIP events on
+    the stream it runs on) against the same loss built from torch operations."""
+    import numpy as np
+    import torch
+
+    from rgp import gsmodel, ops
+    from rgp.losses import bary_loss, hue_separation_loss, palette_compactness_loss
+    from rgp.scene import make_scene, orbit_cameras
+
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sc = make_scene(n, seed=args.seed, extent=3.0, K=5)
+    tcams = [c.to_torch(dev) for c in orbit_cameras(views, 6.0, args.width, args.height)]
+    pws, rots, scales, alphas = T(sc.pws), T(sc.rots), T(sc.scales), T(sc.alphas)
+    params = {"low_shs_w": T(sc.shs[:, :5]).requires_grad_(True), "high_shs_w": T(sc.shs[:, 5:]).requires_grad_(True),
+              "low_rank": False}
+    for k, v in (("hue_base", [0.4]), ("gap_raw", [0.1, -0.2, 0.05]), ("sat_raw", [0.6, 0.3, 0.9]),
+                 ("lit_raw", [0.1, -0.1, 0.2])):
+        params[k] = torch.tensor(v, dtype=torch.float32, device=dev, requires_grad=True)
+    with torch.no_grad():  # frozen renders: the scene's weights mixed with a colour palette
+        shs, pal0 = T(sc.shs), T(sc.palette)
+        frozen = [torch.clamp(ops.mix(gsmodel.render_weights(pws, alphas, scales, rots, tc, 5, shs=shs)[0], pal0), 0, 1)
+                  for tc in tcams]
+        del shs
+    model = gsmodel.FinetuneModel(num_palette=3, use_fast=False)
+    opt = torch.optim.Adam([v for v in params.values() if torch.is_tensor(v)], lr=1e-3)
+
+    def step(i):
+        W_tilde, pal = model(pws, params, alphas, scales, rots, tcams[i % views])
+        loss, _ = bary_loss(W_tilde, pal, frozen[i % views])
+        loss = loss + 0.001 * palette_compactness_loss(pal) + 1.0 * hue_separation_loss(params["gap_raw"], 0.3)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+
+    for i in range(warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(i)
+    torch.cuda.synchronize()
+    sps = steps / (time.perf_counter() - t0)
+
+    # the loss alone: fused kernel vs torch operations (forward + backward)
+    with torch.no_grad():
+        W_tilde, pal = model(pws, params, alphas, scales, rots, tcams[0])
+    Wt = W_tilde.detach().requires_grad_(True)
+    P = pal.detach().requires_grad_(True)
+    C = frozen[0]
+
+    def timed(fn, reps=20):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def fused():
+        loss, _ = bary_loss(Wt, P, C)
+        g = torch.autograd.grad(loss, (Wt, P))
+        return g
+
+    def reference():
+        loss = torch_bary_loss(Wt, P, C)
+        return torch.autograd.grad(loss, (Wt, P))
+
+    ms_fused = timed(fused)
+    ms_torch = timed(reference, reps=5)
+    gf, gr = fused(), reference()
+    hw = args.width * args.height
+    nbytes = hw * (12 + 8 * 5)  # C read, W_tilde read, dL/dW_tilde written (K = 5 channels)
+    out = {"steps_per_s": round(sps, 2), "loss_fwd_bwd_ms": round(ms_fused, 4),
+           "torch_loss_fwd_bwd_ms": round(ms_torch, 3), "loss_speedup_vs_torch": round(ms_torch / ms_fused, 1),
+           "loss_kernel_bytes": nbytes, "loss_kernel_GBps": round(nbytes / (ms_fused * 1e-3) / 1e9, 1),
+           "max_abs_diff_dW_vs_torch": float((gf[0] - gr[0]).abs().max()),
+           "workload": f"finetune.py step: 1M Gaussians, K=5 (3 chromatic), {args.width}x{args.height}, {views} "
+                       "views, barycentric target (tau 1e-9) + L_pos + compactness + hue separation, Adam"}
+    log(f"barycentric finetune step: {sps:.2f} steps/s; loss fwd+bwd {ms_fused:.3f} ms fused vs "
+        f"{ms_torch:.2f} ms torch ({ms_torch / ms_fused:.0f}x)")
+    del sc, pws, rots, scales, alphas, params, frozen, opt, Wt, P
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -190,6 +322,9 @@ def main():
     K = 5
     if args.serve_only:
         print(json.dumps({"serving": measure_serving(args, dev)}))
+        return
+    if args.bary_only:
+        print(json.dumps({"barycentric_step": measure_bary(args, dev)}))
         return
     if args.copy_calibration:
         a = torch.empty(64 << 20, dtype=torch.float32, device=dev)
@@ -304,6 +439,7 @@ def main():
 
     serving = None if args.no_serve else measure_serving(args, dev)
     lowrank = None if args.no_lowrank else measure_lowrank(args, dev)
+    bary = None if args.no_bary else measure_bary(args, dev)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -342,7 +478,7 @@ def main():
         "frames_per_s": round(fps, 2), "ms_per_frame": round(1e3 / fps, 3),
         "pairs_per_view": int(np.mean([w["pairs"] for w in work])),
         "p99_palette_edit_ms": serving["edit_p99_ms"] if serving else None,
-        "serving": serving, "lowrank_rank2": lowrank,
+        "serving": serving, "lowrank_rank2": lowrank, "barycentric_step": bary,
         "kernels": breakdown, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

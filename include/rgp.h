@@ -219,6 +219,13 @@ int rgp_sh_backward_adam_full(int n, int K, int sh_dim, const rgp_camera* cam, c
                               int64_t step, double lr, double beta1, double beta2, double eps,
                               double weight_decay, hipStream_t stream);
 
+/* No counterpart in the reference either: one torch.optim.Adam step (the
+ * optimizer of finetune.py:346-399) over a flat float32 tensor of n elements
+ * with its gradient, in one pass; the same element update and scalars as
+ * rgp_sh_backward_adam_full. */
+int rgp_adam(int64_t n, float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t step,
+             double lr, double beta1, double beta2, double eps, double weight_decay, hipStream_t stream);
+
 /* Low-rank SH backward of gsmodel.py:194-198 (dL/ddc = dw * SH_C0 and
  * sh2Weight_fast_B), Y recomputed from pws and twc. */
 int rgp_sh_backward_lowrank(int n, int K, int P, int Q, int rank, const rgp_camera* cam,

@@ -1155,6 +1155,72 @@ extern "C" int rgp_sh_backward_full(int n, int K, int sh_dim, const rgp_camera* 
   return check_launch("rgp_sh_backward_full");
 }
 
+// Adam scalars as torch.optim.Adam forms them: in double from the double
+// hyper-parameters, then rounded once to float.
+static AdamArgs adam_args(int64_t step, double lr, double beta1, double beta2, double eps, double wd) {
+  const double bc1 = 1.0 - std::pow(beta1, (double)step);
+  const double bc2 = 1.0 - std::pow(beta2, (double)step);
+  AdamArgs a;
+  a.neg_step = (float)(-lr / bc1);
+  a.omb1 = (float)(1.0 - beta1);
+  a.b2 = (float)beta2;
+  a.omb2 = (float)(1.0 - beta2);
+  a.bc2_sqrt = (float)std::sqrt(bc2);
+  a.eps = (float)eps;
+  a.wd = (float)wd;
+  return a;
+}
+
+namespace rgp {
+// One torch.optim.Adam step over a flat tensor with its gradient: one pass,
+// float4 when the four arrays are 16-byte aligned.
+__global__ __launch_bounds__(256) void k_adam4(int64_t n4, float4* __restrict__ p, const float4* __restrict__ g,
+                                               float4* __restrict__ m, float4* __restrict__ v, AdamArgs a) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    float4 pp = p[i], mm = m[i], vv = v[i];
+    const float4 gg = g[i];
+    adam_elem(pp.x, mm.x, vv.x, gg.x, a);
+    adam_elem(pp.y, mm.y, vv.y, gg.y, a);
+    adam_elem(pp.z, mm.z, vv.z, gg.z, a);
+    adam_elem(pp.w, mm.w, vv.w, gg.w, a);
+    p[i] = pp;
+    m[i] = mm;
+    v[i] = vv;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adam1(int64_t lo, int64_t n, float* __restrict__ p,
+                                               const float* __restrict__ g, float* __restrict__ m,
+                                               float* __restrict__ v, AdamArgs a) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    adam_elem(p[i], m[i], v[i], g[i], a);
+}
+}  // namespace rgp
+
+extern "C" int rgp_adam(int64_t n, float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+                        int64_t step, double lr, double beta1, double beta2, double eps, double weight_decay,
+                        hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_adam: n < 0");
+  if (step < 1) return set_error(RGP_ERR_ARG, "rgp_adam: step must be >= 1");
+  if (n == 0) return RGP_OK;
+  const AdamArgs a = adam_args(step, lr, beta1, beta2, eps, weight_decay);
+  const bool al = ((((uintptr_t)param) | ((uintptr_t)grad) | ((uintptr_t)exp_avg) | ((uintptr_t)exp_avg_sq)) & 15) == 0;
+  const int64_t n4 = al ? n >> 2 : 0;
+  auto grid = [](int64_t work) {
+    const int64_t b = (work + 255) / 256;
+    return (int)(b < 1 ? 1 : b > 8192 ? 8192 : b);
+  };
+  if (n4 > 0)
+    hipLaunchKernelGGL(k_adam4, dim3(grid(n4)), dim3(256), 0, s, n4, (float4*)param, (const float4*)grad,
+                       (float4*)exp_avg, (float4*)exp_avg_sq, a);
+  if (4 * n4 < n)
+    hipLaunchKernelGGL(k_adam1, dim3(grid(n - 4 * n4)), dim3(256), 0, s, 4 * n4, n, param, grad, exp_avg,
+                       exp_avg_sq, a);
+  return check_launch("rgp_adam");
+}
+
 extern "C" int rgp_sh_backward_adam_full(int n, int K, int sh_dim, const rgp_camera* cam,
                                          const float* pws, const float* dw, float* shs, float* exp_avg,
                                          float* exp_avg_sq, int64_t step, double lr, double beta1,
@@ -1167,18 +1233,7 @@ extern "C" int rgp_sh_backward_adam_full(int n, int K, int sh_dim, const rgp_cam
   if ((((uintptr_t)shs) | ((uintptr_t)exp_avg) | ((uintptr_t)exp_avg_sq)) & 15)
     return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_full: shs and moments must be 16-byte aligned");
   if (n == 0) return RGP_OK;
-  // scalars as torch.optim.Adam forms them: in double from the double
-  // hyper-parameters, then rounded once to float
-  const double bc1 = 1.0 - std::pow(beta1, (double)step);
-  const double bc2 = 1.0 - std::pow(beta2, (double)step);
-  AdamArgs a;
-  a.neg_step = (float)(-lr / bc1);
-  a.omb1 = (float)(1.0 - beta1);
-  a.b2 = (float)beta2;
-  a.omb2 = (float)(1.0 - beta2);
-  a.bc2_sqrt = (float)std::sqrt(bc2);
-  a.eps = (float)eps;
-  a.wd = (float)weight_decay;
+  const AdamArgs a = adam_args(step, lr, beta1, beta2, eps, weight_decay);
   const int blocks = (n + 63) / 64;
 #define RGP_ADAM(SH)                                                                              \
   RGP_K_SWITCH(K, hipLaunchKernelGGL((k_sh_adam_full<KK, SH>), dim3(blocks), dim3(256), 0, s, n, cam, \

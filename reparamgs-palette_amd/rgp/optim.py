@@ -18,8 +18,9 @@ element by element:
 Parameters listed in `fused_sh` must be the tensor passed as `shs` to
 WeightGSFunctionTest (a float32, contiguous (N, sh_dim*K) CUDA tensor).  Every
 other parameter, and a fused one whose gradient also arrives the usual way
-(several backward passes before one step), is updated by the same formula
-with torch operations.
+(several backward passes before one step), is updated by the same formula:
+in one kernel pass (rgp_adam) for contiguous float32 GPU tensors, with torch
+operations otherwise.
 """
 import math
 
@@ -90,6 +91,12 @@ class PaletteAdam(torch.optim.Optimizer):
                              stream(p.device))
                     continue
                 g = p.grad if pend is None else p.grad + self._materialise(p, pend)
+                if (p.is_cuda and p.dtype == torch.float32 and g.dtype == torch.float32 and p.is_contiguous()
+                        and g.is_contiguous() and m.is_contiguous() and v.is_contiguous()):
+                    # one pass (rgp_adam): the same element update as below
+                    call("rgp_adam", I64(p.numel()), ptr(p), ptr(g), ptr(m), ptr(v), I64(t), float(lr), float(b1),
+                         float(b2), float(eps), float(wd), stream(p.device))
+                    continue
                 if wd != 0:
                     g = g.add(p, alpha=wd)
                 m.lerp_(g, 1 - b1)

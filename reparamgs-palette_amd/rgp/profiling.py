@@ -9,8 +9,13 @@ import torch
 
 
 class Spans:
+    """enabled turns recording on; only (a set of names, or None for all)
+    limits it to some spans.  Each recorded event is a barrier packet on the
+    stream, so a timed region should record as few as it needs."""
+
     def __init__(self):
         self.enabled = False
+        self.only = None
         self.events = {}
 
     def reset(self):
@@ -18,7 +23,7 @@ class Spans:
 
     @contextlib.contextmanager
     def span(self, name):
-        if not self.enabled:
+        if not self.enabled or (self.only is not None and name not in self.only):
             yield
             return
         s = torch.cuda.current_stream()

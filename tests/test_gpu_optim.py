@@ -104,3 +104,30 @@ def test_palette_adam_in_the_finetune_step(device, monkeypatch):
     assert shs.grad is not None
     opt.step()
     assert torch.isfinite(shs).all()
+
+
+@pytest.mark.parametrize("n,offset,wd", [(1 << 20, 0, 0.0), (1000003, 0, 1e-2), (4099, 1, 0.0)])
+def test_flat_adam_matches_torch_adam(device, n, offset, wd):
+    """rgp_adam (one pass; float4 when aligned, the scalar path at a 4-byte
+    offset and for the tail) against torch.optim.Adam over three steps."""
+    print(f"case: rgp_adam n={n} offset={offset} wd={wd}", flush=True)
+    rng = np.random.default_rng(7)
+    base = torch.empty(n + offset, dtype=torch.float32, device=device)
+    p_f = base[offset:]
+    p_f.copy_(T(rng.normal(0, 0.1, n).astype(np.float32), device))
+    m_f, v_f = torch.zeros_like(p_f), torch.zeros_like(p_f)
+    p_t = p_f.clone().requires_grad_(True)
+    ref = torch.optim.Adam([p_t], lr=3e-3, betas=(0.9, 0.999), eps=1e-8, weight_decay=wd, foreach=False)
+    for step in range(1, 4):
+        g = T(rng.normal(0, 1e-2, n).astype(np.float32), device)
+        call("rgp_adam", I64(n), ptr(p_f), ptr(g), ptr(m_f), ptr(v_f), I64(step), 3e-3, 0.9, 0.999, 1e-8, wd,
+             stream(device))
+        p_t.grad = g.clone()
+        ref.step()
+    torch.cuda.synchronize()
+    st = ref.state[p_t]
+    # torch rounds g + wd * p once (a contracted multiply-add), the kernel twice
+    tol = 1e-6 if wd == 0 else 1e-5
+    close(p_f, p_t, "param", tol)
+    close(m_f, st["exp_avg"], "exp_avg", tol)
+    close(v_f, st["exp_avg_sq"], "exp_avg_sq", tol)

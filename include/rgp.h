@@ -173,6 +173,18 @@ int rgp_draw_backward(int width, int height, int dim, const int32_t* ranges, con
                       const float* records, const int32_t* contrib, const float* final_tau,
                       const float* dlayer, float* dfeat, hipStream_t stream);
 
+/* Colour backward with geometry gradients: draw_color_B (kernel.cu:1176-1289,
+ * gausplat.cu:111-155) and the vanilla trainer's drawB
+ * (vanilla_3dgs/gsplatcu/kernel.cu:811-971), which adds dL/dfinal_tau.
+ * records are rgp_pack_records of (us, cinv2ds, alphas, colors) with dim 3;
+ * dimage (3,H,W); dfinal_tau (H,W) or NULL.  dus (N,2), dcinv (N,3),
+ * dalphas (N), dcolors (N,3) must be zero-filled; the derivatives of the 0.99
+ * clamp and of max(0, .) are not taken, as in the reference. */
+int rgp_draw_color_backward(int width, int height, const int32_t* ranges, const int32_t* ids,
+                            const float* records, const int32_t* contrib, const float* final_tau,
+                            const float* dimage, const float* dfinal_tau, float* dus, float* dcinv,
+                            float* dalphas, float* dcolors, hipStream_t stream);
+
 /* Replay form of draw_weight_B's result: the forward recurrence is replayed
  * front to back and every contributing (pixel, entry) adds
  * dlayer[k,p] * tau * alpha' to dfeat, tau being the transmittance in front

@@ -32,6 +32,10 @@ How each function is pinned (tests/test_oracle.py):
                       equals 1 - final_tau up to rounding; FP32 vs FP64.
   draw_backward       float64 central finite differences of sum(dL * layer)
                       w.r.t. the per-Gaussian weights (exactly linear).
+  draw_color_backward float64 central finite differences of
+                      sum(dL/dimage * image) + sum(dL/dtau * final_tau)
+                      w.r.t. every Gaussian's u, cinv, alpha and colour, the
+                      tile lists held fixed, no alpha' near the 0.002 cut.
   mix                 linear identity mix(render(w)) = render(bake(w, P))
                       (gsmodel.py:488-509) and a numpy einsum.
   log1pf              against libm log1p in double: max error <= 4 ulp over
@@ -357,6 +361,22 @@ def draw_backward(width, height, K, ranges, ids, us, cinv, alphas, contrib, fina
         _p(_c(alphas, dt).reshape(-1)), _p(_c(contrib, i32)), _p(_c(final_tau, dt)), _p(_c(dlayer, dt)), mode,
         _p(grad), _p(gabs), _p(grad_t))
     return grad_t if mode == 1 else (grad, gabs)
+
+
+def draw_color_backward(width, height, ranges, ids, us, cinv, alphas, colors, contrib, final_tau, dimg, dtau=None,
+                        dt=f32):
+    """vanilla drawB (vanilla_3dgs/gsplatcu/kernel.cu:811-971): per-Gaussian
+    (N, 9) gradient sums [du (2), dcinv (3), dalpha, dcolor (3)] and the sums of
+    |terms|, in float64."""
+    us, cinv, alphas, colors = _c(us, dt), _c(cinv, dt), _c(alphas, dt), _c(colors, dt)
+    ft, dimg = _c(final_tau, dt), _c(dimg, dt)
+    dtau = None if dtau is None else _c(dtau, dt)
+    n = us.shape[0]
+    grad, gabs = np.zeros((n, 9), f64), np.zeros((n, 9), f64)
+    getattr(lib(), "orc_draw_color_backward_" + _sfx(dt))(width, height, _p(_c(ranges, i32)), _p(_c(ids, i32)), _p(us),
+                                                          _p(cinv), _p(alphas), _p(colors), _p(_c(contrib, i32)),
+                                                          _p(ft), _p(dimg), _p(dtau), _p(grad), _p(gabs))
+    return grad, gabs
 
 
 def mix(palette, W, dt=f32):

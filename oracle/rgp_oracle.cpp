@@ -659,6 +659,72 @@ void draw_backward(int width, int height, int K, const int32_t* ranges, const in
 }
 
 // ---------------------------------------------------------------------------
+// Colour backward with geometry gradients (vanilla_3dgs/gsplatcu/kernel.cu:
+// 811-971, drawB): per pixel, the list in reverse from its contrib, tau
+// recovered by division from final_tau, the colour behind accumulated as
+// gamma = a' c + (1 - a') gamma.  For each contributing (pixel, entry):
+//   dL/da' = dot(dL/dgamma, tau (c - gamma)) + dL/dfinal_tau (-final_tau / (1 - a'))
+//   dL/dalpha = dL/da' g            dL/dc = dL/dgamma (a' tau)
+//   dL/du = dL/da' ((-cx dx - cy dy) a', (-cy dx - cz dy) a')
+//   dL/dcinv = dL/da' (-0.5 a' dx^2, -1 a' dx dy, -0.5 a' dy^2)
+// (the derivatives of the 0.99 clamp and of the max(0, .) are not taken,
+// as in the reference).  grad/gabs (N, 9) = [du (2), dcinv (3), dalpha,
+// dcolor (3)], each term summed in double.  dtau may be null (zero).
+// ---------------------------------------------------------------------------
+template <typename T>
+void draw_color_backward(int width, int height, const int32_t* ranges, const int32_t* ids, const T* us,
+                         const T* cinv, const T* alphas, const T* colors, const int32_t* contrib,
+                         const T* final_tau, const T* dimg, const T* dtau, double* grad, double* gabs) {
+  const int gx = (width + BLOCK - 1) / BLOCK, gy = (height + BLOCK - 1) / BLOCK;
+  const size_t HW = (size_t)width * height;
+  for (int tile = 0; tile < gx * gy; ++tile) {
+    const int tx = tile % gx, ty = tile / gx;
+    const int r0 = ranges[2 * tile], r1 = ranges[2 * tile + 1];
+    const int gs_num = r1 - r0;
+    if (gs_num == 0) continue;
+    for (int py = ty * BLOCK; py < std::min(height, ty * BLOCK + BLOCK); ++py)
+      for (int px = tx * BLOCK; px < std::min(width, tx * BLOCK + BLOCK); ++px) {
+        const size_t pix = (size_t)py * width + px;
+        const T dl[3] = {dimg[pix], dimg[HW + pix], dimg[2 * HW + pix]};
+        const T dlt = dtau ? dtau[pix] : (T)0;
+        const T ft = final_tau[pix];
+        T tau = ft;
+        T gam[3] = {0, 0, 0};
+        const int cont = contrib[pix];
+        for (int i = 0; i < gs_num; ++i) {
+          if (i < gs_num - cont) continue;
+          const int g = ids[r1 - i - 1];
+          const T dx = us[2 * g] - (T)px, dy = us[2 * g + 1] - (T)py;
+          const T cx = cinv[3 * g], cy = cinv[3 * g + 1], cz = cinv[3 * g + 2];
+          const T gg = ex((T)-0.5f * maha_sq(cx, cy, cz, dx, dy));
+          const T ap = fmn((T)0.99f, alphas[g] * gg);
+          if (ap < (T)0.002f) continue;
+          tau = tau / ((T)1 - ap);
+          const T* c = colors + 3 * g;
+          const T dg[3] = {tau * (c[0] - gam[0]), tau * (c[1] - gam[1]), tau * (c[2] - gam[2])};
+          const T dap_c = (dl[0] * dg[0] + dl[1] * dg[1]) + dl[2] * dg[2];
+          const T dap_t = dlt * (-ft / ((T)1 - ap));
+          const T dap = dap_c + dap_t;
+          const T at = ap * tau;
+          T term[9];
+          term[0] = dap * ((-cx * dx - cy * dy) * ap);
+          term[1] = dap * ((-cy * dx - cz * dy) * ap);
+          term[2] = dap * (((T)-0.5f * ap) * (dx * dx));
+          term[3] = dap * (((T)-1.0f * ap) * (dx * dy));
+          term[4] = dap * (((T)-0.5f * ap) * (dy * dy));
+          term[5] = dap * gg;
+          for (int k = 0; k < 3; ++k) term[6 + k] = dl[k] * at;
+          for (int e = 0; e < 9; ++e) {
+            grad[(size_t)g * 9 + e] += (double)term[e];
+            gabs[(size_t)g * 9 + e] += std::fabs((double)term[e]);
+          }
+          for (int k = 0; k < 3; ++k) gam[k] = ap * c[k] + ((T)1 - ap) * gam[k];
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // A13 palette mix: RGB[c, p] = sum_k P[k, c] W[k, p], k ascending, no
 // contraction.  Equal in exact arithmetic to baking the palette into the
 // per-Gaussian colours and re-rendering (gsmodel.py:488-509 + 13-26).
@@ -975,6 +1041,13 @@ void sh2weight_jac(int n, int K, int sh_dim, const T* shs, const T* pws, const T
                              double* gabs, T* grad_t) {                                           \
     orc::draw_backward<T>(w, h, K, ranges, ids, us, cinv, alphas, contrib, final_tau, dlayer,    \
                           mode, grad, gabs, grad_t);                                              \
+  }                                                                                               \
+  void orc_draw_color_backward_##S(int w, int h, const int32_t* ranges, const int32_t* ids,         \
+                                   const T* us, const T* cinv, const T* alphas, const T* colors,  \
+                                   const int32_t* contrib, const T* final_tau, const T* dimg,     \
+                                   const T* dtau, double* grad, double* gabs) {                   \
+    orc::draw_color_backward<T>(w, h, ranges, ids, us, cinv, alphas, colors, contrib, final_tau,  \
+                                dimg, dtau, grad, gabs);                                          \
   }                                                                                               \
   void orc_mix_##S(int K, int64_t HW, const T* pal, const T* W, T* rgb) {                         \
     orc::mix<T>(K, HW, pal, W, rgb);                                                              \

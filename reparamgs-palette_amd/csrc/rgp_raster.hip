@@ -318,6 +318,146 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
 }
 
 // ---------------------------------------------------------------------------
+// Colour backward with geometry gradients (vanilla drawB,
+// vanilla_3dgs/gsplatcu/kernel.cu:811-971): the reference form's reverse
+// traversal with tau by division from final_tau, plus the colour behind each
+// entry, gamma = a' c + (1 - a') gamma.  Per contributing (pixel, entry):
+//   dL/da' = dot(dL/dgamma, tau (c - gamma)) + dL/dtau_f (-tau_f / (1 - a'))
+//   du = dL/da' ((-cx dx - cy dy) a', (-cy dx - cz dy) a')
+//   dcinv = dL/da' (-0.5 a' dx^2, -a' dx dy, -0.5 a' dy^2)
+//   dalpha = dL/da' g,  dcolor = dL/dgamma a' tau
+// (the 0.99 clamp and the max(0, .) are not differentiated, as in the
+// reference).  The nine per-entry sums go out as one atomic instruction from
+// lanes 0..8, each lane to its own array.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void flush_geom(const float* part, float* __restrict__ dus, float* __restrict__ dcinv,
+                                           float* __restrict__ dalpha, float* __restrict__ dcolor, int g,
+                                           int lane) {
+  float v = 0.f;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const float r = row_sum16(part[k]);
+    v = (lane & 15) == k ? r : v;
+  }
+  v = across_rows(v);
+  if (lane < 9 && v != 0.f) {
+    float* dst = lane < 2 ? dus + (size_t)g * 2 + lane
+               : lane < 5 ? dcinv + (size_t)g * 3 + (lane - 2)
+               : lane == 5 ? dalpha + g
+                           : dcolor + (size_t)g * 3 + (lane - 6);
+    unsafeAtomicAdd(dst, v);
+  }
+}
+
+__global__ __launch_bounds__(64) void k_draw_color_bwd(
+    int W, int H, int gx, int tiles, const int2* __restrict__ ranges, const int32_t* __restrict__ ids,
+    const float4* __restrict__ recs, const int32_t* __restrict__ contrib, const float* __restrict__ final_tau,
+    const float* __restrict__ dimg, const float* __restrict__ dtau, float* __restrict__ dus,
+    float* __restrict__ dcinv, float* __restrict__ dalpha, float* __restrict__ dcolor) {
+  constexpr int F4 = RecStride<3>::f4;
+  __shared__ float4 s_rec[64 * F4];
+  __shared__ int32_t s_id[64];
+  const int tile = xcd_tile(blockIdx.x, tiles);
+  const int ty = tile / gx, tx = tile - ty * gx;
+  const int lane = threadIdx.x;
+  const int px = tx * kTile + (lane & 15);
+  const int y0 = ty * kTile + (lane >> 4);
+  const float fpx = (float)px;
+  const int2 rg = ranges[tile];
+  if (rg.y - rg.x <= 0) return;
+
+  const size_t HW = (size_t)W * H;
+  float dl[kSlots][3], gam[kSlots][3];
+  float tau[kSlots], ft[kSlots], dlt[kSlots], fpy[kSlots];
+  int cont[kSlots];
+  int maxc = 0;
+  uint32_t smax_bits[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    const int py = y0 + 4 * j;
+    const bool in = px < W && py < H;
+    const size_t pix = (size_t)py * W + px;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      dl[j][k] = in ? dimg[k * HW + pix] : 0.f;
+      gam[j][k] = 0.f;
+    }
+    ft[j] = in ? final_tau[pix] : 0.f;
+    dlt[j] = in && dtau ? dtau[pix] : 0.f;
+    tau[j] = ft[j];
+    cont[j] = in ? contrib[pix] : 0;
+    fpy[j] = (float)py;
+    int c = cont[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c = max(c, __shfl_xor(c, o));
+    smax_bits[j] = (uint32_t)c;
+    maxc = max(maxc, c);
+  }
+  int nid = 0;
+  RecRegs<F4> nxt = load_rec<F4>(recs, ids, rg.x + maxc - 1 - lane, rg.x, rg.x + maxc, &nid);
+  for (int top = rg.x + maxc; top > rg.x; top -= 64) {
+    const int lo = max(rg.x, top - 64);
+    const int cnt = top - lo;
+    __syncthreads();
+    if (lane < cnt) {
+      stage_rec<F4>(s_rec, lane, nxt, ty);
+      s_id[lane] = nid;
+    }
+    __syncthreads();
+    nxt = load_rec<F4>(recs, ids, top - 65 - lane, rg.x, rg.y, &nid);
+    for (int jj = 0; jj < cnt; ++jj) {
+      const int fi = top - 1 - jj - rg.x;  // index of the entry in forward order
+      const float4 g0 = s_rec[jj];
+      const float4 g1 = s_rec[64 + jj];
+      const float4 g2 = s_rec[128 + jj];
+      uint32_t slots = uniform_bits(g1.w);
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j)
+        if ((uint32_t)fi >= smax_bits[j]) slots &= ~(1u << j);
+      if (slots == 0) continue;
+      const float cx = g0.z, cy = 0.5f * g0.w, cz = g1.x;
+      const float dx = g0.x - fpx;
+      const float ta0 = cx * dx * dx;
+      const float tc0 = g0.w * dx;
+      const float col[3] = {g2.x, g2.y, g2.z};
+      float part[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) part[k] = 0.f;
+      bool on = false;
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) {
+        if (!(slots & (1u << j))) continue;
+        if (fi >= cont[j]) continue;
+        const float dy = g0.y - fpy[j];
+        const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(cz * dy, dy, ta0)));
+        if (m > g1.z) continue;
+        const float gg = gauss_of(m);
+        const float ap = fminf(0.99f, g1.y * gg);
+        if (ap < 0.002f) continue;
+        tau[j] = tau[j] / (1.f - ap);
+        const float t = tau[j];
+        const float dg0 = t * (col[0] - gam[j][0]), dg1 = t * (col[1] - gam[j][1]), dg2 = t * (col[2] - gam[j][2]);
+        const float dap_c = (dl[j][0] * dg0 + dl[j][1] * dg1) + dl[j][2] * dg2;
+        const float dap = dap_c + dlt[j] * (-ft[j] / (1.f - ap));
+        const float at = ap * t;
+        part[0] += dap * ((-cx * dx - cy * dy) * ap);
+        part[1] += dap * ((-cy * dx - cz * dy) * ap);
+        part[2] += dap * ((-0.5f * ap) * (dx * dx));
+        part[3] += dap * ((-1.0f * ap) * (dx * dy));
+        part[4] += dap * ((-0.5f * ap) * (dy * dy));
+        part[5] += dap * gg;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) part[6 + k] += dl[j][k] * at;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gam[j][k] = ap * col[k] + (1.f - ap) * gam[j][k];
+        on = true;
+      }
+      if (__any(on)) flush_geom(part, dus, dcinv, dalpha, dcolor, s_id[jj], lane);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // backward, replay form: the forward recurrence is run again front to back;
 // the term of a contributing entry is dL/dW_k * tau * alpha' with tau the
 // transmittance in front of it, the factor the forward applied to the
@@ -557,6 +697,20 @@ extern "C" int rgp_draw_backward(int width, int height, int dim, const int32_t* 
                                          (const float4*)records, contrib, final_tau, dlayer,
                                          dfeat));
   return check_launch("rgp_draw_backward");
+}
+
+extern "C" int rgp_draw_color_backward(int width, int height, const int32_t* ranges, const int32_t* ids,
+                                       const float* records, const int32_t* contrib, const float* final_tau,
+                                       const float* dimage, const float* dfinal_tau, float* dus, float* dcinv,
+                                       float* dalphas, float* dcolors, hipStream_t s) {
+  int gx, tiles;
+  if (int rc = draw_grid("rgp_draw_color_backward", width, height, 3, &gx, &tiles)) return rc;
+  if (!dus || !dcinv || !dalphas || !dcolors)
+    return set_error(RGP_ERR_ARG, "rgp_draw_color_backward: null gradient buffer");
+  hipLaunchKernelGGL(k_draw_color_bwd, dim3(tiles), dim3(64), 0, s, width, height, gx, tiles, (const int2*)ranges,
+                     ids, (const float4*)records, contrib, final_tau, dimage, dfinal_tau, dus, dcinv, dalphas,
+                     dcolors);
+  return check_launch("rgp_draw_color_backward");
 }
 
 extern "C" int rgp_draw_backward_replay(int width, int height, int dim, const int32_t* ranges,

@@ -220,9 +220,38 @@ def splat_color(height, width, us, cinv2ds, alphas, depths, colors, areas):
     return _splat(int(height), int(width), 3, us, cinv2ds, alphas, depths, colors, areas, 1.0)
 
 
-def splat_color_B(*args, **kwargs):
-    raise NotImplementedError("gsplatcu.splat_color_B (geometry gradients of the colour path) is not part "
-                              "of the palette-weight path and is not provided by this build")
+def _color_B(height, width, us, cinv2ds, alphas, colors, contrib, final_tau, patch_range_per_tile, gsid_per_patch,
+             dloss_dgammas, dloss_dfinal_tau):
+    require_cuda(us, cinv2ds, alphas, colors, contrib, final_tau, patch_range_per_tile, dloss_dgammas,
+                 dloss_dfinal_tau)
+    n = us.shape[0]
+    records = ops.pack_records(us, cinv2ds, alphas, colors.reshape(n, 3))
+    ids = gsid_per_patch if gsid_per_patch.numel() > 0 else torch.zeros(1, dtype=torch.int32, device=us.device)
+    dus, dcinv, dalpha, dcolor = ops.draw_color_backward(int(width), int(height), patch_range_per_tile, ids, records,
+                                                         contrib, final_tau, dloss_dgammas, dloss_dfinal_tau, n)
+    return [dus.reshape(n, 1, 2), dcinv.reshape(n, 1, 3), dalpha.reshape(n, 1, 1), dcolor.reshape(n, 1, 3)]
+
+
+def splat_color_B(height, width, us, cinv2ds, alphas, depths, colors, contrib, final_tau, patch_range_per_tile,
+                  gsid_per_patch, dloss_dgammas):
+    """gausplat.cu:111-155 (draw_color_B, kernel.cu:1176-1289): [dL/dus (N,1,2),
+    dL/dcinv2ds (N,1,3), dL/dalphas (N,1,1), dL/dcolors (N,1,3)]."""
+    return _color_B(height, width, us, cinv2ds, alphas, colors, contrib, final_tau, patch_range_per_tile,
+                    gsid_per_patch, dloss_dgammas, None)
+
+
+def splat(height, width, us, cinv2ds, alphas, depths, colors, areas):
+    """The vanilla module's name for the colour forward
+    (vanilla_3dgs/gsplatcu/gausplat.cu:24-112): splat_color."""
+    return splat_color(height, width, us, cinv2ds, alphas, depths, colors, areas)
+
+
+def splatB(height, width, us, cinv2ds, alphas, depths, colors, contrib, final_tau, patch_range_per_tile,
+           gsid_per_patch, dloss_dgammas, dloss_dfinal_tau):
+    """vanilla_3dgs/gsplatcu/gausplat.cu:114-159 (drawB, kernel.cu:811-971):
+    splat_color_B plus the gradient of a loss on final_tau, dL/dfinal_tau (H,W)."""
+    return _color_B(height, width, us, cinv2ds, alphas, colors, contrib, final_tau, patch_range_per_tile,
+                    gsid_per_patch, dloss_dgammas, dloss_dfinal_tau)
 
 
 def lab2rgb(lab_image):

@@ -154,6 +154,22 @@ def draw_backward_replay(width, height, dim, ranges, ids, records, dlayer, n):
     return dfeat
 
 
+def draw_color_backward(width, height, ranges, ids, records, contrib, final_tau, dimage, dfinal_tau, n):
+    """Colour backward with geometry gradients (rgp_draw_color_backward):
+    dL/dus (N,2), dL/dcinv2ds (N,3), dL/dalphas (N,), dL/dcolors (N,3)."""
+    dev = records.device
+    dus = torch.zeros((n, 2), dtype=torch.float32, device=dev)
+    dcinv = torch.zeros((n, 3), dtype=torch.float32, device=dev)
+    dalpha = torch.zeros((n,), dtype=torch.float32, device=dev)
+    dcolor = torch.zeros((n, 3), dtype=torch.float32, device=dev)
+    ranges, ids, contrib, final_tau, dimage = i32c(ranges), i32c(ids), i32c(contrib), f32c(final_tau), f32c(dimage)
+    dtau = None if dfinal_tau is None else f32c(dfinal_tau)
+    with SPANS.span("draw_color_backward"):
+        call("rgp_draw_color_backward", int(width), int(height), ptr(ranges), ptr(ids), ptr(records), ptr(contrib),
+             ptr(final_tau), ptr(dimage), ptr(dtau), ptr(dus), ptr(dcinv), ptr(dalpha), ptr(dcolor), stream(dev))
+    return dus, dcinv, dalpha, dcolor
+
+
 class CapacityCache:
     """Pair capacity per (n, width, height): grows to 1.25x the largest pair
     count seen, so repeated renders of a scene reuse one workspace size."""

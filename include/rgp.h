@@ -108,9 +108,10 @@ int rgp_sh2weight_fast_backward(int n, int K, int P, int Q, int rank, const floa
                                 float* dA, float* dB, hipStream_t stream);
 
 /* gsc.sh2Color, ext.cpp:109-112 / gausplat.cu:547-584 / kernel.cu:1040-1169
- * (colour path, used to render frozen targets). */
+ * (colour path, used to render frozen targets).  calc_J outputs (both or
+ * neither): dcolor_dshs (N, sh_dim), dcolor_dpws (N, 3, 3). */
 int rgp_sh2color(int n, int sh_dim, const float* shs, const float* pws, const float* twc,
-                 float* colors, hipStream_t stream);
+                 float* colors, float* dcolor_dshs, float* dcolor_dpws, hipStream_t stream);
 
 /* ---- binning: getRects + scan + createKeys + sort + getRanges ----------- */
 

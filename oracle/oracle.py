@@ -23,6 +23,9 @@ How each function is pinned (tests/test_oracle.py):
                       at rank 1, A (N,P,1) / B (N,1,Q) reduce to (N,P) / (N,Q).
   sh2weight_fast_bwd  adjoint identity <dw, J dtheta> = <J^T dw, dtheta> and
                       float64 central finite differences.
+  sh2color_jac        float64 central differences of sh2color in pws (plus
+                      the reference's dY_9/dy term, kernel.cu:1152, pinned
+                      exactly) and in the coefficients (linear).
   get_rects/bin_sort  brute force on tiny inputs: per-Gaussian tile sets from
                       an independent enumeration, a Python stable sort of
                       (tile, uint32(depth*1000)) keys, ranges by scanning,
@@ -302,6 +305,16 @@ def sh2weight_jac(shs, pws, twc, K, dt=f32):
     out = np.zeros((n, K, 3), dt)
     getattr(lib(), "orc_sh2weight_jac_" + _sfx(dt))(n, K, sh_dim, _p(shs), _p(pws), _p(twc), _p(out))
     return out
+
+
+def sh2color_jac(shs, pws, twc, dt=f32):
+    """sh2Color's calc_J outputs: dcolor_dshs (N,1,sh_dim), dcolor_dpws (N,3,3)."""
+    shs, pws, twc = _c(shs, dt), _c(pws, dt), _c(twc, dt)
+    n = pws.shape[0]
+    sh_dim = shs.shape[1] // 3
+    dshs, dpws = np.zeros((n, 1, sh_dim), dt), np.zeros((n, 3, 3), dt)
+    getattr(lib(), "orc_sh2color_jac_" + _sfx(dt))(n, sh_dim, _p(shs), _p(pws), _p(twc), _p(dshs), _p(dpws))
+    return dshs, dpws
 
 
 def grid_dims(width, height):

@@ -416,6 +416,72 @@ void sh2color(int n, int sh_dim, const T* shs, const T* pws, const T* twc, T* co
   }
 }
 
+// sh2Color's calc_J outputs (kernel.cu:1112-1167): dcolor_dshs (N, sh_dim) =
+// (SH_C0_0, Y_1, ...), and dcolor_dpws (N,3,3) = dc_dr dr_dpw accumulated from
+// 0.  dc_dr is common.cuh's float3 arithmetic per colour channel: a double
+// constant times a float is a double product rounded to float where it meets
+// the float3, then float products and sums left to right (F below).  The Y_9
+// term of dc_dr1 is C30 (3x^2 - 2y^2) (kernel.cu:1152), as in sh2Weight.
+template <typename T>
+void sh2color_jac(int n, int sh_dim, const T* shs, const T* pws, const T* twc, T* dshs, T* dpws) {
+  auto F = [](double v) { return (T)v; };
+  for (int i = 0; i < n; ++i) {
+    const T* sh = shs + (size_t)i * sh_dim * 3;
+    T Y[16] = {0};
+    if (sh_dim > 1) sh_dir_basis(pws + i * 3, twc, sh_dim, Y);
+    dshs[(size_t)i * sh_dim] = (T)SH_C0_0;
+    for (int l = 1; l < sh_dim; ++l) dshs[(size_t)i * sh_dim + l] = Y[l];
+    T out[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (sh_dim > 1) {
+      const T d0 = pws[3 * i] - twc[0], d1 = pws[3 * i + 1] - twc[1], d2 = pws[3 * i + 2] - twc[2];
+      const T ni = (T)1 / std::sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+      const T x = d0 * ni, y = d1 * ni, z = d2 * ni;
+      const T xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+      const T n3 = (ni * ni) * ni;
+      const T r00 = ((-d0) * d0) * n3 + ni, r11 = ((-d1) * d1) * n3 + ni, r22 = ((-d2) * d2) * n3 + ni;
+      const T r01 = ((-d0) * d1) * n3, r02 = ((-d0) * d2) * n3, r12 = ((-d1) * d2) * n3;
+      const T R[9] = {r00, r01, r02, r01, r11, r12, r02, r12, r22};
+      for (int c = 0; c < 3; ++c) {
+        auto h = [&](int l) { return sh[3 * l + c]; };
+        T D[3] = {0, 0, 0};
+        D[0] = D[0] + h(3) * F(SH_C1_2);
+        D[1] = D[1] + h(1) * F(SH_C1_0);
+        D[2] = D[2] + h(2) * F(SH_C1_1);
+        if (sh_dim > 4) {
+          const double X = (double)x, Yd = (double)y, Z = (double)z;
+          D[0] = D[0] + (((h(4) * F(SH_C2_0 * Yd) - h(6) * F((SH_C2_2 * 2) * X)) + h(7) * F(SH_C2_3 * Z)) +
+                         h(8) * F((SH_C2_4 * 2) * X));
+          D[1] = D[1] + (((h(4) * F(SH_C2_0 * X) + h(5) * F(SH_C2_1 * Z)) - h(6) * F((SH_C2_2 * 2.0) * Yd)) -
+                         h(8) * F((SH_C2_4 * 2) * Yd));
+          D[2] = D[2] + ((h(5) * F(SH_C2_1 * Yd) + h(6) * F((SH_C2_2 * 4.0) * Z)) + h(7) * F(SH_C2_3 * X));
+          if (sh_dim > 9) {
+            const T a[7] = {((h(9) * F(6.0 * SH_C3_0)) * x) * y, (h(10) * F(SH_C3_1)) * yz,
+                            (h(11) * F(2 * SH_C3_2)) * xy,         (h(12) * F(6.0 * SH_C3_3)) * xz,
+                            (h(13) * F(SH_C3_4)) * (((T)4 * zz - (T)3 * xx) - yy),
+                            (h(14) * F(2 * SH_C3_5)) * xz,         (h(15) * F(SH_C3_6)) * ((T)3 * xx - (T)3 * yy)};
+            D[0] = D[0] + ((((((a[0] + a[1]) - a[2]) - a[3]) + a[4]) + a[5]) + a[6]);
+            const T b[7] = {(h(9) * F(SH_C3_0)) * ((T)3 * xx - (T)2 * yy), (h(10) * F(SH_C3_1)) * xz,
+                            (h(11) * F(SH_C3_2)) * (((T)4 * zz - xx) - (T)3 * yy),
+                            (h(12) * F(6.0 * SH_C3_3)) * yz,     (h(13) * F(SH_C3_4)) * ((T)-2 * xy),
+                            (h(14) * F(2 * SH_C3_5)) * yz,       (h(15) * F(6.0 * SH_C3_6)) * xy};
+            D[1] = D[1] + ((((((b[0] + b[1]) + b[2]) - b[3]) + b[4]) - b[5]) - b[6]);
+            const T cc[5] = {(h(10) * F(SH_C3_1)) * xy, (h(11) * F(8.0 * SH_C3_2)) * yz,
+                             (h(12) * F(SH_C3_3)) * (((T)6 * zz - (T)3 * xx) - (T)3 * yy),
+                             (h(13) * F(8.0 * SH_C3_4)) * xz, (h(14) * F(SH_C3_5)) * (xx - yy)};
+            D[2] = D[2] + ((((cc[0] + cc[1]) + cc[2]) + cc[3]) + cc[4]);
+          }
+        }
+        for (int j = 0; j < 3; ++j) {
+          T acc = 0;
+          for (int k = 0; k < 3; ++k) acc = acc + D[k] * R[3 * k + j];
+          out[3 * c + j] = acc;
+        }
+      }
+    }
+    for (int e = 0; e < 9; ++e) dpws[(size_t)i * 9 + e] = out[e];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // A7 getRects (kernel.cu:127-162): rect = (x0, y0, x1, y1) and tile count,
 // culling empty rects in place.  DIV_ROUND_UP on a float argument expands to
@@ -1019,6 +1085,10 @@ void sh2weight_jac(int n, int K, int sh_dim, const T* shs, const T* pws, const T
   }                                                                                               \
   void orc_sh2color_##S(int n, int sh_dim, const T* shs, const T* pws, const T* twc, T* colors) { \
     orc::sh2color<T>(n, sh_dim, shs, pws, twc, colors);                                           \
+  }                                                                                               \
+  void orc_sh2color_jac_##S(int n, int sh_dim, const T* shs, const T* pws, const T* twc, T* dshs,  \
+                            T* dpws) {                                                            \
+    orc::sh2color_jac<T>(n, sh_dim, shs, pws, twc, dshs, dpws);                                   \
   }                                                                                               \
   void orc_get_rects_##S(int n, const T* us, int32_t* areas, T* depths, int gx, int gy,           \
                          uint32_t* rects, uint32_t* counts) {                                     \

@@ -137,6 +137,98 @@ __global__ __launch_bounds__(256) void k_sh2weight_fast(int n, int P, int Q, int
     for (int l = 0; l < L; ++l) ylm[(size_t)i * L + l] = Y[l + 1];
 }
 
+// sh2Color's calc_J outputs (kernel.cu:1112-1167).  dcolor_dshs (N, sh_dim):
+// SH_C0_0 and the basis values the forward multiplies the coefficients by.
+// dcolor_dpws (N, 3, 3) = dc_dr (colour c, direction component r) times
+// dr_dpw, accumulated from 0 (matrix.cuh:53-70).  dc_dr follows the float3
+// arithmetic of common.cuh: a double constant times a float is formed in
+// double and rounded to float when it meets a float3 (operator* takes
+// const float&), the float3 products and sums are float, left to right.
+// Its Y_9 term is C30 (3x^2 - 2y^2) (kernel.cu:1152), as in sh2Weight.
+__global__ __launch_bounds__(256) void k_sh2color_jac(int n, int sh_dim, const float* __restrict__ shs,
+                                                      const float* __restrict__ pws,
+                                                      const float* __restrict__ twc,
+                                                      float* __restrict__ dcolor_dshs,
+                                                      float* __restrict__ dcolor_dpws) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* sh = shs + (size_t)i * sh_dim * 3;
+  float Y[16];
+#pragma unroll
+  for (int l = 0; l < 16; ++l) Y[l] = 0.f;
+  const float tw[3] = {twc[0], twc[1], twc[2]};
+  if (sh_dim > 1) sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], tw, sh_dim, Y);
+  dcolor_dshs[(size_t)i * sh_dim] = (float)SH_C0_0;
+  for (int l = 1; l < sh_dim; ++l) dcolor_dshs[(size_t)i * sh_dim + l] = Y[l];
+  float out[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) out[e] = 0.f;
+  if (sh_dim > 1) {
+    const float d0 = pws[3 * i] - tw[0], d1 = pws[3 * i + 1] - tw[1], d2 = pws[3 * i + 2] - tw[2];
+    const float normd_inv = 1.f / sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+    const float x = d0 * normd_inv, y = d1 * normd_inv, z = d2 * normd_inv;
+    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+    const float n3 = (normd_inv * normd_inv) * normd_inv;
+    const float r00 = ((-d0) * d0) * n3 + normd_inv, r11 = ((-d1) * d1) * n3 + normd_inv;
+    const float r22 = ((-d2) * d2) * n3 + normd_inv, r01 = ((-d0) * d1) * n3;
+    const float r02 = ((-d0) * d2) * n3, r12 = ((-d1) * d2) * n3;
+    const float R[9] = {r00, r01, r02, r01, r11, r12, r02, r12, r22};
+    auto F = [](double v) { return (float)v; };
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float* h = sh + c;  // coefficient l of channel c: h[3 * l]
+      float dr0 = 0.f, dr1 = 0.f, dr2 = 0.f;
+      dr0 = dr0 + h[9] * F(SH_C1_2);
+      dr1 = dr1 + h[3] * F(SH_C1_0);
+      dr2 = dr2 + h[6] * F(SH_C1_1);
+      if (sh_dim > 4) {
+        dr0 = dr0 + (((h[12] * F(SH_C2_0 * (double)y) - h[18] * F((SH_C2_2 * 2) * (double)x)) +
+                      h[21] * F(SH_C2_3 * (double)z)) +
+                     h[24] * F((SH_C2_4 * 2) * (double)x));
+        dr1 = dr1 + (((h[12] * F(SH_C2_0 * (double)x) + h[15] * F(SH_C2_1 * (double)z)) -
+                      h[18] * F((SH_C2_2 * 2.0) * (double)y)) -
+                     h[24] * F((SH_C2_4 * 2) * (double)y));
+        dr2 = dr2 + ((h[15] * F(SH_C2_1 * (double)y) + h[18] * F((SH_C2_2 * 4.0) * (double)z)) +
+                     h[21] * F(SH_C2_3 * (double)x));
+        if (sh_dim > 9) {
+          const float a0 = ((h[27] * F(6.0 * SH_C3_0)) * x) * y;
+          const float a1 = (h[30] * F(SH_C3_1)) * yz;
+          const float a2 = (h[33] * F(2 * SH_C3_2)) * xy;
+          const float a3 = (h[36] * F(6.0 * SH_C3_3)) * xz;
+          const float a4 = (h[39] * F(SH_C3_4)) * ((4.f * zz - 3.f * xx) - yy);
+          const float a5 = (h[42] * F(2 * SH_C3_5)) * xz;
+          const float a6 = (h[45] * F(SH_C3_6)) * (3 * xx - 3 * yy);
+          dr0 = dr0 + ((((((a0 + a1) - a2) - a3) + a4) + a5) + a6);
+          const float b0 = (h[27] * F(SH_C3_0)) * (3.f * xx - 2.f * yy);
+          const float b1 = (h[30] * F(SH_C3_1)) * xz;
+          const float b2 = (h[33] * F(SH_C3_2)) * ((4.f * zz - xx) - 3.f * yy);
+          const float b3 = (h[36] * F(6.0 * SH_C3_3)) * yz;
+          const float b4 = (h[39] * F(SH_C3_4)) * (-2 * xy);
+          const float b5 = (h[42] * F(2 * SH_C3_5)) * yz;
+          const float b6 = (h[45] * F(6.0 * SH_C3_6)) * xy;
+          dr1 = dr1 + ((((((b0 + b1) + b2) - b3) + b4) - b5) - b6);
+          const float c0 = (h[30] * F(SH_C3_1)) * xy;
+          const float c1 = (h[33] * F(8.0 * SH_C3_2)) * yz;
+          const float c2 = (h[36] * F(SH_C3_3)) * ((6.f * zz - 3.f * xx) - 3.f * yy);
+          const float c3 = (h[39] * F(8.0 * SH_C3_4)) * xz;
+          const float c4 = (h[42] * F(SH_C3_5)) * (xx - yy);
+          dr2 = dr2 + ((((c0 + c1) + c2) + c3) + c4);
+        }
+      }
+      const float D[3] = {dr0, dr1, dr2};
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc = acc + D[k] * R[3 * k + j];
+        out[3 * c + j] = acc;
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) dcolor_dpws[(size_t)i * 9 + e] = out[e];
+}
+
 __global__ __launch_bounds__(256) void k_sh2color(int n, int sh_dim, const float* __restrict__ shs,
                                                   const float* __restrict__ pws,
                                                   const float* __restrict__ twc,
@@ -1055,13 +1147,19 @@ extern "C" int rgp_sh2weight_fast_backward(int n, int K, int P, int Q, int rank,
 }
 
 extern "C" int rgp_sh2color(int n, int sh_dim, const float* shs, const float* pws,
-                            const float* twc, float* colors, hipStream_t s) {
+                            const float* twc, float* colors, float* dcolor_dshs, float* dcolor_dpws,
+                            hipStream_t s) {
   if (n < 0) return set_error(RGP_ERR_ARG, "rgp_sh2color: n < 0");
   if (!(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
     return set_error(RGP_ERR_ARG, "rgp_sh2color: sh_dim must be 1, 4, 9 or 16");
+  if (!dcolor_dshs != !dcolor_dpws)
+    return set_error(RGP_ERR_ARG, "rgp_sh2color: give both Jacobian outputs or neither");
   if (n == 0) return RGP_OK;
   hipLaunchKernelGGL(k_sh2color, dim3(grid1(n)), dim3(256), 0, s, n, sh_dim, shs, pws, twc,
                      colors);
+  if (dcolor_dshs)
+    hipLaunchKernelGGL(k_sh2color_jac, dim3(grid1(n)), dim3(256), 0, s, n, sh_dim, shs, pws, twc,
+                       dcolor_dshs, dcolor_dpws);
   return check_launch("rgp_sh2color");
 }
 

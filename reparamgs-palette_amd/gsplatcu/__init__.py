@@ -15,11 +15,12 @@ Differences from the reference, all deliberate:
   reference's rank-1 (N,P), (N,Q), and calc_J=False returns the weights
   instead of dereferencing null (kernel.cu:911, 961);
 * the calc_J Jacobians of project / computeCov3D / computeCov2D /
-  inverseCov2D / sh2Weight are computed by kernels that follow the
-  reference's formulas (kernel.cu:402-414, 463-500, 557-607, 656-660,
-  769-869), including sh2Weight's dY_9/dy = C30 (3x^2 - 2y^2)
-  (kernel.cu:840); sh2Color's (colour path, kernel.cu:1114-1167) is not
-  provided and asking raises.
+  inverseCov2D / sh2Weight / sh2Color are computed by kernels that follow
+  the reference's formulas (kernel.cu:402-414, 463-500, 557-607, 656-660,
+  769-869, 1112-1167), including dY_9/dy = C30 (3x^2 - 2y^2) in sh2Weight
+  and sh2Color (kernel.cu:840, 1152);
+* splat_color_B and the vanilla module's splat / splatB (with the gradient
+  of a loss on final_tau) are provided.
 """
 import torch
 
@@ -28,12 +29,7 @@ from rgp._lib import I64, RgpError, call, f32c, i32c, ptr, require_cuda, stream
 
 __all__ = ["project", "computeCov3D", "computeCov2D", "inverseCov2D", "sh2Weight", "sh2Weight_fast",
            "sh2Weight_fast_B", "sh2Color", "splat_weight", "splat_weight_B", "splat_color",
-           "splat_color_B", "lab2rgb"]
-
-
-def _no_jacobian(name):
-    raise NotImplementedError(f"gsplatcu.{name}: calc_J=True Jacobians are not provided by this build "
-                              "(the palette-weight path calls it with calc_J=False)")
+           "splat_color_B", "splat", "splatB", "lab2rgb"]
 
 
 def _inplace_f32(t):
@@ -166,16 +162,18 @@ def sh2Weight_fast_B(shs_A, shs_B, Ylm_stored, dloss_dw, num_palette, Q):
 
 
 def sh2Color(shs, pws, twc, calc_J):
-    """gausplat.cu:547-584."""
-    if calc_J:
-        _no_jacobian("sh2Color")
+    """gausplat.cu:547-584: [colors (N,3)] or, with calc_J, [colors,
+    dcolor_dshs (N,1,sh_dim), dcolor_dpws (N,3,3)]."""
     require_cuda(shs, pws, twc)
     n = pws.shape[0]
     sh_dim = shs.shape[1] // 3
-    colors = torch.zeros((n, 3), dtype=torch.float32, device=pws.device)
-    call("rgp_sh2color", n, sh_dim, ptr(f32c(shs)), ptr(f32c(pws)), ptr(f32c(twc)), ptr(colors),
-         stream(pws.device))
-    return [colors]
+    dev = pws.device
+    colors = torch.zeros((n, 3), dtype=torch.float32, device=dev)
+    dshs = torch.zeros((n, 1, sh_dim), dtype=torch.float32, device=dev) if calc_J else None
+    dpws = torch.zeros((n, 3, 3), dtype=torch.float32, device=dev) if calc_J else None
+    call("rgp_sh2color", n, sh_dim, ptr(f32c(shs)), ptr(f32c(pws)), ptr(f32c(twc)), ptr(colors), ptr(dshs),
+         ptr(dpws), stream(dev))
+    return [colors, dshs, dpws] if calc_J else [colors]
 
 
 def _splat(height, width, dim, us, cinv2ds, alphas, depths, feat, areas, empty_tau):

@@ -31,7 +31,7 @@ _SIGS = {
     "rgp_sh2weight": (I, [I, I, I, P, P, P, P, P, P, P]),
     "rgp_sh2weight_fast": (I, [I, I, I, I, I, P, P, P, P, P, P, P, P, P]),
     "rgp_sh2weight_fast_backward": (I, [I, I, I, I, I, P, P, P, P, P, P, P]),
-    "rgp_sh2color": (I, [I, I, P, P, P, P, P]),
+    "rgp_sh2color": (I, [I, I, P, P, P, P, P, P, P]),
     "rgp_bin_workspace_bytes": (SZ, [I, I64, I, I]),
     "rgp_bin_rects": (I, [I, P, P, P, I, I, P, I64, P]),
     "rgp_bin_scan": (I, [I, I, I, P, I64, P]),

@@ -1,0 +1,54 @@
+"""The vanilla 3DGS training step's autograd function on the gfx950 kernels.
+
+GSFunction restates vanilla_3dgs/gsplat/gsmodel.py:6-85: the forward runs the
+per-Gaussian stages with their calc_J Jacobians and the colour rasterizer;
+the backward runs the colour backward with geometry gradients
+(gsplatcu.splatB, rgp_draw_color_backward, including the gradient of a loss on
+final_tau) and pulls dL/d(us, cinv2d, alpha, colour) back through the
+Jacobians to dL/d(pws, shs, alphas, scales, rots) with the same products the
+reference forms (batched float32 matrix products).  The derivatives the
+reference leaves out stay out: the +-1.3 tan_fov clamp of computeCov2D, the
+0.99 clamp of alpha', max(0, .) of the Mahalanobis distance, and the
+sh2Color term dY_9/dy = C30 (3x^2 - 2y^2) (kernel.cu:1152).
+"""
+import torch
+
+import gsplatcu as gsc
+
+
+class GSFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pws, shs, alphas, scales, rots, us, cam):
+        us, pcs, depths, du_dpcs = gsc.project(pws, cam.Rcw, cam.tcw, cam.fx, cam.fy, cam.cx, cam.cy, True)
+        cov3ds, dcov3d_drots, dcov3d_dscales = gsc.computeCov3D(rots, scales, depths, True)
+        cov2ds, dcov2d_dcov3ds, dcov2d_dpcs = gsc.computeCov2D(cov3ds, pcs, cam.Rcw, depths, cam.fx, cam.fy,
+                                                               cam.width, cam.height, True)
+        colors, dcolor_dshs, dcolor_dpws = gsc.sh2Color(shs, pws, cam.twc, True)
+        cinv2ds, areas, dcinv2d_dcov2ds = gsc.inverseCov2D(cov2ds, depths, True)
+        image, contrib, final_tau, ranges, ids = gsc.splat(cam.height, cam.width, us, cinv2ds, alphas, depths, colors,
+                                                           areas)
+        ctx.cam = cam
+        ctx.save_for_backward(us, cinv2ds, alphas, depths, colors, contrib, final_tau, ranges, ids, dcinv2d_dcov2ds,
+                              dcov2d_dcov3ds, dcov3d_drots, dcov3d_dscales, dcolor_dshs, du_dpcs, dcov2d_dpcs,
+                              dcolor_dpws)
+        return image, final_tau, depths > 0.2
+
+    @staticmethod
+    def backward(ctx, dloss_dgammas, dloss_dfinal_tau, _):
+        cam = ctx.cam
+        (us, cinv2ds, alphas, depths, colors, contrib, final_tau, ranges, ids, dcinv2d_dcov2ds, dcov2d_dcov3ds,
+         dcov3d_drots, dcov3d_dscales, dcolor_dshs, du_dpcs, dcov2d_dpcs, dcolor_dpws) = ctx.saved_tensors
+        dtau = None if dloss_dfinal_tau is None else dloss_dfinal_tau.contiguous()
+        dus, dcinv, dalphas, dcolors = gsc.splatB(cam.height, cam.width, us, cinv2ds, alphas, depths, colors, contrib,
+                                                  final_tau, ranges, ids, dloss_dgammas.contiguous(), dtau)
+        n = us.shape[0]
+        R = cam.Rcw.reshape(1, 3, 3).expand(n, 3, 3)
+        dcov2d = torch.bmm(dcinv, dcinv2d_dcov2ds)                           # (N,1,3)
+        dcov3d = torch.bmm(dcov2d, dcov2d_dcov3ds)                           # (N,1,6)
+        drots = torch.bmm(dcov3d, dcov3d_drots)                              # (N,1,4)
+        dscales = torch.bmm(dcov3d, dcov3d_dscales)                          # (N,1,3)
+        dshs = torch.bmm(dcolors.permute(0, 2, 1).contiguous(), dcolor_dshs).permute(0, 2, 1).reshape(n, -1)
+        dpws = (torch.bmm(torch.bmm(dus, du_dpcs), R) + torch.bmm(dcolors, dcolor_dpws) +
+                torch.bmm(torch.bmm(dcov2d, dcov2d_dpcs), R))
+        return (dpws.reshape(n, 3), dshs, dalphas.reshape(n, 1), dscales.reshape(n, 3), drots.reshape(n, 4),
+                dus.reshape(n, 2), None)

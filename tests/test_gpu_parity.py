@@ -334,7 +334,7 @@ def test_binning_sort_paths(device, case):
 @pytest.mark.parametrize("sh_dim", [16, 9, 4, 1])
 def test_calc_j_jacobians_bitexact(device, sh_dim):
     """calc_J=True outputs of project, computeCov3D, computeCov2D,
-    inverseCov2D and sh2Weight against the oracle, bit for bit, on a scene
+    inverseCov2D, sh2Weight and sh2Color against the oracle, bit for bit, on a scene
     with Gaussians behind the camera and outside the clamp (their entries
     keep the zero fill); the oracle's Jacobians are pinned by float64 finite
     differences."""
@@ -366,3 +366,10 @@ def test_calc_j_jacobians_bitexact(device, sh_dim):
     same(N(dci), O.inverse_jac(o_c2, o_dep), "dcinv_dcov2ds")
     same(N(dwdp), O.sh2weight_jac(shs, sc.pws, cam.twc, K), "dweight_dpws")
     same(N(basis).reshape(sc.n, sh_dim), O.sh2weight(shs, sc.pws, cam.twc, K)[1], "basis")
+    # sh2Color with calc_J (colour coefficients: the first 3 sh_dim values of each row)
+    cshs = np.ascontiguousarray(sc.shs[:, : sh_dim * 3])
+    col, dcs, dcp = gsc.sh2Color(T(cshs, device), pws, tc.twc, True)
+    o_dcs, o_dcp = O.sh2color_jac(cshs, sc.pws, cam.twc)
+    same(N(col), O.sh2color(cshs, sc.pws, cam.twc), "colors")
+    same(N(dcs), o_dcs, "dcolor_dshs")
+    same(N(dcp), o_dcp, "dcolor_dpws")

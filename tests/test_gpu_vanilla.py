@@ -78,3 +78,37 @@ def test_colour_backward_200k_640x480(device):
     dimg = rng.normal(size=(3, 480, 640)).astype(np.float32)
     dtau = rng.normal(size=(480, 640)).astype(np.float32)
     check(f, cam, dimg, dtau, True, device)
+
+
+def test_gsfunction_backward_chain(device):
+    """rgp.vanilla.GSFunction (the vanilla training step's autograd function)
+    on C0 with colour coefficients: the forward image and final_tau equal the
+    oracle's bit for bit; dL/d(pws, shs, alphas, scales, rots) for
+    L = sum(G * image) + sum(G_tau * final_tau) lie within 1e-5 of the bound
+    |J| |terms| of the same chain assembled in float64 from the FP32 oracle
+    (its Jacobians bit-exact with the kernels, tests/test_gpu_parity.py)."""
+    from rgp.vanilla import GSFunction
+    from test_oracle_vanilla import oracle_chain, oracle_forward
+    print("case: vanilla GSFunction backward chain (C0)", flush=True)
+    scene, cam = workload_c0()
+    rng = np.random.default_rng(12)
+    shs = rng.normal(0, 0.3, (scene.n, 48)).astype(np.float32)
+    sc = dict(pws=scene.pws, rots=scene.rots, scales=scene.scales, alphas=scene.alphas)
+    f = oracle_forward(sc, cam, shs, np.float32)
+    dimg = rng.normal(size=(3, cam.height, cam.width)).astype(np.float32)
+    dtau = rng.normal(size=(cam.height, cam.width)).astype(np.float32)
+    ref, bound = oracle_chain(sc, cam, shs, f, dimg, dtau, np.float32)
+    leaves = {k: T(v, device).requires_grad_(True) for k, v in
+              (("pws", scene.pws), ("shs", shs), ("alphas", scene.alphas.reshape(-1, 1)), ("scales", scene.scales),
+               ("rots", scene.rots))}
+    us0 = torch.zeros((scene.n, 2), device=device)
+    img, ft, mask = GSFunction.apply(leaves["pws"], leaves["shs"], leaves["alphas"], leaves["scales"],
+                                     leaves["rots"], us0, cam.to_torch(device))
+    assert np.array_equal(N(img), f["img"]) and np.array_equal(N(ft), f["ft"])
+    ((img * T(dimg, device)).sum() + (ft * T(dtau, device)).sum()).backward()
+    for k in ("pws", "shs", "alphas", "scales", "rots"):
+        got = N(leaves[k].grad).astype(np.float64).reshape(ref[k].shape)
+        err = np.abs(got - ref[k])
+        lim = 1e-5 * bound[k] + 1e-30
+        assert (err <= lim).all(), (k, float(np.max(err / lim)))
+        assert np.count_nonzero(ref[k]) > 0

@@ -496,6 +496,36 @@ def test_jacobians_fd64():
         assert np.abs(dws - fds).max() <= 1e-7 * (np.abs(fdw).max() + 1e-12), sh_dim
 
 
+def test_sh2color_jacobians_fd64():
+    """sh2Color's calc_J outputs against central differences of the float64
+    colour: dcolor/dpw carries the reference's dY_9/dy = C30 (3x^2 - 2y^2)
+    (kernel.cu:1152), exceeding the true derivative by C30 sh_9 y^2 ddir_y/dpw;
+    dcolor/dshs is the basis (the colour is linear in the coefficients)."""
+    d = np.float64
+    sc = make_scene(300, seed=19, extent=1.0, K=5)
+    cam = orbit_cameras(1, 3.5, 160, 120)[0]
+    n = sc.n
+    pws = sc.pws.astype(d)
+    shs = np.random.default_rng(5).normal(0, 0.3, (n, 48)).astype(d)
+    dshs, dpws = O.sh2color_jac(shs, pws, cam.twc, d)
+    fd = _fd(lambda p: O.sh2color(shs, p, cam.twc, d), pws, 1e-6)
+    dv = pws - cam.twc.astype(d)
+    nrm = np.linalg.norm(dv, axis=1)
+    dirv = dv / nrm[:, None]
+    ddir_y = (np.eye(3)[1][None, :] - dirv[:, 1:2] * dirv) / nrm[:, None]
+    quirk = (-0.5900435899266435 * shs[:, 27:30] * (dirv[:, 1:2] ** 2))[:, :, None] * ddir_y[:, None, :]
+    assert _rel(dpws, fd + quirk) < 1e-7
+    assert np.abs(quirk).max() > 1e-3 * np.abs(fd).max()
+    fds = _fd(lambda h: O.sh2color(h, pws, cam.twc, d), shs, 1e-6)  # (n, 3, 48)
+    for c in range(3):
+        np.testing.assert_allclose(fds[:, c, c::3], dshs[:, 0, :], rtol=0, atol=1e-8)
+    for sh_dim in (1, 4, 9):
+        sub = np.ascontiguousarray(shs[:, : 3 * sh_dim])
+        _, dp = O.sh2color_jac(sub, pws, cam.twc, d)
+        fdp = _fd(lambda p: O.sh2color(sub, p, cam.twc, d), pws, 1e-6)
+        assert np.abs(dp - fdp).max() <= 1e-7 * (np.abs(fd).max() + 1e-12), sh_dim
+
+
 def test_jacobians_fp32_vs_fp64():
     """The FP32 Jacobians follow their float64 values within FP32 rounding
     (cancellation aside: the bar is relative to the largest entry)."""

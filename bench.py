@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-lowrank", action="store_true", help="skip the rank-2 1M / 3M finetune steps")
     ap.add_argument("--no-bary", action="store_true", help="skip the barycentric-loss finetune step")
     ap.add_argument("--bary-only", action="store_true", help="run only the barycentric-loss finetune step (profiling)")
+    ap.add_argument("--no-vanilla", action="store_true", help="skip the vanilla 3DGS training step")
+    ap.add_argument("--vanilla-only", action="store_true", help="run only the vanilla 3DGS training step (profiling)")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--serve-rate", type=float, default=500.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -305,6 +307,55 @@ def measure_bary(args, dev, n=1_000_000, views=8, steps=24, warmup=3):
     return out
 
 
+def measure_vanilla(args, dev, n=1_000_000, views=8, steps=16, warmup=2):
+    """The vanilla 3DGS training step (vanilla_3dgs/gsplat/gsmodel.py:6-85 and
+    train.py's loop): colour render with the calc_J Jacobians, L1+SSIM against a
+    target plus an alpha term on final_tau, the colour backward with geometry
+    gradients, the Jacobian chain, Adam on pws/shs/alphas/scales/rots."""
+    import numpy as np
+    import torch
+
+    from rgp.losses import L1SSIMLoss
+    from rgp.optim import PaletteAdam
+    from rgp.scene import make_scene, orbit_cameras
+    from rgp.vanilla import GSFunction
+
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sc = make_scene(n, seed=args.seed, extent=3.0, K=5)
+    rng = np.random.default_rng(args.seed + 3)
+    tcams = [c.to_torch(dev) for c in orbit_cameras(views, 6.0, args.width, args.height)]
+    leaves = [T(sc.pws), T(rng.normal(0, 0.3, (n, 48)).astype(np.float32)), T(sc.alphas.reshape(-1, 1)),
+              T(sc.scales), T(sc.rots)]
+    for t in leaves:
+        t.requires_grad_(True)
+    us0 = torch.zeros((n, 2), device=dev)
+    with torch.no_grad():
+        targets = [GSFunction.apply(*leaves, us0, tc)[0] * 0.9 + 0.05 for tc in tcams]
+    opt = PaletteAdam(leaves, lr=1e-3)
+
+    def step(i):
+        img, ft, _ = GSFunction.apply(*leaves, us0, tcams[i % views])
+        loss = L1SSIMLoss.apply(img, targets[i % views], 0.2) + 0.1 * (1.0 - ft).mean()
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+
+    for i in range(warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(i)
+    torch.cuda.synchronize()
+    sps = steps / (time.perf_counter() - t0)
+    log(f"vanilla training step, {n} Gaussians: {sps:.2f} steps/s")
+    del sc, leaves, targets, opt
+    torch.cuda.empty_cache()
+    return {"steps_per_s": round(sps, 2),
+            "workload": f"vanilla 3DGS step: 1M Gaussians, 48 colour SH coefficients, {args.width}x{args.height}, "
+                        f"{views} views, L1+SSIM + final_tau term, geometry gradients, Adam on all five leaves"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -327,6 +378,9 @@ def main():
         return
     if args.bary_only:
         print(json.dumps({"barycentric_step": measure_bary(args, dev)}))
+        return
+    if args.vanilla_only:
+        print(json.dumps({"vanilla_step": measure_vanilla(args, dev)}))
         return
     if args.copy_calibration:
         a = torch.empty(64 << 20, dtype=torch.float32, device=dev)
@@ -462,6 +516,7 @@ def main():
     serving = None if args.no_serve else measure_serving(args, dev)
     lowrank = None if args.no_lowrank else measure_lowrank(args, dev)
     bary = None if args.no_bary else measure_bary(args, dev)
+    vanilla = None if args.no_vanilla else measure_vanilla(args, dev)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -500,7 +555,7 @@ def main():
         "frames_per_s": round(fps, 2), "ms_per_frame": round(1e3 / fps, 3),
         "pairs_per_view": int(np.mean([w["pairs"] for w in work])),
         "p99_palette_edit_ms": serving["edit_p99_ms"] if serving else None,
-        "serving": serving, "lowrank_rank2": lowrank, "barycentric_step": bary,
+        "serving": serving, "lowrank_rank2": lowrank, "barycentric_step": bary, "vanilla_step": vanilla,
         "kernels": breakdown, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

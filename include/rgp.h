@@ -186,6 +186,18 @@ int rgp_draw_color_backward(int width, int height, const int32_t* ranges, const 
                             const float* dimage, const float* dfinal_tau, float* dus, float* dcinv,
                             float* dalphas, float* dcolors, hipStream_t stream);
 
+/* The vanilla training step's backward chain (vanilla_3dgs/gsplat/gsmodel.py:
+ * 59-73) in one kernel instead of batched matrix products: from the colour
+ * backward's dus (N,2), dcinv (N,3), dcolors (N,3) and the forward's calc_J
+ * Jacobians (shapes of gausplat.cu's outputs), writes dpws (N,3),
+ * dshs (N, sh_dim*3), drots (N,4), dscales (N,3).  Rcw is the (3,3) device
+ * camera rotation; every product accumulates from 0 in index order. */
+int rgp_vanilla_chain(int n, int sh_dim, const float* Rcw, const float* dus, const float* dcinv,
+                      const float* dcolors, const float* du_dpcs, const float* dcinv_dcov2d,
+                      const float* dcov2d_dcov3d, const float* dcov2d_dpc, const float* dcov3d_drot,
+                      const float* dcov3d_dscale, const float* dcolor_dshs, const float* dcolor_dpw,
+                      float* dpws, float* dshs, float* drots, float* dscales, hipStream_t stream);
+
 /* Replay form of draw_weight_B's result: the forward recurrence is replayed
  * front to back and every contributing (pixel, entry) adds
  * dlayer[k,p] * tau * alpha' to dfeat, tau being the transmittance in front

@@ -1297,6 +1297,78 @@ __global__ __launch_bounds__(256) void k_adam1(int64_t lo, int64_t n, float* __r
 }
 }  // namespace rgp
 
+namespace rgp {
+// The vanilla training step's backward chain (vanilla_3dgs/gsplat/gsmodel.py:
+// 59-73) for one Gaussian per thread: the reference's batched products, each
+// accumulated from 0 in index order.
+//   dcov2d = dcinv J_inv;  dcov3d = dcov2d J_c2;  drot = dcov3d J_rot;
+//   dscale = dcov3d J_scale;  dshs[l, c] = dcolor[c] dcolor_dshs[l];
+//   dpw = ((du J_u) R + dcolor J_colpw) + (dcov2d J_c2pc) R
+template <int I, int J>
+__device__ __forceinline__ void vec_mat(const float* v, const float* __restrict__ M, float* out) {
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    float acc = 0.f;
+#pragma unroll
+    for (int i = 0; i < I; ++i) acc = acc + v[i] * M[i * J + j];
+    out[j] = acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_vanilla_chain(
+    int n, int sh_dim, const float* __restrict__ Rcw, const float* __restrict__ dus,
+    const float* __restrict__ dcinv, const float* __restrict__ dcolors, const float* __restrict__ du_dpcs,
+    const float* __restrict__ dcinv_dcov2d, const float* __restrict__ dcov2d_dcov3d,
+    const float* __restrict__ dcov2d_dpc, const float* __restrict__ dcov3d_drot,
+    const float* __restrict__ dcov3d_dscale, const float* __restrict__ dcolor_dshs,
+    const float* __restrict__ dcolor_dpw, float* __restrict__ dpws, float* __restrict__ dshs,
+    float* __restrict__ drots, float* __restrict__ dscales) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float R[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) R[e] = Rcw[e];
+  const float du[2] = {dus[2 * i], dus[2 * i + 1]};
+  const float dci[3] = {dcinv[3 * i], dcinv[3 * i + 1], dcinv[3 * i + 2]};
+  const float dco[3] = {dcolors[3 * i], dcolors[3 * i + 1], dcolors[3 * i + 2]};
+  float dc2[3], dc3[6], dr[4], dsc[3];
+  vec_mat<3, 3>(dci, dcinv_dcov2d + (size_t)i * 9, dc2);
+  vec_mat<3, 6>(dc2, dcov2d_dcov3d + (size_t)i * 18, dc3);
+  vec_mat<6, 4>(dc3, dcov3d_drot + (size_t)i * 24, dr);
+  vec_mat<6, 3>(dc3, dcov3d_dscale + (size_t)i * 18, dsc);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) drots[(size_t)i * 4 + e] = dr[e];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) dscales[(size_t)i * 3 + e] = dsc[e];
+  for (int l = 0; l < sh_dim; ++l) {
+    const float y = dcolor_dshs[(size_t)i * sh_dim + l];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dshs[((size_t)i * sh_dim + l) * 3 + c] = dco[c] * y;
+  }
+  float a[3], b[3], c[3], d[3], e3[3];
+  vec_mat<2, 3>(du, du_dpcs + (size_t)i * 6, a);
+  vec_mat<3, 3>(a, R, b);
+  vec_mat<3, 3>(dco, dcolor_dpw + (size_t)i * 9, c);
+  vec_mat<3, 3>(dc2, dcov2d_dpc + (size_t)i * 9, d);
+  vec_mat<3, 3>(d, R, e3);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) dpws[(size_t)i * 3 + k] = (b[k] + c[k]) + e3[k];
+}
+}  // namespace rgp
+
+extern "C" int rgp_vanilla_chain(int n, int sh_dim, const float* Rcw, const float* dus, const float* dcinv,
+                                 const float* dcolors, const float* du_dpcs, const float* dcinv_dcov2d,
+                                 const float* dcov2d_dcov3d, const float* dcov2d_dpc, const float* dcov3d_drot,
+                                 const float* dcov3d_dscale, const float* dcolor_dshs, const float* dcolor_dpw,
+                                 float* dpws, float* dshs, float* drots, float* dscales, hipStream_t s) {
+  if (n < 0 || sh_dim < 1 || sh_dim > 16) return set_error(RGP_ERR_ARG, "rgp_vanilla_chain: bad n or sh_dim");
+  if (n == 0) return RGP_OK;
+  hipLaunchKernelGGL(k_vanilla_chain, dim3(grid1(n)), dim3(256), 0, s, n, sh_dim, Rcw, dus, dcinv, dcolors,
+                     du_dpcs, dcinv_dcov2d, dcov2d_dcov3d, dcov2d_dpc, dcov3d_drot, dcov3d_dscale, dcolor_dshs,
+                     dcolor_dpw, dpws, dshs, drots, dscales);
+  return check_launch("rgp_vanilla_chain");
+}
+
 extern "C" int rgp_adam(int64_t n, float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
                         int64_t step, double lr, double beta1, double beta2, double eps, double weight_decay,
                         hipStream_t s) {

@@ -49,6 +49,7 @@ _SIGS = {
     "rgp_sh_backward_adam_full": (I, [I, I, I, P, P, P, P, P, P, I64, D, D, D, D, D, P]),
     "rgp_adam": (I, [I64, P, P, P, P, I64, D, D, D, D, D, P]),
     "rgp_draw_color_backward": (I, [I, I, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "rgp_vanilla_chain": (I, [I, I] + [P] * 16 + [P]),
     "rgp_sh_backward_lowrank": (I, [I, I, I, I, I, P, P, P, P, P, P, P, P, P]),
     "rgp_mix_batched": (I, [I, I, P, P, P, P, I64, P]),
     "rgp_mix": (I, [I, I64, P, P, P, P]),

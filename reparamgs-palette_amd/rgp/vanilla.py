@@ -5,8 +5,9 @@ per-Gaussian stages with their calc_J Jacobians and the colour rasterizer;
 the backward runs the colour backward with geometry gradients
 (gsplatcu.splatB, rgp_draw_color_backward, including the gradient of a loss on
 final_tau) and pulls dL/d(us, cinv2d, alpha, colour) back through the
-Jacobians to dL/d(pws, shs, alphas, scales, rots) with the same products the
-reference forms (batched float32 matrix products).  The derivatives the
+Jacobians to dL/d(pws, shs, alphas, scales, rots) with the reference's
+products (its batched float32 matrix products, here one kernel,
+rgp_vanilla_chain, one Gaussian per thread).  The derivatives the
 reference leaves out stay out: the +-1.3 tan_fov clamp of computeCov2D, the
 0.99 clamp of alpha', max(0, .) of the Mahalanobis distance, and the
 sh2Color term dY_9/dy = C30 (3x^2 - 2y^2) (kernel.cu:1152).
@@ -14,6 +15,8 @@ sh2Color term dY_9/dy = C30 (3x^2 - 2y^2) (kernel.cu:1152).
 import torch
 
 import gsplatcu as gsc
+
+from ._lib import call, f32c, ptr, stream
 
 
 class GSFunction(torch.autograd.Function):
@@ -42,13 +45,15 @@ class GSFunction(torch.autograd.Function):
         dus, dcinv, dalphas, dcolors = gsc.splatB(cam.height, cam.width, us, cinv2ds, alphas, depths, colors, contrib,
                                                   final_tau, ranges, ids, dloss_dgammas.contiguous(), dtau)
         n = us.shape[0]
-        R = cam.Rcw.reshape(1, 3, 3).expand(n, 3, 3)
-        dcov2d = torch.bmm(dcinv, dcinv2d_dcov2ds)                           # (N,1,3)
-        dcov3d = torch.bmm(dcov2d, dcov2d_dcov3ds)                           # (N,1,6)
-        drots = torch.bmm(dcov3d, dcov3d_drots)                              # (N,1,4)
-        dscales = torch.bmm(dcov3d, dcov3d_dscales)                          # (N,1,3)
-        dshs = torch.bmm(dcolors.permute(0, 2, 1).contiguous(), dcolor_dshs).permute(0, 2, 1).reshape(n, -1)
-        dpws = (torch.bmm(torch.bmm(dus, du_dpcs), R) + torch.bmm(dcolors, dcolor_dpws) +
-                torch.bmm(torch.bmm(dcov2d, dcov2d_dpcs), R))
-        return (dpws.reshape(n, 3), dshs, dalphas.reshape(n, 1), dscales.reshape(n, 3), drots.reshape(n, 4),
-                dus.reshape(n, 2), None)
+        sh_dim = dcolor_dshs.shape[-1]
+        dev = us.device
+        dpws = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        dshs = torch.empty((n, 3 * sh_dim), dtype=torch.float32, device=dev)
+        drots = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        dscales = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        # the chain of gsmodel.py:59-73 (dcov2d = dcinv J_inv, ..., dpw) in one kernel
+        R = f32c(cam.Rcw).to(dev)
+        call("rgp_vanilla_chain", n, sh_dim, ptr(R), ptr(dus), ptr(dcinv), ptr(dcolors), ptr(du_dpcs),
+             ptr(dcinv2d_dcov2ds), ptr(dcov2d_dcov3ds), ptr(dcov2d_dpcs), ptr(dcov3d_drots), ptr(dcov3d_dscales),
+             ptr(dcolor_dshs), ptr(dcolor_dpws), ptr(dpws), ptr(dshs), ptr(drots), ptr(dscales), stream(dev))
+        return dpws, dshs, dalphas.reshape(n, 1), dscales, drots, dus.reshape(n, 2), None

@@ -50,13 +50,19 @@ __device__ __forceinline__ RecRegs<F4> load_rec(const float4* __restrict__ recs,
   return r;
 }
 
+// Bit of the staged mask word set when the record's cutoff mcut is not at
+// most 170 (or is NaN).  Below it every distance that passes m <= mcut lies
+// in [0, 170], where gauss_of(m) is gauss_half(m), so the range test is a
+// scalar branch per entry instead of a per-lane one.
+constexpr uint32_t kWideExp = 1u << 4;
+
 // Stages one record in LDS, replacing its ry by the mask of strips of tile
-// row ty it can reach (bit j: rows ty*16 + 4j .. + 3).  Float4 q of slot s
+// row ty it can reach (bit j: rows ty*16 + 4j .. + 3) and kWideExp.  Float4 q of slot s
 // sits at s_rec[q * 64 + s]: the wave's 16-byte stores cover the banks once
 // per quarter-wave, and a read of one slot is a broadcast.
 template <int F4>
 __device__ __forceinline__ void stage_rec(float4* s_rec, int slot, const RecRegs<F4>& r, int ty) {
-  uint32_t mask = 0;
+  uint32_t mask = r.v[1].z <= 170.0f ? 0u : kWideExp;
   const float uy = r.v[0].y, ry = r.v[1].w;
 #pragma unroll
   for (int j = 0; j < kSlots; ++j) {
@@ -67,6 +73,13 @@ __device__ __forceinline__ void stage_rec(float4* s_rec, int slot, const RecRegs
   s_rec[64 + slot] = make_float4(r.v[1].x, r.v[1].y, r.v[1].z, __uint_as_float(mask));
 #pragma unroll
   for (int q = 2; q < F4; ++q) s_rec[q * 64 + slot] = r.v[q];
+}
+
+// exp(-m/2) for an m that passed m <= mcut; wide (wavefront-uniform) is the
+// entry's kWideExp bit, so the branch is a scalar one.
+__device__ __forceinline__ float gauss_in(float m, bool wide) {
+  if (__builtin_expect(wide, 0)) return gauss_of(m);
+  return gauss_half(m);
 }
 
 __device__ __forceinline__ uint32_t uniform_bits(float v) {
@@ -135,11 +148,13 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
       for (int jj = 0; jj < cnt; ++jj) {
         const float4 g0 = s_rec[jj];
         const float4 g1 = s_rec[64 + jj];
-        const uint32_t slots = uniform_bits(g1.w) & live;
+        const uint32_t bits = uniform_bits(g1.w);
+        const uint32_t slots = bits & live;
         if (slots == 0) continue;
         const float dx = g0.x - fpx;
         const float ta0 = g0.z * dx * dx;  // (cinv.x dx) dx
         const float tc0 = g0.w * dx;       // (2 cinv.y) dx
+        const bool wide = bits & kWideExp;
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) {
           if (!(slots & (1u << j))) continue;
@@ -147,7 +162,7 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
           const float dy = g0.y - fpy[j];
           const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
           if (m > g1.z) continue;
-          const float ap = fminf(0.99f, g1.y * gauss_of(m));
+          const float ap = fminf(0.99f, g1.y * gauss_in(m, wide));
           if (ap < 0.002f) continue;
           const float ta = tau[j] * ap;
 #pragma unroll
@@ -285,11 +300,13 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
       const int fi = top - 1 - jj - rg.x;  // index of the entry in forward order
       const float4 g0 = s_rec[jj];
       const float4 g1 = s_rec[64 + jj];
-      uint32_t slots = uniform_bits(g1.w);
+      const uint32_t bits = uniform_bits(g1.w);
+      uint32_t slots = bits & 15u;
 #pragma unroll
       for (int j = 0; j < kSlots; ++j)
         if ((uint32_t)fi >= smax_bits[j]) slots &= ~(1u << j);
       if (slots == 0) continue;
+      const bool wide = bits & kWideExp;
       const float dx = g0.x - fpx;
       const float ta0 = g0.z * dx * dx;
       const float tc0 = g0.w * dx;
@@ -304,7 +321,7 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
         const float dy = g0.y - fpy[j];
         const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
         if (m > g1.z) continue;
-        const float ap = fminf(0.99f, g1.y * gauss_of(m));
+        const float ap = fminf(0.99f, g1.y * gauss_in(m, wide));
         if (ap < 0.002f) continue;
         tau[j] = tau[j] / (1.f - ap);
         const float gv = ap * tau[j];
@@ -410,11 +427,13 @@ __global__ __launch_bounds__(64) void k_draw_color_bwd(
       const float4 g0 = s_rec[jj];
       const float4 g1 = s_rec[64 + jj];
       const float4 g2 = s_rec[128 + jj];
-      uint32_t slots = uniform_bits(g1.w);
+      const uint32_t bits = uniform_bits(g1.w);
+      uint32_t slots = bits & 15u;
 #pragma unroll
       for (int j = 0; j < kSlots; ++j)
         if ((uint32_t)fi >= smax_bits[j]) slots &= ~(1u << j);
       if (slots == 0) continue;
+      const bool wide = bits & kWideExp;
       const float cx = g0.z, cy = 0.5f * g0.w, cz = g1.x;
       const float dx = g0.x - fpx;
       const float ta0 = cx * dx * dx;
@@ -431,7 +450,7 @@ __global__ __launch_bounds__(64) void k_draw_color_bwd(
         const float dy = g0.y - fpy[j];
         const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(cz * dy, dy, ta0)));
         if (m > g1.z) continue;
-        const float gg = gauss_of(m);
+        const float gg = gauss_in(m, wide);
         const float ap = fminf(0.99f, g1.y * gg);
         if (ap < 0.002f) continue;
         tau[j] = tau[j] / (1.f - ap);
@@ -496,6 +515,12 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
     tau[j] = in ? 1.f : 0.f;
     fpy[j] = (float)py;
   }
+  // The per-entry partial sums are zero outside a flush: they start at zero
+  // and are cleared after each one, so an entry without contributions leaves
+  // them untouched and the accumulation needs no per-entry re-zeroing.
+  float part[DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) part[k] = 0.f;
   int nid = 0;
   RecRegs<F4> nxt = load_rec<F4>(recs, ids, rg.x + lane, rg.x, rg.y, &nid);
   for (int b0 = rg.x; b0 < rg.y; b0 += 64) {
@@ -510,14 +535,13 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
     for (int jj = 0; jj < cnt; ++jj) {
       const float4 g0 = s_rec[jj];
       const float4 g1 = s_rec[64 + jj];
-      const uint32_t slots = uniform_bits(g1.w) & live;
+      const uint32_t bits = uniform_bits(g1.w);
+      const uint32_t slots = bits & live;
       if (slots == 0) continue;
+      const bool wide = bits & kWideExp;
       const float dx = g0.x - fpx;
       const float ta0 = g0.z * dx * dx;
       const float tc0 = g0.w * dx;
-      float part[DIM];
-#pragma unroll
-      for (int k = 0; k < DIM; ++k) part[k] = 0.f;
       bool on = false;
 #pragma unroll
       for (int j = 0; j < kSlots; ++j) {
@@ -526,7 +550,7 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
         const float dy = g0.y - fpy[j];
         const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
         if (m > g1.z) continue;
-        const float ap = fminf(0.99f, g1.y * gauss_of(m));
+        const float ap = fminf(0.99f, g1.y * gauss_in(m, wide));
         if (ap < 0.002f) continue;
         const float ta = tau[j] * ap;
 #pragma unroll
@@ -534,7 +558,11 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
         tau[j] = tau[j] * (1.f - ap);
         on = true;
       }
-      if (__any(on)) flush_grad<DIM>(part, dfeat, s_id[jj], lane);
+      if (__any(on)) {
+        flush_grad<DIM>(part, dfeat, s_id[jj], lane);
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) part[k] = 0.f;
+      }
     }
   }
 }

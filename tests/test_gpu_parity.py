@@ -248,6 +248,40 @@ def test_edge_cases(device):
             assert len(o["ids"]) == 1 and (o["ranges"] == 0).all()
 
 
+@pytest.mark.parametrize("mode", ["replay", "reference"])
+def test_extreme_opacities(device, mode):
+    """Opacities far outside (0, 1): alpha up to 3e38, inf and NaN give
+    cutoffs above 170 (or NaN), where exp(-m/2) is taken by the range-checked
+    path down to denormal results; alpha 1e20 stays on the short path with
+    distances near 100.  Forward bit-exact, both backward forms within the
+    gradient bar."""
+    print(f"case: extreme opacities, {mode} backward, 3000 Gaussians 128x96", flush=True)
+    scene = make_scene(3000, seed=17, extent=1.0, K=5)
+    cam = orbit_cameras(1, 4.0, 128, 96)[0]
+    special = np.array([5e35, 3e38, np.inf, np.nan, 1e20, 0.0, -0.5], dtype=np.float32)
+    scene.alphas[::7] = special[np.arange(len(scene.alphas[::7])) % len(special)]
+    o = O.forward_weight(scene, cam)
+    g = gpu_chain(scene, cam, device)
+    for k in ("depths", "areas", "ranges", "ids", "contrib", "final_tau", "layer"):
+        same(N(g[k]), o[k], k)
+    dl = np.random.default_rng(4).normal(size=o["layer"].shape).astype(np.float32)
+    if mode == "reference":
+        ob = O.backward_weight(scene, o, dl)
+        dw, = gsc.splat_weight_B(cam.height, cam.width, scene.K, g["us"], g["cinv"], T(scene.alphas, device),
+                                 g["depths"], g["weights"], g["contrib"], g["final_tau"], g["ranges"], g["ids"],
+                                 T(dl, device))
+        grad_close(N(dw).reshape(scene.n, scene.K), ob["dweights"], ob["dweights_abs"], "dweights")
+    else:
+        ob = O.backward_weight(scene, o, dl, replay=True)
+        shs = T(scene.shs, device).requires_grad_(True)
+        layer, _ = gsmodel.WeightGSFunctionTest.apply(T(scene.pws, device), shs, T(scene.alphas, device),
+                                                      T(scene.scales, device), T(scene.rots, device),
+                                                      cam.to_torch(device), scene.K)
+        same(N(layer), o["layer"], "layer")
+        layer.backward(T(dl, device))
+        grad_close(N(shs.grad), ob["dshs"], ob["dshs_abs"], "dshs")
+
+
 def test_forward_counts_match_oracle(device, c0):
     """The instrumented forward counts exactly the evaluations and
     contributions of the oracle's per-pixel loop (the roofline's work)."""

@@ -449,6 +449,81 @@ __device__ __forceinline__ float row_sum16(float v) {
   return v;
 }
 
+// Reduce-scatter of N <= 16 per-lane values over the 16 lanes of each row.
+// Level t (xor distance 1, 2, 4, 8) pairs lane l with l ^ d: of each pair of
+// values the lane with bit d clear keeps the first and sends the second, its
+// partner the reverse, so one level halves the values a lane holds (an odd
+// last value is summed in both lanes).  After the four levels every lane
+// holds the row sum of one channel, rs_channel(N, l & 15): about 3 operations
+// per pair and level instead of 4 DPP adds per channel and a select.
+// Exchanges: quad_perm for d = 1, 2; ds_swizzle (bit mode, xor 4) for d = 4;
+// row_ror:8, its own inverse, for d = 8.
+template <int D>
+__device__ __forceinline__ float lane_xor(float v) {
+  if constexpr (D == 1) return RGP_DPP(v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+  if constexpr (D == 2) return RGP_DPP(v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+  if constexpr (D == 4) return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
+  if constexpr (D == 8) return RGP_DPP(v, 0x128, 0xF, 0xF, false);  // row_ror:8
+}
+
+template <int N, int D>
+__device__ __forceinline__ void rs_level(const float* in, float* out, bool bit) {
+#pragma unroll
+  for (int i = 0; i < N / 2; ++i) {
+    const float keep = bit ? in[2 * i + 1] : in[2 * i];
+    const float send = bit ? in[2 * i] : in[2 * i + 1];
+    out[i] = keep + lane_xor<D>(send);
+  }
+  if constexpr ((N & 1) != 0) out[N / 2] = in[N - 1] + lane_xor<D>(in[N - 1]);
+}
+
+template <int N>
+__device__ __forceinline__ float row_reduce_scatter(const float* v, int lane) {
+  static_assert(N >= 1 && N <= 16, "one channel per lane of a row");
+  constexpr int N1 = (N + 1) / 2, N2 = (N1 + 1) / 2, N3 = (N2 + 1) / 2;
+  float a[N1], b[N2], c[N3], d[1];
+  rs_level<N, 1>(v, a, lane & 1);
+  rs_level<N1, 2>(a, b, lane & 2);
+  rs_level<N2, 4>(b, c, lane & 4);
+  rs_level<N3, 8>(c, d, lane & 8);
+  return d[0];
+}
+
+// The channel whose row sum row_reduce_scatter<n> leaves in lane r of a row,
+// and whether r is the lowest lane of its row holding that channel.
+__host__ __device__ constexpr int rs_channel(int n, int r) {
+  int map[16] = {0};
+  for (int i = 0; i < 16; ++i) map[i] = i;
+  for (int t = 0; t < 4; ++t) {
+    const int bit = (r >> t) & 1;
+    const int h = (n + 1) / 2;
+    for (int i = 0; i < n / 2; ++i) map[i] = map[2 * i + bit];
+    if (n & 1) map[n / 2] = map[n - 1];
+    n = h;
+  }
+  return map[0];
+}
+__host__ __device__ constexpr bool rs_owner(int n, int r) {
+  for (int q = 0; q < r; ++q)
+    if (rs_channel(n, q) == rs_channel(n, r)) return false;
+  return true;
+}
+// The same, packed at compile time: 4 bits of channel per row lane, one bit
+// of ownership per row lane.
+template <int N>
+struct RsLanes {
+  static constexpr uint64_t chan = [] {
+    uint64_t t = 0;
+    for (int r = 0; r < 16; ++r) t |= (uint64_t)rs_channel(N, r) << (4 * r);
+    return t;
+  }();
+  static constexpr uint32_t owner = [] {
+    uint32_t m = 0;
+    for (int r = 0; r < 16; ++r) m |= rs_owner(N, r) ? 1u << r : 0u;
+    return m;
+  }();
+};
+
 // Sum of lanes l, l^16, l^32 and l^48 into every one of them: the gfx950
 // v_permlane16_swap exchanges rows 0<->1 and 2<->3 between its two copies,
 // v_permlane32_swap the two halves of the wavefront.

@@ -215,25 +215,16 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
   }
 }
 
-// Adds the four-slot lane sums of an entry's gradient terms into dfeat.  Each
-// channel is summed within the 16-lane rows; lane l then keeps the row sum of
-// channel l & 15, the four rows are added across, and lanes 0..DIM-1 issue
-// one atomic instruction over the entry's DIM contiguous floats.  (A separate
-// single-lane atomic per channel costs five instructions and, through the
-// compiler's atomic scan for a lane mask it cannot prove, some twenty VALU
-// operations more per entry.)
+// Adds the four-slot lane sums of an entry's gradient terms into dfeat: a
+// reduce-scatter over each 16-lane row leaves channel ch = rs_channel(DIM,
+// lane & 15) in the lane, the four rows are added across, and the owner lanes
+// of row 0 (one per channel) issue one atomic instruction over the entry's
+// DIM floats.  ch and owner are per-lane constants set up once per kernel.
 template <int DIM>
-__device__ __forceinline__ void flush_grad(const float* part, float* __restrict__ dfeat, int g,
-                                           int lane) {
-  static_assert(DIM <= 16, "one channel per lane of a row");
-  float v = 0.f;
-#pragma unroll
-  for (int k = 0; k < DIM; ++k) {
-    const float r = row_sum16(part[k]);
-    v = (lane & 15) == k ? r : v;
-  }
-  v = across_rows(v);
-  if (lane < DIM && v != 0.f) unsafeAtomicAdd(&dfeat[(size_t)g * DIM + lane], v);
+__device__ __forceinline__ void flush_grad(const float* part, float* __restrict__ dfeat, int g, int lane,
+                                           int ch, bool owner) {
+  const float v = across_rows(row_reduce_scatter<DIM>(part, lane));
+  if (owner && v != 0.f) unsafeAtomicAdd(&dfeat[(size_t)g * DIM + ch], v);
 }
 
 // ---------------------------------------------------------------------------
@@ -260,6 +251,8 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
   const int y0 = ty * kTile + (lane >> 4);
   const float fpx = (float)px;
   const int2 rg = ranges[tile];
+  const int ch = (int)((RsLanes<DIM>::chan >> (4 * (lane & 15))) & 15);
+  const bool owner = lane < 16 && ((RsLanes<DIM>::owner >> lane) & 1);
   if (rg.y - rg.x <= 0) return;
 
   const size_t HW = (size_t)W * H;
@@ -329,7 +322,7 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
         for (int k = 0; k < DIM; ++k) part[k] = fmaf(dl[j][k], gv, part[k]);
         on = true;
       }
-      if (__any(on)) flush_grad<DIM>(part, dfeat, s_id[jj], lane);
+      if (__any(on)) flush_grad<DIM>(part, dfeat, s_id[jj], lane, ch, owner);
     }
   }
 }
@@ -349,19 +342,13 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void flush_geom(const float* part, float* __restrict__ dus, float* __restrict__ dcinv,
                                            float* __restrict__ dalpha, float* __restrict__ dcolor, int g,
-                                           int lane) {
-  float v = 0.f;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    const float r = row_sum16(part[k]);
-    v = (lane & 15) == k ? r : v;
-  }
-  v = across_rows(v);
-  if (lane < 9 && v != 0.f) {
-    float* dst = lane < 2 ? dus + (size_t)g * 2 + lane
-               : lane < 5 ? dcinv + (size_t)g * 3 + (lane - 2)
-               : lane == 5 ? dalpha + g
-                           : dcolor + (size_t)g * 3 + (lane - 6);
+                                           int lane, int ch, bool owner) {
+  const float v = across_rows(row_reduce_scatter<9>(part, lane));
+  if (owner && v != 0.f) {
+    float* dst = ch < 2 ? dus + (size_t)g * 2 + ch
+               : ch < 5 ? dcinv + (size_t)g * 3 + (ch - 2)
+               : ch == 5 ? dalpha + g
+                         : dcolor + (size_t)g * 3 + (ch - 6);
     unsafeAtomicAdd(dst, v);
   }
 }
@@ -381,6 +368,8 @@ __global__ __launch_bounds__(64) void k_draw_color_bwd(
   const int y0 = ty * kTile + (lane >> 4);
   const float fpx = (float)px;
   const int2 rg = ranges[tile];
+  const int ch = (int)((RsLanes<9>::chan >> (4 * (lane & 15))) & 15);
+  const bool owner = lane < 16 && ((RsLanes<9>::owner >> lane) & 1);
   if (rg.y - rg.x <= 0) return;
 
   const size_t HW = (size_t)W * H;
@@ -471,7 +460,7 @@ __global__ __launch_bounds__(64) void k_draw_color_bwd(
         for (int k = 0; k < 3; ++k) gam[j][k] = ap * col[k] + (1.f - ap) * gam[j][k];
         on = true;
       }
-      if (__any(on)) flush_geom(part, dus, dcinv, dalpha, dcolor, s_id[jj], lane);
+      if (__any(on)) flush_geom(part, dus, dcinv, dalpha, dcolor, s_id[jj], lane, ch, owner);
     }
   }
 }
@@ -500,6 +489,8 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
   const int y0 = ty * kTile + (lane >> 4);
   const float fpx = (float)px;
   const int2 rg = ranges[tile];
+  const int ch = (int)((RsLanes<DIM>::chan >> (4 * (lane & 15))) & 15);
+  const bool owner = lane < 16 && ((RsLanes<DIM>::owner >> lane) & 1);
   if (rg.y - rg.x <= 0) return;
 
   const size_t HW = (size_t)W * H;
@@ -559,7 +550,7 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
         on = true;
       }
       if (__any(on)) {
-        flush_grad<DIM>(part, dfeat, s_id[jj], lane);
+        flush_grad<DIM>(part, dfeat, s_id[jj], lane, ch, owner);
 #pragma unroll
         for (int k = 0; k < DIM; ++k) part[k] = 0.f;
       }

@@ -162,6 +162,16 @@ def expf(x):
     return y
 
 
+def gauss_half_check(lo, hi):
+    """(checked, differing, first differing m) over every float m in [lo, hi]:
+    exp(-m/2) in the compositing kernels' range-test-free form against
+    expf(-0.5 m)."""
+    n, d, f = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_float()
+    lib().orc_gauss_half_check(ctypes.c_float(lo), ctypes.c_float(hi), ctypes.byref(n), ctypes.byref(d),
+                               ctypes.byref(f))
+    return n.value, d.value, f.value
+
+
 # --------------------------------------------------------------------------
 # per-stage wrappers (shapes follow gsplatcu/ext.cpp:10-124)
 # --------------------------------------------------------------------------

@@ -104,6 +104,29 @@ float rgp_expf(float x) {
   return p * bits2f((uint32_t)(ni + 127) << 23);
 }
 
+// exp(-m/2) as the compositing kernels form it from the Mahalanobis distance
+// m >= 0 without a range test: n = rint(m (-log2e/2)), the reduction against
+// 2 ln2 (head and tail), the Horner partials of rgp_expf with the
+// coefficients scaled by (-1/2)^k, and ldexp by n.  This is rgp_expf(-0.5 m)
+// with every intermediate scaled by a power of two; orc_gauss_half_check
+// compares the two on every float of an interval (test infrastructure: it
+// pins the kernels' use of this form for every distance below the largest
+// cutoff a finite opacity can have).
+float gauss_half_form(float m) {
+  const float n = rintf(m * (-0.5f * 1.44269504088896341f));
+  float r = fmaf(n, 2.0f * 0.693145751953125f, m);
+  r = fmaf(n, 2.0f * 1.42860676533018570e-06f, r);
+  float t = 1.98412698e-4f * -0.0078125f;
+  t = fmaf(t, r, 1.38888889e-3f * 0.015625f);
+  t = fmaf(t, r, 8.33333377e-3f * -0.03125f);
+  t = fmaf(t, r, 4.16666679e-2f * 0.0625f);
+  t = fmaf(t, r, 1.66666672e-1f * -0.125f);
+  t = fmaf(t, r, 0.5f * 0.25f);
+  t = fmaf(t, r, -0.5f);
+  t = fmaf(t, r, 1.0f);
+  return std::ldexp(t, (int)n);
+}
+
 template <typename T> static inline T ex(T x);
 template <> inline float ex<float>(float x) { return rgp_expf(x); }
 template <> inline double ex<double>(double x) { return std::exp(x); }
@@ -1147,6 +1170,30 @@ ORC_BOTH(d, double, d)
 float orc_expf(float x) { return orc::rgp_expf(x); }
 void orc_expf_array(int64_t n, const float* x, float* y) {
   for (int64_t i = 0; i < n; ++i) y[i] = orc::rgp_expf(x[i]);
+}
+// Every float m in [lo, hi] (lo >= 0): counts those where gauss_half_form(m)
+// and rgp_expf(-0.5f * m) differ in any bit; *first gets the smallest such m.
+void orc_gauss_half_check(float lo, float hi, int64_t* checked, int64_t* differ, float* first) {
+  uint32_t a, b;
+  std::memcpy(&a, &lo, 4);
+  std::memcpy(&b, &hi, 4);
+  int64_t nd = 0;
+  uint32_t fb = 0xffffffffu;
+#pragma omp parallel for reduction(+ : nd) reduction(min : fb) schedule(static)
+  for (int64_t u = a; u <= (int64_t)b; ++u) {
+    const float m = orc::bits2f((uint32_t)u);
+    const float g1 = orc::gauss_half_form(m), g2 = orc::rgp_expf(-0.5f * m);
+    uint32_t x1, x2;
+    std::memcpy(&x1, &g1, 4);
+    std::memcpy(&x2, &g2, 4);
+    if (x1 != x2) {
+      ++nd;
+      if ((uint32_t)u < fb) fb = (uint32_t)u;
+    }
+  }
+  *checked = (int64_t)b - (int64_t)a + 1;
+  *differ = nd;
+  *first = nd ? orc::bits2f(fb) : 0.f;
 }
 int orc_num_threads(void) {
 #ifdef _OPENMP

@@ -397,10 +397,17 @@ __device__ __forceinline__ void sh_weights_lowrank_q15(const float* __restrict__
 // dy^2 / cov_yy, so |dy| > sqrt(mcut * cov_yy) implies maha > mcut.  A
 // wavefront whose pixel rows all lie farther than ry from u.y skips the entry.
 // ---------------------------------------------------------------------------
+// For finite alpha the cutoff is at most 1e7: alpha / 0.002 overflows to inf
+// above 6.8e35, and beyond m = 208 exp(-m/2) is already 0 (alpha * 0 < 0.002),
+// so the cap skips nothing that contributes.  Every distance that passes
+// m <= mcut of a finite alpha therefore lies in [0, 1e7], where gauss_half(m)
+// equals gauss_of(m) bit for bit (checked on every float of that interval by
+// the oracle); inf and NaN alpha give alpha' = 0.99 whatever exp returns.
 __device__ __forceinline__ float mcut_of(float alpha) {
   if (alpha != alpha) return alpha;  // NaN: never skip
   if (alpha < 0.002f) return -1.0f;  // alpha * e <= alpha < 0.002 for every e <= 1
-  return 2.0f * logf(alpha / 0.002f) * 1.00001f + 1e-3f;
+  const float mc = 2.0f * logf(alpha / 0.002f) * 1.00001f + 1e-3f;
+  return (alpha <= 3.40282347e38f && !(mc <= 1.0e7f)) ? 1.0e7f : mc;
 }
 
 __device__ __forceinline__ float ry_of(float mcut, float cxx, float cxy, float cyy) {

@@ -53,7 +53,10 @@ __device__ __forceinline__ RecRegs<F4> load_rec(const float4* __restrict__ recs,
 // Bit of the staged mask word set when the record's cutoff mcut is not at
 // most 170 (or is NaN).  Below it every distance that passes m <= mcut lies
 // in [0, 170], where gauss_of(m) is gauss_half(m), so the range test is a
-// scalar branch per entry instead of a per-lane one.
+// scalar branch per entry instead of a per-lane one.  The forward and the
+// replay backward use only alpha' = min(0.99, alpha exp(-m/2)), which
+// gauss_half gives exactly for every record (see mcut_of), so they need no
+// test at all; the colour backward also uses exp(-m/2) itself.
 constexpr uint32_t kWideExp = 1u << 4;
 
 // Stages one record in LDS, replacing its ry by the mask of strips of tile
@@ -148,13 +151,11 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
       for (int jj = 0; jj < cnt; ++jj) {
         const float4 g0 = s_rec[jj];
         const float4 g1 = s_rec[64 + jj];
-        const uint32_t bits = uniform_bits(g1.w);
-        const uint32_t slots = bits & live;
+        const uint32_t slots = uniform_bits(g1.w) & live;
         if (slots == 0) continue;
         const float dx = g0.x - fpx;
         const float ta0 = g0.z * dx * dx;  // (cinv.x dx) dx
         const float tc0 = g0.w * dx;       // (2 cinv.y) dx
-        const bool wide = bits & kWideExp;
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) {
           if (!(slots & (1u << j))) continue;
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
           const float dy = g0.y - fpy[j];
           const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
           if (m > g1.z) continue;
-          const float ap = fminf(0.99f, g1.y * gauss_in(m, wide));
+          const float ap = fminf(0.99f, g1.y * gauss_half(m));
           if (ap < 0.002f) continue;
           const float ta = tau[j] * ap;
 #pragma unroll
@@ -293,13 +294,11 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
       const int fi = top - 1 - jj - rg.x;  // index of the entry in forward order
       const float4 g0 = s_rec[jj];
       const float4 g1 = s_rec[64 + jj];
-      const uint32_t bits = uniform_bits(g1.w);
-      uint32_t slots = bits & 15u;
+      uint32_t slots = uniform_bits(g1.w) & 15u;
 #pragma unroll
       for (int j = 0; j < kSlots; ++j)
         if ((uint32_t)fi >= smax_bits[j]) slots &= ~(1u << j);
       if (slots == 0) continue;
-      const bool wide = bits & kWideExp;
       const float dx = g0.x - fpx;
       const float ta0 = g0.z * dx * dx;
       const float tc0 = g0.w * dx;
@@ -314,7 +313,7 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
         const float dy = g0.y - fpy[j];
         const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
         if (m > g1.z) continue;
-        const float ap = fminf(0.99f, g1.y * gauss_in(m, wide));
+        const float ap = fminf(0.99f, g1.y * gauss_half(m));
         if (ap < 0.002f) continue;
         tau[j] = tau[j] / (1.f - ap);
         const float gv = ap * tau[j];
@@ -526,10 +525,8 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
     for (int jj = 0; jj < cnt; ++jj) {
       const float4 g0 = s_rec[jj];
       const float4 g1 = s_rec[64 + jj];
-      const uint32_t bits = uniform_bits(g1.w);
-      const uint32_t slots = bits & live;
+      const uint32_t slots = uniform_bits(g1.w) & live;
       if (slots == 0) continue;
-      const bool wide = bits & kWideExp;
       const float dx = g0.x - fpx;
       const float ta0 = g0.z * dx * dx;
       const float tc0 = g0.w * dx;
@@ -541,7 +538,7 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
         const float dy = g0.y - fpy[j];
         const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
         if (m > g1.z) continue;
-        const float ap = fminf(0.99f, g1.y * gauss_in(m, wide));
+        const float ap = fminf(0.99f, g1.y * gauss_half(m));
         if (ap < 0.002f) continue;
         const float ta = tau[j] * ap;
 #pragma unroll

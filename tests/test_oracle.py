@@ -46,6 +46,18 @@ def test_expf_special_values():
     assert (np.diff(O.expf(x)) >= 0).all()  # monotone on the compositing range
 
 
+def test_gauss_half_form_exhaustive():
+    """The compositing kernels form exp(-m/2) from the distance m without
+    expf's range handling.  On every float of [0, 1e7] (1.26e9 values: every
+    distance that can pass the cutoff of a finite opacity, capped at 1e7)
+    it must equal expf(-0.5 m) bit for bit, including the subnormal results
+    above m = 175.  A plausible slip in the scaled coefficients or the
+    reduction constants shows up as differing values."""
+    n, d, first = O.gauss_half_check(0.0, 1.0e7)
+    assert n > 1_200_000_000
+    assert d == 0, f"{d} differing values, first at m = {first}"
+
+
 # --------------------------------------------------------------------------
 # geometry
 # --------------------------------------------------------------------------

@@ -29,6 +29,7 @@ extern "C" {
 #define RGP_ERR_LAUNCH (-2) /* a kernel launch failed */
 
 #define RGP_MAX_DIM 8 /* MAX_SPLAT_DIM (kernel.cu:12): channels per Gaussian */
+#define RGP_MAX_VIEWS 16 /* cameras of one multi-view launch (the *_views entry points) */
 
 /* Camera for the fused entry points, resident in device memory so that a
  * captured graph can be replayed with a new pose.  Fields follow the
@@ -41,7 +42,7 @@ typedef struct rgp_camera {
   float fx, fy, cx, cy;
   float tan_fovx, tan_fovy;
   int32_t width, height;
-  int32_t pad[2];
+  int32_t pad[9]; /* 128 bytes: arrays of cameras (the *_views entry points) have a 128-byte stride */
 } rgp_camera;
 
 const char* rgp_last_error(void);
@@ -226,6 +227,56 @@ int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int rank,
                           float* records, void* ws, int64_t capacity, int width, int height,
                           float* us, float* cinv2ds, int32_t* areas, float* weights,
                           hipStream_t stream);
+
+/* ---- multi-view batches (no counterpart in the reference, which renders
+ * one camera per call: gsmodel.py:104-200 called once per view).  `views`
+ * cameras of one resolution are processed in one launch per stage, each
+ * view producing exactly what the single-view entry points produce for its
+ * camera.  Per-view arrays are contiguous: view v's depths at depths + v*n,
+ * records at records + v*n*rgp_record_floats(K), workspace at
+ * ws + v*ws_stride (ws_stride a multiple of 256 and at least
+ * rgp_bin_workspace_bytes), ids at ids + v*capacity, ranges at
+ * ranges + v*tiles*2, images at layer + v*dim*H*W, contrib / final_tau at
+ * + v*H*W, gradients at dfeat + v*n*dim. -------------------------------- */
+
+/* rgp_preprocess_weight for `views` cameras (cams: device array) in one
+ * pass over the Gaussians: the per-Gaussian parameters are read from HBM
+ * once for all views. */
+int rgp_preprocess_weight_views(int views, int n, int K, int mode, int P, int Q, int rank,
+                                const rgp_camera* cams, const float* pws, const float* rots,
+                                const float* scales, const float* alphas, const float* shs,
+                                const float* dc, const float* A, const float* B, float* depths,
+                                float* records, void* ws, size_t ws_stride, int64_t capacity, int width,
+                                int height, hipStream_t stream);
+
+/* rgp_bin_scan / rgp_bin_sort of every view (one launch sequence per view). */
+int rgp_bin_scan_views(int views, int n, int width, int height, void* ws, size_t ws_stride, int64_t capacity,
+                       hipStream_t stream);
+int rgp_bin_sort_views(int views, int n, const float* depths, int width, int height, void* ws,
+                       size_t ws_stride, int64_t capacity, int32_t* ids, int32_t* ranges, hipStream_t stream);
+
+/* rgp_draw_forward of every view in one launch (views x tiles blocks);
+ * id_stride = capacity, n the Gaussian count (record stride). */
+int rgp_draw_forward_views(int views, int width, int height, int dim, const int32_t* ranges,
+                           const int32_t* ids, int64_t id_stride, const float* records, int64_t n,
+                           float empty_tau, float* layer, int32_t* contrib, float* final_tau,
+                           hipStream_t stream);
+
+/* rgp_draw_backward_replay of every view in one launch; view v adds into its
+ * own rows dfeat + v*n*dim (zero-filled by the caller). */
+int rgp_draw_backward_replay_views(int views, int width, int height, int dim, const int32_t* ranges,
+                                   const int32_t* ids, int64_t id_stride, const float* records, int64_t n,
+                                   const float* dlayer, float* dfeat, hipStream_t stream);
+
+/* The SH chains of the backward for a multi-view batch: the gradient of the
+ * coefficients summed over the views, each view with its own basis
+ * Y(pw, twc_v) and weight gradient dw + v*n*K (cams: device array).
+ * Full rank (gsmodel.py:140-143) and low rank (gsmodel.py:194-198). */
+int rgp_sh_backward_full_views(int views, int n, int K, int sh_dim, const rgp_camera* cams, const float* pws,
+                               const float* dw, float* dshs, hipStream_t stream);
+int rgp_sh_backward_lowrank_views(int views, int n, int K, int P, int Q, int rank, const rgp_camera* cams,
+                                  const float* pws, const float* A, const float* B, const float* dw, float* ddc,
+                                  float* dA, float* dB, hipStream_t stream);
 
 /* Full-rank SH backward of gsmodel.py:140-143: dshs[n, lm*K+k] =
  * dw[n,k] * Y_lm(n), Y recomputed from pws and twc (device camera). */

@@ -783,3 +783,23 @@ extern "C" int rgp_fill_u32(void* dst, uint32_t value, int64_t n, hipStream_t s)
                      (uint32_t*)dst, value, n);
   return check_launch("rgp_fill_u32");
 }
+
+extern "C" int rgp_bin_scan_views(int views, int n, int width, int height, void* ws, size_t ws_stride,
+                                  int64_t capacity, hipStream_t s) {
+  if (views < 1 || views > RGP_MAX_VIEWS) return set_error(RGP_ERR_ARG, "rgp_bin_scan_views: bad views");
+  for (int v = 0; v < views; ++v)
+    if (int rc = rgp_bin_scan(n, width, height, (char*)ws + (size_t)v * ws_stride, capacity, s)) return rc;
+  return RGP_OK;
+}
+
+extern "C" int rgp_bin_sort_views(int views, int n, const float* depths, int width, int height, void* ws,
+                                  size_t ws_stride, int64_t capacity, int32_t* ids_out, int32_t* ranges_out,
+                                  hipStream_t s) {
+  if (views < 1 || views > RGP_MAX_VIEWS) return set_error(RGP_ERR_ARG, "rgp_bin_sort_views: bad views");
+  const int64_t tiles = (int64_t)((width + kTile - 1) / kTile) * ((height + kTile - 1) / kTile);
+  for (int v = 0; v < views; ++v)
+    if (int rc = rgp_bin_sort(n, depths + (size_t)v * n, width, height, (char*)ws + (size_t)v * ws_stride,
+                              capacity, ids_out + v * capacity, nullptr, ranges_out + v * tiles * 2, s))
+      return rc;
+  return RGP_OK;
+}

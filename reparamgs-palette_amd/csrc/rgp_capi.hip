@@ -6,6 +6,9 @@
 
 static thread_local char g_err[512] = "";
 
+// The host packs cameras as 32 float32 words; arrays of them have that stride.
+static_assert(sizeof(rgp_camera) == 128, "rgp_camera is 32 words");
+
 namespace rgp {
 
 int set_error(int code, const char* fmt, ...) {

@@ -357,6 +357,91 @@ __global__ __launch_bounds__(256) void k_lowrank_backward_q15(
   for (int j = 0; j < R * Q; ++j) dB[(size_t)i * R * Q + j] = db[j];
 }
 
+// Multi-view forms: the gradient of the factors is the sum over `views`
+// cameras of the single-view chain, each with its own basis Y(pw, twc_v) and
+// weight gradient dw + v*n*K.  Sums run view by view into the same
+// accumulators (dL/ddc: sum_v dw_v * SH_C0).
+template <int K, int R>
+__global__ __launch_bounds__(256) void k_lowrank_backward_q15_views(
+    int views, int n, const float* __restrict__ A, const float* __restrict__ B, const float* __restrict__ pws,
+    const rgp_camera* __restrict__ cams, const float* __restrict__ dw, float* __restrict__ ddc,
+    float* __restrict__ dA, float* __restrict__ dB) {
+  constexpr int Q = 15, P = K;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float a[P * R], b[R * Q], da[P * R], db[R * Q], dd[K];
+#pragma unroll
+  for (int j = 0; j < P * R; ++j) { a[j] = A[(size_t)i * P * R + j]; da[j] = 0.f; }
+#pragma unroll
+  for (int j = 0; j < R * Q; ++j) { b[j] = B[(size_t)i * R * Q + j]; db[j] = 0.f; }
+#pragma unroll
+  for (int k = 0; k < K; ++k) dd[k] = 0.f;
+  const float px = pws[3 * i], py = pws[3 * i + 1], pz = pws[3 * i + 2];
+  for (int v = 0; v < views; ++v) {
+    float Y[16];
+    sh_basis(px, py, pz, cams[v].twc, 16, Y);
+    const float* dwv = dw + ((size_t)v * n + i) * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const float dwk = dwv[k];
+      dd[k] = dd[k] + dwk * (float)SH_C0_0;
+#pragma unroll
+      for (int lm = 0; lm < 15; ++lm) {
+        const int g = lm * K + k, row = g / Q, col = g % Q;
+        const float y = Y[lm + 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          da[row * R + r] = da[row * R + r] + dwk * b[r * Q + col] * y;
+          db[r * Q + col] = db[r * Q + col] + dwk * a[row * R + r] * y;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) ddc[(size_t)i * K + k] = dd[k];
+#pragma unroll
+  for (int j = 0; j < P * R; ++j) dA[(size_t)i * P * R + j] = da[j];
+#pragma unroll
+  for (int j = 0; j < R * Q; ++j) dB[(size_t)i * R * Q + j] = db[j];
+}
+
+__global__ __launch_bounds__(256) void k_lowrank_backward_views(
+    int views, int n, int K, int P, int Q, int rank, const float* __restrict__ A, const float* __restrict__ B,
+    const float* __restrict__ pws, const rgp_camera* __restrict__ cams, const float* __restrict__ dw,
+    float* __restrict__ ddc, float* __restrict__ dA, float* __restrict__ dB) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int L = P * Q / K;
+  const float* a = A + (size_t)i * P * rank;
+  const float* b = B + (size_t)i * rank * Q;
+  float* da = dA + (size_t)i * P * rank;
+  float* db = dB + (size_t)i * rank * Q;
+  for (int j = 0; j < P * rank; ++j) da[j] = 0.f;
+  for (int j = 0; j < rank * Q; ++j) db[j] = 0.f;
+  for (int k = 0; k < K; ++k) ddc[(size_t)i * K + k] = 0.f;
+  for (int v = 0; v < views; ++v) {
+    float Y[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) Y[l] = 0.f;
+    if (L > 0) sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], cams[v].twc, L + 1, Y);
+    const float* dwv = dw + ((size_t)v * n + i) * K;
+    for (int k = 0; k < K; ++k) {
+      const float dwk = dwv[k];
+      ddc[(size_t)i * K + k] = ddc[(size_t)i * K + k] + dwk * (float)SH_C0_0;
+      int row = k / Q, col = k - (k / Q) * Q;
+      for (int lm = 0; lm < L; ++lm) {
+        const float y = Y[lm + 1];
+        for (int r = 0; r < rank; ++r) {
+          da[row * rank + r] = da[row * rank + r] + dwk * b[r * Q + col] * y;
+          db[r * Q + col] = db[r * Q + col] + dwk * a[row * rank + r] * y;
+        }
+        col += K;
+        while (col >= Q) { col -= Q; ++row; }
+      }
+    }
+  }
+}
+
 // Launch the low-rank backward, picking the register form when it applies.
 static void launch_lowrank_backward(int n, int K, int P, int Q, int rank, const float* A,
                                     const float* B, const float* ylm, const float* pws,
@@ -463,6 +548,109 @@ __global__ __launch_bounds__(256) void k_rects(int n, const float* __restrict__ 
 // and, for odd K, the row reads of 8 or 16 lanes hit distinct banks.
 __device__ __forceinline__ int sh_swizzle(int g) { return ((g >> 1) ^ (g >> 3)) & 3; }
 
+// Stages the block's 64 coefficient rows (WMODE 4) in LDS.
+template <int K, int WMODE>
+__device__ __forceinline__ void stage_sh_rows(float4* s_sh, int n, const float* __restrict__ shs) {
+  constexpr int R4 = 4 * K;  // float4 per staged row
+  const int t = threadIdx.x;
+  const int tot = min(64, n - (int)blockIdx.x * 64) * R4;
+  const float4* src = reinterpret_cast<const float4*>(shs) + (size_t)blockIdx.x * 64 * R4;
+  float4 v[R4];
+  if (tot == 64 * R4) {
+#pragma clang loop unroll(full)
+    for (int m = 0; m < R4; ++m) v[m] = src[t + 64 * m];
+#pragma clang loop unroll(full)
+    for (int m = 0; m < R4; ++m) {
+      const int e = t + 64 * m;
+      s_sh[e ^ sh_swizzle(e / R4)] = v[m];
+    }
+  } else {
+    for (int e = t; e < tot; e += 64) s_sh[e ^ sh_swizzle(e / R4)] = src[e];
+  }
+  __syncthreads();
+}
+
+// The whole per-Gaussian chain of one view for Gaussian i (lane t of its
+// block); returns uint32(depth * 1000) of a binned Gaussian, else 0.
+template <int K, int WMODE>
+__device__ __forceinline__ uint32_t preprocess_gaussian(
+    int i, int t, const float4* s_sh, int sh_dim, int P, int Q, int rank, const rgp_camera* __restrict__ cam,
+    const float* __restrict__ pws, const float4* __restrict__ rots, const float* __restrict__ scales,
+    const float* __restrict__ alphas, const float* __restrict__ shs, const float* __restrict__ dc,
+    const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ depths,
+    float4* __restrict__ recs, uint2* __restrict__ rects, uint32_t* __restrict__ counts, int gx, int gy,
+    float* __restrict__ us_out, float* __restrict__ cinv_out, int32_t* __restrict__ areas_out,
+    float* __restrict__ w_out) {
+  constexpr int R = WMODE == 2 ? 1 : 2;
+  constexpr int R4 = 4 * K;
+  uint32_t dqv = 0;
+  const float px = pws[3 * i], py = pws[3 * i + 1], pz = pws[3 * i + 2];
+  const Proj p = project_one(cam->Rcw, cam->tcw, px, py, pz, cam->fx, cam->fy, cam->cx, cam->cy);
+  // palette weights are evaluated for every Gaussian (kernel.cu:670-767
+  // has no depth test)
+  float Y[16];
+#pragma unroll
+  for (int l = 0; l < 16; ++l) Y[l] = 0.f;
+  float w[K];
+  if (WMODE == 0) {
+    if (sh_dim > 1) sh_basis(px, py, pz, cam->twc, sh_dim, Y);
+    sh_weights_full<K>(shs + (size_t)i * sh_dim * K, Y, sh_dim, w);
+  } else if (WMODE == 4) {
+    sh_basis(px, py, pz, cam->twc, 16, Y);
+    float row[16 * K];
+    const int sw = sh_swizzle(t);
+#pragma unroll
+    for (int q = 0; q < R4; ++q) {
+      const float4 c = s_sh[t * R4 + (q ^ sw)];
+      row[4 * q] = c.x; row[4 * q + 1] = c.y; row[4 * q + 2] = c.z; row[4 * q + 3] = c.w;
+    }
+    sh_weights_full<K>(row, Y, 16, w);
+  } else if (WMODE == 1) {
+    const int L = P * Q / K;
+    if (L > 0) sh_basis(px, py, pz, cam->twc, L + 1, Y);
+    sh_weights_lowrank<K>(dc + (size_t)i * K, A + (size_t)i * P * rank,
+                          B + (size_t)i * rank * Q, Y, L, Q, rank, w);
+  } else {
+    sh_basis(px, py, pz, cam->twc, 16, Y);
+    sh_weights_lowrank_q15<K, R>(dc + (size_t)i * K, A + (size_t)i * K * R, B + (size_t)i * R * 15, Y,
+                                 w);
+  }
+  if (w_out) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) w_out[(size_t)i * K + k] = w[k];
+  }
+  float depth = kBadMarker;
+  uint32_t cnt = 0;
+  if (p.ok) {
+    depth = p.z;
+    if (us_out) { us_out[2 * i] = p.ux; us_out[2 * i + 1] = p.uy; }
+    const float4 q = rots[i];
+    float c3[6], c2[3], ci[3];
+    int ar[2];
+    cov3d_one(q.x, q.y, q.z, q.w, scales[3 * i], scales[3 * i + 1], scales[3 * i + 2], c3);
+    cov2d_one(c3, p.x, p.y, p.z, cam->Rcw, cam->fx, cam->fy, cam->tan_fovx, cam->tan_fovy, c2);
+    if (!inverse_one(c2[0], c2[1], c2[2], ci, ar)) {
+      depth = kBadMarker;
+    } else {
+      if (cinv_out) { cinv_out[3 * i] = ci[0]; cinv_out[3 * i + 1] = ci[1]; cinv_out[3 * i + 2] = ci[2]; }
+      uint2 r;
+      cnt = rect_of(p.ux, p.uy, ar[0], ar[1], gx, gy, &r);
+      if (cnt == 0) {
+        depth = kBadMarker;
+        ar[0] = 0; ar[1] = 0;
+      } else {
+        rects[i] = r;
+        dqv = sat_u32(depth * 1000.f);
+        store_record<K>(recs + (size_t)i * RecStride<K>::f4, p.ux, p.uy, ci, alphas[i], w);
+      }
+      if (areas_out) { areas_out[2 * i] = ar[0]; areas_out[2 * i + 1] = ar[1]; }
+    }
+  }
+  depths[i] = depth;
+  counts[i] = cnt;
+  return dqv;
+}
+
 template <int K, int WMODE>
 __global__ __launch_bounds__(64) void k_preprocess(
     int n, int sh_dim, int P, int Q, int rank, const rgp_camera* __restrict__ cam,
@@ -473,97 +661,48 @@ __global__ __launch_bounds__(64) void k_preprocess(
     uint2* __restrict__ rects, uint32_t* __restrict__ counts, uint32_t* __restrict__ dqw,
     int gx, int gy, float* __restrict__ us_out, float* __restrict__ cinv_out,
     int32_t* __restrict__ areas_out, float* __restrict__ w_out) {
-  constexpr int R = WMODE == 2 ? 1 : 2;
-  constexpr int R4 = 4 * K;  // float4 per staged row
-  __shared__ float4 s_sh[WMODE == 4 ? 64 * R4 : 1];
+  __shared__ float4 s_sh[WMODE == 4 ? 64 * 4 * K : 1];
   const int t = threadIdx.x;
   const int i = blockIdx.x * 64 + t;
-  if constexpr (WMODE == 4) {
-    const int tot = min(64, n - (int)blockIdx.x * 64) * R4;
-    const float4* src = reinterpret_cast<const float4*>(shs) + (size_t)blockIdx.x * 64 * R4;
-    float4 v[R4];
-    if (tot == 64 * R4) {
-#pragma clang loop unroll(full)
-      for (int m = 0; m < R4; ++m) v[m] = src[t + 64 * m];
-#pragma clang loop unroll(full)
-      for (int m = 0; m < R4; ++m) {
-        const int e = t + 64 * m;
-        s_sh[e ^ sh_swizzle(e / R4)] = v[m];
-      }
-    } else {
-      for (int e = t; e < tot; e += 64) s_sh[e ^ sh_swizzle(e / R4)] = src[e];
-    }
-    __syncthreads();
-  }
+  if constexpr (WMODE == 4) stage_sh_rows<K, WMODE>(s_sh, n, shs);
   uint32_t dqv = 0;
-  if (i < n) {
-    const float px = pws[3 * i], py = pws[3 * i + 1], pz = pws[3 * i + 2];
-    const Proj p = project_one(cam->Rcw, cam->tcw, px, py, pz, cam->fx, cam->fy, cam->cx, cam->cy);
-    // palette weights are evaluated for every Gaussian (kernel.cu:670-767
-    // has no depth test)
-    float Y[16];
-#pragma unroll
-    for (int l = 0; l < 16; ++l) Y[l] = 0.f;
-    float w[K];
-    if (WMODE == 0) {
-      if (sh_dim > 1) sh_basis(px, py, pz, cam->twc, sh_dim, Y);
-      sh_weights_full<K>(shs + (size_t)i * sh_dim * K, Y, sh_dim, w);
-    } else if (WMODE == 4) {
-      sh_basis(px, py, pz, cam->twc, 16, Y);
-      float row[16 * K];
-      const int sw = sh_swizzle(t);
-#pragma unroll
-      for (int q = 0; q < R4; ++q) {
-        const float4 c = s_sh[t * R4 + (q ^ sw)];
-        row[4 * q] = c.x; row[4 * q + 1] = c.y; row[4 * q + 2] = c.z; row[4 * q + 3] = c.w;
-      }
-      sh_weights_full<K>(row, Y, 16, w);
-    } else if (WMODE == 1) {
-      const int L = P * Q / K;
-      if (L > 0) sh_basis(px, py, pz, cam->twc, L + 1, Y);
-      sh_weights_lowrank<K>(dc + (size_t)i * K, A + (size_t)i * P * rank,
-                            B + (size_t)i * rank * Q, Y, L, Q, rank, w);
-    } else {
-      sh_basis(px, py, pz, cam->twc, 16, Y);
-      sh_weights_lowrank_q15<K, R>(dc + (size_t)i * K, A + (size_t)i * K * R, B + (size_t)i * R * 15, Y,
-                                   w);
-    }
-    if (w_out) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) w_out[(size_t)i * K + k] = w[k];
-    }
-    float depth = kBadMarker;
-    uint32_t cnt = 0;
-    if (p.ok) {
-      depth = p.z;
-      if (us_out) { us_out[2 * i] = p.ux; us_out[2 * i + 1] = p.uy; }
-      const float4 q = rots[i];
-      float c3[6], c2[3], ci[3];
-      int ar[2];
-      cov3d_one(q.x, q.y, q.z, q.w, scales[3 * i], scales[3 * i + 1], scales[3 * i + 2], c3);
-      cov2d_one(c3, p.x, p.y, p.z, cam->Rcw, cam->fx, cam->fy, cam->tan_fovx, cam->tan_fovy, c2);
-      if (!inverse_one(c2[0], c2[1], c2[2], ci, ar)) {
-        depth = kBadMarker;
-      } else {
-        if (cinv_out) { cinv_out[3 * i] = ci[0]; cinv_out[3 * i + 1] = ci[1]; cinv_out[3 * i + 2] = ci[2]; }
-        uint2 r;
-        cnt = rect_of(p.ux, p.uy, ar[0], ar[1], gx, gy, &r);
-        if (cnt == 0) {
-          depth = kBadMarker;
-          ar[0] = 0; ar[1] = 0;
-        } else {
-          rects[i] = r;
-          dqv = sat_u32(depth * 1000.f);
-          store_record<K>(recs + (size_t)i * RecStride<K>::f4, p.ux, p.uy, ci, alphas[i], w);
-        }
-        if (areas_out) { areas_out[2 * i] = ar[0]; areas_out[2 * i + 1] = ar[1]; }
-      }
-    }
-    depths[i] = depth;
-    counts[i] = cnt;
-  }
+  if (i < n)
+    dqv = preprocess_gaussian<K, WMODE>(i, t, s_sh, sh_dim, P, Q, rank, cam, pws, rots, scales, alphas, shs, dc,
+                                        A, B, depths, recs, rects, counts, gx, gy, us_out, cinv_out, areas_out,
+                                        w_out);
   dqv = wave_max_u32(dqv);
   if (t == 0) dqw[blockIdx.x] = dqv;
+}
+
+// The same chain for `views` cameras of one resolution in one pass: the
+// block's parameters come from HBM once (the coefficient rows staged in LDS
+// once; the other per-Gaussian loads of later views hit the caches), and
+// view v writes depths + v*n, recs + v*n*F4 and the binning regions of the
+// workspace at ws + v*ws_stride.
+template <int K, int WMODE>
+__global__ __launch_bounds__(64) void k_preprocess_views(
+    int views, int n, int sh_dim, int P, int Q, int rank, const rgp_camera* __restrict__ cams,
+    const float* __restrict__ pws, const float4* __restrict__ rots, const float* __restrict__ scales,
+    const float* __restrict__ alphas, const float* __restrict__ shs, const float* __restrict__ dc,
+    const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ depths,
+    float4* __restrict__ recs, char* __restrict__ ws, size_t ws_stride, size_t off_rects, size_t off_counts,
+    size_t off_dqw, int gx, int gy) {
+  __shared__ float4 s_sh[WMODE == 4 ? 64 * 4 * K : 1];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * 64 + t;
+  if constexpr (WMODE == 4) stage_sh_rows<K, WMODE>(s_sh, n, shs);
+  for (int v = 0; v < views; ++v) {
+    char* w = ws + (size_t)v * ws_stride;
+    uint32_t dqv = 0;
+    if (i < n)
+      dqv = preprocess_gaussian<K, WMODE>(i, t, s_sh, sh_dim, P, Q, rank, cams + v, pws, rots, scales, alphas,
+                                          shs, dc, A, B, depths + (size_t)v * n,
+                                          recs + (size_t)v * n * RecStride<K>::f4, (uint2*)(w + off_rects),
+                                          (uint32_t*)(w + off_counts), gx, gy, nullptr, nullptr, nullptr,
+                                          nullptr);
+    dqv = wave_max_u32(dqv);
+    if (t == 0) ((uint32_t*)(w + off_dqw))[blockIdx.x] = dqv;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -607,6 +746,41 @@ __global__ __launch_bounds__(256) void k_sh_backward_full(int n, int sh_dim,
     const int lm = r / K, k = r - lm * K;
     out[e] = sW[gl][k] * sY[gl][lm];
   }
+}
+
+// dshs[n, lm*K + k] = sum_v dw_v[n,k] * Y_lm(pw, twc_v) over `views`
+// cameras (view by view), with dw_v at dw + v*n*K.
+template <int K>
+__global__ __launch_bounds__(256) void k_sh_backward_full_views(int views, int n, int sh_dim,
+                                                                const rgp_camera* __restrict__ cams,
+                                                                const float* __restrict__ pws,
+                                                                const float* __restrict__ dw,
+                                                                float* __restrict__ dshs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int row = sh_dim * K;
+  float acc[16 * K];
+#pragma unroll
+  for (int e = 0; e < 16 * K; ++e) acc[e] = 0.f;
+  const float px = pws[3 * i], py = pws[3 * i + 1], pz = pws[3 * i + 2];
+  for (int v = 0; v < views; ++v) {
+    float Y[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) Y[l] = 0.f;
+    if (sh_dim > 1) sh_basis(px, py, pz, cams[v].twc, sh_dim, Y);
+    Y[0] = (float)SH_C0_0;
+    const float* dwv = dw + ((size_t)v * n + i) * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const float d = dwv[k];
+#pragma unroll
+      for (int lm = 0; lm < 16; ++lm) acc[lm * K + k] = acc[lm * K + k] + d * Y[lm];
+    }
+  }
+  float* out = dshs + (size_t)i * row;
+#pragma unroll
+  for (int e = 0; e < 16 * K; ++e)
+    if (e < row) out[e] = acc[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -1239,6 +1413,52 @@ extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int r
   return check_launch("rgp_preprocess_weight");
 }
 
+extern "C" int rgp_preprocess_weight_views(int views, int n, int K, int mode, int P, int Q, int rank,
+                                           const rgp_camera* cams, const float* pws, const float* rots,
+                                           const float* scales, const float* alphas, const float* shs,
+                                           const float* dc, const float* A, const float* B, float* depths,
+                                           float* records, void* ws, size_t ws_stride, int64_t capacity,
+                                           int width, int height, hipStream_t s) {
+  if (views < 1 || views > RGP_MAX_VIEWS || n < 0 || K < 1 || K > RGP_MAX_DIM || width <= 0 || height <= 0)
+    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight_views: bad views, size or K");
+  if (mode == 1) {
+    int rc = check_lowrank("rgp_preprocess_weight_views", K, P, Q, rank);
+    if (rc) return rc;
+  } else if (mode != 0) {
+    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight_views: mode must be 0 or 1");
+  }
+  const int sh_dim = mode == 0 ? P : 16;
+  if (mode == 0 && !(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
+    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight_views: sh_dim (P) must be 1, 4, 9 or 16");
+  BinLayout L = bin_layout(n, capacity, width, height);
+  if (ws_stride < L.total || (ws_stride & 255) != 0)
+    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight_views: ws_stride %zu below the workspace size %zu "
+                     "or not a multiple of 256", ws_stride, L.total);
+  char* base = (char*)ws;
+  for (int v = 0; v < views; ++v) {
+    int rc = bin_reset_header(base + (size_t)v * ws_stride, s);
+    if (rc) return rc;
+  }
+  if (n == 0) return RGP_OK;
+  int wmode = mode;
+  if (mode == 1 && Q == 15 && P == K && (rank == 1 || rank == 2)) wmode = rank + 1;
+  if (mode == 0 && sh_dim == 16 && ((uintptr_t)shs & 15) == 0) wmode = 4;
+#define RGP_PREV(WM)                                                                                    \
+  RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess_views<KK, WM>), dim3((n + 63) / 64), dim3(64), 0, s,  \
+                                     views, n, sh_dim, P, Q, rank, cams, pws, (const float4*)rots, scales, \
+                                     alphas, shs, dc, A, B, depths, (float4*)records, base, ws_stride,    \
+                                     L.rects, L.counts, L.dqw, L.gx, L.gy))
+  switch (wmode) {
+    case 0: RGP_PREV(0); break;
+    case 1: RGP_PREV(1); break;
+    case 2: RGP_PREV(2); break;
+    case 3: RGP_PREV(3); break;
+    default: RGP_PREV(4); break;
+  }
+#undef RGP_PREV
+  return check_launch("rgp_preprocess_weight_views");
+}
+
 extern "C" int rgp_sh_backward_full(int n, int K, int sh_dim, const rgp_camera* cam,
                                     const float* pws, const float* dw, float* dshs,
                                     hipStream_t s) {
@@ -1430,4 +1650,38 @@ extern "C" int rgp_sh_backward_lowrank(int n, int K, int P, int Q, int rank,
   const float* twc = (const float*)((const char*)cam + offsetof(rgp_camera, twc));
   rgp::launch_lowrank_backward(n, K, P, Q, rank, A, B, nullptr, pws, twc, dw, ddc, dA, dB, s);
   return check_launch("rgp_sh_backward_lowrank");
+}
+
+extern "C" int rgp_sh_backward_lowrank_views(int views, int n, int K, int P, int Q, int rank,
+                                             const rgp_camera* cams, const float* pws, const float* A,
+                                             const float* B, const float* dw, float* ddc, float* dA, float* dB,
+                                             hipStream_t s) {
+  if (views < 1 || views > RGP_MAX_VIEWS || n < 0)
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_lowrank_views: bad views or n");
+  int rc = check_lowrank("rgp_sh_backward_lowrank_views", K, P, Q, rank);
+  if (rc) return rc;
+  if (n == 0) return RGP_OK;
+  if (Q == 15 && P == K && rank == 1) {
+    RGP_K_SWITCH(K, hipLaunchKernelGGL((k_lowrank_backward_q15_views<KK, 1>), dim3(grid1(n)), dim3(256), 0, s,
+                                       views, n, A, B, pws, cams, dw, ddc, dA, dB));
+  } else if (Q == 15 && P == K && rank == 2) {
+    RGP_K_SWITCH(K, hipLaunchKernelGGL((k_lowrank_backward_q15_views<KK, 2>), dim3(grid1(n)), dim3(256), 0, s,
+                                       views, n, A, B, pws, cams, dw, ddc, dA, dB));
+  } else {
+    hipLaunchKernelGGL(k_lowrank_backward_views, dim3(grid1(n)), dim3(256), 0, s, views, n, K, P, Q, rank, A, B,
+                       pws, cams, dw, ddc, dA, dB);
+  }
+  return check_launch("rgp_sh_backward_lowrank_views");
+}
+
+extern "C" int rgp_sh_backward_full_views(int views, int n, int K, int sh_dim, const rgp_camera* cams,
+                                          const float* pws, const float* dw, float* dshs, hipStream_t s) {
+  if (views < 1 || views > RGP_MAX_VIEWS || n < 0 || K < 1 || K > RGP_MAX_DIM)
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_full_views: bad views, n or K");
+  if (!(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_full_views: sh_dim must be 1, 4, 9 or 16");
+  if (n == 0) return RGP_OK;
+  RGP_K_SWITCH(K, hipLaunchKernelGGL(k_sh_backward_full_views<KK>, dim3(grid1(n)), dim3(256), 0, s, views, n,
+                                     sh_dim, cams, pws, dw, dshs));
+  return check_launch("rgp_sh_backward_full_views");
 }

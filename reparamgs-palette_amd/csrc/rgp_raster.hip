@@ -103,18 +103,13 @@ __device__ __forceinline__ uint32_t live_slots(const float* tau) {
 // pixel evaluates before it stops (counts[0]) and its contributions
 // (counts[1]).
 // ---------------------------------------------------------------------------
-template <int DIM, bool COUNT = false>
-__global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles,
-                                                 const int2* __restrict__ ranges,
-                                                 const int32_t* __restrict__ ids,
-                                                 const float4* __restrict__ recs, float empty_tau,
-                                                 float* __restrict__ layer,
-                                                 int32_t* __restrict__ contrib,
-                                                 float* __restrict__ final_tau,
-                                                 unsigned long long* __restrict__ counts) {
+template <int DIM, bool COUNT>
+__device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, const int2* __restrict__ ranges,
+                                              const int32_t* __restrict__ ids, const float4* __restrict__ recs,
+                                              float empty_tau, float* __restrict__ layer,
+                                              int32_t* __restrict__ contrib, float* __restrict__ final_tau,
+                                              unsigned long long* __restrict__ counts, float4* s_rec) {
   constexpr int F4 = RecStride<DIM>::f4;
-  __shared__ float4 s_rec[64 * F4];
-  const int tile = xcd_tile(blockIdx.x, tiles);
   const int ty = tile / gx, tx = tile - ty * gx;
   const int lane = threadIdx.x;
   const int px = tx * kTile + (lane & 15);
@@ -214,6 +209,41 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
       atomicAdd(&counts[1], cc);
     }
   }
+}
+
+template <int DIM, bool COUNT = false>
+__global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles,
+                                                 const int2* __restrict__ ranges,
+                                                 const int32_t* __restrict__ ids,
+                                                 const float4* __restrict__ recs, float empty_tau,
+                                                 float* __restrict__ layer,
+                                                 int32_t* __restrict__ contrib,
+                                                 float* __restrict__ final_tau,
+                                                 unsigned long long* __restrict__ counts) {
+  __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
+  draw_fwd_tile<DIM, COUNT>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau, layer, contrib,
+                            final_tau, counts, s_rec);
+}
+
+// `views` images of one resolution in one launch: block b composites tile
+// t % tiles of view t / tiles (t the XCD-banded position of b); view v reads
+// ranges + v*tiles, ids + v*id_stride, recs + v*rec_stride (float4 units)
+// and writes the v-th (DIM,H,W) layer and (H,W) contrib / final_tau.
+template <int DIM>
+__global__ __launch_bounds__(64) void k_draw_fwd_views(int W, int H, int gx, int tiles, int views,
+                                                       const int2* __restrict__ ranges,
+                                                       const int32_t* __restrict__ ids, int64_t id_stride,
+                                                       const float4* __restrict__ recs, int64_t rec_stride,
+                                                       float empty_tau, float* __restrict__ layer,
+                                                       int32_t* __restrict__ contrib,
+                                                       float* __restrict__ final_tau) {
+  __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
+  const int t = xcd_tile(blockIdx.x, tiles * views);
+  const int v = t / tiles, tile = t - v * tiles;
+  const size_t HW = (size_t)W * H;
+  draw_fwd_tile<DIM, false>(W, H, gx, tile, ranges + (size_t)v * tiles, ids + v * id_stride, recs + v * rec_stride,
+                            empty_tau, layer + (size_t)v * DIM * HW, contrib + v * HW, final_tau + v * HW, nullptr,
+                            s_rec);
 }
 
 // Adds the four-slot lane sums of an entry's gradient terms into dfeat: a
@@ -472,16 +502,13 @@ __global__ __launch_bounds__(64) void k_draw_color_bwd(
 // contributing (pixel, entry) pairs is the forward's (same stop rule).
 // ---------------------------------------------------------------------------
 template <int DIM>
-__global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, int tiles,
-                                                        const int2* __restrict__ ranges,
-                                                        const int32_t* __restrict__ ids,
-                                                        const float4* __restrict__ recs,
-                                                        const float* __restrict__ dlayer,
-                                                        float* __restrict__ dfeat) {
+__device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int tile,
+                                                     const int2* __restrict__ ranges,
+                                                     const int32_t* __restrict__ ids,
+                                                     const float4* __restrict__ recs,
+                                                     const float* __restrict__ dlayer,
+                                                     float* __restrict__ dfeat, float4* s_rec, int32_t* s_id) {
   constexpr int F4 = RecStride<DIM>::f4;
-  __shared__ float4 s_rec[64 * F4];
-  __shared__ int32_t s_id[64];
-  const int tile = xcd_tile(blockIdx.x, tiles);
   const int ty = tile / gx, tx = tile - ty * gx;
   const int lane = threadIdx.x;
   const int px = tx * kTile + (lane & 15);
@@ -553,6 +580,37 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
       }
     }
   }
+}
+
+template <int DIM>
+__global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, int tiles,
+                                                        const int2* __restrict__ ranges,
+                                                        const int32_t* __restrict__ ids,
+                                                        const float4* __restrict__ recs,
+                                                        const float* __restrict__ dlayer,
+                                                        float* __restrict__ dfeat) {
+  __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
+  __shared__ int32_t s_id[64];
+  draw_bwd_replay_tile<DIM>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, dlayer, dfeat, s_rec, s_id);
+}
+
+// The replay backward of `views` images in one launch, laid out as in
+// k_draw_fwd_views; view v reads dlayer + v*DIM*H*W and adds into its own
+// gradient rows dfeat + v*n*DIM (the SH chain behind them depends on the view).
+template <int DIM>
+__global__ __launch_bounds__(64) void k_draw_bwd_replay_views(int W, int H, int gx, int tiles, int views,
+                                                              const int2* __restrict__ ranges,
+                                                              const int32_t* __restrict__ ids, int64_t id_stride,
+                                                              const float4* __restrict__ recs, int64_t rec_stride,
+                                                              const float* __restrict__ dlayer,
+                                                              float* __restrict__ dfeat, int64_t n) {
+  __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
+  __shared__ int32_t s_id[64];
+  const int t = xcd_tile(blockIdx.x, tiles * views);
+  const int v = t / tiles, tile = t - v * tiles;
+  const size_t HW = (size_t)W * H;
+  draw_bwd_replay_tile<DIM>(W, H, gx, tile, ranges + (size_t)v * tiles, ids + v * id_stride, recs + v * rec_stride,
+                            dlayer + (size_t)v * DIM * HW, dfeat + (size_t)v * n * DIM, s_rec, s_id);
 }
 
 // ---------------------------------------------------------------------------
@@ -687,6 +745,39 @@ extern "C" int rgp_draw_forward(int width, int height, int dim, const int32_t* r
                                          (const float4*)records, empty_tau, layer, contrib,
                                          final_tau, (unsigned long long*)nullptr));
   return check_launch("rgp_draw_forward");
+}
+
+static int views_grid(const char* who, int views, int width, int height, int dim, int64_t id_stride,
+                      int* gx, int* tiles) {
+  if (int rc = draw_grid(who, width, height, dim, gx, tiles)) return rc;
+  if (views < 1 || views > RGP_MAX_VIEWS || (int64_t)views * *tiles > INT32_MAX || id_stride < 1)
+    return set_error(RGP_ERR_ARG, "%s: bad views (%d) or id stride", who, views);
+  return RGP_OK;
+}
+
+extern "C" int rgp_draw_forward_views(int views, int width, int height, int dim, const int32_t* ranges,
+                                      const int32_t* ids, int64_t id_stride, const float* records,
+                                      int64_t n, float empty_tau, float* layer, int32_t* contrib,
+                                      float* final_tau, hipStream_t s) {
+  int gx, tiles;
+  if (int rc = views_grid("rgp_draw_forward_views", views, width, height, dim, id_stride, &gx, &tiles)) return rc;
+  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL(k_draw_fwd_views<DD>, dim3(tiles * views), dim3(64), 0, s, width,
+                                         height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
+                                         (const float4*)records, n * RecStride<DD>::f4, empty_tau, layer,
+                                         contrib, final_tau));
+  return check_launch("rgp_draw_forward_views");
+}
+
+extern "C" int rgp_draw_backward_replay_views(int views, int width, int height, int dim, const int32_t* ranges,
+                                              const int32_t* ids, int64_t id_stride, const float* records,
+                                              int64_t n, const float* dlayer, float* dfeat, hipStream_t s) {
+  int gx, tiles;
+  if (int rc = views_grid("rgp_draw_backward_replay_views", views, width, height, dim, id_stride, &gx, &tiles))
+    return rc;
+  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL(k_draw_bwd_replay_views<DD>, dim3(tiles * views), dim3(64), 0, s,
+                                         width, height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
+                                         (const float4*)records, n * RecStride<DD>::f4, dlayer, dfeat, n));
+  return check_launch("rgp_draw_backward_replay_views");
 }
 
 extern "C" int rgp_draw_forward_counts(int width, int height, int dim, const int32_t* ranges,

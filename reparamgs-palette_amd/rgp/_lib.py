@@ -35,6 +35,13 @@ _SIGS = {
     "rgp_bin_workspace_bytes": (SZ, [I, I64, I, I]),
     "rgp_bin_rects": (I, [I, P, P, P, I, I, P, I64, P]),
     "rgp_bin_scan": (I, [I, I, I, P, I64, P]),
+    "rgp_preprocess_weight_views": (I, [I, I, I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, SZ, I64, I, I, P]),
+    "rgp_bin_scan_views": (I, [I, I, I, I, P, SZ, I64, P]),
+    "rgp_bin_sort_views": (I, [I, I, P, I, I, P, SZ, I64, P, P, P]),
+    "rgp_draw_forward_views": (I, [I, I, I, I, P, P, I64, P, I64, F, P, P, P, P]),
+    "rgp_draw_backward_replay_views": (I, [I, I, I, I, P, P, I64, P, I64, P, P, P]),
+    "rgp_sh_backward_full_views": (I, [I, I, I, I, P, P, P, P, P]),
+    "rgp_sh_backward_lowrank_views": (I, [I, I, I, I, I, I, P, P, P, P, P, P, P, P, P]),
     "rgp_bin_sort": (I, [I, P, I, I, P, I64, P, P, P, P]),
     "rgp_bin_header": (P, [P]),
     "rgp_record_floats": (I, [I]),

@@ -13,6 +13,7 @@ runs the reverse compositing kernel and one SH-chain kernel that recomputes
 the basis from the positions rather than storing it.  The values are the ones
 of the reference chain.
 """
+import ctypes
 import math
 
 import torch
@@ -80,6 +81,105 @@ def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None,
         return layer, depths, dict(records=records, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids,
                                    camd=camd, m=m, mode=mode, P=P_, Q=Q_, rank=r_)
     return layer, depths
+
+
+def _coef_args(K, shs, dc, A, B, Q, sh_dim):
+    """(mode, P, Q, rank, shs, dc, A, B) of the preprocess entry points."""
+    if shs is not None:
+        shs_c = f32c(shs)
+        sd = shs_c.shape[1] // K if sh_dim is None else sh_dim
+        return 0, sd, 0, 1, shs_c, None, None, None
+    dc_c, A_c, B_c = f32c(dc), f32c(A), f32c(B)
+    P, r = (A_c.shape[1], 1) if A_c.dim() == 2 else (A_c.shape[1], A_c.shape[2])
+    return 1, P, int(Q), r, None, dc_c, A_c, B_c
+
+
+def render_weights_views(pws, alphas, scales, rots, cams, K, shs=None, dc=None, A=None, B=None, Q=None,
+                         sh_dim=None, return_state=False, exact=False):
+    """render_weights for several cameras of one resolution at once (no
+    reference counterpart: the reference renders one camera per call).  One
+    preprocess launch reads the Gaussians once for all views, and one
+    compositing launch covers every view's tiles.  Returns layers (V,K,H,W)
+    and depths (V,N); each view's values are those render_weights gives for
+    its camera alone."""
+    require_cuda(pws, alphas, scales, rots)
+    dev = pws.device
+    n, V = pws.shape[0], len(cams)
+    W, H = int(cams[0].width), int(cams[0].height)
+    if any(int(c.width) != W or int(c.height) != H for c in cams):
+        raise ValueError("render_weights_views: all cameras must share one resolution")
+    camsd = ops.camera_structs(cams, dev)
+    pws_c, rots_c, scales_c, alphas_c = f32c(pws), f32c(rots), f32c(scales), f32c(alphas).reshape(-1)
+    depths = torch.empty((V, n), dtype=torch.float32, device=dev)
+    records = torch.empty((V, n, ops.record_floats(K)), dtype=torch.float32, device=dev)
+    mode, P_, Q_, r_, shs_c, dc_c, A_c, B_c = _coef_args(K, shs, dc, A, B, Q, sh_dim)
+
+    def fill(ws, ws_stride, cap):
+        call("rgp_preprocess_weight_views", V, n, K, mode, P_, Q_, r_, ptr(camsd), ptr(pws_c), ptr(rots_c),
+             ptr(scales_c), ptr(alphas_c), ptr(shs_c), ptr(dc_c), ptr(A_c), ptr(B_c), ptr(depths), ptr(records),
+             ptr(ws), ctypes.c_size_t(ws_stride), I64(int(cap)), W, H, stream(dev))
+
+    layer, contrib, final_tau, ranges, ids, ms, _, _ = ops.splat_pipeline_views(n, V, W, H, K, records, depths, fill,
+                                                                                0.0, dev, exact=exact)
+    if return_state:
+        return layer, depths, dict(records=records, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids,
+                                   camsd=camsd, m=ms, mode=mode, P=P_, Q=Q_, rank=r_)
+    return layer, depths
+
+
+class WeightGSFunctionTestViews(torch.autograd.Function):
+    """WeightGSFunctionTest over a list of cameras of one resolution in one
+    batch: returns (layers (V,K,H,W), masks (V,N)); the gradient of the
+    full-rank coefficients is the sum over the views."""
+
+    @staticmethod
+    def forward(ctx, pws, shs, alphas, scales, rots, cams, num_palette):
+        K = int(num_palette)
+        layer, depths, st = render_weights_views(pws, alphas, scales, rots, cams, K, shs=shs, return_state=True)
+        ctx.meta = (int(cams[0].width), int(cams[0].height), K, pws.shape[0], st["P"], len(cams))
+        ctx.save_for_backward(f32c(pws), st["records"], st["ranges"], st["ids"], st["camsd"])
+        return layer, depths > 0.2
+
+    @staticmethod
+    def backward(ctx, dloss_dlayer, _):
+        W, H, K, n, sh_dim, V = ctx.meta
+        pws, records, ranges, ids, camsd = ctx.saved_tensors
+        dw = ops.draw_backward_replay_views(V, W, H, K, ranges, ids, records, dloss_dlayer.contiguous(), n)
+        dshs = torch.empty((n, sh_dim * K), dtype=torch.float32, device=pws.device)
+        with SPANS.span("sh_backward"):
+            call("rgp_sh_backward_full_views", V, n, K, sh_dim, ptr(camsd), ptr(pws), ptr(dw), ptr(dshs),
+                 stream(pws.device))
+        return (None, dshs, None, None, None, None, None)
+
+
+class WeightGSFunctionFastViews(torch.autograd.Function):
+    """WeightGSFunctionFast over a list of cameras of one resolution in one
+    batch: returns (layers (V,K,H,W), masks (V,N)); the gradients of dc, A, B
+    are sums over the views."""
+
+    @staticmethod
+    def forward(ctx, pws, shs_dc, shs_A, shs_B, alphas, scales, rots, cams, num_palette, Q):
+        K = int(num_palette)
+        layer, depths, st = render_weights_views(pws, alphas, scales, rots, cams, K, dc=shs_dc, A=shs_A, B=shs_B,
+                                                 Q=Q, return_state=True)
+        ctx.meta = (int(cams[0].width), int(cams[0].height), K, pws.shape[0], st["P"], st["Q"], st["rank"],
+                    len(cams))
+        ctx.save_for_backward(f32c(pws), f32c(shs_A), f32c(shs_B), st["records"], st["ranges"], st["ids"],
+                              st["camsd"])
+        return layer, depths > 0.2
+
+    @staticmethod
+    def backward(ctx, dloss_dlayer, _):
+        W, H, K, n, P, Q, r, V = ctx.meta
+        pws, A, B, records, ranges, ids, camsd = ctx.saved_tensors
+        dw = ops.draw_backward_replay_views(V, W, H, K, ranges, ids, records, dloss_dlayer.contiguous(), n)
+        ddc = torch.empty((n, K), dtype=torch.float32, device=pws.device)
+        dA = torch.empty(A.shape, dtype=torch.float32, device=pws.device)
+        dB = torch.empty(B.shape, dtype=torch.float32, device=pws.device)
+        with SPANS.span("sh_backward"):
+            call("rgp_sh_backward_lowrank_views", V, n, K, P, Q, r, ptr(camsd), ptr(pws), ptr(A), ptr(B), ptr(dw),
+                 ptr(ddc), ptr(dA), ptr(dB), stream(pws.device))
+        return (None, ddc, dA, dB, None, None, None, None, None, None)
 
 
 class WeightGSFunctionTest(torch.autograd.Function):

@@ -6,6 +6,8 @@ number of (tile, Gaussian) pairs read it back once, as the reference does
 (gsplatcu/gausplat.cu:201); the capacity-based variants used for graph capture
 never synchronise.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -231,6 +233,80 @@ def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, d
                 cap = CAPACITY.get(n, width, height)
                 continue
         return layer, contrib, final_tau, ranges, ids, m, ws
+
+
+def camera_structs(cams, device):
+    """Contiguous device array (V, 32) of rgp_camera structs, one per camera."""
+    return torch.stack([camera_struct(c, device) for c in cams])
+
+
+def _align256(x):
+    return (int(x) + 255) & ~255
+
+
+def splat_pipeline_views(n, views, width, height, dim, records, depths, ws_fill, empty_tau, device,
+                         exact=False):
+    """splat_pipeline for `views` cameras of one resolution: one preprocess
+    launch (ws_fill(ws, stride, cap)), the binning of every view, and one
+    compositing launch over all views' tiles.  Returns layers (V,dim,H,W),
+    contrib (V,H,W), final_tau (V,H,W), ranges (V,tiles,2), ids (V,cap),
+    the pair counts per view, the workspace and its per-view stride.  The
+    overflow check follows splat_pipeline's (exact=False: after the draw)."""
+    gx, gy = grid_dims(width, height)
+    cap = CAPACITY.get(n, width, height)
+    while True:
+        stride = _align256(workspace_bytes(n, cap, width, height))
+        ws = torch.empty(stride * views, dtype=torch.uint8, device=device)
+        with SPANS.span("preprocess"):
+            ws_fill(ws, stride, cap)
+        with SPANS.span("scan"):
+            call("rgp_bin_scan_views", int(views), int(n), int(width), int(height), ptr(ws), ctypes.c_size_t(stride),
+                 I64(int(cap)), stream(device))
+        hdrs = ws.view(views, stride)[:, :8].contiguous().view(torch.int32)  # (V, 2): M, overflow
+        if exact:
+            ms = hdrs[:, 0].tolist()
+            for m in ms:
+                CAPACITY.update(n, width, height, m)
+            if max(ms) > cap:
+                cap = CAPACITY.get(n, width, height)
+                continue
+        else:
+            host = torch.empty((views, 2), dtype=torch.int32, pin_memory=True)
+            host.copy_(hdrs, non_blocking=True)
+            scanned = torch.cuda.Event()
+            scanned.record(torch.cuda.current_stream(device))
+        ids = torch.empty((views, max(int(cap), 1)), dtype=torch.int32, device=device)
+        ranges = torch.empty((views, gx * gy, 2), dtype=torch.int32, device=device)
+        with SPANS.span("sort"):
+            call("rgp_bin_sort_views", int(views), int(n), ptr(depths), int(width), int(height), ptr(ws),
+                 ctypes.c_size_t(stride), I64(int(cap)), ptr(ids), ptr(ranges), stream(device))
+        layer = torch.empty((views, dim, height, width), dtype=torch.float32, device=device)
+        contrib = torch.empty((views, height, width), dtype=torch.int32, device=device)
+        final_tau = torch.empty((views, height, width), dtype=torch.float32, device=device)
+        with SPANS.span("draw_forward"):
+            call("rgp_draw_forward_views", int(views), int(width), int(height), int(dim), ptr(ranges), ptr(ids),
+                 I64(ids.shape[1]), ptr(records), I64(int(n)), float(empty_tau), ptr(layer), ptr(contrib),
+                 ptr(final_tau), stream(device))
+        if not exact:
+            scanned.synchronize()
+            ms = host[:, 0].tolist()
+            for m in ms:
+                CAPACITY.update(n, width, height, m)
+            if int(host[:, 1].max()) != 0:
+                cap = CAPACITY.get(n, width, height)
+                continue
+        return layer, contrib, final_tau, ranges, ids, ms, ws, stride
+
+
+def draw_backward_replay_views(views, width, height, dim, ranges, ids, records, dlayer, n):
+    """Replay backward of a multi-view batch: dfeat (V, n, dim), one row set per view."""
+    dev = records.device
+    dfeat = torch.zeros((views, n, dim), dtype=torch.float32, device=dev)
+    dlayer = f32c(dlayer)
+    with SPANS.span("draw_backward"):
+        call("rgp_draw_backward_replay_views", int(views), int(width), int(height), int(dim), ptr(ranges), ptr(ids),
+             I64(ids.shape[1]), ptr(records), I64(int(n)), ptr(dlayer), ptr(dfeat), stream(dev))
+    return dfeat
 
 
 def mix(W, palette):

@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-bary", action="store_true", help="skip the barycentric-loss finetune step")
     ap.add_argument("--bary-only", action="store_true", help="run only the barycentric-loss finetune step (profiling)")
     ap.add_argument("--no-vanilla", action="store_true", help="skip the vanilla 3DGS training step")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 6M / 4K / 4-view batched finetune step")
+    ap.add_argument("--c4-only", action="store_true", help="run only the 6M / 4K / 4-view step (profiling)")
     ap.add_argument("--vanilla-only", action="store_true", help="run only the vanilla 3DGS training step (profiling)")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--serve-rate", type=float, default=500.0)
@@ -174,6 +176,71 @@ def measure_lowrank(args, dev, sizes=(1_000_000, 3_000_000), steps=16, warmup=2)
         torch.cuda.empty_cache()
     return {"steps_per_s": out, "workload": "rank-2 factors A (N,5,2), B (N,2,15), K=5, 1920x1080, 8 orbit views, "
                                             "L1+SSIM, Adam (one pass per tensor) on dc/A/B"}
+
+
+def measure_c4(args, dev, n=6_000_000, views=4, steps=4, warmup=1):
+    """The large finetune configuration: 6M Gaussians, K=7, rank-2 factors
+    P=7 x Q=15, 3840x2160, 4 views per step.  Batched: one
+    WeightGSFunctionFastViews call (one preprocess launch for the four
+    cameras, one compositing launch per direction over the four views' tiles)
+    against the same step with one WeightGSFunctionFast call per view; both
+    with the palette mix, L1+SSIM over the four images and Adam on dc/A/B."""
+    import numpy as np
+    import torch
+
+    from rgp import gsmodel, ops
+    from rgp.losses import L1SSIMLoss
+    from rgp.optim import PaletteAdam
+    from rgp.scene import make_scene, orbit_cameras
+
+    K, Q = 7, 15
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    sc = make_scene(n, seed=4, extent=3.0, K=K, lowrank=True, rank=2, P=7, Q=Q)
+    cams = orbit_cameras(views, 6.0, 3840, 2160)
+    tcams = [c.to_torch(dev) for c in cams]
+    pws, rots, scales, alphas = T(sc.pws), T(sc.rots), T(sc.scales), T(sc.alphas)
+    dc, A, B = (T(x).requires_grad_(True) for x in (sc.dc, sc.A, sc.B))
+    pal = T(sc.palette)
+    with torch.no_grad():
+        lay, _ = gsmodel.render_weights_views(pws, alphas, scales, rots, cams, K, dc=dc, A=A, B=B, Q=Q)
+        target = torch.cat([ops.mix(lay[v], torch.clamp(pal + 0.05, 0, 1)) for v in range(views)], 0)
+        del lay
+    opt = PaletteAdam([dc, A, B], lr=1e-3)
+
+    def step_batched():
+        layers, _ = gsmodel.WeightGSFunctionFastViews.apply(pws, dc, A, B, alphas, scales, rots, cams, K, Q)
+        rgb = torch.cat([ops.Mix.apply(layers[v], pal) for v in range(views)], 0)
+        loss = L1SSIMLoss.apply(rgb, target, 0.2)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+
+    def step_serial():
+        rgb = []
+        for tc in tcams:
+            layer, _ = gsmodel.WeightGSFunctionFast.apply(pws, dc, A, B, alphas, scales, rots, tc, K, Q)
+            rgb.append(ops.Mix.apply(layer, pal))
+        loss = L1SSIMLoss.apply(torch.cat(rgb, 0), target, 0.2)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+
+    out = {}
+    for name, fn in (("batched", step_batched), ("one_view_per_launch", step_serial)):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        out[name] = round(steps / (time.perf_counter() - t0), 3)
+        log(f"C4 6M K=7 rank-2 4K x{views} views, {name}: {out[name]:.3f} steps/s")
+    del sc, pws, rots, scales, alphas, dc, A, B, opt, target
+    torch.cuda.empty_cache()
+    return {"steps_per_s": out, "views_per_s": round(out["batched"] * views, 2),
+            "workload": f"{n} Gaussians, K=7 rank-2 (A (N,7,2), B (N,2,15)), 3840x2160, {views} views per step; "
+                        "mix, L1+SSIM over the views, Adam (one pass per tensor) on dc/A/B"}
 
 
 def torch_bary_loss(Wt, P, C, tau=1e-9, eps=1e-12, lam_pos=0.5):
@@ -382,6 +449,9 @@ def main():
     if args.vanilla_only:
         print(json.dumps({"vanilla_step": measure_vanilla(args, dev)}))
         return
+    if args.c4_only:
+        print(json.dumps({"large_finetune_4views": measure_c4(args, dev)}))
+        return
     if args.copy_calibration:
         a = torch.empty(64 << 20, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -517,6 +587,7 @@ def main():
     lowrank = None if args.no_lowrank else measure_lowrank(args, dev)
     bary = None if args.no_bary else measure_bary(args, dev)
     vanilla = None if args.no_vanilla else measure_vanilla(args, dev)
+    c4 = None if args.no_c4 else measure_c4(args, dev)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -556,6 +627,7 @@ def main():
         "pairs_per_view": int(np.mean([w["pairs"] for w in work])),
         "p99_palette_edit_ms": serving["edit_p99_ms"] if serving else None,
         "serving": serving, "lowrank_rank2": lowrank, "barycentric_step": bary, "vanilla_step": vanilla,
+        "large_finetune_4views": c4,
         "kernels": breakdown, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -78,21 +78,46 @@ def expected_outputs(engine, reqs):
     return out
 
 
+def test_default_workload_with_batch_slots_fits_200gb():
+    from rgp.serving import default_specs
+    store = SceneStore(200 * 2**30, batch_views=8)
+    for s in default_specs():
+        store.admit(s)
+    assert store.used < 200 * 2**30
+    assert store.used > SceneStore(200 * 2**30).admit(default_specs()[0])
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["batched", "serial", "serial_rerender"])
+@pytest.mark.parametrize("mode", ["batched", "batched_oneview", "batched_smallcap", "serial", "serial_rerender"])
 def test_results_alone_equal_in_mix(device, mode, monkeypatch):
+    """batched: poses of a scene and resolution pending together render as
+    multi-view batches; _oneview: one pose per launch; _smallcap: pair buffers
+    planned far too small, so batches overflow, grow and re-render the
+    requests that overflowed."""
     import rgp.serving as S
     monkeypatch.setattr(S, "RESOLUTIONS", SMALL_RES)
     print(f"case: serving {mode}", flush=True)
-    eng = S.Engine(SMALL, device, n_poses=8)
+    kw = dict(batch_views=0) if mode == "batched_oneview" else {}
+    if mode == "batched_smallcap":
+        kw = dict(cap_margin=0.05)
+    eng = S.Engine(SMALL, device, n_poses=8, **kw)
     reqs = S.make_requests(SMALL, 120, rate=0.0, seed=5, n_poses=8)
     eng.prepare(reqs)
-    rep = eng.run(reqs, mode=mode.split("_")[0], realtime=False, rerender_edits=mode.endswith("rerender"))
+    # 32 requests admitted per loop iteration: the run spans several iterations
+    rep = eng.run(reqs, mode=mode.split("_")[0], realtime=False, rerender_edits=mode.endswith("rerender"),
+                  max_batch=32)
     assert rep["requests"] == 120 and rep["edits"] > 40
+    assert all(r.done is not None for r in reqs)
     exp = expected_outputs(eng, reqs)
     for r in reqs:
         got = r.out.cpu().numpy()
         assert np.array_equal(got, exp[r.rid]), (r.rid, r.kind, r.scene)
+    if mode == "batched":
+        assert any(v > 1 for bs in eng.bslots.values() for v in bs.graphs)  # some batches held several views
+        # the same engine then serves one request at a time, with the same results
+        eng.run(reqs, mode="serial", realtime=False, max_batch=32)
+        for r in reqs:
+            assert np.array_equal(r.out.cpu().numpy(), exp[r.rid]), ("serial after batched", r.rid)
 
 
 @pytest.mark.gpu

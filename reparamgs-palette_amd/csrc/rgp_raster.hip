@@ -154,10 +154,9 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) {
           if (!(slots & (1u << j))) continue;
-          if (tau[j] < 0.0001f) continue;
           const float dy = g0.y - fpy[j];
           const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
-          if (m > g1.z) continue;
+          if (tau[j] < 0.0001f || m > g1.z) continue;  // stopped pixel, or beyond the cutoff
           const float ap = fminf(0.99f, g1.y * gauss_half(m));
           if (ap < 0.002f) continue;
           const float ta = tau[j] * ap;
@@ -561,10 +560,9 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
 #pragma unroll
       for (int j = 0; j < kSlots; ++j) {
         if (!(slots & (1u << j))) continue;
-        if (tau[j] < 0.0001f) continue;
         const float dy = g0.y - fpy[j];
         const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
-        if (m > g1.z) continue;
+        if (tau[j] < 0.0001f || m > g1.z) continue;  // stopped pixel, or beyond the cutoff
         const float ap = fminf(0.99f, g1.y * gauss_half(m));
         if (ap < 0.002f) continue;
         const float ta = tau[j] * ap;

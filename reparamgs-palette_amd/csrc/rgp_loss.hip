@@ -44,12 +44,71 @@ __device__ __forceinline__ float block_sum_f(float v, float* lds) {
 // with a float64 reference within a tolerance, not bit for bit).
 constexpr int kSG = kST / 4;  // four-output groups per tile edge
 
+// Blurs three maps of the staged tile: horizontally into hb (the products of
+// an input pair formed once per input, not once per tap), then vertically
+// into mu[m][o] for this thread's four outputs (column q, rows r0..r0+3).
+// MAPS selects (x, y, x^2) or (y^2, x y).
+template <int MAPS>
+__device__ __forceinline__ void ssim_blur_phase(const float (*sx)[kSI + 1], const float (*sy)[kSI + 1],
+                                                float (*hb)[kSI][kST + 1], const Win11& win, int q, int r0,
+                                                float (*mu)[4]) {
+  constexpr int NM = MAPS == 0 ? 3 : 2;
+  for (int i = threadIdx.x; i < kSI * kSG; i += blockDim.x) {
+    const int r = i % kSI, c0 = 4 * (i / kSI);
+    float in[NM][14];
+#pragma unroll
+    for (int j = 0; j < 14; ++j) {
+      const float a = sx[r][c0 + j], b = sy[r][c0 + j];
+      if (MAPS == 0) {
+        in[0][j] = a;
+        in[1][j] = b;
+        in[2][j] = a * a;
+      } else {
+        in[0][j] = b * b;
+        in[1][j] = a * b;
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      float acc[NM];
+#pragma unroll
+      for (int m = 0; m < NM; ++m) acc[m] = 0.f;
+#pragma unroll
+      for (int j = 0; j < 11; ++j)
+#pragma unroll
+        for (int m = 0; m < NM; ++m) acc[m] = fmaf(win.g[j], in[m][o + j], acc[m]);
+#pragma unroll
+      for (int m = 0; m < NM; ++m) hb[m][r][c0 + o] = acc[m];
+    }
+  }
+  __syncthreads();
+  float v[NM][14];
+#pragma unroll
+  for (int m = 0; m < NM; ++m)
+#pragma unroll
+    for (int j = 0; j < 14; ++j) v[m][j] = hb[m][r0 + j][q];
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m) mu[m][o] = 0.f;
+#pragma unroll
+    for (int j = 0; j < 11; ++j)
+#pragma unroll
+      for (int m = 0; m < NM; ++m) mu[m][o] = fmaf(win.g[j], v[m][o + j], mu[m][o]);
+  }
+  __syncthreads();  // hb is reused by the next phase
+}
+
+// One 32x32 output tile per block of 256 threads; thread t owns column
+// t % 32, rows 4 (t / 32) .. + 3.  The five blurred maps are formed in two
+// phases through one three-map LDS buffer (31 KB per block in all, five
+// blocks per CU).
 __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __restrict__ img,
                                                   const float* __restrict__ tgt, Win11 win,
                                                   float C1, float C2, float* __restrict__ abc,
                                                   double* __restrict__ sums) {
   __shared__ float sx[kSI][kSI + 1], sy[kSI][kSI + 1];
-  __shared__ float hb[5][kSI][kST + 1];
+  __shared__ float hb[3][kSI][kST + 1];
   __shared__ float red[8];
   const int c = blockIdx.z;
   const size_t HW = (size_t)H * W;
@@ -64,64 +123,28 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __r
     sy[r][q] = in ? Y[(size_t)gy * W + gx] : 0.f;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kSI * kSG; i += blockDim.x) {
-    const int r = i % kSI, q0 = 4 * (i / kSI);
-    float a[14], b[14];
-#pragma unroll
-    for (int j = 0; j < 14; ++j) {
-      a[j] = sx[r][q0 + j];
-      b[j] = sy[r][q0 + j];
-    }
-#pragma unroll
-    for (int o = 0; o < 4; ++o) {
-      float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f;
-#pragma unroll
-      for (int j = 0; j < 11; ++j) {
-        const float g = win.g[j], u = a[o + j], v = b[o + j];
-        m0 = fmaf(g, u, m0);
-        m1 = fmaf(g, v, m1);
-        m2 = fmaf(g, u * u, m2);
-        m3 = fmaf(g, v * v, m3);
-        m4 = fmaf(g, u * v, m4);
-      }
-      hb[0][r][q0 + o] = m0; hb[1][r][q0 + o] = m1; hb[2][r][q0 + o] = m2;
-      hb[3][r][q0 + o] = m3; hb[4][r][q0 + o] = m4;
-    }
-  }
-  __syncthreads();
+  const int q = threadIdx.x % kST, r0 = 4 * (threadIdx.x / kST);
+  float m3[3][4], m2[2][4];
+  ssim_blur_phase<0>(sx, sy, hb, win, q, r0, m3);
+  ssim_blur_phase<1>(sx, sy, hb, win, q, r0, m2);
   float s_sum = 0.f, l1_sum = 0.f;
-  for (int i = threadIdx.x; i < kSG * kST; i += blockDim.x) {
-    const int q = i % kST, r0 = 4 * (i / kST);
-    float v[5][14];
+  const int gx = x0 + kSR + q;
 #pragma unroll
-    for (int m = 0; m < 5; ++m)
-#pragma unroll
-      for (int j = 0; j < 14; ++j) v[m][j] = hb[m][r0 + j][q];
-    const int gx = x0 + kSR + q;
-#pragma unroll
-    for (int o = 0; o < 4; ++o) {
-      const int r = r0 + o, gy = y0 + kSR + r;
-      if (gy >= H || gx >= W) continue;
-      float mu[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int j = 0; j < 11; ++j) {
-        const float g = win.g[j];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) mu[m] = fmaf(g, v[m][o + j], mu[m]);
-      }
-      const float mx = mu[0], my = mu[1], exx = mu[2], eyy = mu[3], exy = mu[4];
-      const float A1 = 2.f * mx * my + C1, A2 = 2.f * (exy - mx * my) + C2;
-      const float B1 = mx * mx + my * my + C1, B2 = (exx - mx * mx) + (eyy - my * my) + C2;
-      const float D = B1 * B2, Dinv = 1.f / D;
-      const float sv = A1 * A2 * Dinv;
-      const float dN = 2.f * my * (A2 - A1), dD = 2.f * mx * (B2 - B1);
-      const size_t o_ = c * HW + (size_t)gy * W + gx;
-      abc[o_] = (dN - sv * dD) * Dinv;     // ds/dmu_x
-      abc[3 * HW + o_] = -sv * B1 * Dinv;  // ds/dE[x^2]
-      abc[6 * HW + o_] = 2.f * A1 * Dinv;  // ds/dE[xy]
-      s_sum += sv;
-      l1_sum += fabsf(sx[r + kSR][q + kSR] - sy[r + kSR][q + kSR]);
-    }
+  for (int o = 0; o < 4; ++o) {
+    const int r = r0 + o, gy = y0 + kSR + r;
+    if (gy >= H || gx >= W) continue;
+    const float mx = m3[0][o], my = m3[1][o], exx = m3[2][o], eyy = m2[0][o], exy = m2[1][o];
+    const float A1 = 2.f * mx * my + C1, A2 = 2.f * (exy - mx * my) + C2;
+    const float B1 = mx * mx + my * my + C1, B2 = (exx - mx * mx) + (eyy - my * my) + C2;
+    const float D = B1 * B2, Dinv = 1.f / D;
+    const float sv = A1 * A2 * Dinv;
+    const float dN = 2.f * my * (A2 - A1), dD = 2.f * mx * (B2 - B1);
+    const size_t o_ = c * HW + (size_t)gy * W + gx;
+    abc[o_] = (dN - sv * dD) * Dinv;     // ds/dmu_x
+    abc[3 * HW + o_] = -sv * B1 * Dinv;  // ds/dE[x^2]
+    abc[6 * HW + o_] = 2.f * A1 * Dinv;  // ds/dE[xy]
+    s_sum += sv;
+    l1_sum += fabsf(sx[r + kSR][q + kSR] - sy[r + kSR][q + kSR]);
   }
   const float bs = block_sum_f(s_sum, red);
   __syncthreads();

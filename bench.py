@@ -308,8 +308,10 @@ def measure_bary(args, dev, n=1_000_000, views=8, steps=24, warmup=3):
         frozen = [torch.clamp(ops.mix(gsmodel.render_weights(pws, alphas, scales, rots, tc, 5, shs=shs)[0], pal0), 0, 1)
                   for tc in tcams]
         del shs
-    model = gsmodel.FinetuneModel(num_palette=3, use_fast=False)
-    opt = PaletteAdam([v for v in params.values() if torch.is_tensor(v)], lr=1e-3)
+    model = gsmodel.FinetuneModel(num_palette=3, use_fast=False)  # two leaves, rendered without concatenation
+    # Adam on every parameter; the two SH leaves take the update fused with their backward
+    opt = PaletteAdam([v for v in params.values() if torch.is_tensor(v)], lr=1e-3,
+                      fused_sh=[(params["low_shs_w"], params["high_shs_w"])])
 
     def step(i):
         W_tilde, pal = model(pws, params, alphas, scales, rots, tcams[i % views])

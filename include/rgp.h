@@ -217,7 +217,9 @@ int rgp_draw_backward_replay(int width, int height, int dim, const int32_t* rang
  * getRects of WeightGSFunctionTest/Fast.forward (gsmodel.py:107-118 /
  * 155-170) in one pass.  mode 0: full rank, shs (N, P*K) with P the number
  * of SH coefficients per channel (1, 4, 9 or 16; Q and rank unused); mode 1:
- * low rank, dc (N,K), A (N,P,r), B (N,r,Q).  Writes depths (N, with -1 culls), records
+ * low rank, dc (N,K), A (N,P,r), B (N,r,Q); mode 2: full rank held as the
+ * reference's two leaves, dc = low_shs_w (N,K) and shs = high_shs_w
+ * (N,(P-1)*K), giving the values of mode 0 on their concatenation.  Writes depths (N, with -1 culls), records
  * (N, rgp_record_floats(K)), and rects/counts into ws.  us/cinv2ds/areas/
  * weights (nullable) receive the unfused outputs for inspection. */
 int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int rank,
@@ -294,6 +296,16 @@ int rgp_sh_backward_adam_full(int n, int K, int sh_dim, const rgp_camera* cam, c
                               const float* dw, float* shs, float* exp_avg, float* exp_avg_sq,
                               int64_t step, double lr, double beta1, double beta2, double eps,
                               double weight_decay, hipStream_t stream);
+
+/* No counterpart in the reference: the same fused update for the
+ * reference's two coefficient leaves (gsplat/utils.py:157-158 concatenates
+ * low_shs_w (N,K) and high_shs_w (N,15K) before every render): low holds
+ * band 0 and high bands 1-15, each with its own Adam moments; the values are
+ * those of rgp_sh_backward_adam_full on the concatenated tensor. */
+int rgp_sh_backward_adam_split(int n, int K, const rgp_camera* cam, const float* pws, const float* dw, float* low,
+                               float* low_exp_avg, float* low_exp_avg_sq, float* high, float* high_exp_avg,
+                               float* high_exp_avg_sq, int64_t step, double lr, double beta1, double beta2,
+                               double eps, double weight_decay, hipStream_t stream);
 
 /* No counterpart in the reference either: one torch.optim.Adam step (the
  * optimizer of finetune.py:346-399) over a flat float32 tensor of n elements

@@ -570,6 +570,22 @@ __device__ __forceinline__ void stage_sh_rows(float4* s_sh, int n, const float* 
   __syncthreads();
 }
 
+// Stages the block's 64 rows of the higher bands (15*K floats each, WMODE 5:
+// the reference's second coefficient leaf) in LDS as one contiguous span.
+// Row g's floats sit at s[g*15K ..]: with 15K odd the row reads of a
+// wavefront fall in distinct banks.
+template <int K>
+__device__ __forceinline__ void stage_high_rows(float* s_hi, int n, const float* __restrict__ high) {
+  constexpr int ROW = 15 * K;
+  const int t = threadIdx.x;
+  const int tot = min(64, n - (int)blockIdx.x * 64) * ROW;
+  const float* src = high + (size_t)blockIdx.x * 64 * ROW;
+  const int t4 = tot >> 2;
+  for (int e = t; e < t4; e += 64) reinterpret_cast<float4*>(s_hi)[e] = reinterpret_cast<const float4*>(src)[e];
+  for (int e = 4 * t4 + t; e < tot; e += 64) s_hi[e] = src[e];
+  __syncthreads();
+}
+
 // The whole per-Gaussian chain of one view for Gaussian i (lane t of its
 // block); returns uint32(depth * 1000) of a binned Gaussian, else 0.
 template <int K, int WMODE>
@@ -605,6 +621,22 @@ __device__ __forceinline__ uint32_t preprocess_gaussian(
       row[4 * q] = c.x; row[4 * q + 1] = c.y; row[4 * q + 2] = c.z; row[4 * q + 3] = c.w;
     }
     sh_weights_full<K>(row, Y, 16, w);
+  } else if (WMODE == 5 || WMODE == 6) {
+    // the reference's two leaves, low (N,K) in dc and the higher bands
+    // (N,(sh_dim-1)K) in shs: the row they concatenate to (gsplat/utils.py:157)
+    if (sh_dim > 1) sh_basis(px, py, pz, cam->twc, sh_dim, Y);
+    float row[16 * K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) row[k] = dc[(size_t)i * K + k];
+    if (WMODE == 5) {
+      const float* s_hi = reinterpret_cast<const float*>(s_sh);
+#pragma unroll
+      for (int e = 0; e < 15 * K; ++e) row[K + e] = s_hi[t * 15 * K + e];
+    } else {
+      const int hr = (sh_dim - 1) * K;
+      for (int e = 0; e < hr; ++e) row[K + e] = shs[(size_t)i * hr + e];
+    }
+    sh_weights_full<K>(row, Y, sh_dim, w);
   } else if (WMODE == 1) {
     const int L = P * Q / K;
     if (L > 0) sh_basis(px, py, pz, cam->twc, L + 1, Y);
@@ -661,10 +693,11 @@ __global__ __launch_bounds__(64) void k_preprocess(
     uint2* __restrict__ rects, uint32_t* __restrict__ counts, uint32_t* __restrict__ dqw,
     int gx, int gy, float* __restrict__ us_out, float* __restrict__ cinv_out,
     int32_t* __restrict__ areas_out, float* __restrict__ w_out) {
-  __shared__ float4 s_sh[WMODE == 4 ? 64 * 4 * K : 1];
+  __shared__ float4 s_sh[WMODE == 4 ? 64 * 4 * K : WMODE == 5 ? (64 * 15 * K + 3) / 4 : 1];
   const int t = threadIdx.x;
   const int i = blockIdx.x * 64 + t;
   if constexpr (WMODE == 4) stage_sh_rows<K, WMODE>(s_sh, n, shs);
+  if constexpr (WMODE == 5) stage_high_rows<K>(reinterpret_cast<float*>(s_sh), n, shs);
   uint32_t dqv = 0;
   if (i < n)
     dqv = preprocess_gaussian<K, WMODE>(i, t, s_sh, sh_dim, P, Q, rank, cam, pws, rots, scales, alphas, shs, dc,
@@ -687,10 +720,11 @@ __global__ __launch_bounds__(64) void k_preprocess_views(
     const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ depths,
     float4* __restrict__ recs, char* __restrict__ ws, size_t ws_stride, size_t off_rects, size_t off_counts,
     size_t off_dqw, int gx, int gy) {
-  __shared__ float4 s_sh[WMODE == 4 ? 64 * 4 * K : 1];
+  __shared__ float4 s_sh[WMODE == 4 ? 64 * 4 * K : WMODE == 5 ? (64 * 15 * K + 3) / 4 : 1];
   const int t = threadIdx.x;
   const int i = blockIdx.x * 64 + t;
   if constexpr (WMODE == 4) stage_sh_rows<K, WMODE>(s_sh, n, shs);
+  if constexpr (WMODE == 5) stage_high_rows<K>(reinterpret_cast<float*>(s_sh), n, shs);
   for (int v = 0; v < views; ++v) {
     char* w = ws + (size_t)v * ws_stride;
     uint32_t dqv = 0;
@@ -1088,14 +1122,17 @@ __device__ __forceinline__ void adam_elem(float& p, float& m, float& v, float g,
   p = p + a.neg_step * (m / denom);
 }
 
-template <int K, int SH>
-__global__ __launch_bounds__(256) void k_sh_adam_full(int n, const rgp_camera* __restrict__ cam,
-                                                      const float* __restrict__ pws,
-                                                      const float* __restrict__ dw, float* __restrict__ p,
-                                                      float* __restrict__ m, float* __restrict__ v,
-                                                      AdamArgs a) {
-  constexpr int G = 64, ROW = SH * K;
-  __shared__ float sY[G][SH + 1];
+// The update of one coefficient tensor holding bands LM0 .. LM0 + NB - 1 of
+// each Gaussian (rows of NB*K floats): the full-rank tensor is LM0 = 0,
+// NB = SH; the reference's two leaves are the DC band (0, 1) and the rest
+// (1, 15).
+template <int K, int LM0, int NB>
+__device__ __forceinline__ void sh_adam_span(int n, const rgp_camera* __restrict__ cam,
+                                             const float* __restrict__ pws, const float* __restrict__ dw,
+                                             float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                                             const AdamArgs& a) {
+  constexpr int G = 64, ROW = NB * K, SH = LM0 + NB;
+  __shared__ float sY[G][NB + 1];
   __shared__ float sW[G][K];
   const int g0 = blockIdx.x * G, t = threadIdx.x;
   if (t < G) {
@@ -1110,7 +1147,7 @@ __global__ __launch_bounds__(256) void k_sh_adam_full(int n, const rgp_camera* _
     }
     Y[0] = (float)SH_C0_0;
 #pragma unroll
-    for (int l = 0; l < SH; ++l) sY[t][l] = Y[l];
+    for (int l = 0; l < NB; ++l) sY[t][l] = Y[LM0 + l];
   }
   __syncthreads();
   const int cnt = min(G, n - g0);
@@ -1163,6 +1200,24 @@ __global__ __launch_bounds__(256) void k_sh_adam_full(int n, const rgp_camera* _
   }
   for (int e = 4 * n4 + t; e < total; e += blockDim.x)
     adam_elem(p[base + e], m[base + e], v[base + e], grad(e), a);
+}
+
+template <int K, int SH>
+__global__ __launch_bounds__(256) void k_sh_adam_full(int n, const rgp_camera* __restrict__ cam,
+                                                      const float* __restrict__ pws,
+                                                      const float* __restrict__ dw, float* __restrict__ p,
+                                                      float* __restrict__ m, float* __restrict__ v,
+                                                      AdamArgs a) {
+  sh_adam_span<K, 0, SH>(n, cam, pws, dw, p, m, v, a);
+}
+
+template <int K, int LM0, int NB>
+__global__ __launch_bounds__(256) void k_sh_adam_bands(int n, const rgp_camera* __restrict__ cam,
+                                                       const float* __restrict__ pws,
+                                                       const float* __restrict__ dw, float* __restrict__ p,
+                                                       float* __restrict__ m, float* __restrict__ v,
+                                                       AdamArgs a) {
+  sh_adam_span<K, LM0, NB>(n, cam, pws, dw, p, m, v, a);
 }
 
 }  // namespace rgp
@@ -1380,12 +1435,13 @@ extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int r
   if (mode == 1) {
     int rc = check_lowrank("rgp_preprocess_weight", K, P, Q, rank);
     if (rc) return rc;
-  } else if (mode != 0) {
-    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight: mode must be 0 or 1");
+  } else if (mode != 0 && mode != 2) {
+    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight: mode must be 0, 1 or 2");
   }
-  // mode 0: P carries the number of SH coefficients per channel
-  const int sh_dim = mode == 0 ? P : 16;
-  if (mode == 0 && !(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
+  // modes 0 and 2: P carries the number of SH coefficients per channel
+  const int sh_dim = mode != 1 ? P : 16;
+  if (mode == 2 && sh_dim > 1 && !shs) return set_error(RGP_ERR_ARG, "rgp_preprocess_weight: mode 2 needs shs (higher bands)");
+  if (mode != 1 && !(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
     return set_error(RGP_ERR_ARG, "rgp_preprocess_weight: sh_dim (P) must be 1, 4, 9 or 16");
   BinLayout L = bin_layout(n, capacity, width, height);
   char* base = (char*)ws;
@@ -1395,6 +1451,7 @@ extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int r
   int wmode = mode;
   if (mode == 1 && Q == 15 && P == K && (rank == 1 || rank == 2)) wmode = rank + 1;
   if (mode == 0 && sh_dim == 16 && ((uintptr_t)shs & 15) == 0) wmode = 4;
+  if (mode == 2) wmode = (sh_dim == 16 && ((uintptr_t)shs & 15) == 0) ? 5 : 6;
 #define RGP_PRE(WM)                                                                               \
   RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess<KK, WM>), dim3((n + 63) / 64), dim3(64), 0, s, n, \
                                      sh_dim, P, Q, rank, cam, pws, (const float4*)rots, scales,   \
@@ -1407,6 +1464,8 @@ extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int r
     case 1: RGP_PRE(1); break;
     case 2: RGP_PRE(2); break;
     case 3: RGP_PRE(3); break;
+    case 5: RGP_PRE(5); break;
+    case 6: RGP_PRE(6); break;
     default: RGP_PRE(4); break;
   }
 #undef RGP_PRE
@@ -1424,11 +1483,12 @@ extern "C" int rgp_preprocess_weight_views(int views, int n, int K, int mode, in
   if (mode == 1) {
     int rc = check_lowrank("rgp_preprocess_weight_views", K, P, Q, rank);
     if (rc) return rc;
-  } else if (mode != 0) {
-    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight_views: mode must be 0 or 1");
+  } else if (mode != 0 && mode != 2) {
+    return set_error(RGP_ERR_ARG, "rgp_preprocess_weight_views: mode must be 0, 1 or 2");
   }
-  const int sh_dim = mode == 0 ? P : 16;
-  if (mode == 0 && !(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
+  const int sh_dim = mode != 1 ? P : 16;
+  if (mode == 2 && sh_dim > 1 && !shs) return set_error(RGP_ERR_ARG, "rgp_preprocess_weight_views: mode 2 needs shs (higher bands)");
+  if (mode != 1 && !(sh_dim == 1 || sh_dim == 4 || sh_dim == 9 || sh_dim == 16))
     return set_error(RGP_ERR_ARG, "rgp_preprocess_weight_views: sh_dim (P) must be 1, 4, 9 or 16");
   BinLayout L = bin_layout(n, capacity, width, height);
   if (ws_stride < L.total || (ws_stride & 255) != 0)
@@ -1443,6 +1503,7 @@ extern "C" int rgp_preprocess_weight_views(int views, int n, int K, int mode, in
   int wmode = mode;
   if (mode == 1 && Q == 15 && P == K && (rank == 1 || rank == 2)) wmode = rank + 1;
   if (mode == 0 && sh_dim == 16 && ((uintptr_t)shs & 15) == 0) wmode = 4;
+  if (mode == 2) wmode = (sh_dim == 16 && ((uintptr_t)shs & 15) == 0) ? 5 : 6;
 #define RGP_PREV(WM)                                                                                    \
   RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess_views<KK, WM>), dim3((n + 63) / 64), dim3(64), 0, s,  \
                                      views, n, sh_dim, P, Q, rank, cams, pws, (const float4*)rots, scales, \
@@ -1453,6 +1514,8 @@ extern "C" int rgp_preprocess_weight_views(int views, int n, int K, int mode, in
     case 1: RGP_PREV(1); break;
     case 2: RGP_PREV(2); break;
     case 3: RGP_PREV(3); break;
+    case 5: RGP_PREV(5); break;
+    case 6: RGP_PREV(6); break;
     default: RGP_PREV(4); break;
   }
 #undef RGP_PREV
@@ -1636,6 +1699,27 @@ extern "C" int rgp_sh_backward_adam_full(int n, int K, int sh_dim, const rgp_cam
   }
 #undef RGP_ADAM
   return check_launch("rgp_sh_backward_adam_full");
+}
+
+extern "C" int rgp_sh_backward_adam_split(int n, int K, const rgp_camera* cam, const float* pws, const float* dw,
+                                          float* low, float* low_exp_avg, float* low_exp_avg_sq, float* high,
+                                          float* high_exp_avg, float* high_exp_avg_sq, int64_t step, double lr,
+                                          double beta1, double beta2, double eps, double weight_decay,
+                                          hipStream_t s) {
+  if (n < 0 || K < 1 || K > RGP_MAX_DIM)
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_split: bad n or K");
+  if (step < 1) return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_split: step must be >= 1");
+  if ((((uintptr_t)low) | ((uintptr_t)low_exp_avg) | ((uintptr_t)low_exp_avg_sq) | ((uintptr_t)high) |
+       ((uintptr_t)high_exp_avg) | ((uintptr_t)high_exp_avg_sq)) & 15)
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_split: leaves and moments must be 16-byte aligned");
+  if (n == 0) return RGP_OK;
+  const AdamArgs a = adam_args(step, lr, beta1, beta2, eps, weight_decay);
+  const int blocks = (n + 63) / 64;
+  RGP_K_SWITCH(K, hipLaunchKernelGGL((k_sh_adam_bands<KK, 0, 1>), dim3(blocks), dim3(256), 0, s, n, cam, pws, dw,
+                                     low, low_exp_avg, low_exp_avg_sq, a));
+  RGP_K_SWITCH(K, hipLaunchKernelGGL((k_sh_adam_bands<KK, 1, 15>), dim3(blocks), dim3(256), 0, s, n, cam, pws, dw,
+                                     high, high_exp_avg, high_exp_avg_sq, a));
+  return check_launch("rgp_sh_backward_adam_split");
 }
 
 extern "C" int rgp_sh_backward_lowrank(int n, int K, int P, int Q, int rank,

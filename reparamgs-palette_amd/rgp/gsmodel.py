@@ -44,11 +44,13 @@ def _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dlayer):
 
 
 def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None, B=None, Q=None,
-                   sh_dim=None, return_state=False, exact=False):
+                   sh_dim=None, return_state=False, exact=False, low=None, high=None):
     """Forward of the palette-weight rasterizer.  Full rank when `shs`
-    (N, sh_dim*K) is given, low rank (dc, A (N,P,r), B (N,r,Q)) otherwise.
-    The pair buffers keep the cached capacity unless exact=True (see
-    ops.splat_pipeline); the results do not depend on it."""
+    (N, sh_dim*K) is given, or the reference's two leaves low (N,K) and high
+    (N,(sh_dim-1)K) (the values of their concatenation, without forming it);
+    low rank (dc, A (N,P,r), B (N,r,Q)) otherwise.  The pair buffers keep the
+    cached capacity unless exact=True (see ops.splat_pipeline); the results
+    do not depend on it."""
     require_cuda(pws, alphas, scales, rots)
     dev = pws.device
     n = pws.shape[0]
@@ -57,18 +59,7 @@ def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None,
     pws_c, rots_c, scales_c, alphas_c = f32c(pws), f32c(rots), f32c(scales), f32c(alphas).reshape(-1)
     depths = torch.empty((n,), dtype=torch.float32, device=dev)
     records = _rec(n, K, dev)
-    if shs is not None:
-        shs_c = f32c(shs)
-        sd = shs_c.shape[1] // K if sh_dim is None else sh_dim
-        args = (0, sd, 0, 1, shs_c, None, None, None)
-    else:
-        dc_c, A_c, B_c = f32c(dc), f32c(A), f32c(B)
-        if A_c.dim() == 2:
-            P, r = A_c.shape[1], 1
-        else:
-            P, r = A_c.shape[1], A_c.shape[2]
-        args = (1, P, int(Q), r, None, dc_c, A_c, B_c)
-    mode, P_, Q_, r_, shs_c, dc_c, A_c, B_c = args
+    mode, P_, Q_, r_, shs_c, dc_c, A_c, B_c = _coef_args(K, shs, dc, A, B, Q, sh_dim, low, high)
 
     def fill(ws, cap):
         call("rgp_preprocess_weight", n, K, mode, P_, Q_, r_, ptr(camd), ptr(pws_c), ptr(rots_c), ptr(scales_c),
@@ -83,8 +74,12 @@ def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None,
     return layer, depths
 
 
-def _coef_args(K, shs, dc, A, B, Q, sh_dim):
+def _coef_args(K, shs, dc, A, B, Q, sh_dim, low=None, high=None):
     """(mode, P, Q, rank, shs, dc, A, B) of the preprocess entry points."""
+    if low is not None:  # mode 2: the two leaves, high bands in the shs slot, DC band in the dc slot
+        high_c = f32c(high) if high is not None and high.numel() > 0 else None
+        sd = 1 + (high.shape[1] // K if high_c is not None else 0)
+        return 2, sd, 0, 1, high_c, f32c(low), None, None
     if shs is not None:
         shs_c = f32c(shs)
         sd = shs_c.shape[1] // K if sh_dim is None else sh_dim
@@ -180,6 +175,40 @@ class WeightGSFunctionFastViews(torch.autograd.Function):
             call("rgp_sh_backward_lowrank_views", V, n, K, P, Q, r, ptr(camsd), ptr(pws), ptr(A), ptr(B), ptr(dw),
                  ptr(ddc), ptr(dA), ptr(dB), stream(pws.device))
         return (None, ddc, dA, dB, None, None, None, None, None, None)
+
+
+class WeightGSFunctionSplit(torch.autograd.Function):
+    """WeightGSFunctionTest on the reference's two coefficient leaves
+    (FinetuneModel's low_shs_w (N,K) and high_shs_w (N,15K), which
+    gsplat/utils.py:157-158 concatenates before every render): the same
+    (layer, mask) and the same gradients, split over the leaves, without the
+    concatenation.  An optimizer that fuses the SH update (rgp.optim.PaletteAdam
+    with the pair in fused_sh) takes dL/dw instead of the two gradients."""
+
+    @staticmethod
+    def forward(ctx, pws, low, high, alphas, scales, rots, cam, num_palette):
+        K = int(num_palette)
+        layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, low=low, high=high, return_state=True)
+        ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"])
+        fused = getattr(low, "_rgp_fused_adam", None)
+        ctx.fused = (fused, low, high) if fused is not None else None
+        ctx.save_for_backward(f32c(pws), st["records"], st["contrib"], st["final_tau"], st["ranges"], st["ids"],
+                              st["camd"])
+        return layer, depths > 0.2
+
+    @staticmethod
+    def backward(ctx, dloss_dlayer, _):
+        W, H, K, n, sh_dim = ctx.meta
+        pws, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
+        dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())
+        if ctx.fused is not None and ctx.fused[0].take_split(ctx.fused[1], ctx.fused[2], dw, camd, pws, K, sh_dim):
+            ctx.fused = None
+            return (None, None, None, None, None, None, None, None)
+        ctx.fused = None
+        dshs = torch.empty((n, sh_dim * K), dtype=torch.float32, device=pws.device)
+        with SPANS.span("sh_backward"):
+            call("rgp_sh_backward_full", n, K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(dshs), stream(pws.device))
+        return (None, dshs[:, :K], dshs[:, K:], None, None, None, None, None)
 
 
 class WeightGSFunctionTest(torch.autograd.Function):
@@ -283,14 +312,17 @@ def get_shs(low_shs, high_shs):
 
 
 class FinetuneModel(torch.nn.Module):
-    """gsmodel.py:292-329."""
+    """gsmodel.py:292-329.  split_leaves: full-rank coefficients held as two
+    leaves render through WeightGSFunctionSplit instead of concatenating them
+    (get_shs) every forward; the values are the same."""
 
-    def __init__(self, sh_order=3, num_palette=4, use_fast=True):
+    def __init__(self, sh_order=3, num_palette=4, use_fast=True, split_leaves=True):
         super().__init__()
         self.sh_order = sh_order
         self.num_palette = num_palette
         self.splat_dim = num_palette + 2
         self.use_fast = use_fast
+        self.split_leaves = split_leaves
         self.mask = None
 
     def forward(self, pws, params, alphas, scales, rots, cam):
@@ -298,6 +330,9 @@ class FinetuneModel(torch.nn.Module):
             W_tilde, self.mask = WeightGSFunctionFast.apply(pws, params["low_shs_w"], params["high_shs_A"],
                                                             params["high_shs_B"], alphas, scales, rots, cam,
                                                             self.splat_dim, params["reshape_Q"])
+        elif not params["low_rank"] and self.split_leaves:
+            W_tilde, self.mask = WeightGSFunctionSplit.apply(pws, params["low_shs_w"], params["high_shs_w"], alphas,
+                                                             scales, rots, cam, self.splat_dim)
         else:
             shs_w = get_shs(params["low_shs_w"], reconstruct_high_shs(params))
             W_tilde, self.mask = WeightGSFunctionTest.apply(pws, shs_w, alphas, scales, rots, cam, self.splat_dim)

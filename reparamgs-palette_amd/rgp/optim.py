@@ -16,7 +16,9 @@ element by element:
     p += -(lr / (1 - beta1^t)) * (m / (sqrt(v) / sqrt(1 - beta2^t) + eps))
 
 Parameters listed in `fused_sh` must be the tensor passed as `shs` to
-WeightGSFunctionTest (a float32, contiguous (N, sh_dim*K) CUDA tensor).  Every
+WeightGSFunctionTest (a float32, contiguous (N, sh_dim*K) CUDA tensor), or a
+pair (low, high) passed to WeightGSFunctionSplit ((N,K) and (N,15K), updated
+together by rgp_sh_backward_adam_split with the same arithmetic).  Every
 other parameter, and a fused one whose gradient also arrives the usual way
 (several backward passes before one step), is updated by the same formula:
 in one kernel pass (rgp_adam) for contiguous float32 GPU tensors, with torch
@@ -35,13 +37,21 @@ class PaletteAdam(torch.optim.Optimizer):
         defaults = dict(lr=lr, betas=betas, eps=eps, weight_decay=weight_decay)
         super().__init__(params, defaults)
         self._pending = {}
+        self._pair_of = {}  # id(high) -> low, for the pairs of split leaves
         known = {id(p) for g in self.param_groups for p in g["params"]}
-        for p in fused_sh:
-            if id(p) not in known:
-                raise ValueError("PaletteAdam: a fused_sh tensor is not among the optimised parameters")
-            if not (p.is_cuda and p.dtype == torch.float32 and p.dim() == 2 and p.is_contiguous()):
-                raise ValueError("PaletteAdam: fused_sh tensors must be contiguous float32 (N, sh_dim*K) on the GPU")
-            p._rgp_fused_adam = self
+        for entry in fused_sh:
+            pair = tuple(entry) if isinstance(entry, (tuple, list)) else (entry,)
+            for p in pair:
+                if id(p) not in known:
+                    raise ValueError("PaletteAdam: a fused_sh tensor is not among the optimised parameters")
+                if not (p.is_cuda and p.dtype == torch.float32 and p.dim() == 2 and p.is_contiguous()):
+                    raise ValueError("PaletteAdam: fused_sh tensors must be contiguous 2-D float32 on the GPU")
+            if len(pair) == 2:
+                low, high = pair
+                if high.shape[0] != low.shape[0] or high.shape[1] != 15 * low.shape[1]:
+                    raise ValueError("PaletteAdam: a split pair must be (N,K) and (N,15K)")
+                self._pair_of[id(high)] = low
+            pair[0]._rgp_fused_adam = self
 
     # -- called by WeightGSFunctionTest.backward ---------------------------
     def take(self, p, dw, camd, pws, K, sh_dim):
@@ -50,6 +60,14 @@ class PaletteAdam(torch.optim.Optimizer):
         if id(p) in self._pending:
             return False
         self._pending[id(p)] = (dw, camd, pws, int(K), int(sh_dim))
+        return True
+
+    def take_split(self, low, high, dw, camd, pws, K, sh_dim):
+        """The same for a pair of split leaves (updated together at low's turn)."""
+        if id(low) in self._pending or sh_dim != 16 or self._pair_of.get(id(high)) is not low:
+            return False
+        self._pending[id(low)] = ("split", high, dw, camd, pws, int(K))
+        self._pending[id(high)] = ("split_high",)
         return True
 
     def zero_grad(self, set_to_none=True):
@@ -63,23 +81,57 @@ class PaletteAdam(torch.optim.Optimizer):
         call("rgp_sh_backward_full", p.shape[0], K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(g), stream(p.device))
         return g
 
+    def _state(self, p):
+        st = self.state[p]
+        if not st:
+            st["step"] = 0
+            st["exp_avg"] = torch.zeros_like(p, memory_format=torch.preserve_format)
+            st["exp_avg_sq"] = torch.zeros_like(p, memory_format=torch.preserve_format)
+        return st
+
     @torch.no_grad()
     def step(self, closure=None):
         loss = None
         if closure is not None:
             with torch.enable_grad():
                 loss = closure()
+        extra = {}  # id -> gradient of a split leaf whose pair also holds an ordinary gradient
+        for group in self.param_groups:
+            for p in group["params"]:
+                pend = self._pending.get(id(p))
+                if pend is None or pend[0] != "split":
+                    continue
+                _, high, dw, camd, pws, K = pend
+                if p.grad is None and high.grad is None:
+                    continue
+                full = self._materialise(torch.empty((p.shape[0], 16 * K), dtype=p.dtype, device=p.device),
+                                         (dw, camd, pws, K, 16))
+                extra[id(p)], extra[id(high)] = full[:, :K].contiguous(), full[:, K:].contiguous()
+                self._pending.pop(id(p))
+                self._pending.pop(id(high), None)
         for group in self.param_groups:
             lr, (b1, b2), eps, wd = group["lr"], group["betas"], group["eps"], group["weight_decay"]
             for p in group["params"]:
                 pend = self._pending.pop(id(p), None)
+                if pend is not None and pend[0] == "split_high":
+                    continue  # updated with its low leaf
+                if pend is not None and pend[0] == "split":
+                    _, high, dw, camd, pws, K = pend
+                    sl, sh = self._state(p), self._state(high)
+                    sl["step"] += 1
+                    sh["step"] += 1
+                    with SPANS.span("sh_backward_adam"):
+                        call("rgp_sh_backward_adam_split", p.shape[0], K, ptr(camd), ptr(pws), ptr(dw), ptr(p),
+                             ptr(sl["exp_avg"]), ptr(sl["exp_avg_sq"]), ptr(high), ptr(sh["exp_avg"]),
+                             ptr(sh["exp_avg_sq"]), I64(sl["step"]), float(lr), float(b1), float(b2), float(eps),
+                             float(wd), stream(p.device))
+                    continue
+                if id(p) in extra:
+                    g = extra.pop(id(p))
+                    p.grad = g if p.grad is None else p.grad + g
                 if pend is None and p.grad is None:
                     continue
-                st = self.state[p]
-                if not st:
-                    st["step"] = 0
-                    st["exp_avg"] = torch.zeros_like(p, memory_format=torch.preserve_format)
-                    st["exp_avg_sq"] = torch.zeros_like(p, memory_format=torch.preserve_format)
+                st = self._state(p)
                 st["step"] += 1
                 t = st["step"]
                 m, v = st["exp_avg"], st["exp_avg_sq"]

@@ -425,6 +425,19 @@ struct RecStride {
   static constexpr int f4 = 2 + (DIM + 3) / 4;
 };
 
+// Non-temporal 16-byte loads and stores for streams touched once (optimizer
+// state): the stores go out to HBM during the kernel instead of leaving
+// dirty lines in the caches for the next kernel to write back.
+typedef float rgp_f4n __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_nt(const float4* a) {
+  const rgp_f4n x = __builtin_nontemporal_load(reinterpret_cast<const rgp_f4n*>(a));
+  return make_float4(x.x, x.y, x.z, x.w);
+}
+__device__ __forceinline__ void st_nt(float4* a, float4 v) {
+  const rgp_f4n x = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(x, reinterpret_cast<rgp_f4n*>(a));
+}
+
 // Blocks of one launch go round-robin to the 8 XCDs; give each XCD a
 // contiguous run of tiles so neighbouring tiles share its L2.
 __device__ __forceinline__ int xcd_tile(int b, int nblocks) {

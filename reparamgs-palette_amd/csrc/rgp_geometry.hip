@@ -1168,9 +1168,9 @@ __device__ __forceinline__ void sh_adam_span(int n, const rgp_camera* __restrict
       float4 pp[PER], mm[PER], vv[PER];
 #pragma unroll
       for (int j = 0; j < PER; ++j) {
-        pp[j] = p4[t + 256 * j];
-        mm[j] = m4[t + 256 * j];
-        vv[j] = v4[t + 256 * j];
+        pp[j] = ld_nt(&p4[t + 256 * j]);
+        mm[j] = ld_nt(&m4[t + 256 * j]);
+        vv[j] = ld_nt(&v4[t + 256 * j]);
       }
       asm volatile("" ::: "memory");  // keep the loads ahead of the updates
 #pragma unroll
@@ -1180,9 +1180,9 @@ __device__ __forceinline__ void sh_adam_span(int n, const rgp_camera* __restrict
         adam_elem(pp[j].y, mm[j].y, vv[j].y, grad(4 * q + 1), a);
         adam_elem(pp[j].z, mm[j].z, vv[j].z, grad(4 * q + 2), a);
         adam_elem(pp[j].w, mm[j].w, vv[j].w, grad(4 * q + 3), a);
-        p4[q] = pp[j];
-        m4[q] = mm[j];
-        v4[q] = vv[j];
+        st_nt(&p4[q], pp[j]);
+        st_nt(&m4[q], mm[j]);
+        st_nt(&v4[q], vv[j]);
       }
       return;
     }
@@ -1559,15 +1559,15 @@ __global__ __launch_bounds__(256) void k_adam4(int64_t n4, float4* __restrict__ 
                                                float4* __restrict__ m, float4* __restrict__ v, AdamArgs a) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-    float4 pp = p[i], mm = m[i], vv = v[i];
-    const float4 gg = g[i];
+    float4 pp = ld_nt(&p[i]), mm = ld_nt(&m[i]), vv = ld_nt(&v[i]);
+    const float4 gg = ld_nt(&g[i]);
     adam_elem(pp.x, mm.x, vv.x, gg.x, a);
     adam_elem(pp.y, mm.y, vv.y, gg.y, a);
     adam_elem(pp.z, mm.z, vv.z, gg.z, a);
     adam_elem(pp.w, mm.w, vv.w, gg.w, a);
-    p[i] = pp;
-    m[i] = mm;
-    v[i] = vv;
+    st_nt(&p[i], pp);
+    st_nt(&m[i], mm);
+    st_nt(&v[i], vv);
   }
 }
 

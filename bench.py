@@ -280,7 +280,7 @@ def torch_bary_loss(Wt, P, C, tau=1e-9, eps=1e-12, lam_pos=0.5):
     return F.mse_loss(Wt, Wb) + lam_pos * torch.mean(F.relu(-Wt))
 
 
-def measure_bary(args, dev, n=1_000_000, views=8, steps=24, warmup=3):
+def measure_bary(args, dev, n=1_000_000, views=8, steps=96, warmup=4):
     """The reference's own finetune step (finetune.py:346-399): palette-weight
     render, barycentric target of the frozen render, L_geo + L_pos +
     compactness + hue separation, backward, Adam on the SH weights and the

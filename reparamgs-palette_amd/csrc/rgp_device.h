@@ -298,6 +298,15 @@ __device__ __forceinline__ void sh_basis(float px, float py, float pz, const flo
   }
 }
 
+// One Gaussian's coefficient row held as two leaves: the K band-0 values
+// (registers) and the (sh_dim-1)*K higher-band values (LDS or global).
+template <int K>
+struct SplitRow {
+  const float* lo;
+  const float* hi;
+  __device__ __forceinline__ float operator[](int j) const { return j < K ? lo[j] : hi[j - K]; }
+};
+
 // Full-rank weights (kernel.cu:712-767).  sh: band-major coefficients of one
 // Gaussian, sh[lm * K + k], read through an accessor (global memory or a
 // staged LDS row); the expression tree is the same for both.

@@ -580,9 +580,24 @@ __device__ __forceinline__ void stage_high_rows(float* s_hi, int n, const float*
   const int t = threadIdx.x;
   const int tot = min(64, n - (int)blockIdx.x * 64) * ROW;
   const float* src = high + (size_t)blockIdx.x * 64 * ROW;
-  const int t4 = tot >> 2;
-  for (int e = t; e < t4; e += 64) reinterpret_cast<float4*>(s_hi)[e] = reinterpret_cast<const float4*>(src)[e];
-  for (int e = 4 * t4 + t; e < tot; e += 64) s_hi[e] = src[e];
+  if (tot == 64 * ROW) {
+    // a full block: 16*ROW float4, every load issued before the first store
+    // (a load-store loop waits out one HBM latency per iteration)
+    constexpr int F4 = 16 * ROW, PER = (F4 + 63) / 64;
+    const float4* src4 = reinterpret_cast<const float4*>(src);
+    const bool tail = t + 64 * (PER - 1) < F4;
+    float4 v[PER - 1], last = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int m = 0; m < PER - 1; ++m) v[m] = src4[t + 64 * m];
+    if (tail) last = src4[t + 64 * (PER - 1)];
+#pragma unroll
+    for (int m = 0; m < PER - 1; ++m) reinterpret_cast<float4*>(s_hi)[t + 64 * m] = v[m];
+    if (tail) reinterpret_cast<float4*>(s_hi)[t + 64 * (PER - 1)] = last;
+  } else {
+    const int t4 = tot >> 2;
+    for (int e = t; e < t4; e += 64) reinterpret_cast<float4*>(s_hi)[e] = reinterpret_cast<const float4*>(src)[e];
+    for (int e = 4 * t4 + t; e < tot; e += 64) s_hi[e] = src[e];
+  }
   __syncthreads();
 }
 
@@ -624,19 +639,15 @@ __device__ __forceinline__ uint32_t preprocess_gaussian(
   } else if (WMODE == 5 || WMODE == 6) {
     // the reference's two leaves, low (N,K) in dc and the higher bands
     // (N,(sh_dim-1)K) in shs: the row they concatenate to (gsplat/utils.py:157)
+    // Both leaves are read where sh_weights_full uses them, at constant
+    // offsets (a local copy of the 16K-float row would live in scratch).
     if (sh_dim > 1) sh_basis(px, py, pz, cam->twc, sh_dim, Y);
-    float row[16 * K];
+    float lo[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) row[k] = dc[(size_t)i * K + k];
-    if (WMODE == 5) {
-      const float* s_hi = reinterpret_cast<const float*>(s_sh);
-#pragma unroll
-      for (int e = 0; e < 15 * K; ++e) row[K + e] = s_hi[t * 15 * K + e];
-    } else {
-      const int hr = (sh_dim - 1) * K;
-      for (int e = 0; e < hr; ++e) row[K + e] = shs[(size_t)i * hr + e];
-    }
-    sh_weights_full<K>(row, Y, sh_dim, w);
+    for (int k = 0; k < K; ++k) lo[k] = dc[(size_t)i * K + k];
+    const float* hi = WMODE == 5 ? reinterpret_cast<const float*>(s_sh) + t * 15 * K
+                                 : shs + (size_t)i * (sh_dim - 1) * K;
+    sh_weights_full<K>(SplitRow<K>{lo, hi}, Y, sh_dim, w);
   } else if (WMODE == 1) {
     const int L = P * Q / K;
     if (L > 0) sh_basis(px, py, pz, cam->twc, L + 1, Y);
@@ -1160,21 +1171,33 @@ __device__ __forceinline__ void sh_adam_span(int n, const rgp_camera* __restrict
   float4* p4 = (float4*)(p + base);
   float4* m4 = (float4*)(m + base);
   float4* v4 = (float4*)(v + base);
-  if constexpr ((G * ROW / 4) % 256 == 0) {
-    // a full block is PER float4 of each array per thread: all 3 PER loads
-    // are issued before the first update
-    constexpr int PER = G * ROW / 4 / 256;
+  {
+    // a full block is F4 float4 of each array, PER per thread (the last one
+    // on the first F4 - 256 (PER-1) threads): all loads are issued before
+    // the first update.  Non-temporal: p, m and v are touched once per step,
+    // and stores left dirty in the caches would be written back during the
+    // next kernel.
+    constexpr int F4 = G * ROW / 4, PER = (F4 + 255) / 256;
     if (cnt == G) {
+      const bool tail = t + 256 * (PER - 1) < F4;
+      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
       float4 pp[PER], mm[PER], vv[PER];
 #pragma unroll
-      for (int j = 0; j < PER; ++j) {
+      for (int j = 0; j < PER - 1; ++j) {
         pp[j] = ld_nt(&p4[t + 256 * j]);
         mm[j] = ld_nt(&m4[t + 256 * j]);
         vv[j] = ld_nt(&v4[t + 256 * j]);
       }
+      pp[PER - 1] = mm[PER - 1] = vv[PER - 1] = z;
+      if (tail) {
+        pp[PER - 1] = ld_nt(&p4[t + 256 * (PER - 1)]);
+        mm[PER - 1] = ld_nt(&m4[t + 256 * (PER - 1)]);
+        vv[PER - 1] = ld_nt(&v4[t + 256 * (PER - 1)]);
+      }
       asm volatile("" ::: "memory");  // keep the loads ahead of the updates
 #pragma unroll
       for (int j = 0; j < PER; ++j) {
+        if (j == PER - 1 && !tail) break;
         const int q = t + 256 * j;
         adam_elem(pp[j].x, mm[j].x, vv[j].x, grad(4 * q), a);
         adam_elem(pp[j].y, mm[j].y, vv[j].y, grad(4 * q + 1), a);

@@ -130,6 +130,7 @@ class WeightGSFunctionTestViews(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pws, shs, alphas, scales, rots, cams, num_palette):
         K = int(num_palette)
+        ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights_views(pws, alphas, scales, rots, cams, K, shs=shs, return_state=True)
         ctx.meta = (int(cams[0].width), int(cams[0].height), K, pws.shape[0], st["P"], len(cams))
         ctx.save_for_backward(f32c(pws), st["records"], st["ranges"], st["ids"], st["camsd"])
@@ -137,6 +138,8 @@ class WeightGSFunctionTestViews(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, dloss_dlayer, _):
+        if dloss_dlayer is None:
+            return (None,) * 7
         W, H, K, n, sh_dim, V = ctx.meta
         pws, records, ranges, ids, camsd = ctx.saved_tensors
         dw = ops.draw_backward_replay_views(V, W, H, K, ranges, ids, records, dloss_dlayer.contiguous(), n)
@@ -155,6 +158,7 @@ class WeightGSFunctionFastViews(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pws, shs_dc, shs_A, shs_B, alphas, scales, rots, cams, num_palette, Q):
         K = int(num_palette)
+        ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights_views(pws, alphas, scales, rots, cams, K, dc=shs_dc, A=shs_A, B=shs_B,
                                                  Q=Q, return_state=True)
         ctx.meta = (int(cams[0].width), int(cams[0].height), K, pws.shape[0], st["P"], st["Q"], st["rank"],
@@ -165,6 +169,8 @@ class WeightGSFunctionFastViews(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, dloss_dlayer, _):
+        if dloss_dlayer is None:
+            return (None,) * 10
         W, H, K, n, P, Q, r, V = ctx.meta
         pws, A, B, records, ranges, ids, camsd = ctx.saved_tensors
         dw = ops.draw_backward_replay_views(V, W, H, K, ranges, ids, records, dloss_dlayer.contiguous(), n)
@@ -188,6 +194,7 @@ class WeightGSFunctionSplit(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pws, low, high, alphas, scales, rots, cam, num_palette):
         K = int(num_palette)
+        ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, low=low, high=high, return_state=True)
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"])
         fused = getattr(low, "_rgp_fused_adam", None)
@@ -198,6 +205,9 @@ class WeightGSFunctionSplit(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, dloss_dlayer, _):
+        if dloss_dlayer is None:
+            ctx.fused = None
+            return (None,) * 8
         W, H, K, n, sh_dim = ctx.meta
         pws, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
         dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())
@@ -217,6 +227,7 @@ class WeightGSFunctionTest(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pws, shs, alphas, scales, rots, cam, num_palette):
         K = int(num_palette)
+        ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, shs=shs, return_state=True)
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"])
         # an optimizer that fuses the SH update (rgp.optim.PaletteAdam) takes dL/dw instead of dL/dshs
@@ -228,6 +239,9 @@ class WeightGSFunctionTest(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, dloss_dlayer, _):
+        if dloss_dlayer is None:
+            ctx.fused = None
+            return (None,) * 7
         W, H, K, n, sh_dim = ctx.meta
         pws, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
         dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())
@@ -248,6 +262,7 @@ class WeightGSFunctionFast(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pws, shs_dc, shs_A, shs_B, alphas, scales, rots, cam, num_palette, Q):
         K = int(num_palette)
+        ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, dc=shs_dc, A=shs_A, B=shs_B, Q=Q,
                                            return_state=True)
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"], st["Q"], st["rank"])
@@ -257,6 +272,8 @@ class WeightGSFunctionFast(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, dloss_dlayer, _):
+        if dloss_dlayer is None:
+            return (None,) * 10
         W, H, K, n, P, Q, r = ctx.meta
         pws, A, B, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
         dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())

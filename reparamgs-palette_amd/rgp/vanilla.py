@@ -22,6 +22,7 @@ from ._lib import call, f32c, ptr, stream
 class GSFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pws, shs, alphas, scales, rots, us, cam):
+        ctx.set_materialize_grads(False)  # an unused final_tau or mask gets no zero-filled gradient
         us, pcs, depths, du_dpcs = gsc.project(pws, cam.Rcw, cam.tcw, cam.fx, cam.fy, cam.cx, cam.cy, True)
         cov3ds, dcov3d_drots, dcov3d_dscales = gsc.computeCov3D(rots, scales, depths, True)
         cov2ds, dcov2d_dcov3ds, dcov2d_dpcs = gsc.computeCov2D(cov3ds, pcs, cam.Rcw, depths, cam.fx, cam.fy,
@@ -41,6 +42,10 @@ class GSFunction(torch.autograd.Function):
         cam = ctx.cam
         (us, cinv2ds, alphas, depths, colors, contrib, final_tau, ranges, ids, dcinv2d_dcov2ds, dcov2d_dcov3ds,
          dcov3d_drots, dcov3d_dscales, dcolor_dshs, du_dpcs, dcov2d_dpcs, dcolor_dpws) = ctx.saved_tensors
+        if dloss_dgammas is None and dloss_dfinal_tau is None:
+            return (None,) * 7
+        if dloss_dgammas is None:
+            dloss_dgammas = torch.zeros((3, cam.height, cam.width), dtype=torch.float32, device=us.device)
         dtau = None if dloss_dfinal_tau is None else dloss_dfinal_tau.contiguous()
         dus, dcinv, dalphas, dcolors = gsc.splatB(cam.height, cam.width, us, cinv2ds, alphas, depths, colors, contrib,
                                                   final_tau, ranges, ids, dloss_dgammas.contiguous(), dtau)

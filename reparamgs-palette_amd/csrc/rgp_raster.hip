@@ -662,7 +662,11 @@ __global__ __launch_bounds__(256) void k_mix(MixArgs a) {
   }
 }
 
-template <int K>
+// dL/dW_k = P_k . dL/drgb per pixel, and dL/dP_kc = sum_p W_k dL/drgb_c.
+// VEC: hw is a multiple of 4 and the planes are 16-byte aligned; each thread
+// then takes two float4 groups of pixels (all 2 (K+3) loads issued before the
+// first use).  Else one pixel per thread and step of a grid-stride loop.
+template <int K, bool VEC>
 __global__ __launch_bounds__(256) void k_mix_bwd(int64_t hw, const float* __restrict__ Wp,
                                                  const float* __restrict__ pal,
                                                  const float* __restrict__ drgb,
@@ -679,16 +683,52 @@ __global__ __launch_bounds__(256) void k_mix_bwd(int64_t hw, const float* __rest
   for (int k = 0; k < K; ++k)
 #pragma unroll
     for (int c = 0; c < 3; ++c) gp[k][c] = 0.f;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < hw; p += stride) {
-    const float d0 = drgb[p], d1 = drgb[hw + p], d2 = drgb[2 * hw + p];
+  if constexpr (VEC) {
+    constexpr int G = 2;
+    const int64_t n4 = hw >> 2;
+    const int64_t q0 = (int64_t)blockIdx.x * (256 * G) + threadIdx.x;
+    const float4* d4 = reinterpret_cast<const float4*>(drgb);
+    const float4* w4 = reinterpret_cast<const float4*>(Wp);
+    float4* o4 = reinterpret_cast<float4*>(dW);
+    float4 d[G][3], w[G][K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const float wk = Wp[k * hw + p];
-      dW[k * hw + p] = (P[k][0] * d0 + P[k][1] * d1) + P[k][2] * d2;
-      gp[k][0] = gp[k][0] + wk * d0;
-      gp[k][1] = gp[k][1] + wk * d1;
-      gp[k][2] = gp[k][2] + wk * d2;
+    for (int i = 0; i < G; ++i) {
+      const int64_t q = q0 + 256 * i;
+      const bool ok = q < n4;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) d[i][c] = ok ? d4[c * n4 + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < K; ++k) w[i][k] = ok ? w4[k * n4 + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int64_t q = q0 + 256 * i;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        float4 r;
+        r.x = (P[k][0] * d[i][0].x + P[k][1] * d[i][1].x) + P[k][2] * d[i][2].x;
+        r.y = (P[k][0] * d[i][0].y + P[k][1] * d[i][1].y) + P[k][2] * d[i][2].y;
+        r.z = (P[k][0] * d[i][0].z + P[k][1] * d[i][1].z) + P[k][2] * d[i][2].z;
+        r.w = (P[k][0] * d[i][0].w + P[k][1] * d[i][1].w) + P[k][2] * d[i][2].w;
+        if (q < n4) o4[k * n4 + q] = r;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          gp[k][c] = gp[k][c] + ((w[i][k].x * d[i][c].x + w[i][k].y * d[i][c].y) +
+                                 (w[i][k].z * d[i][c].z + w[i][k].w * d[i][c].w));
+      }
+    }
+  } else {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < hw; p += stride) {
+      const float d0 = drgb[p], d1 = drgb[hw + p], d2 = drgb[2 * hw + p];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float wk = Wp[k * hw + p];
+        dW[k * hw + p] = (P[k][0] * d0 + P[k][1] * d1) + P[k][2] * d2;
+        gp[k][0] = gp[k][0] + wk * d0;
+        gp[k][1] = gp[k][1] + wk * d1;
+        gp[k][2] = gp[k][2] + wk * d2;
+      }
     }
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -847,9 +887,17 @@ extern "C" int rgp_mix_backward(int K, int64_t hw, const float* W, const float* 
   if (K < 1 || K > RGP_MAX_DIM || hw < 0)
     return set_error(RGP_ERR_ARG, "rgp_mix_backward: bad K or size");
   if (hw == 0) return RGP_OK;
-  int64_t blocks = (hw + 255) / 256;
-  if (blocks > 2048) blocks = 2048;
-  RGP_DIM_SWITCH(K, hipLaunchKernelGGL(k_mix_bwd<DD>, dim3((unsigned)blocks), dim3(256), 0, s, hw,
-                                       W, palette, drgb, dW, dpalette));
+  const bool vec = hw % 4 == 0 && (((uintptr_t)W | (uintptr_t)drgb | (uintptr_t)dW) & 15) == 0;
+  if (vec) {
+    const int64_t blocks = (hw / 4 + 511) / 512;
+    if (blocks > 0x7fffffff) return set_error(RGP_ERR_ARG, "rgp_mix_backward: image too large");
+    RGP_DIM_SWITCH(K, hipLaunchKernelGGL((k_mix_bwd<DD, true>), dim3((unsigned)blocks), dim3(256), 0, s, hw,
+                                         W, palette, drgb, dW, dpalette));
+  } else {
+    int64_t blocks = (hw + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    RGP_DIM_SWITCH(K, hipLaunchKernelGGL((k_mix_bwd<DD, false>), dim3((unsigned)blocks), dim3(256), 0, s, hw,
+                                         W, palette, drgb, dW, dpalette));
+  }
   return check_launch("rgp_mix_backward");
 }

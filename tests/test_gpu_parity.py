@@ -207,6 +207,39 @@ def test_mix_matches_bake(device, c0):
     assert diff.max() < 2e-4, diff.max()
 
 
+@pytest.mark.parametrize("shape,offset", [((5, 48, 64), 0), ((5, 47, 63), 0), ((3, 48, 64), 1),
+                                          ((5, 1080, 1920), 0)])
+def test_mix_backward(device, shape, offset):
+    """Mix.backward: dL/dW_k = (P_k0 d_0 + P_k1 d_1) + P_k2 d_2 bit for bit
+    against the same float32 operations in torch (the float4 path when
+    H*W % 4 == 0 and the planes are 16-byte aligned, else the scalar one:
+    odd sizes, and W read at a 4-byte offset), and dL/dP_kc = sum_p W_k d_c
+    within 1e-5 of sum_p |W_k d_c| against float64."""
+    print(f"case: mix backward {shape} offset {offset}", flush=True)
+    K, H, W = shape
+    rng = np.random.default_rng(11)
+    Wn = rng.random(shape, dtype=np.float32)
+    P = rng.random((K, 3), dtype=np.float32)
+    d = rng.normal(size=(3, H, W)).astype(np.float32)
+    if offset:
+        flat = torch.empty(Wn.size + offset, dtype=torch.float32, device=device)
+        Wt = flat[offset:].view(shape)
+        Wt.copy_(T(Wn, device))
+    else:
+        Wt = T(Wn, device)
+    Wt.requires_grad_(True)
+    Pt = T(P, device).requires_grad_(True)
+    rgb = ops.Mix.apply(Wt, Pt)
+    rgb.backward(T(d, device))
+    dt, Pc = T(d, device), T(P, device)
+    ref_dW = (Pc[:, 0, None, None] * dt[0] + Pc[:, 1, None, None] * dt[1]) + Pc[:, 2, None, None] * dt[2]
+    same(N(Wt.grad), N(ref_dW), "dW")
+    terms = Wn.reshape(K, 1, -1).astype(np.float64) * d.reshape(1, 3, -1).astype(np.float64)
+    ref_dP, ref_abs = terms.sum(-1), np.abs(terms).sum(-1)
+    err = np.abs(N(Pt.grad).astype(np.float64) - ref_dP)
+    assert (err <= 1e-5 * ref_abs).all(), float(np.max(err / ref_abs))
+
+
 def test_binning_many_tiles(device):
     """1080p with a dense scene: two tile bytes and two depth bytes in the
     sort, hundreds of sort blocks."""

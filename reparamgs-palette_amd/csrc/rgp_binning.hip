@@ -456,7 +456,9 @@ __global__ __launch_bounds__(kTbThreads) void k_tb_count(int n, int chunk, int t
                                                          const uint2* __restrict__ rects,
                                                          uint32_t* __restrict__ tbpart) {
   extern __shared__ uint32_t s_hist[];  // tiles
-  const int b = blockIdx.x;
+  // consecutive chunks on one XCD: their columns of tbpart, and their slots in
+  // a tile's list, are adjacent words that combine in that XCD's L2
+  const int b = xcd_tile(blockIdx.x, nb);
   for (int q = threadIdx.x; q < tiles; q += kTbThreads) s_hist[q] = 0;
   __syncthreads();
   const int g0 = b * chunk, g1 = min(n, g0 + chunk);
@@ -477,7 +479,7 @@ __global__ __launch_bounds__(kTbThreads) void k_tb_scatter(int n, int chunk, int
                                                            int32_t* __restrict__ ids) {
   if (hdr[kHdrMEff] == 0) return;
   extern __shared__ uint32_t s_cur[];  // tiles
-  const int b = blockIdx.x;
+  const int b = xcd_tile(blockIdx.x, nb);
   for (int q = threadIdx.x; q < tiles; q += kTbThreads) s_cur[q] = tbpart[(size_t)q * nb + b];
   __syncthreads();
   const int g0 = b * chunk, g1 = min(n, g0 + chunk);

@@ -44,6 +44,18 @@ __device__ __forceinline__ float block_sum_f(float v, float* lds) {
 // with a float64 reference within a tolerance, not bit for bit).
 constexpr int kSG = kST / 4;  // four-output groups per tile edge
 
+// The block's output tile (bx, by) of channel c.  The grid is one dimension,
+// tiles in row-major order per channel, and each XCD takes a contiguous band
+// of them: the 5-pixel halos a tile shares with its neighbours are then read
+// through the same L2 (blocks go round-robin to the 8 XCDs).
+__device__ __forceinline__ void ssim_block(int H, int W, int* bx, int* by, int* c) {
+  const int nbx = (W + kST - 1) / kST, nby = (H + kST - 1) / kST;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  *bx = t % nbx;
+  *by = (t / nbx) % nby;
+  *c = t / (nbx * nby);
+}
+
 // Blurs three maps of the staged tile: horizontally into hb (the products of
 // an input pair formed once per input, not once per tap), then vertically
 // into mu[m][o] for this thread's four outputs (column q, rows r0..r0+3).
@@ -110,11 +122,12 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __r
   __shared__ float sx[kSI][kSI + 1], sy[kSI][kSI + 1];
   __shared__ float hb[3][kSI][kST + 1];
   __shared__ float red[8];
-  const int c = blockIdx.z;
+  int bx, by, c;
+  ssim_block(H, W, &bx, &by, &c);
   const size_t HW = (size_t)H * W;
   const float* X = img + c * HW;
   const float* Y = tgt + c * HW;
-  const int x0 = blockIdx.x * kST - kSR, y0 = blockIdx.y * kST - kSR;
+  const int x0 = bx * kST - kSR, y0 = by * kST - kSR;
   for (int i = threadIdx.x; i < kSI * kSI; i += blockDim.x) {
     const int r = i / kSI, q = i - r * kSI;
     const int gy = y0 + r, gx = x0 + q;
@@ -151,7 +164,7 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __r
   const float bl = block_sum_f(l1_sum, red);
   if (threadIdx.x == 0) {
     // 64 accumulator pairs so that blocks do not queue on one address
-    const int slot = (blockIdx.x + blockIdx.y * gridDim.x + blockIdx.z * gridDim.x * gridDim.y) & 63;
+    const int slot = blockIdx.x & 63;
     unsafeAtomicAdd(&sums[2 * slot], (double)bs);
     unsafeAtomicAdd(&sums[2 * slot + 1], (double)bl);
   }
@@ -174,9 +187,10 @@ __global__ __launch_bounds__(256) void k_ssim_bwd(int H, int W, const float* __r
                                                   float* __restrict__ dimg) {
   __shared__ float sa[3][kSI][kSI + 1];
   __shared__ float hb[3][kSI][kST + 1];
-  const int c = blockIdx.z;
+  int bx, by, c;
+  ssim_block(H, W, &bx, &by, &c);
   const size_t HW = (size_t)H * W;
-  const int x0 = blockIdx.x * kST - kSR, y0 = blockIdx.y * kST - kSR;
+  const int x0 = bx * kST - kSR, y0 = by * kST - kSR;
   for (int i = threadIdx.x; i < kSI * kSI; i += blockDim.x) {
     const int r = i / kSI, q = i - r * kSI;
     const int gy = y0 + r, gx = x0 + q;
@@ -253,7 +267,9 @@ extern "C" int rgp_l1_ssim_forward(int channels, int height, int width, const fl
                                    hipStream_t s) {
   if (channels <= 0 || height <= 0 || width <= 0 || channels > 65535 || !window11)
     return set_error(RGP_ERR_ARG, "rgp_l1_ssim_forward: bad size or window");
-  const dim3 grid((width + kST - 1) / kST, (height + kST - 1) / kST, channels);
+  const int64_t blocks = (int64_t)((width + kST - 1) / kST) * ((height + kST - 1) / kST) * channels;
+  if (blocks > 0x7fffffff) return set_error(RGP_ERR_ARG, "rgp_l1_ssim_forward: image too large");
+  const dim3 grid((unsigned)blocks);
   hipLaunchKernelGGL(k_ssim_fwd, grid, dim3(256), 0, s, height, width, img, target,
                      make_win(window11), C1, C2, abc, sums);
   hipLaunchKernelGGL(k_ssim_finish, dim3(1), dim3(64), 0, s, (const double*)sums,
@@ -267,7 +283,9 @@ extern "C" int rgp_l1_ssim_backward(int channels, int height, int width, const f
                                     hipStream_t s) {
   if (channels <= 0 || height <= 0 || width <= 0 || channels > 65535 || !window11)
     return set_error(RGP_ERR_ARG, "rgp_l1_ssim_backward: bad size or window");
-  const dim3 grid((width + kST - 1) / kST, (height + kST - 1) / kST, channels);
+  const int64_t blocks = (int64_t)((width + kST - 1) / kST) * ((height + kST - 1) / kST) * channels;
+  if (blocks > 0x7fffffff) return set_error(RGP_ERR_ARG, "rgp_l1_ssim_backward: image too large");
+  const dim3 grid((unsigned)blocks);
   hipLaunchKernelGGL(k_ssim_bwd, grid, dim3(256), 0, s, height, width, img, target, abc,
                      make_win(window11), lam, (double)channels * height * width, grad_loss, dimg);
   return check_launch("rgp_l1_ssim_backward");

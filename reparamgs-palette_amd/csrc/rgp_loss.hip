@@ -37,12 +37,59 @@ __device__ __forceinline__ float block_sum_f(float v, float* lds) {
   return s;
 }
 
-// Horizontal pass items: a row r of the staged tile and a group of four
-// output columns; vertical pass: a group of four output rows and a column.
-// Each item loads the 14 inputs its four outputs share once, so products and
+// Horizontal pass items: a row r of the staged tile and a group of eight
+// output columns (42 x 4 items, one per thread of the first 168); vertical
+// pass: a group of four output rows and a column.  Each item loads the inputs
+// its outputs share once (18 for eight outputs, 14 for four), so products and
 // LDS reads are not repeated per tap; sums use fmaf (the loss is compared
 // with a float64 reference within a tolerance, not bit for bit).
-constexpr int kSG = kST / 4;  // four-output groups per tile edge
+constexpr int kSG = kST / 4;   // four-output groups per tile edge (vertical)
+constexpr int kHO = 8;         // outputs per horizontal item
+constexpr int kHG = kST / kHO;  // horizontal items per staged row
+constexpr int kSB = 256;       // threads per block
+static_assert(kSI * kHG <= kSB, "one horizontal item per thread");
+
+// Stages the (kSI x kSI) windows at (x0, y0) of NP planes, zero outside the
+// image, into d[p][r][q]: every load of a thread (7 per plane) is issued
+// before the first LDS store.  With kSB = 6 kSI + 4 the row and column of a
+// thread's next element follow from the current ones without a division.
+template <int NP>
+__device__ __forceinline__ void ssim_stage(const float* const (&P)[NP], int H, int W, int x0, int y0,
+                                           float (*const (&d)[NP])[kSI + 1]) {
+  static_assert(kSB == 6 * kSI + 4, "stepping below assumes 256 = 6 * 42 + 4");
+  constexpr int IT = (kSI * kSI + kSB - 1) / kSB;
+  const int t = threadIdx.x;
+  float v[NP][IT];
+  int r = t / kSI, q = t - r * kSI;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int gy = y0 + r, gx = x0 + q;
+    const bool in = t + it * kSB < kSI * kSI && gy >= 0 && gy < H && gx >= 0 && gx < W;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) v[p][it] = in ? P[p][gy * W + gx] : 0.f;
+    r += 6;
+    q += 4;
+    if (q >= kSI) {
+      q -= kSI;
+      ++r;
+    }
+  }
+  r = t / kSI;
+  q = t - r * kSI;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    if (t + it * kSB < kSI * kSI) {
+#pragma unroll
+      for (int p = 0; p < NP; ++p) d[p][r][q] = v[p][it];
+    }
+    r += 6;
+    q += 4;
+    if (q >= kSI) {
+      q -= kSI;
+      ++r;
+    }
+  }
+}
 
 // The block's output tile (bx, by) of channel c.  The grid is one dimension,
 // tiles in row-major order per channel, and each XCD takes a contiguous band
@@ -65,11 +112,12 @@ __device__ __forceinline__ void ssim_blur_phase(const float (*sx)[kSI + 1], cons
                                                 float (*hb)[kSI][kST + 1], const Win11& win, int q, int r0,
                                                 float (*mu)[4]) {
   constexpr int NM = MAPS == 0 ? 3 : 2;
-  for (int i = threadIdx.x; i < kSI * kSG; i += blockDim.x) {
-    const int r = i % kSI, c0 = 4 * (i / kSI);
-    float in[NM][14];
+  constexpr int NI = kHO + 10;
+  if (threadIdx.x < kSI * kHG) {
+    const int r = threadIdx.x % kSI, c0 = kHO * (threadIdx.x / kSI);
+    float in[NM][NI];
 #pragma unroll
-    for (int j = 0; j < 14; ++j) {
+    for (int j = 0; j < NI; ++j) {
       const float a = sx[r][c0 + j], b = sy[r][c0 + j];
       if (MAPS == 0) {
         in[0][j] = a;
@@ -81,7 +129,7 @@ __device__ __forceinline__ void ssim_blur_phase(const float (*sx)[kSI + 1], cons
       }
     }
 #pragma unroll
-    for (int o = 0; o < 4; ++o) {
+    for (int o = 0; o < kHO; ++o) {
       float acc[NM];
 #pragma unroll
       for (int m = 0; m < NM; ++m) acc[m] = 0.f;
@@ -128,12 +176,10 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __r
   const float* X = img + c * HW;
   const float* Y = tgt + c * HW;
   const int x0 = bx * kST - kSR, y0 = by * kST - kSR;
-  for (int i = threadIdx.x; i < kSI * kSI; i += blockDim.x) {
-    const int r = i / kSI, q = i - r * kSI;
-    const int gy = y0 + r, gx = x0 + q;
-    const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
-    sx[r][q] = in ? X[(size_t)gy * W + gx] : 0.f;
-    sy[r][q] = in ? Y[(size_t)gy * W + gx] : 0.f;
+  {
+    const float* const planes[2] = {X, Y};
+    float (*const dst[2])[kSI + 1] = {sx, sy};
+    ssim_stage<2>(planes, H, W, x0, y0, dst);
   }
   __syncthreads();
   const int q = threadIdx.x % kST, r0 = 4 * (threadIdx.x / kST);
@@ -191,25 +237,21 @@ __global__ __launch_bounds__(256) void k_ssim_bwd(int H, int W, const float* __r
   ssim_block(H, W, &bx, &by, &c);
   const size_t HW = (size_t)H * W;
   const int x0 = bx * kST - kSR, y0 = by * kST - kSR;
-  for (int i = threadIdx.x; i < kSI * kSI; i += blockDim.x) {
-    const int r = i / kSI, q = i - r * kSI;
-    const int gy = y0 + r, gx = x0 + q;
-    const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
-    const size_t o = c * HW + (size_t)gy * W + gx;
-    sa[0][r][q] = in ? abc[o] : 0.f;
-    sa[1][r][q] = in ? abc[3 * HW + o] : 0.f;
-    sa[2][r][q] = in ? abc[6 * HW + o] : 0.f;
+  {
+    const float* const planes[3] = {abc + c * HW, abc + 3 * HW + c * HW, abc + 6 * HW + c * HW};
+    float (*const dst[3])[kSI + 1] = {sa[0], sa[1], sa[2]};
+    ssim_stage<3>(planes, H, W, x0, y0, dst);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kSI * kSG; i += blockDim.x) {
-    const int r = i % kSI, q0 = 4 * (i / kSI);
-    float v[3][14];
+  if (threadIdx.x < kSI * kHG) {
+    const int r = threadIdx.x % kSI, q0 = kHO * (threadIdx.x / kSI);
+    float v[3][kHO + 10];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
-      for (int j = 0; j < 14; ++j) v[m][j] = sa[m][r][q0 + j];
+      for (int j = 0; j < kHO + 10; ++j) v[m][j] = sa[m][r][q0 + j];
 #pragma unroll
-    for (int o = 0; o < 4; ++o) {
+    for (int o = 0; o < kHO; ++o) {
       float acc[3] = {0.f, 0.f, 0.f};
 #pragma unroll
       for (int j = 0; j < 11; ++j) {
@@ -267,6 +309,7 @@ extern "C" int rgp_l1_ssim_forward(int channels, int height, int width, const fl
                                    hipStream_t s) {
   if (channels <= 0 || height <= 0 || width <= 0 || channels > 65535 || !window11)
     return set_error(RGP_ERR_ARG, "rgp_l1_ssim_forward: bad size or window");
+  if ((int64_t)height * width > 0x7fffffff) return set_error(RGP_ERR_ARG, "rgp_l1_ssim_forward: image too large");
   const int64_t blocks = (int64_t)((width + kST - 1) / kST) * ((height + kST - 1) / kST) * channels;
   if (blocks > 0x7fffffff) return set_error(RGP_ERR_ARG, "rgp_l1_ssim_forward: image too large");
   const dim3 grid((unsigned)blocks);
@@ -283,6 +326,7 @@ extern "C" int rgp_l1_ssim_backward(int channels, int height, int width, const f
                                     hipStream_t s) {
   if (channels <= 0 || height <= 0 || width <= 0 || channels > 65535 || !window11)
     return set_error(RGP_ERR_ARG, "rgp_l1_ssim_backward: bad size or window");
+  if ((int64_t)height * width > 0x7fffffff) return set_error(RGP_ERR_ARG, "rgp_l1_ssim_backward: image too large");
   const int64_t blocks = (int64_t)((width + kST - 1) / kST) * ((height + kST - 1) / kST) * channels;
   if (blocks > 0x7fffffff) return set_error(RGP_ERR_ARG, "rgp_l1_ssim_backward: image too large");
   const dim3 grid((unsigned)blocks);

@@ -307,6 +307,21 @@ int rgp_sh_backward_adam_split(int n, int K, const rgp_camera* cam, const float*
                                float* high_exp_avg_sq, int64_t step, double lr, double beta1, double beta2,
                                double eps, double weight_decay, hipStream_t stream);
 
+/* The low-rank SH backward fused with torch.optim.Adam's update of the three
+ * factor tensors, for WeightGSFunctionFast's register path (Q = 15, P = K,
+ * rank 1 or 2): replaces rgp_sh_backward_lowrank (gsmodel.py:194-198 and
+ * sh2Weight_fast_B, kernel.cu:991-1032) followed by torch.optim.Adam on
+ * shs_dc (N,K), shs_A (N,K,rank) and shs_B (N,rank,15) (the optimizer of
+ * finetune.py:303).  Each tensor has its own exp_avg / exp_avg_sq;
+ * the update is torch.optim.Adam's, bit-identical to rgp_sh_backward_lowrank
+ * then rgp_adam on each tensor.  All nine arrays 16-byte aligned; other
+ * shapes return RGP_ERR_ARG (use the unfused pair). */
+int rgp_sh_backward_adam_lowrank(int n, int K, int P, int Q, int rank, const rgp_camera* cam, const float* pws,
+                                 const float* dw, float* dc, float* dc_exp_avg, float* dc_exp_avg_sq, float* A,
+                                 float* A_exp_avg, float* A_exp_avg_sq, float* B, float* B_exp_avg,
+                                 float* B_exp_avg_sq, int64_t step, double lr, double beta1, double beta2,
+                                 double eps, double weight_decay, hipStream_t stream);
+
 /* No counterpart in the reference either: one torch.optim.Adam step (the
  * optimizer of finetune.py:346-399) over a flat float32 tensor of n elements
  * with its gradient, in one pass; the same element update and scalars as

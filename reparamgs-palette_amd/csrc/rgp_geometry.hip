@@ -314,6 +314,33 @@ __global__ __launch_bounds__(256) void k_lowrank_backward(
 // The same with Q = 15, P = K and rank R known at compile time: factors and
 // gradient accumulators in registers, one store per output.  Same operations
 // in the same order.
+// dL/d(dc, A, B) of one Gaussian for Q = 15, P = K: Y[1..15] its basis, a
+// (P x R) and b (R x Q) its factors, dw its K weight gradients.
+template <int K, int R>
+__device__ __forceinline__ void lowrank_q15_grad(const float* Y, const float* a, const float* b,
+                                                 const float* dw, float* ddc, float* da, float* db) {
+  constexpr int Q = 15, P = K;
+#pragma unroll
+  for (int j = 0; j < P * R; ++j) da[j] = 0.f;
+#pragma unroll
+  for (int j = 0; j < R * Q; ++j) db[j] = 0.f;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const float dwk = dw[k];
+    ddc[k] = dwk * (float)SH_C0_0;
+#pragma unroll
+    for (int lm = 0; lm < 15; ++lm) {
+      const int g = lm * K + k, row = g / Q, col = g % Q;
+      const float y = Y[lm + 1];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        da[row * R + r] = da[row * R + r] + dwk * b[r * Q + col] * y;
+        db[r * Q + col] = db[r * Q + col] + dwk * a[row * R + r] * y;
+      }
+    }
+  }
+}
+
 template <int K, int R>
 __global__ __launch_bounds__(256) void k_lowrank_backward_q15(
     int n, const float* __restrict__ A, const float* __restrict__ B,
@@ -331,25 +358,17 @@ __global__ __launch_bounds__(256) void k_lowrank_backward_q15(
     const float tw[3] = {twc[0], twc[1], twc[2]};
     sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], tw, 16, Y);
   }
-  float a[P * R], b[R * Q], da[P * R], db[R * Q];
+  float a[P * R], b[R * Q], dwr[K], gdc[K], da[P * R], db[R * Q];
 #pragma unroll
-  for (int j = 0; j < P * R; ++j) { a[j] = A[(size_t)i * P * R + j]; da[j] = 0.f; }
+  for (int j = 0; j < P * R; ++j) a[j] = A[(size_t)i * P * R + j];
 #pragma unroll
-  for (int j = 0; j < R * Q; ++j) { b[j] = B[(size_t)i * R * Q + j]; db[j] = 0.f; }
+  for (int j = 0; j < R * Q; ++j) b[j] = B[(size_t)i * R * Q + j];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const float dwk = dw[(size_t)i * K + k];
-    if (ddc) ddc[(size_t)i * K + k] = dwk * (float)SH_C0_0;
+  for (int k = 0; k < K; ++k) dwr[k] = dw[(size_t)i * K + k];
+  lowrank_q15_grad<K, R>(Y, a, b, dwr, gdc, da, db);
+  if (ddc) {
 #pragma unroll
-    for (int lm = 0; lm < 15; ++lm) {
-      const int g = lm * K + k, row = g / Q, col = g % Q;
-      const float y = Y[lm + 1];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        da[row * R + r] = da[row * R + r] + dwk * b[r * Q + col] * y;
-        db[r * Q + col] = db[r * Q + col] + dwk * a[row * R + r] * y;
-      }
-    }
+    for (int k = 0; k < K; ++k) ddc[(size_t)i * K + k] = gdc[k];
   }
 #pragma unroll
   for (int j = 0; j < P * R; ++j) dA[(size_t)i * P * R + j] = da[j];
@@ -1243,6 +1262,102 @@ __global__ __launch_bounds__(256) void k_sh_adam_bands(int n, const rgp_camera* 
   sh_adam_span<K, LM0, NB>(n, cam, pws, dw, p, m, v, a);
 }
 
+// Adam over one factor tensor's span of a 64-Gaussian block (rows of ROW
+// floats) with the gradients the block formed in LDS (row stride GS, this
+// tensor's at offset GOFF): a full block as PER float4 per thread, every
+// load issued first, non-temporal as in sh_adam_span; a partial last block
+// element by element.
+template <int ROW, int GOFF, int GS>
+__device__ __forceinline__ void lowrank_adam_span(float* __restrict__ p, float* __restrict__ m,
+                                                  float* __restrict__ v, int cnt, const float* sG,
+                                                  const AdamArgs& a) {
+  constexpr int F4 = 64 * ROW / 4, PER = (F4 + 63) / 64;
+  const int t = threadIdx.x;
+  auto grad = [&](int e) {
+    const int gl = e / ROW;
+    return sG[gl * GS + GOFF + (e - gl * ROW)];
+  };
+  float4* p4 = reinterpret_cast<float4*>(p);
+  float4* m4 = reinterpret_cast<float4*>(m);
+  float4* v4 = reinterpret_cast<float4*>(v);
+  if (cnt == 64) {
+    const bool tail = t + 64 * (PER - 1) < F4;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 pp[PER], mm[PER], vv[PER];
+#pragma unroll
+    for (int j = 0; j < PER - 1; ++j) {
+      pp[j] = ld_nt(&p4[t + 64 * j]);
+      mm[j] = ld_nt(&m4[t + 64 * j]);
+      vv[j] = ld_nt(&v4[t + 64 * j]);
+    }
+    pp[PER - 1] = mm[PER - 1] = vv[PER - 1] = z;
+    if (tail) {
+      pp[PER - 1] = ld_nt(&p4[t + 64 * (PER - 1)]);
+      mm[PER - 1] = ld_nt(&m4[t + 64 * (PER - 1)]);
+      vv[PER - 1] = ld_nt(&v4[t + 64 * (PER - 1)]);
+    }
+    asm volatile("" ::: "memory");  // keep the loads ahead of the updates
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if (j == PER - 1 && !tail) break;
+      const int q = t + 64 * j;
+      adam_elem(pp[j].x, mm[j].x, vv[j].x, grad(4 * q), a);
+      adam_elem(pp[j].y, mm[j].y, vv[j].y, grad(4 * q + 1), a);
+      adam_elem(pp[j].z, mm[j].z, vv[j].z, grad(4 * q + 2), a);
+      adam_elem(pp[j].w, mm[j].w, vv[j].w, grad(4 * q + 3), a);
+      st_nt(&p4[q], pp[j]);
+      st_nt(&m4[q], mm[j]);
+      st_nt(&v4[q], vv[j]);
+    }
+    return;
+  }
+  for (int e = t; e < cnt * ROW; e += 64) adam_elem(p[e], m[e], v[e], grad(e), a);
+}
+
+// The low-rank SH backward (gsmodel.py:194-198, sh2Weight_fast_B) fused with
+// torch.optim.Adam's update of dc (N,K), A (N,K,R) and B (N,R,15): a block
+// of 64 threads forms the gradients of its 64 Gaussians (one per thread,
+// lowrank_q15_grad, from the factors before the update) into LDS, then
+// updates the block's rows of the three tensors.  The gradients never reach
+// HBM.  Blocks own disjoint rows, so reading A and B before updating them
+// needs only the block barrier.
+template <int K, int R>
+__global__ __launch_bounds__(64) void k_lowrank_adam_q15(
+    int n, const rgp_camera* __restrict__ cam, const float* __restrict__ pws, const float* __restrict__ dw,
+    float* __restrict__ dc, float* __restrict__ mdc, float* __restrict__ vdc, float* __restrict__ A,
+    float* __restrict__ mA, float* __restrict__ vA, float* __restrict__ B, float* __restrict__ mB,
+    float* __restrict__ vB, AdamArgs a) {
+  constexpr int Q = 15, P = K, GS = K + P * R + R * Q + 1;
+  __shared__ float sG[64 * GS];
+  const int g0 = blockIdx.x * 64, t = threadIdx.x, i = g0 + t;
+  if (i < n) {
+    float Y[16];
+    sh_basis(pws[3 * i], pws[3 * i + 1], pws[3 * i + 2], cam->twc, 16, Y);
+    float ar[P * R], br[R * Q], dwr[K], gdc[K], da[P * R], db[R * Q];
+#pragma unroll
+    for (int j = 0; j < P * R; ++j) ar[j] = A[(size_t)i * P * R + j];
+#pragma unroll
+    for (int j = 0; j < R * Q; ++j) br[j] = B[(size_t)i * R * Q + j];
+#pragma unroll
+    for (int k = 0; k < K; ++k) dwr[k] = dw[(size_t)i * K + k];
+    lowrank_q15_grad<K, R>(Y, ar, br, dwr, gdc, da, db);
+    float* row = sG + t * GS;
+#pragma unroll
+    for (int k = 0; k < K; ++k) row[k] = gdc[k];
+#pragma unroll
+    for (int j = 0; j < P * R; ++j) row[K + j] = da[j];
+#pragma unroll
+    for (int j = 0; j < R * Q; ++j) row[K + P * R + j] = db[j];
+  }
+  __syncthreads();
+  const int cnt = min(64, n - g0);
+  lowrank_adam_span<K, 0, GS>(dc + (size_t)g0 * K, mdc + (size_t)g0 * K, vdc + (size_t)g0 * K, cnt, sG, a);
+  lowrank_adam_span<P * R, K, GS>(A + (size_t)g0 * P * R, mA + (size_t)g0 * P * R, vA + (size_t)g0 * P * R,
+                                  cnt, sG, a);
+  lowrank_adam_span<R * Q, K + P * R, GS>(B + (size_t)g0 * R * Q, mB + (size_t)g0 * R * Q,
+                                          vB + (size_t)g0 * R * Q, cnt, sG, a);
+}
+
 }  // namespace rgp
 
 using namespace rgp;
@@ -1743,6 +1858,39 @@ extern "C" int rgp_sh_backward_adam_split(int n, int K, const rgp_camera* cam, c
   RGP_K_SWITCH(K, hipLaunchKernelGGL((k_sh_adam_bands<KK, 1, 15>), dim3(blocks), dim3(256), 0, s, n, cam, pws, dw,
                                      high, high_exp_avg, high_exp_avg_sq, a));
   return check_launch("rgp_sh_backward_adam_split");
+}
+
+extern "C" int rgp_sh_backward_adam_lowrank(int n, int K, int P, int Q, int rank, const rgp_camera* cam,
+                                            const float* pws, const float* dw, float* dc, float* dc_exp_avg,
+                                            float* dc_exp_avg_sq, float* A, float* A_exp_avg, float* A_exp_avg_sq,
+                                            float* B, float* B_exp_avg, float* B_exp_avg_sq, int64_t step,
+                                            double lr, double beta1, double beta2, double eps, double weight_decay,
+                                            hipStream_t s) {
+  if (n < 0) return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_lowrank: n < 0");
+  int rc = check_lowrank("rgp_sh_backward_adam_lowrank", K, P, Q, rank);
+  if (rc) return rc;
+  if (!(Q == 15 && P == K && (rank == 1 || rank == 2)))
+    return set_error(RGP_ERR_ARG,
+                     "rgp_sh_backward_adam_lowrank: needs Q = 15, P = K and rank 1 or 2 (else "
+                     "rgp_sh_backward_lowrank + rgp_adam)");
+  if (step < 1) return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_lowrank: step must be >= 1");
+  if ((((uintptr_t)dc) | ((uintptr_t)dc_exp_avg) | ((uintptr_t)dc_exp_avg_sq) | ((uintptr_t)A) |
+       ((uintptr_t)A_exp_avg) | ((uintptr_t)A_exp_avg_sq) | ((uintptr_t)B) | ((uintptr_t)B_exp_avg) |
+       ((uintptr_t)B_exp_avg_sq)) & 15)
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_lowrank: tensors and moments must be 16-byte aligned");
+  if (n == 0) return RGP_OK;
+  const AdamArgs a = adam_args(step, lr, beta1, beta2, eps, weight_decay);
+  const int blocks = (n + 63) / 64;
+  if (rank == 1) {
+    RGP_K_SWITCH(K, hipLaunchKernelGGL((k_lowrank_adam_q15<KK, 1>), dim3(blocks), dim3(64), 0, s, n, cam, pws, dw,
+                                       dc, dc_exp_avg, dc_exp_avg_sq, A, A_exp_avg, A_exp_avg_sq, B, B_exp_avg,
+                                       B_exp_avg_sq, a));
+  } else {
+    RGP_K_SWITCH(K, hipLaunchKernelGGL((k_lowrank_adam_q15<KK, 2>), dim3(blocks), dim3(64), 0, s, n, cam, pws, dw,
+                                       dc, dc_exp_avg, dc_exp_avg_sq, A, A_exp_avg, A_exp_avg_sq, B, B_exp_avg,
+                                       B_exp_avg_sq, a));
+  }
+  return check_launch("rgp_sh_backward_adam_lowrank");
 }
 
 extern "C" int rgp_sh_backward_lowrank(int n, int K, int P, int Q, int rank,

@@ -266,6 +266,10 @@ class WeightGSFunctionFast(torch.autograd.Function):
         layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, dc=shs_dc, A=shs_A, B=shs_B, Q=Q,
                                            return_state=True)
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"], st["Q"], st["rank"])
+        # an optimizer that fuses the factor update (rgp.optim.PaletteAdam with the triple in
+        # fused_lowrank) takes dL/dw instead of the three gradients
+        fused = getattr(shs_dc, "_rgp_fused_adam", None)
+        ctx.fused = (fused, shs_dc, shs_A, shs_B) if fused is not None else None
         ctx.save_for_backward(f32c(pws), f32c(shs_A), f32c(shs_B), st["records"], st["contrib"], st["final_tau"],
                               st["ranges"], st["ids"], st["camd"])
         return layer, depths > 0.2
@@ -273,10 +277,14 @@ class WeightGSFunctionFast(torch.autograd.Function):
     @staticmethod
     def backward(ctx, dloss_dlayer, _):
         if dloss_dlayer is None:
+            ctx.fused = None
             return (None,) * 10
         W, H, K, n, P, Q, r = ctx.meta
         pws, A, B, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
         dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())
+        fused, ctx.fused = ctx.fused, None
+        if fused is not None and fused[0].take_lowrank(fused[1], fused[2], fused[3], dw, camd, pws, K, P, Q, r):
+            return (None,) * 10
         ddc = torch.empty((n, K), dtype=torch.float32, device=pws.device)
         dA = torch.empty(A.shape, dtype=torch.float32, device=pws.device)
         dB = torch.empty(B.shape, dtype=torch.float32, device=pws.device)

@@ -18,8 +18,11 @@ element by element:
 Parameters listed in `fused_sh` must be the tensor passed as `shs` to
 WeightGSFunctionTest (a float32, contiguous (N, sh_dim*K) CUDA tensor), or a
 pair (low, high) passed to WeightGSFunctionSplit ((N,K) and (N,15K), updated
-together by rgp_sh_backward_adam_split with the same arithmetic).  Every
-other parameter, and a fused one whose gradient also arrives the usual way
+together by rgp_sh_backward_adam_split with the same arithmetic).  Triples
+in `fused_lowrank` are the (shs_dc, shs_A, shs_B) of WeightGSFunctionFast
+with Q = 15, P = K and rank 1 or 2: rgp_sh_backward_adam_lowrank forms their
+gradients from dL/dw (N,K) and updates all three in one kernel.  Every other
+parameter, and a fused one whose gradient also arrives the usual way
 (several backward passes before one step), is updated by the same formula:
 in one kernel pass (rgp_adam) for contiguous float32 GPU tensors, with torch
 operations otherwise.
@@ -33,12 +36,32 @@ from .profiling import SPANS
 
 
 class PaletteAdam(torch.optim.Optimizer):
-    def __init__(self, params, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, weight_decay=0.0, fused_sh=()):
+    def __init__(self, params, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, weight_decay=0.0, fused_sh=(),
+                 fused_lowrank=()):
         defaults = dict(lr=lr, betas=betas, eps=eps, weight_decay=weight_decay)
         super().__init__(params, defaults)
         self._pending = {}
         self._pair_of = {}  # id(high) -> low, for the pairs of split leaves
+        self._triple_of = {}  # id(dc) -> (A, B), for the low-rank triples
         known = {id(p) for g in self.param_groups for p in g["params"]}
+        group_of = {id(p): gi for gi, g in enumerate(self.param_groups) for p in g["params"]}
+        for triple in fused_lowrank:
+            if len(triple) != 3:
+                raise ValueError("PaletteAdam: fused_lowrank entries are (shs_dc, shs_A, shs_B)")
+            if len({group_of.get(id(p)) for p in triple}) != 1:
+                raise ValueError("PaletteAdam: a fused_lowrank triple must sit in one parameter group")
+            for p in triple:
+                if id(p) not in known:
+                    raise ValueError("PaletteAdam: a fused_lowrank tensor is not among the optimised parameters")
+                if not (p.is_cuda and p.dtype == torch.float32 and p.is_contiguous()):
+                    raise ValueError("PaletteAdam: fused_lowrank tensors must be contiguous float32 on the GPU")
+            dc, A, B = triple
+            n, K = dc.shape[0], dc.shape[-1]
+            if dc.dim() != 2 or A.shape[0] != n or B.shape[0] != n or A.numel() % (n * K) or \
+                    B.numel() != (A.numel() // (n * K)) * 15 * n or A.numel() // (n * K) not in (1, 2):
+                raise ValueError("PaletteAdam: a low-rank triple must be (N,K), (N,K[,r]) and ([N,r,]15), r = 1 or 2")
+            self._triple_of[id(dc)] = (A, B)
+            dc._rgp_fused_adam = self
         for entry in fused_sh:
             pair = tuple(entry) if isinstance(entry, (tuple, list)) else (entry,)
             for p in pair:
@@ -70,6 +93,18 @@ class PaletteAdam(torch.optim.Optimizer):
         self._pending[id(high)] = ("split_high",)
         return True
 
+    def take_lowrank(self, dc, A, B, dw, camd, pws, K, P, Q, rank):
+        """The same for a low-rank triple (updated together at dc's turn)."""
+        if id(dc) in self._pending or Q != 15 or P != K or rank not in (1, 2):
+            return False
+        tri = self._triple_of.get(id(dc))
+        if tri is None or tri[0] is not A or tri[1] is not B:
+            return False
+        self._pending[id(dc)] = ("lowrank", A, B, dw, camd, pws, int(K), int(rank))
+        self._pending[id(A)] = ("lowrank_part",)
+        self._pending[id(B)] = ("lowrank_part",)
+        return True
+
     def zero_grad(self, set_to_none=True):
         self._pending.clear()
         super().zero_grad(set_to_none=set_to_none)
@@ -99,6 +134,23 @@ class PaletteAdam(torch.optim.Optimizer):
         for group in self.param_groups:
             for p in group["params"]:
                 pend = self._pending.get(id(p))
+                if pend is None or pend[0] != "lowrank":
+                    continue
+                _, A, B, dw, camd, pws, K, r = pend
+                if p.grad is None and A.grad is None and B.grad is None:
+                    continue
+                # a second gradient arrived the usual way: materialise the fused one
+                n = p.shape[0]
+                gdc = torch.empty((n, K), dtype=torch.float32, device=p.device)
+                gA, gB = torch.empty_like(A), torch.empty_like(B)
+                call("rgp_sh_backward_lowrank", n, K, K, 15, r, ptr(camd), ptr(pws), ptr(A), ptr(B), ptr(dw),
+                     ptr(gdc), ptr(gA), ptr(gB), stream(p.device))
+                extra[id(p)], extra[id(A)], extra[id(B)] = gdc.view_as(p), gA, gB
+                for t in (p, A, B):
+                    self._pending.pop(id(t), None)
+        for group in self.param_groups:
+            for p in group["params"]:
+                pend = self._pending.get(id(p))
                 if pend is None or pend[0] != "split":
                     continue
                 _, high, dw, camd, pws, K = pend
@@ -113,8 +165,20 @@ class PaletteAdam(torch.optim.Optimizer):
             lr, (b1, b2), eps, wd = group["lr"], group["betas"], group["eps"], group["weight_decay"]
             for p in group["params"]:
                 pend = self._pending.pop(id(p), None)
-                if pend is not None and pend[0] == "split_high":
-                    continue  # updated with its low leaf
+                if pend is not None and pend[0] in ("split_high", "lowrank_part"):
+                    continue  # updated with its low leaf / its dc
+                if pend is not None and pend[0] == "lowrank":
+                    _, A, B, dw, camd, pws, K, r = pend
+                    sd, sa, sb = self._state(p), self._state(A), self._state(B)
+                    for st_ in (sd, sa, sb):
+                        st_["step"] += 1
+                    with SPANS.span("sh_backward_adam"):
+                        call("rgp_sh_backward_adam_lowrank", p.shape[0], K, K, 15, r, ptr(camd), ptr(pws), ptr(dw),
+                             ptr(p), ptr(sd["exp_avg"]), ptr(sd["exp_avg_sq"]), ptr(A), ptr(sa["exp_avg"]),
+                             ptr(sa["exp_avg_sq"]), ptr(B), ptr(sb["exp_avg"]), ptr(sb["exp_avg_sq"]),
+                             I64(sd["step"]), float(lr), float(b1), float(b2), float(eps), float(wd),
+                             stream(p.device))
+                    continue
                 if pend is not None and pend[0] == "split":
                     _, high, dw, camd, pws, K = pend
                     sl, sh = self._state(p), self._state(high)

@@ -68,10 +68,19 @@ __global__ __launch_bounds__(256) void k_scan_reduce(int n, const uint32_t* __re
   __shared__ uint32_t lds[16];
   const int64_t base = (int64_t)blockIdx.x * kScanChunk;
   uint32_t s = 0;
+  if (base + kScanChunk <= n) {  // a full chunk: coalesced 16-byte loads (integer sum, any order)
+    const uint4* c4 = reinterpret_cast<const uint4*>(counts + base);
 #pragma unroll
-  for (int j = 0; j < kScanChunk / 256; ++j) {
-    const int64_t i = base + j * 256 + threadIdx.x;
-    if (i < n) s += counts[i];
+    for (int j = 0; j < kScanChunk / 1024; ++j) {
+      const uint4 x = c4[j * 256 + threadIdx.x];
+      s += (x.x + x.y) + (x.z + x.w);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < kScanChunk / 256; ++j) {
+      const int64_t i = base + j * 256 + threadIdx.x;
+      if (i < n) s += counts[i];
+    }
   }
   if (dqpart && threadIdx.x < 64) {
     const int64_t g = (int64_t)blockIdx.x * 64 + threadIdx.x;
@@ -136,20 +145,43 @@ __global__ __launch_bounds__(256) void k_scan_down(int n, uint32_t* __restrict__
                                                    const uint32_t* __restrict__ partials) {
   __shared__ uint32_t lds[16];
   constexpr int PER = kScanChunk / 256;
+  static_assert(PER % 4 == 0, "a thread's counts are whole 16-byte groups");
   const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * PER;
+  // a full chunk (the workspace arrays are 256-byte aligned): 16-byte loads
+  // and stores, without a bounds test per element
+  const bool full = (int64_t)(blockIdx.x + 1) * kScanChunk <= n;
+  uint4* c4 = reinterpret_cast<uint4*>(counts + base);
   uint32_t v[PER];
   uint32_t s = 0;
+  if (full) {
 #pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    v[j] = (base + j < n) ? counts[base + j] : 0u;
-    s += v[j];
+    for (int q = 0; q < PER / 4; ++q) {
+      const uint4 x = c4[q];
+      v[4 * q] = x.x;
+      v[4 * q + 1] = x.y;
+      v[4 * q + 2] = x.z;
+      v[4 * q + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < PER; ++j) v[j] = (base + j < n) ? counts[base + j] : 0u;
   }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) s += v[j];
   uint32_t tot;
   uint32_t run = block_excl_scan(s, &tot, lds) + partials[blockIdx.x];
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
     run += v[j];
-    if (base + j < n) counts[base + j] = run;
+    v[j] = run;
+  }
+  if (full) {
+#pragma unroll
+    for (int q = 0; q < PER / 4; ++q) c4[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (base + j < n) counts[base + j] = v[j];
   }
 }
 

@@ -206,7 +206,7 @@ def measure_c4(args, dev, n=6_000_000, views=4, steps=4, warmup=1):
         lay, _ = gsmodel.render_weights_views(pws, alphas, scales, rots, cams, K, dc=dc, A=A, B=B, Q=Q)
         target = torch.cat([ops.mix(lay[v], torch.clamp(pal + 0.05, 0, 1)) for v in range(views)], 0)
         del lay
-    opt = PaletteAdam([dc, A, B], lr=1e-3)
+    opt = PaletteAdam([dc, A, B], lr=1e-3, fused_lowrank=[(dc, A, B)])  # update fused with the factor backward
 
     def step_batched():
         layers, _ = gsmodel.WeightGSFunctionFastViews.apply(pws, dc, A, B, alphas, scales, rots, cams, K, Q)
@@ -241,7 +241,7 @@ def measure_c4(args, dev, n=6_000_000, views=4, steps=4, warmup=1):
     torch.cuda.empty_cache()
     return {"steps_per_s": out, "views_per_s": round(out["batched"] * views, 2),
             "workload": f"{n} Gaussians, K=7 rank-2 (A (N,7,2), B (N,2,15)), 3840x2160, {views} views per step; "
-                        "mix, L1+SSIM over the views, Adam (one pass per tensor) on dc/A/B"}
+                        "mix, L1+SSIM over the views, Adam on dc/A/B fused with their backward"}
 
 
 def torch_bary_loss(Wt, P, C, tau=1e-9, eps=1e-12, lam_pos=0.5):

@@ -322,6 +322,18 @@ int rgp_sh_backward_adam_lowrank(int n, int K, int P, int Q, int rank, const rgp
                                  float* B_exp_avg_sq, int64_t step, double lr, double beta1, double beta2,
                                  double eps, double weight_decay, hipStream_t stream);
 
+/* rgp_sh_backward_adam_lowrank for a multi-view batch: the factor gradients
+ * are summed over `views` cameras (cams: device array, dw: (views, N, K)) as
+ * rgp_sh_backward_lowrank_views forms them, then updated as above; equal bit
+ * for bit to rgp_sh_backward_lowrank_views followed by rgp_adam per tensor.
+ * No reference counterpart (the reference renders one camera per call). */
+int rgp_sh_backward_adam_lowrank_views(int views, int n, int K, int P, int Q, int rank, const rgp_camera* cams,
+                                       const float* pws, const float* dw, float* dc, float* dc_exp_avg,
+                                       float* dc_exp_avg_sq, float* A, float* A_exp_avg, float* A_exp_avg_sq,
+                                       float* B, float* B_exp_avg, float* B_exp_avg_sq, int64_t step, double lr,
+                                       double beta1, double beta2, double eps, double weight_decay,
+                                       hipStream_t stream);
+
 /* No counterpart in the reference either: one torch.optim.Adam step (the
  * optimizer of finetune.py:346-399) over a flat float32 tensor of n elements
  * with its gradient, in one pass; the same element update and scalars as

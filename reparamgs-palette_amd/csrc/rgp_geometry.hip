@@ -1358,6 +1358,64 @@ __global__ __launch_bounds__(64) void k_lowrank_adam_q15(
                                           vB + (size_t)g0 * R * Q, cnt, sG, a);
 }
 
+// The multi-view form: the gradients are the sums over `views` cameras of the
+// single-view chain (as k_lowrank_backward_q15_views forms them, view by view
+// into the same accumulators), then the same update.
+template <int K, int R>
+__global__ __launch_bounds__(64) void k_lowrank_adam_q15_views(
+    int views, int n, const rgp_camera* __restrict__ cams, const float* __restrict__ pws,
+    const float* __restrict__ dw, float* __restrict__ dc, float* __restrict__ mdc, float* __restrict__ vdc,
+    float* __restrict__ A, float* __restrict__ mA, float* __restrict__ vA, float* __restrict__ B,
+    float* __restrict__ mB, float* __restrict__ vB, AdamArgs a) {
+  constexpr int Q = 15, P = K, GS = K + P * R + R * Q + 1;
+  __shared__ float sG[64 * GS];
+  const int g0 = blockIdx.x * 64, t = threadIdx.x, i = g0 + t;
+  if (i < n) {
+    float ar[P * R], br[R * Q], da[P * R], db[R * Q], dd[K];
+#pragma unroll
+    for (int j = 0; j < P * R; ++j) { ar[j] = A[(size_t)i * P * R + j]; da[j] = 0.f; }
+#pragma unroll
+    for (int j = 0; j < R * Q; ++j) { br[j] = B[(size_t)i * R * Q + j]; db[j] = 0.f; }
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) dd[kk] = 0.f;
+    const float px = pws[3 * i], py = pws[3 * i + 1], pz = pws[3 * i + 2];
+    for (int v = 0; v < views; ++v) {
+      float Y[16];
+      sh_basis(px, py, pz, cams[v].twc, 16, Y);
+      const float* dwv = dw + ((size_t)v * n + i) * K;
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const float dwk = dwv[kk];
+        dd[kk] = dd[kk] + dwk * (float)SH_C0_0;
+#pragma unroll
+        for (int lm = 0; lm < 15; ++lm) {
+          const int g = lm * K + kk, row = g / Q, col = g % Q;
+          const float y = Y[lm + 1];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            da[row * R + r] = da[row * R + r] + dwk * br[r * Q + col] * y;
+            db[r * Q + col] = db[r * Q + col] + dwk * ar[row * R + r] * y;
+          }
+        }
+      }
+    }
+    float* row = sG + t * GS;
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) row[kk] = dd[kk];
+#pragma unroll
+    for (int j = 0; j < P * R; ++j) row[K + j] = da[j];
+#pragma unroll
+    for (int j = 0; j < R * Q; ++j) row[K + P * R + j] = db[j];
+  }
+  __syncthreads();
+  const int cnt = min(64, n - g0);
+  lowrank_adam_span<K, 0, GS>(dc + (size_t)g0 * K, mdc + (size_t)g0 * K, vdc + (size_t)g0 * K, cnt, sG, a);
+  lowrank_adam_span<P * R, K, GS>(A + (size_t)g0 * P * R, mA + (size_t)g0 * P * R, vA + (size_t)g0 * P * R,
+                                  cnt, sG, a);
+  lowrank_adam_span<R * Q, K + P * R, GS>(B + (size_t)g0 * R * Q, mB + (size_t)g0 * R * Q,
+                                          vB + (size_t)g0 * R * Q, cnt, sG, a);
+}
+
 }  // namespace rgp
 
 using namespace rgp;
@@ -1891,6 +1949,41 @@ extern "C" int rgp_sh_backward_adam_lowrank(int n, int K, int P, int Q, int rank
                                        B_exp_avg_sq, a));
   }
   return check_launch("rgp_sh_backward_adam_lowrank");
+}
+
+extern "C" int rgp_sh_backward_adam_lowrank_views(int views, int n, int K, int P, int Q, int rank,
+                                                  const rgp_camera* cams, const float* pws, const float* dw,
+                                                  float* dc, float* dc_exp_avg, float* dc_exp_avg_sq, float* A,
+                                                  float* A_exp_avg, float* A_exp_avg_sq, float* B, float* B_exp_avg,
+                                                  float* B_exp_avg_sq, int64_t step, double lr, double beta1,
+                                                  double beta2, double eps, double weight_decay, hipStream_t s) {
+  if (views < 1 || views > RGP_MAX_VIEWS || n < 0)
+    return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_lowrank_views: bad views or n");
+  int rc = check_lowrank("rgp_sh_backward_adam_lowrank_views", K, P, Q, rank);
+  if (rc) return rc;
+  if (!(Q == 15 && P == K && (rank == 1 || rank == 2)))
+    return set_error(RGP_ERR_ARG,
+                     "rgp_sh_backward_adam_lowrank_views: needs Q = 15, P = K and rank 1 or 2 (else "
+                     "rgp_sh_backward_lowrank_views + rgp_adam)");
+  if (step < 1) return set_error(RGP_ERR_ARG, "rgp_sh_backward_adam_lowrank_views: step must be >= 1");
+  if ((((uintptr_t)dc) | ((uintptr_t)dc_exp_avg) | ((uintptr_t)dc_exp_avg_sq) | ((uintptr_t)A) |
+       ((uintptr_t)A_exp_avg) | ((uintptr_t)A_exp_avg_sq) | ((uintptr_t)B) | ((uintptr_t)B_exp_avg) |
+       ((uintptr_t)B_exp_avg_sq)) & 15)
+    return set_error(RGP_ERR_ARG,
+                     "rgp_sh_backward_adam_lowrank_views: tensors and moments must be 16-byte aligned");
+  if (n == 0) return RGP_OK;
+  const AdamArgs a = adam_args(step, lr, beta1, beta2, eps, weight_decay);
+  const int blocks = (n + 63) / 64;
+  if (rank == 1) {
+    RGP_K_SWITCH(K, hipLaunchKernelGGL((k_lowrank_adam_q15_views<KK, 1>), dim3(blocks), dim3(64), 0, s, views, n,
+                                       cams, pws, dw, dc, dc_exp_avg, dc_exp_avg_sq, A, A_exp_avg, A_exp_avg_sq, B,
+                                       B_exp_avg, B_exp_avg_sq, a));
+  } else {
+    RGP_K_SWITCH(K, hipLaunchKernelGGL((k_lowrank_adam_q15_views<KK, 2>), dim3(blocks), dim3(64), 0, s, views, n,
+                                       cams, pws, dw, dc, dc_exp_avg, dc_exp_avg_sq, A, A_exp_avg, A_exp_avg_sq, B,
+                                       B_exp_avg, B_exp_avg_sq, a));
+  }
+  return check_launch("rgp_sh_backward_adam_lowrank_views");
 }
 
 extern "C" int rgp_sh_backward_lowrank(int n, int K, int P, int Q, int rank,

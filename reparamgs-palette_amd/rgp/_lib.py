@@ -44,6 +44,8 @@ _SIGS = {
     "rgp_sh_backward_adam_split": (I, [I, I, P, P, P, P, P, P, P, P, P, I64, D, D, D, D, D, P]),
     "rgp_sh_backward_adam_lowrank": (I, [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, D, D, D, D,
                                          P]),
+    "rgp_sh_backward_adam_lowrank_views": (I, [I, I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, D, D,
+                                               D, D, P]),
     "rgp_sh_backward_lowrank_views": (I, [I, I, I, I, I, I, P, P, P, P, P, P, P, P, P]),
     "rgp_bin_sort": (I, [I, P, I, I, P, I64, P, P, P, P]),
     "rgp_bin_header": (P, [P]),

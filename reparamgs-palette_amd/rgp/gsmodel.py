@@ -163,6 +163,8 @@ class WeightGSFunctionFastViews(torch.autograd.Function):
                                                  Q=Q, return_state=True)
         ctx.meta = (int(cams[0].width), int(cams[0].height), K, pws.shape[0], st["P"], st["Q"], st["rank"],
                     len(cams))
+        fused = getattr(shs_dc, "_rgp_fused_adam", None)  # PaletteAdam with the triple in fused_lowrank
+        ctx.fused = (fused, shs_dc, shs_A, shs_B) if fused is not None else None
         ctx.save_for_backward(f32c(pws), f32c(shs_A), f32c(shs_B), st["records"], st["ranges"], st["ids"],
                               st["camsd"])
         return layer, depths > 0.2
@@ -170,10 +172,15 @@ class WeightGSFunctionFastViews(torch.autograd.Function):
     @staticmethod
     def backward(ctx, dloss_dlayer, _):
         if dloss_dlayer is None:
+            ctx.fused = None
             return (None,) * 10
         W, H, K, n, P, Q, r, V = ctx.meta
         pws, A, B, records, ranges, ids, camsd = ctx.saved_tensors
         dw = ops.draw_backward_replay_views(V, W, H, K, ranges, ids, records, dloss_dlayer.contiguous(), n)
+        fused, ctx.fused = ctx.fused, None
+        if fused is not None and fused[0].take_lowrank(fused[1], fused[2], fused[3], dw, camsd, pws, K, P, Q, r,
+                                                       views=V):
+            return (None,) * 10
         ddc = torch.empty((n, K), dtype=torch.float32, device=pws.device)
         dA = torch.empty(A.shape, dtype=torch.float32, device=pws.device)
         dB = torch.empty(B.shape, dtype=torch.float32, device=pws.device)

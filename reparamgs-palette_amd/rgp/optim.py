@@ -93,14 +93,15 @@ class PaletteAdam(torch.optim.Optimizer):
         self._pending[id(high)] = ("split_high",)
         return True
 
-    def take_lowrank(self, dc, A, B, dw, camd, pws, K, P, Q, rank):
-        """The same for a low-rank triple (updated together at dc's turn)."""
+    def take_lowrank(self, dc, A, B, dw, camd, pws, K, P, Q, rank, views=1):
+        """The same for a low-rank triple (updated together at dc's turn);
+        views > 1: dw is (views, N, K) and camd the device array of cameras."""
         if id(dc) in self._pending or Q != 15 or P != K or rank not in (1, 2):
             return False
         tri = self._triple_of.get(id(dc))
         if tri is None or tri[0] is not A or tri[1] is not B:
             return False
-        self._pending[id(dc)] = ("lowrank", A, B, dw, camd, pws, int(K), int(rank))
+        self._pending[id(dc)] = ("lowrank", A, B, dw, camd, pws, int(K), int(rank), int(views))
         self._pending[id(A)] = ("lowrank_part",)
         self._pending[id(B)] = ("lowrank_part",)
         return True
@@ -136,15 +137,19 @@ class PaletteAdam(torch.optim.Optimizer):
                 pend = self._pending.get(id(p))
                 if pend is None or pend[0] != "lowrank":
                     continue
-                _, A, B, dw, camd, pws, K, r = pend
+                _, A, B, dw, camd, pws, K, r, V = pend
                 if p.grad is None and A.grad is None and B.grad is None:
                     continue
                 # a second gradient arrived the usual way: materialise the fused one
                 n = p.shape[0]
                 gdc = torch.empty((n, K), dtype=torch.float32, device=p.device)
                 gA, gB = torch.empty_like(A), torch.empty_like(B)
-                call("rgp_sh_backward_lowrank", n, K, K, 15, r, ptr(camd), ptr(pws), ptr(A), ptr(B), ptr(dw),
-                     ptr(gdc), ptr(gA), ptr(gB), stream(p.device))
+                if V == 1:
+                    call("rgp_sh_backward_lowrank", n, K, K, 15, r, ptr(camd), ptr(pws), ptr(A), ptr(B), ptr(dw),
+                         ptr(gdc), ptr(gA), ptr(gB), stream(p.device))
+                else:
+                    call("rgp_sh_backward_lowrank_views", V, n, K, K, 15, r, ptr(camd), ptr(pws), ptr(A), ptr(B),
+                         ptr(dw), ptr(gdc), ptr(gA), ptr(gB), stream(p.device))
                 extra[id(p)], extra[id(A)], extra[id(B)] = gdc.view_as(p), gA, gB
                 for t in (p, A, B):
                     self._pending.pop(id(t), None)
@@ -168,12 +173,13 @@ class PaletteAdam(torch.optim.Optimizer):
                 if pend is not None and pend[0] in ("split_high", "lowrank_part"):
                     continue  # updated with its low leaf / its dc
                 if pend is not None and pend[0] == "lowrank":
-                    _, A, B, dw, camd, pws, K, r = pend
+                    _, A, B, dw, camd, pws, K, r, V = pend
                     sd, sa, sb = self._state(p), self._state(A), self._state(B)
                     for st_ in (sd, sa, sb):
                         st_["step"] += 1
+                    head = ("rgp_sh_backward_adam_lowrank",) if V == 1 else ("rgp_sh_backward_adam_lowrank_views", V)
                     with SPANS.span("sh_backward_adam"):
-                        call("rgp_sh_backward_adam_lowrank", p.shape[0], K, K, 15, r, ptr(camd), ptr(pws), ptr(dw),
+                        call(*head, p.shape[0], K, K, 15, r, ptr(camd), ptr(pws), ptr(dw),
                              ptr(p), ptr(sd["exp_avg"]), ptr(sd["exp_avg_sq"]), ptr(A), ptr(sa["exp_avg"]),
                              ptr(sa["exp_avg_sq"]), ptr(B), ptr(sb["exp_avg"]), ptr(sb["exp_avg_sq"]),
                              I64(sd["step"]), float(lr), float(b1), float(b2), float(eps), float(wd),

@@ -62,6 +62,33 @@ def test_kernel_equals_backward_then_adam(device, K, rank):
     assert not np.array_equal(N(fused[1]), sc.A)  # the update moved the factors
 
 
+@pytest.mark.parametrize("views,K,rank", [(3, 5, 2), (4, 7, 2), (2, 5, 1)])
+def test_views_kernel_equals_backward_then_adam(device, views, K, rank):
+    print(f"case: fused low-rank adam, {views} views K={K} rank={rank}", flush=True)
+    n = 3000 + 11
+    rng = np.random.default_rng(views * 100 + K * 10 + rank)
+    sc = make_scene(n, seed=6, extent=1.0, K=K, lowrank=True, rank=rank, P=K, Q=15)
+    camsd = ops.camera_structs(orbit_cameras(views, 4.0, 96, 64), device)
+    pws = T(sc.pws, device)
+    fused = [T(sc.dc, device), T(sc.A, device), T(sc.B, device)]
+    ref = [t.clone() for t in fused]
+    mf, vf, mr, vr = ([torch.zeros_like(t) for t in fused] for _ in range(4))
+    for step in range(1, 4):
+        dw = T(rng.normal(0, 1e-2, (views, n, K)).astype(np.float32), device)
+        g = [torch.empty_like(t) for t in ref]
+        call("rgp_sh_backward_lowrank_views", views, n, K, K, 15, rank, ptr(camsd), ptr(pws), ptr(ref[1]),
+             ptr(ref[2]), ptr(dw), ptr(g[0]), ptr(g[1]), ptr(g[2]), stream(device))
+        for t, gt, m, v in zip(ref, g, mr, vr):
+            call("rgp_adam", I64(t.numel()), ptr(t), ptr(gt), ptr(m), ptr(v), I64(step), 3e-3, 0.9, 0.999, 1e-8,
+                 0.0, stream(device))
+        call("rgp_sh_backward_adam_lowrank_views", views, n, K, K, 15, rank, ptr(camsd), ptr(pws), ptr(dw),
+             ptr(fused[0]), ptr(mf[0]), ptr(vf[0]), ptr(fused[1]), ptr(mf[1]), ptr(vf[1]), ptr(fused[2]), ptr(mf[2]),
+             ptr(vf[2]), I64(step), 3e-3, 0.9, 0.999, 1e-8, 0.0, stream(device))
+    for name, a, b in (("dc", fused[0], ref[0]), ("A", fused[1], ref[1]), ("B", fused[2], ref[2]),
+                       ("m_A", mf[1], mr[1]), ("v_B", vf[2], vr[2]), ("v_dc", vf[0], vr[0])):
+        assert np.array_equal(N(a), N(b)), name
+
+
 @pytest.mark.parametrize("second", [False, True])
 def test_palette_adam_fused_lowrank(device, second):
     """Two steps of PaletteAdam with the triple in fused_lowrank, each

@@ -655,9 +655,10 @@ void draw(int width, int height, int dim, const int32_t* ranges, const int32_t* 
 // ---------------------------------------------------------------------------
 // A14 draw_weight_B (kernel.cu:1291-1379).  Per (pixel, contributing entry)
 // the term dL/dW_k * alpha' * tau is formed exactly as in the reference, in
-// type T.  mode 0: terms are summed serially in double into grad (N,K) and
-// their absolute values into gabs (N,K), which bounds the reordering error of
-// any atomic summation.  mode 1: OpenMP over tiles, atomic adds in T.
+// type T.  mode 0: terms are summed in double into grad (N,K) and their
+// absolute values into gabs (N,K), which bounds the reordering error of any
+// atomic summation (OpenMP over tiles; the double sums' own order changes
+// them by ~1e-16 relative).  mode 1: OpenMP over tiles, atomic adds in T.
 // ---------------------------------------------------------------------------
 // replay: the same terms with tau taken from a front-to-back replay of the
 // forward recurrence (tau before the entry, the value the forward multiplied
@@ -669,6 +670,7 @@ void draw_backward_replay(int width, int height, int K, const int32_t* ranges, c
                           double* grad, double* gabs) {
   const int gx = (width + BLOCK - 1) / BLOCK, gy = (height + BLOCK - 1) / BLOCK;
   const size_t HW = (size_t)width * height;
+#pragma omp parallel for schedule(dynamic, 4)
   for (int tile = 0; tile < gx * gy; ++tile) {
     const int tx = tile % gx, ty = tile / gx;
     const int r0 = ranges[2 * tile], r1 = ranges[2 * tile + 1];
@@ -687,7 +689,9 @@ void draw_backward_replay(int width, int height, int K, const int32_t* ranges, c
           const T ta = tau * ap;
           for (int k = 0; k < K; ++k) {
             const T term = dlayer[k * HW + pix] * ta;
+#pragma omp atomic
             grad[(size_t)g * K + k] += (double)term;
+#pragma omp atomic
             gabs[(size_t)g * K + k] += std::fabs((double)term);
           }
           tau *= ((T)1 - ap);
@@ -708,7 +712,7 @@ void draw_backward(int width, int height, int K, const int32_t* ranges, const in
   }
   const int gx = (width + BLOCK - 1) / BLOCK, gy = (height + BLOCK - 1) / BLOCK;
   const size_t HW = (size_t)width * height;
-#pragma omp parallel for schedule(dynamic, 4) if (mode == 1)
+#pragma omp parallel for schedule(dynamic, 4)
   for (int tile = 0; tile < gx * gy; ++tile) {
     const int tx = tile % gx, ty = tile / gx;
     const int r0 = ranges[2 * tile], r1 = ranges[2 * tile + 1];
@@ -735,7 +739,9 @@ void draw_backward(int width, int height, int K, const int32_t* ranges, const in
           for (int k = 0; k < K; ++k) {
             const T term = dl[k] * dgdw;
             if (mode == 0) {
+#pragma omp atomic
               grad[(size_t)g * K + k] += (double)term;
+#pragma omp atomic
               gabs[(size_t)g * K + k] += std::fabs((double)term);
             } else {
 #pragma omp atomic

@@ -621,20 +621,14 @@ struct MixArgs {
   const float* pal;
 };
 
-template <int K>
-__global__ __launch_bounds__(256) void k_mix(MixArgs a) {
-  const int img = blockIdx.y;
-  const int64_t hw = a.hw[img];
-  const float* __restrict__ Wp = a.W[img];
-  float* __restrict__ out = a.rgb[img];
-  float P[K][3];
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) P[k][c] = a.pal[((size_t)img * K + k) * 3 + c];
-  const int64_t p4 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-  if (p4 >= hw) return;
-  if ((hw & 3) == 0) {
+// RGB of the four pixels p4 .. p4+3 (those below hw): rgb_c = sum_k P_kc W_k,
+// summed in channel order k = 0 .. K-1 as the oracle does.  VEC: hw is a
+// multiple of 4 and both planes are 16-byte aligned, so the group is one
+// float4 per plane; else pixel by pixel.
+template <int K, bool VEC>
+__device__ __forceinline__ void mix_group(int64_t hw, const float* __restrict__ Wp, const float (&P)[K][3],
+                                          float* __restrict__ out, int64_t p4) {
+  if constexpr (VEC) {
     float4 wv[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) wv[k] = *(const float4*)(Wp + k * hw + p4);
@@ -660,6 +654,45 @@ __global__ __launch_bounds__(256) void k_mix(MixArgs a) {
         out[c * hw + p] = r;
       }
   }
+}
+
+__device__ __forceinline__ bool mix_vec_ok(int64_t hw, const void* W, const void* rgb) {
+  return (hw & 3) == 0 && (((uintptr_t)W | (uintptr_t)rgb) & 15) == 0;
+}
+
+// Batched mix: image blockIdx.y, with its own size, planes and palette; the
+// float4 test is per image (uniform over the block).
+template <int K>
+__global__ __launch_bounds__(256) void k_mix(MixArgs a) {
+  const int img = blockIdx.y;
+  const int64_t hw = a.hw[img];
+  const float* __restrict__ Wp = a.W[img];
+  float* __restrict__ out = a.rgb[img];
+  float P[K][3];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) P[k][c] = a.pal[((size_t)img * K + k) * 3 + c];
+  const int64_t p4 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (p4 >= hw) return;
+  if (mix_vec_ok(hw, Wp, out))
+    mix_group<K, true>(hw, Wp, P, out, p4);
+  else
+    mix_group<K, false>(hw, Wp, P, out, p4);
+}
+
+// Single-image mix with its arguments by value (no device pointer arrays).
+template <int K, bool VEC>
+__global__ __launch_bounds__(256) void k_mix1(int64_t hw, const float* __restrict__ Wp,
+                                              const float* __restrict__ pal, float* __restrict__ out) {
+  float P[K][3];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) P[k][c] = pal[k * 3 + c];
+  const int64_t p4 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (p4 >= hw) return;
+  mix_group<K, VEC>(hw, Wp, P, out, p4);
 }
 
 // dL/dW_k = P_k . dL/drgb per pixel, and dL/dP_kc = sum_p W_k dL/drgb_c.
@@ -880,6 +913,22 @@ extern "C" int rgp_mix_batched(int count, int K, const float* const* W, float* c
   const unsigned gxm = (unsigned)((max_hw + 1023) / 1024);
   RGP_DIM_SWITCH(K, hipLaunchKernelGGL(k_mix<DD>, dim3(gxm, count), dim3(256), 0, s, a));
   return check_launch("rgp_mix_batched");
+}
+
+extern "C" int rgp_mix(int K, int64_t hw, const float* W, const float* palette, float* rgb,
+                       hipStream_t s) {
+  if (K < 1 || K > RGP_MAX_DIM || hw < 0) return set_error(RGP_ERR_ARG, "rgp_mix: bad K or size");
+  if (hw == 0) return RGP_OK;
+  const int64_t blocks = (hw + 1023) / 1024;
+  if (blocks > 0x7fffffff) return set_error(RGP_ERR_ARG, "rgp_mix: image too large");
+  if (((hw & 3) == 0) && ((((uintptr_t)W) | ((uintptr_t)rgb)) & 15) == 0) {
+    RGP_DIM_SWITCH(K, hipLaunchKernelGGL((k_mix1<DD, true>), dim3((unsigned)blocks), dim3(256), 0, s, hw, W,
+                                         palette, rgb));
+  } else {
+    RGP_DIM_SWITCH(K, hipLaunchKernelGGL((k_mix1<DD, false>), dim3((unsigned)blocks), dim3(256), 0, s, hw, W,
+                                         palette, rgb));
+  }
+  return check_launch("rgp_mix");
 }
 
 extern "C" int rgp_mix_backward(int K, int64_t hw, const float* W, const float* palette,

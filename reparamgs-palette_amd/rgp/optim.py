@@ -22,8 +22,10 @@ together by rgp_sh_backward_adam_split with the same arithmetic).  Triples
 in `fused_lowrank` are the (shs_dc, shs_A, shs_B) of WeightGSFunctionFast
 with Q = 15, P = K and rank 1 or 2: rgp_sh_backward_adam_lowrank forms their
 gradients from dL/dw (N,K) and updates all three in one kernel.  Every other
-parameter, and a fused one whose gradient also arrives the usual way
-(several backward passes before one step), is updated by the same formula:
+parameter, a fused one whose gradient also arrives the usual way
+(several backward passes before one step), and the tensors of a fused group
+whose step counts differ (one of them was updated alone earlier) are updated
+by the same formula:
 in one kernel pass (rgp_adam) for contiguous float32 GPU tensors, with torch
 operations otherwise.
 """
@@ -117,6 +119,11 @@ class PaletteAdam(torch.optim.Optimizer):
         call("rgp_sh_backward_full", p.shape[0], K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(g), stream(p.device))
         return g
 
+    def _same_step(self, *ts):
+        """Whether the tensors have taken the same number of Adam steps (a
+        fused group update uses one step count for all of them)."""
+        return len({self.state[t].get("step", 0) for t in ts}) == 1
+
     def _state(self, p):
         st = self.state[p]
         if not st:
@@ -138,9 +145,11 @@ class PaletteAdam(torch.optim.Optimizer):
                 if pend is None or pend[0] != "lowrank":
                     continue
                 _, A, B, dw, camd, pws, K, r, V = pend
-                if p.grad is None and A.grad is None and B.grad is None:
+                if p.grad is None and A.grad is None and B.grad is None and self._same_step(p, A, B):
                     continue
-                # a second gradient arrived the usual way: materialise the fused one
+                # a second gradient arrived the usual way, or the three tensors have taken
+                # different numbers of steps (the fused kernel applies one bias correction):
+                # materialise the fused gradients and update each tensor on its own
                 n = p.shape[0]
                 gdc = torch.empty((n, K), dtype=torch.float32, device=p.device)
                 gA, gB = torch.empty_like(A), torch.empty_like(B)
@@ -159,7 +168,7 @@ class PaletteAdam(torch.optim.Optimizer):
                 if pend is None or pend[0] != "split":
                     continue
                 _, high, dw, camd, pws, K = pend
-                if p.grad is None and high.grad is None:
+                if p.grad is None and high.grad is None and self._same_step(p, high):
                     continue
                 full = self._materialise(torch.empty((p.shape[0], 16 * K), dtype=p.dtype, device=p.device),
                                          (dw, camd, pws, K, 16))

@@ -341,14 +341,19 @@ class Engine:
         self.base_pal = {}
 
     # -- cameras -----------------------------------------------------------
+    def pose_camera(self, pose, width, height):
+        """The host camera of pose `pose` (of n_poses on the orbit) at a resolution."""
+        th = 2.0 * math.pi * pose / self.n_poses
+        el = 0.3 * math.sin(3.0 * th)
+        pos = (self.radius * math.cos(el) * math.sin(th), self.radius * math.sin(el),
+               -self.radius * math.cos(el) * math.cos(th))
+        return look_at_camera(pos, width, height)
+
     def camera(self, pose, width, height):
+        """The pose's rgp_camera struct in pinned host memory (cached)."""
         key = (pose, width, height)
         if key not in self.cams:
-            th = 2.0 * math.pi * pose / self.n_poses
-            el = 0.3 * math.sin(3.0 * th)
-            pos = (self.radius * math.cos(el) * math.sin(th), self.radius * math.sin(el),
-                   -self.radius * math.cos(el) * math.cos(th))
-            c = look_at_camera(pos, width, height)
+            c = self.pose_camera(pose, width, height)
             host = ops.camera_struct_host(c.Rcw, c.tcw, c.twc, c.fx, c.fy, c.cx, c.cy, width, height)
             self.cams[key] = torch.from_numpy(host).pin_memory()
         return self.cams[key]

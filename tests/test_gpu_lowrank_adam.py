@@ -133,6 +133,91 @@ def test_palette_adam_fused_lowrank(device, second):
             assert np.array_equal(N(opt.state[params[i]]["exp_avg_sq"]), N(vr[i])), (it, what, "exp_avg_sq")
 
 
+@pytest.mark.parametrize("second,wd", [(False, 1e-3), (True, 1e-3), (False, 0.0)])
+def test_palette_adam_fused_lowrank_views(device, second, wd):
+    """The multi-view path through Python: WeightGSFunctionFastViews over 3
+    views hands the (views, N, K) weight gradient to PaletteAdam, whose step
+    runs rgp_sh_backward_adam_lowrank_views; two steps against copies updated
+    by the view-summed backward (rgp_sh_backward_lowrank_views) then rgp_adam,
+    bit for bit, params and both moments.  second=True adds 0.5 |B|^2, so B
+    also gets a gradient the usual way and the step materialises the fused
+    one (the V > 1 branch)."""
+    print(f"case: PaletteAdam fused_lowrank over 3 views, second={second} wd={wd}", flush=True)
+    K, rank, n, V = 5, 2, 3000, 3
+    sc = make_scene(n, seed=18, extent=1.0, K=K, lowrank=True, rank=rank, P=K, Q=15)
+    cams = orbit_cameras(6, 4.0, 160, 128)[1:1 + V]
+    geo = (T(sc.alphas, device), T(sc.scales, device), T(sc.rots, device))
+    pws = T(sc.pws, device)
+    params = [T(x, device).requires_grad_(True) for x in (sc.dc, sc.A, sc.B)]
+    opt = PaletteAdam(params, lr=3e-3, weight_decay=wd, fused_lowrank=[tuple(params)])
+    mr, vr = [torch.zeros_like(p) for p in params], [torch.zeros_like(p) for p in params]
+    dl = None
+    for it in range(2):
+        layers, _ = gsmodel.WeightGSFunctionFastViews.apply(pws, *params, *geo, cams, K, 15)
+        if dl is None:
+            dl = T(np.random.default_rng(6).normal(size=tuple(layers.shape)).astype(np.float32), device)
+        loss = (layers * dl).sum() + (0.5 * (params[2] ** 2).sum() if second else 0.0)
+        opt.zero_grad()
+        loss.backward()
+        pend = opt._pending[id(params[0])]
+        assert pend[0] == "lowrank" and pend[-1] == V
+        dw, camsd = pend[3], pend[4]
+        ref = [p.detach().clone() for p in params]
+        g = [torch.empty_like(p) for p in ref]
+        call("rgp_sh_backward_lowrank_views", V, n, K, K, 15, rank, ptr(camsd), ptr(pws), ptr(ref[1]), ptr(ref[2]),
+             ptr(dw), ptr(g[0]), ptr(g[1]), ptr(g[2]), stream(device))
+        if second:
+            g[2] = params[2].grad + g[2]
+        for t, gt, m, v in zip(ref, g, mr, vr):
+            call("rgp_adam", I64(t.numel()), ptr(t), ptr(gt), ptr(m), ptr(v), I64(it + 1), 3e-3, 0.9, 0.999, 1e-8,
+                 wd, stream(device))
+        opt.step()
+        for i, what in enumerate(("dc", "A", "B")):
+            assert np.array_equal(N(params[i]), N(ref[i])), (it, what)
+            assert np.array_equal(N(opt.state[params[i]]["exp_avg"]), N(mr[i])), (it, what, "exp_avg")
+            assert np.array_equal(N(opt.state[params[i]]["exp_avg_sq"]), N(vr[i])), (it, what, "exp_avg_sq")
+
+
+def test_palette_adam_fused_lowrank_unequal_steps(device):
+    """A step that updates A alone (a regulariser without a render) leaves the
+    triple with different step counts; the next fused step then falls back to
+    the materialised gradients and each tensor's own bias correction, equal
+    to the unfused update bit for bit."""
+    print("case: PaletteAdam fused_lowrank with unequal step counts", flush=True)
+    K, rank, n = 5, 2, 2500
+    sc = make_scene(n, seed=28, extent=1.0, K=K, lowrank=True, rank=rank, P=K, Q=15)
+    cam = orbit_cameras(4, 4.0, 160, 128)[3]
+    geo = (T(sc.alphas, device), T(sc.scales, device), T(sc.rots, device), cam.to_torch(device))
+    pws = T(sc.pws, device)
+    params = [T(x, device).requires_grad_(True) for x in (sc.dc, sc.A, sc.B)]
+    opt = PaletteAdam(params, lr=3e-3, fused_lowrank=[tuple(params)])
+    ref = [p.detach().clone() for p in params]
+    mr, vr = [torch.zeros_like(p) for p in params], [torch.zeros_like(p) for p in params]
+    opt.zero_grad()
+    (0.5 * (params[1] ** 2).sum()).backward()
+    opt.step()
+    call("rgp_adam", I64(ref[1].numel()), ptr(ref[1]), ptr(ref[1].clone()), ptr(mr[1]), ptr(vr[1]), I64(1), 3e-3,
+         0.9, 0.999, 1e-8, 0.0, stream(device))
+    assert [opt.state[p].get("step", 0) for p in params] == [0, 1, 0]
+    layer, _ = gsmodel.WeightGSFunctionFast.apply(pws, *params, *geo, K, 15)
+    dl = T(np.random.default_rng(8).normal(size=tuple(layer.shape)).astype(np.float32), device)
+    opt.zero_grad()
+    (layer * dl).sum().backward()
+    dw, camd = opt._pending[id(params[0])][3], opt._pending[id(params[0])][4]
+    g = [torch.empty_like(p) for p in ref]
+    call("rgp_sh_backward_lowrank", n, K, K, 15, rank, ptr(camd), ptr(pws), ptr(ref[1]), ptr(ref[2]), ptr(dw),
+         ptr(g[0]), ptr(g[1]), ptr(g[2]), stream(device))
+    for t, gt, m, v, step in zip(ref, g, mr, vr, (1, 2, 1)):
+        call("rgp_adam", I64(t.numel()), ptr(t), ptr(gt), ptr(m), ptr(v), I64(step), 3e-3, 0.9, 0.999, 1e-8, 0.0,
+             stream(device))
+    opt.step()
+    assert [opt.state[p]["step"] for p in params] == [1, 2, 1]
+    for i, what in enumerate(("dc", "A", "B")):
+        assert np.array_equal(N(params[i]), N(ref[i])), what
+        assert np.array_equal(N(opt.state[params[i]]["exp_avg"]), N(mr[i])), (what, "exp_avg")
+        assert np.array_equal(N(opt.state[params[i]]["exp_avg_sq"]), N(vr[i])), (what, "exp_avg_sq")
+
+
 def test_unsupported_shapes_refused(device):
     z = torch.zeros(64, dtype=torch.float32, device=device)
     with pytest.raises(RgpError):

@@ -1,7 +1,8 @@
 """The full-rank SH backward fused with Adam (rgp_sh_backward_adam_full,
-rgp.optim.PaletteAdam) against torch.optim.Adam applied to the materialised
-gradient dL/dshs = dL/dw * Y (rgp_sh_backward_full, itself bit-exact with the
-oracle): same dL/dw in, parameters and both moments out.
+rgp.optim.PaletteAdam) against torch.optim.Adam applied to the oracle's
+gradient dL/dshs[n, lm*K+k] = dL/dw[n,k] * Y_lm(n) (gsmodel.py:135-144: one
+float32 product per element, Y from the oracle's sh2Weight basis): same
+dL/dw in, parameters and both moments out.
 
 torch's elementwise kernels may contract a multiply-add into one rounding
 where the fused kernel rounds twice: a few float32 ulp of the operands, which
@@ -12,6 +13,7 @@ import numpy as np
 import pytest
 import torch
 
+from oracle import oracle as O
 from rgp import gsmodel
 from rgp._lib import I64, call, ptr, stream
 from rgp.optim import PaletteAdam
@@ -22,6 +24,15 @@ pytestmark = pytest.mark.gpu
 
 def T(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def oracle_dshs(pws, twc, dw, K, sh_dim, dev):
+    """dL/dshs from dL/dw (N,K) with the oracle's SH basis, in float32."""
+    dw = np.asarray(dw, np.float32)
+    n = dw.shape[0]
+    _, basis = O.sh2weight(np.zeros((n, sh_dim * K), np.float32), np.asarray(pws, np.float32), twc, K)
+    g = (dw[:, None, :] * basis.reshape(n, sh_dim)[:, :, None].astype(np.float32)).reshape(n, sh_dim * K)
+    return T(g.astype(np.float32), dev)
 
 
 def close(a, b, what, rtol=1e-6):
@@ -48,9 +59,7 @@ def test_fused_kernel_matches_torch_adam(device, sh_dim, K, wd):
         dw = T(rng.normal(0, 1e-3, (n, K)).astype(np.float32), device)
         call("rgp_sh_backward_adam_full", n, K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(p_f), ptr(m_f), ptr(v_f),
              I64(step), 3e-3, 0.9, 0.999, 1e-8, wd, stream(device))
-        g = torch.empty_like(p_f)
-        call("rgp_sh_backward_full", n, K, sh_dim, ptr(camd), ptr(pws), ptr(dw), ptr(g), stream(device))
-        p_t.grad = g
+        p_t.grad = oracle_dshs(scene.pws[:n], cam.twc, dw.cpu().numpy(), K, sh_dim, device)
         ref.step()
     st = ref.state[p_t]
     # with weight decay g = dw*Y + wd*p can cancel to near 0, where Adam's
@@ -89,10 +98,8 @@ def test_palette_adam_in_the_finetune_step(device, monkeypatch):
         assert shs.grad is None and len(seen) == step + 1
         opt.step()
         opt.zero_grad()
-        dw, camd, pws_s = seen[-1]
-        g = torch.empty_like(shs_ref)
-        call("rgp_sh_backward_full", scene.n, scene.K, 16, ptr(camd), ptr(pws_s), ptr(dw), ptr(g), stream(device))
-        shs_ref.grad = g
+        dw = seen[-1][0]
+        shs_ref.grad = oracle_dshs(scene.pws, cam.twc, dw.cpu().numpy(), scene.K, 16, device)
         ref.step()
     close(shs, shs_ref, "shs")
     close(opt.state[shs]["exp_avg"], ref.state[shs_ref]["exp_avg"], "exp_avg")

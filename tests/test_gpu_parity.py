@@ -202,9 +202,53 @@ def test_mix_matches_bake(device, c0):
     img = gsmodel.render_vanilla(T(scene.pws, device), T(baked.reshape(scene.n, 48), device),
                                  T(scene.alphas, device), T(scene.scales, device), T(scene.rots, device),
                                  cam.to_torch(device))
-    # empty tiles: the colour path leaves final_tau = 1 but colour 0, the weight path 0 and 0
-    diff = np.abs(N(img) - N(rgb))
-    assert diff.max() < 2e-4, diff.max()
+    # Equal in exact arithmetic (both paths see the same alpha' and tau per entry; empty tiles give 0 in
+    # both).  Bound: 1e-5 of sum_k |P_kc| sum_i tau_i alpha'_i sum_lm |Y_lm| |w_lm,k|, the magnitude of every
+    # term either side sums (rendered by the oracle with per-Gaussian weights sum_lm |Y_lm||w_lm,k|).
+    wabs = np.einsum("nl,nlk->nk", np.abs(o["basis"].astype(np.float64)),
+                     np.abs(shs_w.astype(np.float64))).astype(np.float32)
+    MW, _, _, _, _ = O.draw(cam.width, cam.height, o["ranges"], o["ids"], o["us"], o["cinv"], scene.alphas, wabs)
+    M = np.einsum("kc,khw->chw", np.abs(pal.astype(np.float64)), MW.astype(np.float64))
+    diff = np.abs(N(img).astype(np.float64) - N(rgb))
+    assert (diff <= 1e-5 * M).all(), float(np.max(diff / np.maximum(M, 1e-300)))
+    assert M.max() > 0.5
+
+
+@pytest.mark.parametrize("shape,w_off,rgb_off", [((5, 48, 64), 0, 0), ((5, 47, 63), 0, 0), ((5, 48, 64), 1, 0),
+                                                  ((5, 48, 64), 0, 3), ((7, 47, 63), 1, 2), ((5, 1080, 1920), 0, 0)])
+def test_mix_forward(device, shape, w_off, rgb_off):
+    """rgp_mix and rgp_mix_batched bit for bit against the oracle's mix
+    (gsmodel.py:488-509 semantics), with W and rgb at 4-byte offsets and odd
+    H*W: the float4 path only when H*W % 4 == 0 and both planes are 16-byte
+    aligned, the scalar one otherwise."""
+    from rgp._lib import I64, call, ptr, stream
+    print(f"case: mix forward {shape} W offset {w_off} rgb offset {rgb_off}", flush=True)
+    K, H, W = shape
+    rng = np.random.default_rng(5 + w_off + 7 * rgb_off)
+    Wn = rng.random(shape, dtype=np.float32)
+    P = rng.random((K, 3), dtype=np.float32)
+    ref = O.mix(P, Wn)
+    Wflat = torch.empty(Wn.size + w_off, dtype=torch.float32, device=device)
+    Wt = Wflat[w_off:].view(shape)
+    Wt.copy_(T(Wn, device))
+    outs = []
+    for _ in range(2):
+        oflat = torch.full((3 * H * W + rgb_off,), float("nan"), dtype=torch.float32, device=device)
+        outs.append(oflat[rgb_off:].view(3, H, W))
+    Pt = T(P, device)
+    call("rgp_mix", K, I64(H * W), ptr(Wt), ptr(Pt), ptr(outs[0]), stream(device))
+    # the batched form: the same image twice (second output), plus a 1-pixel image
+    tiny_w = T(Wn[:, :1, :1], device)
+    tiny_o = torch.empty((3, 1, 1), dtype=torch.float32, device=device)
+    args = torch.tensor([Wt.data_ptr(), tiny_w.data_ptr(), outs[1].data_ptr(), tiny_o.data_ptr(), H * W, 1],
+                        dtype=torch.int64, device=device)
+    pals = T(np.concatenate([P, P]), device)
+    base = args.data_ptr()
+    call("rgp_mix_batched", 2, K, base, base + 16, base + 32, ptr(pals), I64(H * W), stream(device))
+    torch.cuda.synchronize()
+    same(N(outs[0]), ref, "rgp_mix")
+    same(N(outs[1]), ref, "rgp_mix_batched")
+    same(N(tiny_o), O.mix(P, Wn[:, :1, :1].copy()), "rgp_mix_batched 1 pixel")
 
 
 @pytest.mark.parametrize("shape,offset", [((5, 48, 64), 0), ((5, 47, 63), 0), ((3, 48, 64), 1),

@@ -406,6 +406,22 @@ int rgp_bary_loss(int width, int height, int Kc, const float* image, const float
                   float lambda_pos, float lambda_alpha, float* w_bary, float* dw_tilde,
                   float* dpalette, float* loss, void* workspace, hipStream_t stream);
 
+/* rgp_bary_loss without the workspace argument.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The float64 accumulators live in a library-owned device array of
+ * (RGP_MAX_DIM - 2) * 16 + 3 doubles (792 bytes) per GPU, held until the
+ * process ends; its address is looked up on the first call on each device
+ * (make that first call outside a graph capture).  Kc above RGP_MAX_DIM - 2
+ * is refused.  The stream must belong to the current device.  Every call on
+ * one GPU shares that GPU's accumulators (cleared by a fill kernel at the
+ * start of each call), so calls that overlap on one GPU race.  Allocates
+ * nothing. */
+int rgp_bary_loss_noalloc(int width, int height, int Kc, const float* image, const float* w_tilde,
+                          const float* gt_alpha, const float* palette, float tau, int clamp,
+                          float lambda_pos, float lambda_alpha, float* w_bary, float* dw_tilde,
+                          float* dpalette, float* loss, hipStream_t stream);
+
 /* barycentric_coords_3d(C, P, tau) alone: w_bary (Kc+2,H,W). */
 int rgp_bary_coords(int width, int height, int Kc, const float* image, const float* palette,
                     float tau, float* w_bary, hipStream_t stream);

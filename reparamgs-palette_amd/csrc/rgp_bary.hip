@@ -24,6 +24,8 @@
 // sums over zero rows of the reference's matmuls and einsums are skipped:
 // every partial sum stays a non-negative-zero-free finite value, so adding a
 // signed zero product leaves it unchanged.
+#include <mutex>
+
 #include "rgp_internal.h"
 
 namespace rgp {
@@ -414,17 +416,15 @@ static int bary_args(const char* who, int width, int height, int Kc, float tau) 
   return RGP_OK;
 }
 
-extern "C" int rgp_bary_loss(int width, int height, int Kc, const float* image, const float* w_tilde,
-                             const float* gt_alpha, const float* palette, float tau, int clamp,
-                             float lambda_pos, float lambda_alpha, float* w_bary, float* dw_tilde,
-                             float* dpalette, float* loss, void* workspace, hipStream_t s) {
-  if (int rc = bary_args("rgp_bary_loss", width, height, Kc, tau)) return rc;
-  if (!image || !w_tilde || !palette || !dw_tilde || !dpalette || !workspace)
-    return set_error(RGP_ERR_ARG, "rgp_bary_loss: null buffer");
+// The loss pass over the image with its accumulators at acc (Kc * kBaryAcc + 3
+// doubles, cleared here by a fill kernel, not a memset node).
+static int bary_loss_launch(const char* who, int width, int height, int Kc, const float* image,
+                            const float* w_tilde, const float* gt_alpha, const float* palette, float tau,
+                            int clamp, float lambda_pos, float lambda_alpha, float* w_bary, float* dw_tilde,
+                            float* dpalette, float* loss, double* acc, hipStream_t s) {
   const int64_t hw = (int64_t)width * height, nk = hw * (Kc + 2);
   const float cg = (float)(2.0 / (double)nk), cp = (float)((double)lambda_pos / (double)nk);
   const float ca = (float)(2.0 * (double)lambda_alpha / (double)hw);
-  double* acc = (double*)workspace;
   const int nacc = Kc * kBaryAcc + 3;
   hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, s, acc, nacc);
   const int grid = bary_grid(hw);
@@ -441,7 +441,83 @@ extern "C" int rgp_bary_loss(int width, int height, int Kc, const float* image, 
   hipLaunchKernelGGL(k_bary_finish, dim3(1), dim3(64), 0, s, Kc, palette, acc, 1.0 / (double)nk,
                      1.0 / (double)hw, (double)lambda_pos, (double)lambda_alpha, gt_alpha ? 1 : 0,
                      dpalette, loss);
-  return check_launch("rgp_bary_loss");
+  return check_launch(who);
+}
+
+extern "C" int rgp_bary_loss(int width, int height, int Kc, const float* image, const float* w_tilde,
+                             const float* gt_alpha, const float* palette, float tau, int clamp,
+                             float lambda_pos, float lambda_alpha, float* w_bary, float* dw_tilde,
+                             float* dpalette, float* loss, void* workspace, hipStream_t s) {
+  if (int rc = bary_args("rgp_bary_loss", width, height, Kc, tau)) return rc;
+  if (!image || !w_tilde || !palette || !dw_tilde || !dpalette || !workspace)
+    return set_error(RGP_ERR_ARG, "rgp_bary_loss: null buffer");
+  return bary_loss_launch("rgp_bary_loss", width, height, Kc, image, w_tilde, gt_alpha, palette, tau, clamp,
+                          lambda_pos, lambda_alpha, w_bary, dw_tilde, dpalette, loss, (double*)workspace, s);
+}
+
+// ---------------------------------------------------------------------------
+// rgp_bary_loss without a workspace argument, for callers that cannot
+// allocate (code under HIP graph capture, tight serving loops, harnesses that
+// forbid hipMalloc).  Its float64 accumulators live in `scratch`, a
+// file-scope device array of the size the largest palette needs
+// ((RGP_MAX_DIM - 2) * kBaryAcc + 3 doubles, 792 bytes); every GPU the
+// library runs on holds its own copy until the process ends.  The host-side
+// name `scratch` is not a device address: the array's address on the current
+// device is looked up with hipGetSymbolAddress on the first call on that
+// device and kept per device, and only that pointer reaches kernels.  The
+// accumulators are cleared by a fill kernel at the start of every call, and
+// calls that overlap on one GPU race on them.
+// ---------------------------------------------------------------------------
+constexpr int kBaryScratchDoubles = (RGP_MAX_DIM - 2) * kBaryAcc + 3;
+static __device__ __align__(256) double scratch[kBaryScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The device address of `scratch` on the current device (which must own s).
+static int bary_scratch(const char* who, hipStream_t s, double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return set_error(RGP_ERR_ARG, "%s: no current device (or device index above %d)", who, kMaxDevices - 1);
+  if (s != nullptr) {
+    hipDevice_t sdev = -1;
+    if (hipStreamGetDevice(s, &sdev) != hipSuccess)
+      return set_error(RGP_ERR_ARG, "%s: cannot query the stream's device", who);
+    if (sdev != dev)
+      return set_error(RGP_ERR_ARG, "%s: the stream belongs to device %d, the current device is %d", who,
+                       (int)sdev, dev);
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return set_error(RGP_ERR_LAUNCH, "%s: hipGetSymbolAddress(scratch) on device %d: %s", who, dev,
+                       hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return RGP_OK;
+}
+
+extern "C" int rgp_bary_loss_noalloc(int width, int height, int Kc, const float* image, const float* w_tilde,
+                                     const float* gt_alpha, const float* palette, float tau, int clamp,
+                                     float lambda_pos, float lambda_alpha, float* w_bary, float* dw_tilde,
+                                     float* dpalette, float* loss, hipStream_t s) {
+  static const char* who = "rgp_bary_loss_noalloc";
+  if (int rc = bary_args(who, width, height, Kc, tau)) return rc;
+  if (Kc * kBaryAcc + 3 > kBaryScratchDoubles)
+    return set_error(RGP_ERR_ARG, "%s: Kc = %d needs %d accumulators, the scratch holds %d", who, Kc,
+                     Kc * kBaryAcc + 3, kBaryScratchDoubles);
+  if (!image || !w_tilde || !palette || !dw_tilde || !dpalette)
+    return set_error(RGP_ERR_ARG, "%s: null buffer", who);
+  double* acc = nullptr;
+  if (int rc = bary_scratch(who, s, &acc)) return rc;
+  return bary_loss_launch(who, width, height, Kc, image, w_tilde, gt_alpha, palette, tau, clamp, lambda_pos,
+                          lambda_alpha, w_bary, dw_tilde, dpalette, loss, acc, s);
 }
 
 extern "C" int rgp_bary_coords(int width, int height, int Kc, const float* image, const float* palette,

@@ -71,6 +71,7 @@ _SIGS = {
     "rgp_l1_ssim_backward": (I, [I, I, I, P, P, P, P, F, P, P, P]),
     "rgp_bary_workspace_bytes": (ctypes.c_size_t, [I]),
     "rgp_bary_loss": (I, [I, I, I, P, P, P, P, F, I, F, F, P, P, P, P, P, P]),
+    "rgp_bary_loss_noalloc": (I, [I, I, I, P, P, P, P, F, I, F, F, P, P, P, P, P]),
     "rgp_bary_coords": (I, [I, I, I, P, P, F, P, P]),
     "rgp_bary_coords_backward": (I, [I, I, I, P, P, F, P, P, P, P]),
 }

@@ -106,7 +106,7 @@ def measure_serving(args, dev):
 
 
 # bench.py's span names -> the kernel each one times in the default workload
-SPAN_KERNEL = {"draw_backward": "rgp::k_draw_bwd_replay<5>", "draw_forward": "rgp::k_draw_fwd<5, false>",
+SPAN_KERNEL = {"draw_backward": "rgp::k_draw_bwd_replay<5, true>", "draw_forward": "rgp::k_draw_fwd<5, false, true>",
                "sh_backward_adam": "rgp::k_sh_adam_full<5, 16>", "preprocess": "rgp::k_preprocess<5, 4>"}
 PMC_PROFILE = os.path.join(ROOT, "profiles", "pmc_finetune_step_1M_1080p_k5.json")
 

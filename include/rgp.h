@@ -153,10 +153,15 @@ int rgp_pack_records(int n, int dim, const float* us, const float* cinv2ds, cons
 /* draw_weight / draw_color (kernel.cu:194-372, gausplat.cu:89-100, 228-240):
  * per-tile front-to-back compositing of dim channels.  Writes every pixel:
  * layer (dim,H,W), contrib (H,W), final_tau (H,W); empty tiles get 0, 0 and
- * empty_tau (0 for splat_weight, gausplat.cu:178; 1 for splat_color, :39). */
+ * empty_tau (0 for splat_weight, gausplat.cu:178; 1 for splat_color, :39).
+ * strips (nullable, uint8, one per list position like ids): for every entry
+ * a tile composites, the mask of its four 16x4 row strips (bit j: rows
+ * 4j..4j+3 of the tile) in which some pixel took a contribution; entries after
+ * the batch of 64 in which the tile's pixels all stopped are not written.
+ * rgp_draw_backward_replay reads it back.  No reference counterpart. */
 int rgp_draw_forward(int width, int height, int dim, const int32_t* ranges, const int32_t* ids,
                      const float* records, float empty_tau, float* layer, int32_t* contrib,
-                     float* final_tau, hipStream_t stream);
+                     float* final_tau, uint8_t* strips, hipStream_t stream);
 
 /* rgp_draw_forward plus two counters added into counts (int64[2], caller
  * zero-fills): list entries evaluated per pixel before it stops, summed, and
@@ -205,10 +210,13 @@ int rgp_vanilla_chain(int n, int sh_dim, const float* Rcw, const float* dus, con
  * of the entry (the factor the forward applied to the features).  Needs no
  * contrib / final_tau; the contributing set is the forward's.  The values
  * differ from rgp_draw_backward only by the rounding of the reference's
- * division-recovered tau (kernel.cu:1371).  dfeat zero-filled by caller. */
+ * division-recovered tau (kernel.cu:1371).  dfeat zero-filled by caller.
+ * strips (nullable): the masks rgp_draw_forward wrote for the same ranges,
+ * ids and records; only those strips of each entry are evaluated (the same
+ * values, fewer evaluations).  NULL: every strip the entry can reach. */
 int rgp_draw_backward_replay(int width, int height, int dim, const int32_t* ranges,
-                             const int32_t* ids, const float* records, const float* dlayer,
-                             float* dfeat, hipStream_t stream);
+                             const int32_t* ids, const float* records, const uint8_t* strips,
+                             const float* dlayer, float* dfeat, hipStream_t stream);
 
 /* ---- fused paths (no reference counterpart as single calls; they produce
  * the same values as the reference chains they replace) ------------------ */
@@ -258,17 +266,20 @@ int rgp_bin_sort_views(int views, int n, const float* depths, int width, int hei
                        size_t ws_stride, int64_t capacity, int32_t* ids, int32_t* ranges, hipStream_t stream);
 
 /* rgp_draw_forward of every view in one launch (views x tiles blocks);
- * id_stride = capacity, n the Gaussian count (record stride). */
+ * id_stride = capacity, n the Gaussian count (record stride); strips
+ * (nullable) has the stride of ids. */
 int rgp_draw_forward_views(int views, int width, int height, int dim, const int32_t* ranges,
                            const int32_t* ids, int64_t id_stride, const float* records, int64_t n,
                            float empty_tau, float* layer, int32_t* contrib, float* final_tau,
-                           hipStream_t stream);
+                           uint8_t* strips, hipStream_t stream);
 
 /* rgp_draw_backward_replay of every view in one launch; view v adds into its
- * own rows dfeat + v*n*dim (zero-filled by the caller). */
+ * own rows dfeat + v*n*dim (zero-filled by the caller); strips (nullable) as
+ * rgp_draw_forward_views wrote them. */
 int rgp_draw_backward_replay_views(int views, int width, int height, int dim, const int32_t* ranges,
                                    const int32_t* ids, int64_t id_stride, const float* records, int64_t n,
-                                   const float* dlayer, float* dfeat, hipStream_t stream);
+                                   const uint8_t* strips, const float* dlayer, float* dfeat,
+                                   hipStream_t stream);
 
 /* The SH chains of the backward for a multi-view batch: the gradient of the
  * coefficients summed over the views, each view with its own basis

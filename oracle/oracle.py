@@ -356,18 +356,24 @@ def bin_sort(depths, rects, counts, width, height, dt=f32):
     return keys[:M], ids[:M], ranges
 
 
-def draw(width, height, ranges, ids, us, cinv, alphas, feat, final_tau_init=0.0, dt=f32):
+def draw(width, height, ranges, ids, us, cinv, alphas, feat, final_tau_init=0.0, dt=f32, strips=False):
     """draw_weight (feat (N,K)) / draw_color (feat (N,3)).  Returns
-    (layer (dim,H,W), contrib (H,W), final_tau (H,W), n_eval, n_contrib)."""
+    (layer (dim,H,W), contrib (H,W), final_tau (H,W), n_eval, n_contrib), and
+    with strips=True also the per-list-position masks of row strips (uint8,
+    bit (py mod 16) / 4) in which the entry contributed to some pixel."""
     feat = _c(feat, dt)
     dim = feat.shape[1]
     layer = np.zeros((dim, height, width), dt)
     contrib = np.zeros((height, width), i32)
     final_tau = np.full((height, width), final_tau_init, dt)
     ne, nc = ctypes.c_int64(0), ctypes.c_int64(0)
+    smask = np.zeros(max(len(ids), 1), np.uint8) if strips else None
     getattr(lib(), "orc_draw_" + _sfx(dt))(width, height, dim, _p(_c(ranges, i32)), _p(_c(ids, i32)), _p(_c(us, dt)),
                                            _p(_c(cinv, dt)), _p(_c(alphas, dt).reshape(-1)), _p(feat), _p(layer),
-                                           _p(contrib), _p(final_tau), ctypes.byref(ne), ctypes.byref(nc))
+                                           _p(contrib), _p(final_tau), ctypes.byref(ne), ctypes.byref(nc),
+                                           _p(smask) if strips else None)
+    if strips:
+        return layer, contrib, final_tau, ne.value, nc.value, smask[:len(ids)]
     return layer, contrib, final_tau, ne.value, nc.value
 
 

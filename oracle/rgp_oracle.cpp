@@ -604,12 +604,16 @@ static inline T maha_sq(T cx, T cy, T cz, T dx, T dy) {
 // with a non-empty list; the caller initialises the rest (final_tau 0 for the
 // weight path, gausplat.cu:178; 1 for the colour path, gausplat.cu:39).
 // Also returns per-pixel evaluation and contribution counts (for the flop
-// model of the benchmark).
+// model of the benchmark) and, when strips is not null, for every list
+// position the mask of the tile's 16x4 row strips (bit (py mod 16) / 4) in
+// which some pixel took a contribution from that entry (zero-filled by the
+// caller; the compositing kernels' record of which strips the replay backward
+// must evaluate).
 // ---------------------------------------------------------------------------
 template <typename T>
 void draw(int width, int height, int dim, const int32_t* ranges, const int32_t* ids,
           const T* us, const T* cinv, const T* alphas, const T* feat, T* layer,
-          int32_t* contrib, T* final_tau, int64_t* n_eval, int64_t* n_contrib) {
+          int32_t* contrib, T* final_tau, int64_t* n_eval, int64_t* n_contrib, uint8_t* strips) {
   const int gx = (width + BLOCK - 1) / BLOCK, gy = (height + BLOCK - 1) / BLOCK;
   const size_t HW = (size_t)width * height;
   int64_t tot_e = 0, tot_c = 0;
@@ -638,6 +642,7 @@ void draw(int width, int height, int dim, const int32_t* ranges, const int32_t* 
           // W_k += T alpha' w_k, the multiply-add contracted (one rounding)
           const T ta = tau * ap;
           for (int k = 0; k < dim; ++k) acc[k] = std::fma(ta, feat[(size_t)g * dim + k], acc[k]);
+          if (strips) strips[j] |= (uint8_t)(1u << ((py - ty * BLOCK) / 4));
           cont = cont_tmp;
           tau *= ((T)1 - ap);
           if (tau < (T)0.0001f) break;
@@ -1130,9 +1135,9 @@ void sh2weight_jac(int n, int K, int sh_dim, const T* shs, const T* pws, const T
   }                                                                                               \
   void orc_draw_##S(int w, int h, int dim, const int32_t* ranges, const int32_t* ids,             \
                     const T* us, const T* cinv, const T* alphas, const T* feat, T* layer,         \
-                    int32_t* contrib, T* final_tau, int64_t* ne, int64_t* nc) {                   \
+                    int32_t* contrib, T* final_tau, int64_t* ne, int64_t* nc, uint8_t* strips) {  \
     orc::draw<T>(w, h, dim, ranges, ids, us, cinv, alphas, feat, layer, contrib, final_tau, ne,  \
-                 nc);                                                                             \
+                 nc, strips);                                                                     \
   }                                                                                               \
   void orc_draw_backward_##S(int w, int h, int K, const int32_t* ranges, const int32_t* ids,      \
                              const T* us, const T* cinv, const T* alphas, const int32_t* contrib, \

@@ -578,25 +578,174 @@ __device__ __forceinline__ void tb_range(int tile, int nb, const uint32_t* __res
   *end = tbpart[(size_t)tile * nb + nb - 1];
 }
 
-// One block per tile: its range, and the sort of its ids when the list has
-// at most kTbSortSmall entries; longer lists go to the queue of k_tb_sort_big.
-__global__ __launch_bounds__(256) void k_tb_sort(int tiles, int nb, uint32_t* __restrict__ hdr,
-                                                 const uint32_t* __restrict__ tbpart,
-                                                 const float* __restrict__ depths,
-                                                 int32_t* __restrict__ ids,
-                                                 uint64_t* __restrict__ keys_out,
-                                                 int2* __restrict__ ranges,
-                                                 uint32_t* __restrict__ bigq) {
-  __shared__ uint64_t s[kTbSortSmall];
+// ---------------------------------------------------------------------------
+// In-register bitonic sort of one tile list by one wavefront.  Each lane holds
+// KPL keys; key index i = lane * KPL + r (r the register), so compare-exchange
+// stages with index stride below KPL run inside a lane's registers and the
+// others exchange registers between lanes (DPP or swizzle within a row of 16,
+// ds_bpermute across rows).  The network is tb_bitonic's (every comparator
+// puts the smaller key at the lower index: a "flip" with partner i ^ (k - 1)
+// per block size k, then half-cleaners with partner i ^ j), over P = 64 KPL
+// slots; slots past the list hold ~0 and stay last.  No LDS and no barrier.
+// ---------------------------------------------------------------------------
+template <int M>
+__device__ __forceinline__ uint32_t lane_xor_u32(uint32_t v) {
+  if constexpr (M == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+  else if constexpr (M == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+  else if constexpr (M == 3) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x1B, 0xF, 0xF, false);
+  else if constexpr (M == 7) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);
+  else if constexpr (M == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
+  else if constexpr (M == 15) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);
+  else if constexpr (M < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, (M << 10) | 0x1F);
+  else return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((threadIdx.x ^ M) << 2), (int)v);
+}
+
+template <int M>
+__device__ __forceinline__ uint64_t lane_xor_u64(uint64_t v) {
+  return ((uint64_t)lane_xor_u32<M>((uint32_t)(v >> 32)) << 32) | lane_xor_u32<M>((uint32_t)v);
+}
+
+__device__ __forceinline__ void reg_cx(uint64_t& a, uint64_t& b) {
+  const bool sw = b < a;
+  const uint64_t lo = sw ? b : a, hi = sw ? a : b;
+  a = lo;
+  b = hi;
+}
+
+// Flip stage of block size K (K > KPL): register r pairs with register
+// KPL-1-r of lane ^ (K/KPL - 1); the lane whose index bit K/2 is set keeps
+// the larger key.
+template <int KPL, int K>
+__device__ __forceinline__ void wave_flip(uint64_t (&k)[KPL], int lane) {
+  constexpr int LM = K / KPL - 1;
+  const bool upper = (lane & ((K / KPL) >> 1)) != 0;
+#pragma unroll
+  for (int r = 0; r < (KPL + 1) / 2; ++r) {
+    const int q = KPL - 1 - r;
+    const uint64_t tr = lane_xor_u64<LM>(k[q]);  // partner's register r's pair
+    const uint64_t tq = r == q ? tr : lane_xor_u64<LM>(k[r]);
+    // branch-free: the lower lane keeps the smaller key, the upper one the larger (keys are distinct
+    // except the ~0 padding, for which either choice is the same value)
+    const bool take_r = (tr < k[r]) != upper;
+    const bool take_q = (tq < k[q]) != upper;
+    k[r] = take_r ? tr : k[r];
+    if (r != q) k[q] = take_q ? tq : k[q];
+  }
+}
+
+// Half-cleaner of index stride J (J >= KPL): register r pairs with register r
+// of lane ^ (J/KPL); the lane with that bit set keeps the larger key.
+template <int KPL, int J>
+__device__ __forceinline__ void wave_half(uint64_t (&k)[KPL], int lane) {
+  constexpr int LM = J / KPL;
+  const bool upper = (lane & LM) != 0;
+#pragma unroll
+  for (int r = 0; r < KPL; ++r) {
+    const uint64_t t = lane_xor_u64<LM>(k[r]);
+    const bool take = (t < k[r]) != upper;
+    k[r] = take ? t : k[r];
+  }
+}
+
+template <int KPL, int K, int J>
+__device__ __forceinline__ void wave_halves(uint64_t (&k)[KPL], int lane) {
+  if constexpr (J >= 1) {
+    if constexpr (J >= KPL) {
+      wave_half<KPL, J>(k, lane);
+    } else {
+#pragma unroll
+      for (int r = 0; r < KPL; ++r)
+        if ((r & J) == 0) reg_cx(k[r], k[r | J]);
+    }
+    wave_halves<KPL, K, J / 2>(k, lane);
+  }
+}
+
+template <int KPL, int K>
+__device__ __forceinline__ void wave_bitonic_from(uint64_t (&k)[KPL], int lane) {
+  if constexpr (K <= 64 * KPL) {
+    if constexpr (K <= KPL) {
+#pragma unroll
+      for (int r = 0; r < KPL; ++r) {
+        const int q = r ^ (K - 1);
+        if (r < q) reg_cx(k[r], k[q]);
+      }
+    } else {
+      wave_flip<KPL, K>(k, lane);
+    }
+    wave_halves<KPL, K, K / 4>(k, lane);
+    wave_bitonic_from<KPL, K * 2>(k, lane);
+  }
+}
+
+// Sort the tile list ids[start, start + n) (n <= 64 KPL) by (depth_q, id).
+template <int KPL>
+__device__ __forceinline__ void tb_wave_sort(const float* __restrict__ depths, int32_t* __restrict__ ids,
+                                             uint64_t* __restrict__ keys_out, uint64_t thi, uint32_t start,
+                                             uint32_t n, int lane) {
+  uint64_t k[KPL];
+#pragma unroll
+  for (int r = 0; r < KPL; ++r) {
+    const uint32_t e = (uint32_t)(r * 64 + lane);  // coalesced loads; any placement is sorted alike
+    k[r] = e < n ? tb_key(depths, ids[start + e]) : ~0ull;
+  }
+  wave_bitonic_from<KPL, 2>(k, lane);
+#pragma unroll
+  for (int r = 0; r < KPL; ++r) {
+    const uint32_t i = (uint32_t)(lane * KPL + r);
+    if (i < n) {
+      ids[start + i] = (int32_t)(uint32_t)k[r];
+      if (keys_out) keys_out[start + i] = thi | (k[r] >> 32);
+    }
+  }
+}
+
+// One wavefront per tile: its range, and the sort of its ids when the list
+// has at most kTbSortReg entries (in registers, KPL the smallest power of two
+// with 64 KPL >= n).  Longer lists are left to k_tb_sort_mid.
+__global__ __launch_bounds__(64) void k_tb_sort(int tiles, int nb, const uint32_t* __restrict__ hdr,
+                                                const uint32_t* __restrict__ tbpart,
+                                                const float* __restrict__ depths,
+                                                int32_t* __restrict__ ids,
+                                                uint64_t* __restrict__ keys_out,
+                                                int2* __restrict__ ranges) {
+  static_assert(kTbSortReg == 64 * 16, "at most 16 keys per lane");
   const int tile = blockIdx.x;
+  const int lane = threadIdx.x;
   const uint32_t M = hdr[kHdrMEff];
   uint32_t start, end;
   tb_range(tile, nb, tbpart, &start, &end);
   // empty tiles keep (0,0), and so does every tile when M == 1 (kernel.cu:165-187)
-  if (threadIdx.x == 0)
-    ranges[tile] = (M <= 1 || end == start) ? make_int2(0, 0) : make_int2((int)start, (int)end);
+  if (lane == 0) ranges[tile] = (M <= 1 || end == start) ? make_int2(0, 0) : make_int2((int)start, (int)end);
   if (M == 0 || end == start) return;
   const uint32_t n = end - start;
+  if (n > (uint32_t)kTbSortReg) return;
+  const uint64_t thi = (uint64_t)tile << 32;
+  if (n <= 64) tb_wave_sort<1>(depths, ids, keys_out, thi, start, n, lane);
+  else if (n <= 128) tb_wave_sort<2>(depths, ids, keys_out, thi, start, n, lane);
+  else if (n <= 256) tb_wave_sort<4>(depths, ids, keys_out, thi, start, n, lane);
+  else if (n <= 512) tb_wave_sort<8>(depths, ids, keys_out, thi, start, n, lane);
+  else tb_wave_sort<16>(depths, ids, keys_out, thi, start, n, lane);
+}
+
+// Lists of kTbSortReg + 1 .. kTbSortSmall entries: one 256-thread block per
+// tile, the bitonic network in LDS (at these lengths a wavefront would need 32
+// keys per lane, about 200 registers, and ran slower); lists longer than
+// kTbSortSmall go to the queue of k_tb_sort_big.  Blocks of other tiles exit
+// at once.
+__global__ __launch_bounds__(256) void k_tb_sort_mid(int tiles, int nb, uint32_t* __restrict__ hdr,
+                                                     const uint32_t* __restrict__ tbpart,
+                                                     const float* __restrict__ depths,
+                                                     int32_t* __restrict__ ids,
+                                                     uint64_t* __restrict__ keys_out,
+                                                     uint32_t* __restrict__ bigq) {
+  __shared__ uint64_t s[kTbSortSmall];
+  const int tile = blockIdx.x;
+  if (hdr[kHdrMEff] == 0) return;
+  uint32_t start, end;
+  tb_range(tile, nb, tbpart, &start, &end);
+  const uint32_t n = end - start;
+  if (n <= (uint32_t)kTbSortReg) return;
   if (n > (uint32_t)kTbSortSmall) {
     if (threadIdx.x == 0) bigq[atomicAdd(&hdr[kHdrBig], 1u)] = (uint32_t)tile;
     return;
@@ -775,9 +924,11 @@ extern "C" int rgp_bin_sort(int n, const float* depths, int width, int height, v
       hipLaunchKernelGGL(k_tb_scatter, dim3(L.nb), dim3(kTbThreads), (size_t)L.tiles * 4, s, n, L.chunk,
                          L.tiles, L.nb, L.gx, (const uint32_t*)(base + L.counts),
                          (const uint2*)(base + L.rects), part, hdr, ids_out);
+    hipLaunchKernelGGL(k_tb_sort, dim3(L.tiles), dim3(64), 0, s, L.tiles, L.nb, (const uint32_t*)hdr, part,
+                       depths, ids_out, keys_out, (int2*)ranges_out);
     uint32_t* bigq = (uint32_t*)(base + L.bigq);
-    hipLaunchKernelGGL(k_tb_sort, dim3(L.tiles), dim3(256), 0, s, L.tiles, L.nb, hdr_w, part, depths,
-                       ids_out, keys_out, (int2*)ranges_out, bigq);
+    hipLaunchKernelGGL(k_tb_sort_mid, dim3(L.tiles), dim3(256), 0, s, L.tiles, L.nb, hdr_w, part, depths, ids_out,
+                       keys_out, bigq);
     hipLaunchKernelGGL(k_tb_sort_big, dim3(kTbBigBlocks), dim3(256), 0, s, L.nb, (const uint32_t*)hdr, part,
                        (const uint32_t*)bigq, depths, tkeys, ids_out, keys_out);
     return check_launch("rgp_bin_sort");

@@ -52,8 +52,8 @@ constexpr int kTbThreads = 1024;     // threads of the count / scatter blocks
 constexpr int kTbMinChunk = 4096;    // Gaussians per count / scatter block, at least
 constexpr int kTbMaxBlocks = 256;
 constexpr int kTbBig = 32;           // rects above this many tiles are walked per wavefront
-constexpr int kTbSortSmall = 2048;   // lists sorted by the one-block-per-tile kernel (16 KiB of
-                                     // LDS: 8 blocks of 256 threads still fit a CU)
+constexpr int kTbSortReg = 1024;     // lists sorted in registers by the one-wavefront-per-tile kernel
+constexpr int kTbSortSmall = 2048;   // lists sorted in LDS by the one-block-per-tile kernel (16 KiB)
 constexpr int kTbSortLds = 4096;     // keys of one long list sorted in LDS; longer lists
                                      // are merged through global memory
 constexpr int kTbBigBlocks = 1024;   // blocks of the long-list sort (grid-stride over its queue)

@@ -101,14 +101,19 @@ __device__ __forceinline__ uint32_t live_slots(const float* tau) {
 // ---------------------------------------------------------------------------
 // forward.  COUNT: also accumulate, over the launch, the list entries each
 // pixel evaluates before it stops (counts[0]) and its contributions
-// (counts[1]).
+// (counts[1]).  STRIPS: also write, for every list entry the tile composites
+// (every entry of each batch it stages), the mask of the tile's four row
+// strips in which some pixel took a contribution from it (strips[e], one
+// byte per list position).  The replay backward, which repeats exactly this
+// arithmetic, then evaluates only those strips.
 // ---------------------------------------------------------------------------
-template <int DIM, bool COUNT>
+template <int DIM, bool COUNT, bool STRIPS = false>
 __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, const int2* __restrict__ ranges,
                                               const int32_t* __restrict__ ids, const float4* __restrict__ recs,
                                               float empty_tau, float* __restrict__ layer,
                                               int32_t* __restrict__ contrib, float* __restrict__ final_tau,
-                                              unsigned long long* __restrict__ counts, float4* s_rec) {
+                                              unsigned long long* __restrict__ counts,
+                                              uint8_t* __restrict__ strips, float4* s_rec) {
   constexpr int F4 = RecStride<DIM>::f4;
   const int ty = tile / gx, tx = tile - ty * gx;
   const int lane = threadIdx.x;
@@ -143,6 +148,9 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
       __syncthreads();
       nxt = load_rec<F4>(recs, ids, b0 + 64 + lane, rg.x, rg.y, &nid);
       const int cnt = min(64, rg.y - b0);
+      uint64_t sbits[kSlots];  // STRIPS: bit jj of sbits[j] = entry b0 + jj contributed in strip j (uniform)
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) sbits[j] = 0;
       for (int jj = 0; jj < cnt; ++jj) {
         const float4 g0 = s_rec[jj];
         const float4 g1 = s_rec[64 + jj];
@@ -151,6 +159,7 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
         const float dx = g0.x - fpx;
         const float ta0 = g0.z * dx * dx;  // (cinv.x dx) dx
         const float tc0 = g0.w * dx;       // (2 cinv.y) dx
+        const int idx = b0 + jj - rg.x + 1;  // contrib value of a pixel this entry contributes to
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) {
           if (!(slots & (1u << j))) continue;
@@ -168,13 +177,26 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
             if (4 * q + 2 < DIM) acc[j][4 * q + 2] = fmaf(ta, f.z, acc[j][4 * q + 2]);
             if (4 * q + 3 < DIM) acc[j][4 * q + 3] = fmaf(ta, f.w, acc[j][4 * q + 3]);
           }
-          cont[j] = b0 + jj - rg.x + 1;
+          cont[j] = idx;
           tau[j] = tau[j] * (1.f - ap);
           if (COUNT) {
             ++n_con[j];
             if (tau[j] < 0.0001f) n_eval[j] = cont[j];
           }
         }
+        if (STRIPS) {
+          // contrib only grows, so it equals idx exactly where this entry contributed (no branch:
+          // a compare per strip and scalar selects)
+          const uint64_t bit = 1ull << jj;
+#pragma unroll
+          for (int j = 0; j < kSlots; ++j) sbits[j] |= __builtin_amdgcn_ballot_w64(cont[j] == idx) != 0 ? bit : 0ull;
+        }
+      }
+      if (STRIPS && lane < cnt) {
+        uint32_t mine = 0;
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) mine |= (uint32_t)((sbits[j] >> lane) & 1ull) << j;
+        strips[b0 + lane] = (uint8_t)mine;
       }
     }
   }
@@ -210,7 +232,7 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
   }
 }
 
-template <int DIM, bool COUNT = false>
+template <int DIM, bool COUNT = false, bool STRIPS = false>
 __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles,
                                                  const int2* __restrict__ ranges,
                                                  const int32_t* __restrict__ ids,
@@ -218,31 +240,35 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
                                                  float* __restrict__ layer,
                                                  int32_t* __restrict__ contrib,
                                                  float* __restrict__ final_tau,
-                                                 unsigned long long* __restrict__ counts) {
+                                                 unsigned long long* __restrict__ counts,
+                                                 uint8_t* __restrict__ strips) {
   __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
-  draw_fwd_tile<DIM, COUNT>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau, layer, contrib,
-                            final_tau, counts, s_rec);
+  draw_fwd_tile<DIM, COUNT, STRIPS>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau, layer,
+                                    contrib, final_tau, counts, strips, s_rec);
 }
 
 // `views` images of one resolution in one launch: block b composites tile
 // t % tiles of view t / tiles (t the XCD-banded position of b); view v reads
 // ranges + v*tiles, ids + v*id_stride, recs + v*rec_stride (float4 units)
-// and writes the v-th (DIM,H,W) layer and (H,W) contrib / final_tau.
-template <int DIM>
+// and writes the v-th (DIM,H,W) layer and (H,W) contrib / final_tau (and,
+// STRIPS, its strip masks at strips + v*id_stride).
+template <int DIM, bool STRIPS = false>
 __global__ __launch_bounds__(64) void k_draw_fwd_views(int W, int H, int gx, int tiles, int views,
                                                        const int2* __restrict__ ranges,
                                                        const int32_t* __restrict__ ids, int64_t id_stride,
                                                        const float4* __restrict__ recs, int64_t rec_stride,
                                                        float empty_tau, float* __restrict__ layer,
                                                        int32_t* __restrict__ contrib,
-                                                       float* __restrict__ final_tau) {
+                                                       float* __restrict__ final_tau,
+                                                       uint8_t* __restrict__ strips) {
   __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
   const int t = xcd_tile(blockIdx.x, tiles * views);
   const int v = t / tiles, tile = t - v * tiles;
   const size_t HW = (size_t)W * H;
-  draw_fwd_tile<DIM, false>(W, H, gx, tile, ranges + (size_t)v * tiles, ids + v * id_stride, recs + v * rec_stride,
-                            empty_tau, layer + (size_t)v * DIM * HW, contrib + v * HW, final_tau + v * HW, nullptr,
-                            s_rec);
+  draw_fwd_tile<DIM, false, STRIPS>(W, H, gx, tile, ranges + (size_t)v * tiles, ids + v * id_stride,
+                                    recs + v * rec_stride, empty_tau, layer + (size_t)v * DIM * HW,
+                                    contrib + v * HW, final_tau + v * HW, nullptr,
+                                    STRIPS ? strips + v * id_stride : nullptr, s_rec);
 }
 
 // Adds the four-slot lane sums of an entry's gradient terms into dfeat: a
@@ -499,12 +525,17 @@ __global__ __launch_bounds__(64) void k_draw_color_bwd(
 // transmittance in front of it, the factor the forward applied to the
 // weights, so no per-pixel state from the forward is read.  The set of
 // contributing (pixel, entry) pairs is the forward's (same stop rule).
+// STRIPS: the forward's strip masks (draw_fwd_tile) replace the reachable
+// strips of each entry.  The replay repeats the forward's arithmetic, so a
+// strip without a contribution there has none here and is skipped, and an
+// entry with a nonzero mask always has a term to flush.
 // ---------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, bool STRIPS>
 __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int tile,
                                                      const int2* __restrict__ ranges,
                                                      const int32_t* __restrict__ ids,
                                                      const float4* __restrict__ recs,
+                                                     const uint8_t* __restrict__ strips,
                                                      const float* __restrict__ dlayer,
                                                      float* __restrict__ dfeat, float4* s_rec, int32_t* s_id) {
   constexpr int F4 = RecStride<DIM>::f4;
@@ -539,14 +570,24 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
   for (int k = 0; k < DIM; ++k) part[k] = 0.f;
   int nid = 0;
   RecRegs<F4> nxt = load_rec<F4>(recs, ids, rg.x + lane, rg.x, rg.y, &nid);
+  uint32_t nstr = 0;
+  if (STRIPS && rg.x + lane < rg.y) nstr = strips[rg.x + lane];
   for (int b0 = rg.x; b0 < rg.y; b0 += 64) {
     const uint32_t live = live_slots(tau);
     if (live == 0) break;
     __syncthreads();
-    stage_rec<F4>(s_rec, lane, nxt, ty);
+    if (STRIPS) {
+      s_rec[lane] = nxt.v[0];
+      s_rec[64 + lane] = make_float4(nxt.v[1].x, nxt.v[1].y, nxt.v[1].z, __uint_as_float(nstr));
+#pragma unroll
+      for (int q = 2; q < F4; ++q) s_rec[q * 64 + lane] = nxt.v[q];
+    } else {
+      stage_rec<F4>(s_rec, lane, nxt, ty);
+    }
     s_id[lane] = nid;
     __syncthreads();
     nxt = load_rec<F4>(recs, ids, b0 + 64 + lane, rg.x, rg.y, &nid);
+    if (STRIPS) nstr = b0 + 64 + lane < rg.y ? strips[b0 + 64 + lane] : 0u;
     const int cnt = min(64, rg.y - b0);
     for (int jj = 0; jj < cnt; ++jj) {
       const float4 g0 = s_rec[jj];
@@ -569,9 +610,9 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
 #pragma unroll
         for (int k = 0; k < DIM; ++k) part[k] = fmaf(dl[j][k], ta, part[k]);
         tau[j] = tau[j] * (1.f - ap);
-        on = true;
+        if (!STRIPS) on = true;
       }
-      if (__any(on)) {
+      if (STRIPS || __any(on)) {
         flush_grad<DIM>(part, dfeat, s_id[jj], lane, ch, owner);
 #pragma unroll
         for (int k = 0; k < DIM; ++k) part[k] = 0.f;
@@ -580,26 +621,30 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
   }
 }
 
-template <int DIM>
+template <int DIM, bool STRIPS>
 __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, int tiles,
                                                         const int2* __restrict__ ranges,
                                                         const int32_t* __restrict__ ids,
                                                         const float4* __restrict__ recs,
+                                                        const uint8_t* __restrict__ strips,
                                                         const float* __restrict__ dlayer,
                                                         float* __restrict__ dfeat) {
   __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
   __shared__ int32_t s_id[64];
-  draw_bwd_replay_tile<DIM>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, dlayer, dfeat, s_rec, s_id);
+  draw_bwd_replay_tile<DIM, STRIPS>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, strips, dlayer, dfeat,
+                                    s_rec, s_id);
 }
 
 // The replay backward of `views` images in one launch, laid out as in
-// k_draw_fwd_views; view v reads dlayer + v*DIM*H*W and adds into its own
-// gradient rows dfeat + v*n*DIM (the SH chain behind them depends on the view).
-template <int DIM>
+// k_draw_fwd_views; view v reads dlayer + v*DIM*H*W (and its strip masks at
+// strips + v*id_stride) and adds into its own gradient rows dfeat + v*n*DIM
+// (the SH chain behind them depends on the view).
+template <int DIM, bool STRIPS>
 __global__ __launch_bounds__(64) void k_draw_bwd_replay_views(int W, int H, int gx, int tiles, int views,
                                                               const int2* __restrict__ ranges,
                                                               const int32_t* __restrict__ ids, int64_t id_stride,
                                                               const float4* __restrict__ recs, int64_t rec_stride,
+                                                              const uint8_t* __restrict__ strips,
                                                               const float* __restrict__ dlayer,
                                                               float* __restrict__ dfeat, int64_t n) {
   __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
@@ -607,8 +652,9 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay_views(int W, int H, int 
   const int t = xcd_tile(blockIdx.x, tiles * views);
   const int v = t / tiles, tile = t - v * tiles;
   const size_t HW = (size_t)W * H;
-  draw_bwd_replay_tile<DIM>(W, H, gx, tile, ranges + (size_t)v * tiles, ids + v * id_stride, recs + v * rec_stride,
-                            dlayer + (size_t)v * DIM * HW, dfeat + (size_t)v * n * DIM, s_rec, s_id);
+  draw_bwd_replay_tile<DIM, STRIPS>(W, H, gx, tile, ranges + (size_t)v * tiles, ids + v * id_stride,
+                                    recs + v * rec_stride, STRIPS ? strips + v * id_stride : nullptr,
+                                    dlayer + (size_t)v * DIM * HW, dfeat + (size_t)v * n * DIM, s_rec, s_id);
 }
 
 // ---------------------------------------------------------------------------
@@ -807,14 +853,21 @@ static int draw_grid(const char* who, int width, int height, int dim, int* gx, i
 
 extern "C" int rgp_draw_forward(int width, int height, int dim, const int32_t* ranges,
                                 const int32_t* ids, const float* records, float empty_tau,
-                                float* layer, int32_t* contrib, float* final_tau,
+                                float* layer, int32_t* contrib, float* final_tau, uint8_t* strips,
                                 hipStream_t s) {
   int gx, tiles;
   if (int rc = draw_grid("rgp_draw_forward", width, height, dim, &gx, &tiles)) return rc;
-  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false>), dim3(tiles), dim3(64), 0, s,
-                                         width, height, gx, tiles, (const int2*)ranges, ids,
-                                         (const float4*)records, empty_tau, layer, contrib,
-                                         final_tau, (unsigned long long*)nullptr));
+  if (strips) {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false, true>), dim3(tiles), dim3(64), 0, s,
+                                           width, height, gx, tiles, (const int2*)ranges, ids,
+                                           (const float4*)records, empty_tau, layer, contrib,
+                                           final_tau, (unsigned long long*)nullptr, strips));
+  } else {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false, false>), dim3(tiles), dim3(64), 0, s,
+                                           width, height, gx, tiles, (const int2*)ranges, ids,
+                                           (const float4*)records, empty_tau, layer, contrib,
+                                           final_tau, (unsigned long long*)nullptr, (uint8_t*)nullptr));
+  }
   return check_launch("rgp_draw_forward");
 }
 
@@ -829,25 +882,41 @@ static int views_grid(const char* who, int views, int width, int height, int dim
 extern "C" int rgp_draw_forward_views(int views, int width, int height, int dim, const int32_t* ranges,
                                       const int32_t* ids, int64_t id_stride, const float* records,
                                       int64_t n, float empty_tau, float* layer, int32_t* contrib,
-                                      float* final_tau, hipStream_t s) {
+                                      float* final_tau, uint8_t* strips, hipStream_t s) {
   int gx, tiles;
   if (int rc = views_grid("rgp_draw_forward_views", views, width, height, dim, id_stride, &gx, &tiles)) return rc;
-  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL(k_draw_fwd_views<DD>, dim3(tiles * views), dim3(64), 0, s, width,
-                                         height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
-                                         (const float4*)records, n * RecStride<DD>::f4, empty_tau, layer,
-                                         contrib, final_tau));
+  if (strips) {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd_views<DD, true>), dim3(tiles * views), dim3(64), 0, s,
+                                           width, height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
+                                           (const float4*)records, n * RecStride<DD>::f4, empty_tau, layer,
+                                           contrib, final_tau, strips));
+  } else {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd_views<DD, false>), dim3(tiles * views), dim3(64), 0, s,
+                                           width, height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
+                                           (const float4*)records, n * RecStride<DD>::f4, empty_tau, layer,
+                                           contrib, final_tau, (uint8_t*)nullptr));
+  }
   return check_launch("rgp_draw_forward_views");
 }
 
 extern "C" int rgp_draw_backward_replay_views(int views, int width, int height, int dim, const int32_t* ranges,
                                               const int32_t* ids, int64_t id_stride, const float* records,
-                                              int64_t n, const float* dlayer, float* dfeat, hipStream_t s) {
+                                              int64_t n, const uint8_t* strips, const float* dlayer,
+                                              float* dfeat, hipStream_t s) {
   int gx, tiles;
   if (int rc = views_grid("rgp_draw_backward_replay_views", views, width, height, dim, id_stride, &gx, &tiles))
     return rc;
-  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL(k_draw_bwd_replay_views<DD>, dim3(tiles * views), dim3(64), 0, s,
-                                         width, height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
-                                         (const float4*)records, n * RecStride<DD>::f4, dlayer, dfeat, n));
+  if (strips) {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_bwd_replay_views<DD, true>), dim3(tiles * views), dim3(64), 0,
+                                           s, width, height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
+                                           (const float4*)records, n * RecStride<DD>::f4, strips, dlayer, dfeat,
+                                           n));
+  } else {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_bwd_replay_views<DD, false>), dim3(tiles * views), dim3(64), 0,
+                                           s, width, height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
+                                           (const float4*)records, n * RecStride<DD>::f4,
+                                           (const uint8_t*)nullptr, dlayer, dfeat, n));
+  }
   return check_launch("rgp_draw_backward_replay_views");
 }
 
@@ -857,10 +926,10 @@ extern "C" int rgp_draw_forward_counts(int width, int height, int dim, const int
                                        int64_t* counts, hipStream_t s) {
   int gx, tiles;
   if (int rc = draw_grid("rgp_draw_forward_counts", width, height, dim, &gx, &tiles)) return rc;
-  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, true>), dim3(tiles), dim3(64), 0, s,
+  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, true, false>), dim3(tiles), dim3(64), 0, s,
                                          width, height, gx, tiles, (const int2*)ranges, ids,
                                          (const float4*)records, empty_tau, layer, contrib,
-                                         final_tau, (unsigned long long*)counts));
+                                         final_tau, (unsigned long long*)counts, (uint8_t*)nullptr));
   return check_launch("rgp_draw_forward_counts");
 }
 
@@ -892,13 +961,19 @@ extern "C" int rgp_draw_color_backward(int width, int height, const int32_t* ran
 }
 
 extern "C" int rgp_draw_backward_replay(int width, int height, int dim, const int32_t* ranges,
-                                        const int32_t* ids, const float* records,
+                                        const int32_t* ids, const float* records, const uint8_t* strips,
                                         const float* dlayer, float* dfeat, hipStream_t s) {
   int gx, tiles;
   if (int rc = draw_grid("rgp_draw_backward_replay", width, height, dim, &gx, &tiles)) return rc;
-  RGP_DIM_SWITCH(dim, hipLaunchKernelGGL(k_draw_bwd_replay<DD>, dim3(tiles), dim3(64), 0, s,
-                                         width, height, gx, tiles, (const int2*)ranges, ids,
-                                         (const float4*)records, dlayer, dfeat));
+  if (strips) {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_bwd_replay<DD, true>), dim3(tiles), dim3(64), 0, s,
+                                           width, height, gx, tiles, (const int2*)ranges, ids,
+                                           (const float4*)records, strips, dlayer, dfeat));
+  } else {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_bwd_replay<DD, false>), dim3(tiles), dim3(64), 0, s,
+                                           width, height, gx, tiles, (const int2*)ranges, ids,
+                                           (const float4*)records, (const uint8_t*)nullptr, dlayer, dfeat));
+  }
   return check_launch("rgp_draw_backward_replay");
 }
 

@@ -189,7 +189,7 @@ def _splat(height, width, dim, us, cinv2ds, alphas, depths, feat, areas, empty_t
         call("rgp_bin_rects", n, ptr(us_c), ptr(a), ptr(d), int(width), int(height), ptr(ws), I64(int(cap)),
              stream(dev))
 
-    layer, contrib, final_tau, ranges, ids, m, _ = ops.splat_pipeline(
+    layer, contrib, final_tau, ranges, ids, m, _, _ = ops.splat_pipeline(
         n, int(width), int(height), dim, records, d, fill, empty_tau, dev, exact=True)
     return [layer, contrib, final_tau, ranges, ids[:m].clone()]
 

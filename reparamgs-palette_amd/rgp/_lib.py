@@ -38,8 +38,8 @@ _SIGS = {
     "rgp_preprocess_weight_views": (I, [I, I, I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, SZ, I64, I, I, P]),
     "rgp_bin_scan_views": (I, [I, I, I, I, P, SZ, I64, P]),
     "rgp_bin_sort_views": (I, [I, I, P, I, I, P, SZ, I64, P, P, P]),
-    "rgp_draw_forward_views": (I, [I, I, I, I, P, P, I64, P, I64, F, P, P, P, P]),
-    "rgp_draw_backward_replay_views": (I, [I, I, I, I, P, P, I64, P, I64, P, P, P]),
+    "rgp_draw_forward_views": (I, [I, I, I, I, P, P, I64, P, I64, F, P, P, P, P, P]),
+    "rgp_draw_backward_replay_views": (I, [I, I, I, I, P, P, I64, P, I64, P, P, P, P]),
     "rgp_sh_backward_full_views": (I, [I, I, I, I, P, P, P, P, P]),
     "rgp_sh_backward_adam_split": (I, [I, I, P, P, P, P, P, P, P, P, P, I64, D, D, D, D, D, P]),
     "rgp_sh_backward_adam_lowrank": (I, [I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I64, D, D, D, D, D,
@@ -51,10 +51,10 @@ _SIGS = {
     "rgp_bin_header": (P, [P]),
     "rgp_record_floats": (I, [I]),
     "rgp_pack_records": (I, [I, I, P, P, P, P, P, P]),
-    "rgp_draw_forward": (I, [I, I, I, P, P, P, F, P, P, P, P]),
+    "rgp_draw_forward": (I, [I, I, I, P, P, P, F, P, P, P, P, P]),
     "rgp_draw_forward_counts": (I, [I, I, I, P, P, P, F, P, P, P, P, P]),
     "rgp_draw_backward": (I, [I, I, I, P, P, P, P, P, P, P, P]),
-    "rgp_draw_backward_replay": (I, [I, I, I, P, P, P, P, P, P]),
+    "rgp_draw_backward_replay": (I, [I, I, I, P, P, P, P, P, P, P]),
     "rgp_preprocess_weight": (I, [I, I, I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, I64, I, I,
                                   P, P, P, P, P]),
     "rgp_sh_backward_full": (I, [I, I, I, P, P, P, P, P]),

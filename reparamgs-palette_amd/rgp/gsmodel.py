@@ -37,20 +37,21 @@ def _rec(n, K, dev):
 BACKWARD_MODE = "replay"
 
 
-def _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dlayer):
+def _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dlayer, strips=None):
     if BACKWARD_MODE == "reference":
         return ops.draw_backward(W, H, K, ranges, ids, records, contrib, final_tau, dlayer, n)
-    return ops.draw_backward_replay(W, H, K, ranges, ids, records, dlayer, n)
+    return ops.draw_backward_replay(W, H, K, ranges, ids, records, dlayer, n, strips)
 
 
 def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None, B=None, Q=None,
-                   sh_dim=None, return_state=False, exact=False, low=None, high=None):
+                   sh_dim=None, return_state=False, exact=False, low=None, high=None, strips=False):
     """Forward of the palette-weight rasterizer.  Full rank when `shs`
     (N, sh_dim*K) is given, or the reference's two leaves low (N,K) and high
     (N,(sh_dim-1)K) (the values of their concatenation, without forming it);
     low rank (dc, A (N,P,r), B (N,r,Q)) otherwise.  The pair buffers keep the
     cached capacity unless exact=True (see ops.splat_pipeline); the results
-    do not depend on it."""
+    do not depend on it.  strips=True also records the strip masks of the
+    replay backward (state["strips"])."""
     require_cuda(pws, alphas, scales, rots)
     dev = pws.device
     n = pws.shape[0]
@@ -66,11 +67,11 @@ def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None,
              ptr(alphas_c), ptr(shs_c), ptr(dc_c), ptr(A_c), ptr(B_c), ptr(depths), ptr(records), ptr(ws),
              I64(int(cap)), W, H, None, None, None, None, stream(dev))
 
-    layer, contrib, final_tau, ranges, ids, m, _ = ops.splat_pipeline(n, W, H, K, records, depths, fill, 0.0, dev,
-                                                                      exact=exact)
+    layer, contrib, final_tau, ranges, ids, m, _, smask = ops.splat_pipeline(n, W, H, K, records, depths, fill, 0.0,
+                                                                             dev, exact=exact, strips=strips)
     if return_state:
         return layer, depths, dict(records=records, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids,
-                                   camd=camd, m=m, mode=mode, P=P_, Q=Q_, rank=r_)
+                                   camd=camd, m=m, mode=mode, P=P_, Q=Q_, rank=r_, strips=smask)
     return layer, depths
 
 
@@ -90,7 +91,7 @@ def _coef_args(K, shs, dc, A, B, Q, sh_dim, low=None, high=None):
 
 
 def render_weights_views(pws, alphas, scales, rots, cams, K, shs=None, dc=None, A=None, B=None, Q=None,
-                         sh_dim=None, return_state=False, exact=False):
+                         sh_dim=None, return_state=False, exact=False, strips=False):
     """render_weights for several cameras of one resolution at once (no
     reference counterpart: the reference renders one camera per call).  One
     preprocess launch reads the Gaussians once for all views, and one
@@ -114,11 +115,11 @@ def render_weights_views(pws, alphas, scales, rots, cams, K, shs=None, dc=None, 
              ptr(scales_c), ptr(alphas_c), ptr(shs_c), ptr(dc_c), ptr(A_c), ptr(B_c), ptr(depths), ptr(records),
              ptr(ws), ctypes.c_size_t(ws_stride), I64(int(cap)), W, H, stream(dev))
 
-    layer, contrib, final_tau, ranges, ids, ms, _, _ = ops.splat_pipeline_views(n, V, W, H, K, records, depths, fill,
-                                                                                0.0, dev, exact=exact)
+    layer, contrib, final_tau, ranges, ids, ms, _, _, smask = ops.splat_pipeline_views(
+        n, V, W, H, K, records, depths, fill, 0.0, dev, exact=exact, strips=strips)
     if return_state:
         return layer, depths, dict(records=records, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids,
-                                   camsd=camsd, m=ms, mode=mode, P=P_, Q=Q_, rank=r_)
+                                   camsd=camsd, m=ms, mode=mode, P=P_, Q=Q_, rank=r_, strips=smask)
     return layer, depths
 
 
@@ -131,9 +132,10 @@ class WeightGSFunctionTestViews(torch.autograd.Function):
     def forward(ctx, pws, shs, alphas, scales, rots, cams, num_palette):
         K = int(num_palette)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
-        layer, depths, st = render_weights_views(pws, alphas, scales, rots, cams, K, shs=shs, return_state=True)
+        layer, depths, st = render_weights_views(pws, alphas, scales, rots, cams, K, shs=shs, return_state=True,
+                                                 strips=True)
         ctx.meta = (int(cams[0].width), int(cams[0].height), K, pws.shape[0], st["P"], len(cams))
-        ctx.save_for_backward(f32c(pws), st["records"], st["ranges"], st["ids"], st["camsd"])
+        ctx.save_for_backward(f32c(pws), st["records"], st["ranges"], st["ids"], st["camsd"], st["strips"])
         return layer, depths > 0.2
 
     @staticmethod
@@ -141,8 +143,8 @@ class WeightGSFunctionTestViews(torch.autograd.Function):
         if dloss_dlayer is None:
             return (None,) * 7
         W, H, K, n, sh_dim, V = ctx.meta
-        pws, records, ranges, ids, camsd = ctx.saved_tensors
-        dw = ops.draw_backward_replay_views(V, W, H, K, ranges, ids, records, dloss_dlayer.contiguous(), n)
+        pws, records, ranges, ids, camsd, strips = ctx.saved_tensors
+        dw = ops.draw_backward_replay_views(V, W, H, K, ranges, ids, records, dloss_dlayer.contiguous(), n, strips)
         dshs = torch.empty((n, sh_dim * K), dtype=torch.float32, device=pws.device)
         with SPANS.span("sh_backward"):
             call("rgp_sh_backward_full_views", V, n, K, sh_dim, ptr(camsd), ptr(pws), ptr(dw), ptr(dshs),
@@ -160,13 +162,13 @@ class WeightGSFunctionFastViews(torch.autograd.Function):
         K = int(num_palette)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights_views(pws, alphas, scales, rots, cams, K, dc=shs_dc, A=shs_A, B=shs_B,
-                                                 Q=Q, return_state=True)
+                                                 Q=Q, return_state=True, strips=True)
         ctx.meta = (int(cams[0].width), int(cams[0].height), K, pws.shape[0], st["P"], st["Q"], st["rank"],
                     len(cams))
         fused = getattr(shs_dc, "_rgp_fused_adam", None)  # PaletteAdam with the triple in fused_lowrank
         ctx.fused = (fused, shs_dc, shs_A, shs_B) if fused is not None else None
         ctx.save_for_backward(f32c(pws), f32c(shs_A), f32c(shs_B), st["records"], st["ranges"], st["ids"],
-                              st["camsd"])
+                              st["camsd"], st["strips"])
         return layer, depths > 0.2
 
     @staticmethod
@@ -175,8 +177,8 @@ class WeightGSFunctionFastViews(torch.autograd.Function):
             ctx.fused = None
             return (None,) * 10
         W, H, K, n, P, Q, r, V = ctx.meta
-        pws, A, B, records, ranges, ids, camsd = ctx.saved_tensors
-        dw = ops.draw_backward_replay_views(V, W, H, K, ranges, ids, records, dloss_dlayer.contiguous(), n)
+        pws, A, B, records, ranges, ids, camsd, strips = ctx.saved_tensors
+        dw = ops.draw_backward_replay_views(V, W, H, K, ranges, ids, records, dloss_dlayer.contiguous(), n, strips)
         fused, ctx.fused = ctx.fused, None
         if fused is not None and fused[0].take_lowrank(fused[1], fused[2], fused[3], dw, camsd, pws, K, P, Q, r,
                                                        views=V):
@@ -202,12 +204,13 @@ class WeightGSFunctionSplit(torch.autograd.Function):
     def forward(ctx, pws, low, high, alphas, scales, rots, cam, num_palette):
         K = int(num_palette)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
-        layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, low=low, high=high, return_state=True)
+        layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, low=low, high=high, return_state=True,
+                                           strips=BACKWARD_MODE == "replay")
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"])
         fused = getattr(low, "_rgp_fused_adam", None)
         ctx.fused = (fused, low, high) if fused is not None else None
         ctx.save_for_backward(f32c(pws), st["records"], st["contrib"], st["final_tau"], st["ranges"], st["ids"],
-                              st["camd"])
+                              st["camd"], st["strips"])
         return layer, depths > 0.2
 
     @staticmethod
@@ -216,8 +219,8 @@ class WeightGSFunctionSplit(torch.autograd.Function):
             ctx.fused = None
             return (None,) * 8
         W, H, K, n, sh_dim = ctx.meta
-        pws, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
-        dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())
+        pws, records, contrib, final_tau, ranges, ids, camd, strips = ctx.saved_tensors
+        dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous(), strips)
         if ctx.fused is not None and ctx.fused[0].take_split(ctx.fused[1], ctx.fused[2], dw, camd, pws, K, sh_dim):
             ctx.fused = None
             return (None, None, None, None, None, None, None, None)
@@ -235,13 +238,14 @@ class WeightGSFunctionTest(torch.autograd.Function):
     def forward(ctx, pws, shs, alphas, scales, rots, cam, num_palette):
         K = int(num_palette)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
-        layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, shs=shs, return_state=True)
+        layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, shs=shs, return_state=True,
+                                           strips=BACKWARD_MODE == "replay")
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"])
         # an optimizer that fuses the SH update (rgp.optim.PaletteAdam) takes dL/dw instead of dL/dshs
         fused = getattr(shs, "_rgp_fused_adam", None)
         ctx.fused = (fused, shs) if fused is not None else None
         ctx.save_for_backward(f32c(pws), st["records"], st["contrib"], st["final_tau"], st["ranges"], st["ids"],
-                              st["camd"])
+                              st["camd"], st["strips"])
         return layer, depths > 0.2
 
     @staticmethod
@@ -250,8 +254,8 @@ class WeightGSFunctionTest(torch.autograd.Function):
             ctx.fused = None
             return (None,) * 7
         W, H, K, n, sh_dim = ctx.meta
-        pws, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
-        dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())
+        pws, records, contrib, final_tau, ranges, ids, camd, strips = ctx.saved_tensors
+        dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous(), strips)
         if ctx.fused is not None and ctx.fused[0].take(ctx.fused[1], dw, camd, pws, K, sh_dim):
             ctx.fused = None
             return (None, None, None, None, None, None, None)
@@ -271,14 +275,14 @@ class WeightGSFunctionFast(torch.autograd.Function):
         K = int(num_palette)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, dc=shs_dc, A=shs_A, B=shs_B, Q=Q,
-                                           return_state=True)
+                                           return_state=True, strips=BACKWARD_MODE == "replay")
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"], st["Q"], st["rank"])
         # an optimizer that fuses the factor update (rgp.optim.PaletteAdam with the triple in
         # fused_lowrank) takes dL/dw instead of the three gradients
         fused = getattr(shs_dc, "_rgp_fused_adam", None)
         ctx.fused = (fused, shs_dc, shs_A, shs_B) if fused is not None else None
         ctx.save_for_backward(f32c(pws), f32c(shs_A), f32c(shs_B), st["records"], st["contrib"], st["final_tau"],
-                              st["ranges"], st["ids"], st["camd"])
+                              st["ranges"], st["ids"], st["camd"], st["strips"])
         return layer, depths > 0.2
 
     @staticmethod
@@ -287,8 +291,8 @@ class WeightGSFunctionFast(torch.autograd.Function):
             ctx.fused = None
             return (None,) * 10
         W, H, K, n, P, Q, r = ctx.meta
-        pws, A, B, records, contrib, final_tau, ranges, ids, camd = ctx.saved_tensors
-        dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous())
+        pws, A, B, records, contrib, final_tau, ranges, ids, camd, strips = ctx.saved_tensors
+        dw = _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dloss_dlayer.contiguous(), strips)
         fused, ctx.fused = ctx.fused, None
         if fused is not None and fused[0].take_lowrank(fused[1], fused[2], fused[3], dw, camd, pws, K, P, Q, r):
             return (None,) * 10

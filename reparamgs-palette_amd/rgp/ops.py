@@ -110,14 +110,17 @@ def pack_records(us, cinv2ds, alphas, feat):
     return rec
 
 
-def draw_forward(width, height, dim, ranges, ids, records, empty_tau):
+def draw_forward(width, height, dim, ranges, ids, records, empty_tau, strips=None):
+    """Forward compositing; strips (uint8, one per list position) receives,
+    when given, each composited entry's mask of row strips with a
+    contribution, which draw_backward_replay then uses."""
     dev = records.device
     layer = torch.empty((dim, height, width), dtype=torch.float32, device=dev)
     contrib = torch.empty((height, width), dtype=torch.int32, device=dev)
     final_tau = torch.empty((height, width), dtype=torch.float32, device=dev)
     with SPANS.span("draw_forward"):
         call("rgp_draw_forward", int(width), int(height), int(dim), ptr(ranges), ptr(ids), ptr(records),
-             float(empty_tau), ptr(layer), ptr(contrib), ptr(final_tau), stream(dev))
+             float(empty_tau), ptr(layer), ptr(contrib), ptr(final_tau), ptr(strips), stream(dev))
     return layer, contrib, final_tau
 
 
@@ -145,14 +148,16 @@ def draw_backward(width, height, dim, ranges, ids, records, contrib, final_tau, 
     return dfeat
 
 
-def draw_backward_replay(width, height, dim, ranges, ids, records, dlayer, n):
-    """Gradient of the features by a front-to-back replay of the forward."""
+def draw_backward_replay(width, height, dim, ranges, ids, records, dlayer, n, strips=None):
+    """Gradient of the features by a front-to-back replay of the forward
+    (strips: the forward's strip masks, or None to evaluate every reachable
+    strip)."""
     dev = records.device
     dfeat = torch.zeros((n, dim), dtype=torch.float32, device=dev)
     ranges, ids, dlayer = i32c(ranges), i32c(ids), f32c(dlayer)
     with SPANS.span("draw_backward"):
         call("rgp_draw_backward_replay", int(width), int(height), int(dim), ptr(ranges), ptr(ids), ptr(records),
-             ptr(dlayer), ptr(dfeat), stream(dev))
+             ptr(strips), ptr(dlayer), ptr(dfeat), stream(dev))
     return dfeat
 
 
@@ -192,7 +197,7 @@ CAPACITY = CapacityCache()
 
 
 def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, device,
-                   exact=True):
+                   exact=True, strips=False):
     """Scan -> sort -> ranges -> draw.  ws_fill(ws, cap) must write
     rects/counts into a workspace (the getRects stage or the fused
     preprocess).
@@ -203,7 +208,9 @@ def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, d
     are launched at once, and only then is the overflow flag read (the host
     waits for the scan while the GPU runs the sort and draw); on overflow the
     pipeline runs again with the grown capacity.  Both give the same M,
-    ranges, ids[:M] and images."""
+    ranges, ids[:M] and images.  strips=True: the forward also writes the
+    strip masks of the replay backward (uint8, one per pair slot); they are
+    returned last (None otherwise)."""
     cap = CAPACITY.get(n, width, height)
     while True:
         ws = new_workspace(n, cap, width, height, device)
@@ -224,7 +231,8 @@ def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, d
             scanned.record(torch.cuda.current_stream(device))
         with SPANS.span("sort"):
             ids, ranges, _ = bin_sort(n, depths, width, height, ws, cap, device)
-        layer, contrib, final_tau = draw_forward(width, height, dim, ranges, ids, records, empty_tau)
+        smask = torch.empty(ids.shape[0], dtype=torch.uint8, device=device) if strips else None
+        layer, contrib, final_tau = draw_forward(width, height, dim, ranges, ids, records, empty_tau, smask)
         if not exact:
             scanned.synchronize()
             m = int(hdr[0])
@@ -232,7 +240,7 @@ def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, d
             if int(hdr[1]) != 0:
                 cap = CAPACITY.get(n, width, height)
                 continue
-        return layer, contrib, final_tau, ranges, ids, m, ws
+        return layer, contrib, final_tau, ranges, ids, m, ws, smask
 
 
 def camera_structs(cams, device):
@@ -245,13 +253,14 @@ def _align256(x):
 
 
 def splat_pipeline_views(n, views, width, height, dim, records, depths, ws_fill, empty_tau, device,
-                         exact=False):
+                         exact=False, strips=False):
     """splat_pipeline for `views` cameras of one resolution: one preprocess
     launch (ws_fill(ws, stride, cap)), the binning of every view, and one
     compositing launch over all views' tiles.  Returns layers (V,dim,H,W),
     contrib (V,H,W), final_tau (V,H,W), ranges (V,tiles,2), ids (V,cap),
-    the pair counts per view, the workspace and its per-view stride.  The
-    overflow check follows splat_pipeline's (exact=False: after the draw)."""
+    the pair counts per view, the workspace, its per-view stride and the
+    strip masks (V,cap) uint8 when strips=True (else None).  The overflow
+    check follows splat_pipeline's (exact=False: after the draw)."""
     gx, gy = grid_dims(width, height)
     cap = CAPACITY.get(n, width, height)
     while True:
@@ -283,10 +292,11 @@ def splat_pipeline_views(n, views, width, height, dim, records, depths, ws_fill,
         layer = torch.empty((views, dim, height, width), dtype=torch.float32, device=device)
         contrib = torch.empty((views, height, width), dtype=torch.int32, device=device)
         final_tau = torch.empty((views, height, width), dtype=torch.float32, device=device)
+        smask = torch.empty(ids.shape, dtype=torch.uint8, device=device) if strips else None
         with SPANS.span("draw_forward"):
             call("rgp_draw_forward_views", int(views), int(width), int(height), int(dim), ptr(ranges), ptr(ids),
                  I64(ids.shape[1]), ptr(records), I64(int(n)), float(empty_tau), ptr(layer), ptr(contrib),
-                 ptr(final_tau), stream(device))
+                 ptr(final_tau), ptr(smask), stream(device))
         if not exact:
             scanned.synchronize()
             ms = host[:, 0].tolist()
@@ -295,17 +305,18 @@ def splat_pipeline_views(n, views, width, height, dim, records, depths, ws_fill,
             if int(host[:, 1].max()) != 0:
                 cap = CAPACITY.get(n, width, height)
                 continue
-        return layer, contrib, final_tau, ranges, ids, ms, ws, stride
+        return layer, contrib, final_tau, ranges, ids, ms, ws, stride, smask
 
 
-def draw_backward_replay_views(views, width, height, dim, ranges, ids, records, dlayer, n):
-    """Replay backward of a multi-view batch: dfeat (V, n, dim), one row set per view."""
+def draw_backward_replay_views(views, width, height, dim, ranges, ids, records, dlayer, n, strips=None):
+    """Replay backward of a multi-view batch: dfeat (V, n, dim), one row set per
+    view (strips: the forward's (V, id stride) strip masks, or None)."""
     dev = records.device
     dfeat = torch.zeros((views, n, dim), dtype=torch.float32, device=dev)
     dlayer = f32c(dlayer)
     with SPANS.span("draw_backward"):
         call("rgp_draw_backward_replay_views", int(views), int(width), int(height), int(dim), ptr(ranges), ptr(ids),
-             I64(ids.shape[1]), ptr(records), I64(int(n)), ptr(dlayer), ptr(dfeat), stream(dev))
+             I64(ids.shape[1]), ptr(records), I64(int(n)), ptr(strips), ptr(dlayer), ptr(dfeat), stream(dev))
     return dfeat
 
 

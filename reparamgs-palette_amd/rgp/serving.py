@@ -194,7 +194,7 @@ class RenderSlot:
         call("rgp_bin_sort", sc.n, ptr(self.depths), W, H, ptr(self.ws), I64(self.cap), ptr(self.ids), None,
              ptr(self.ranges), st)
         call("rgp_draw_forward", W, H, sc.K, ptr(self.ranges), ptr(self.ids), ptr(self.records), 0.0,
-             ptr(self.layer), ptr(self.contrib), ptr(self.final_tau), st)
+             ptr(self.layer), ptr(self.contrib), ptr(self.final_tau), None, st)
 
     def render(self, cam_host):
         """Enqueue a render on the current stream with the pose in cam_host
@@ -240,7 +240,7 @@ class BatchSlot:
         call("rgp_bin_sort_views", v, sc.n, ptr(self.depths), W, H, ptr(self.ws), sz, I64(self.cap), ptr(self.ids),
              ptr(self.ranges), st)
         call("rgp_draw_forward_views", v, W, H, sc.K, ptr(self.ranges), ptr(self.ids), I64(self.ids.shape[1]),
-             ptr(self.records), I64(sc.n), 0.0, ptr(self.layer), ptr(self.contrib), ptr(self.final_tau), st)
+             ptr(self.records), I64(sc.n), 0.0, ptr(self.layer), ptr(self.contrib), ptr(self.final_tau), None, st)
 
     def capture(self, v):
         """The graph of the v-view render (captured once; the capture

@@ -8,7 +8,9 @@ gsplat/pytorch_ssim.py:17-66.
 For two of the 32 views, against the FP32 oracle over the whole frame:
 * depths, the visibility mask, every tile's range and the whole sorted id
   list, bit for bit;
-* layer, contrib and final_tau of every pixel, bit for bit;
+* layer, contrib and final_tau of every pixel, bit for bit, and the
+  forward's per-entry masks of row strips with a contribution (which the
+  replay backward evaluates), bit for bit wherever the forward writes them;
 * dL/dshs of every Gaussian for L = sum(G * layer), G ~ N(0,1), within
   1e-5 * sum|terms| of the reference's own backward (tau recovered by
   division from final_tau, kernel.cu:1291-1379; the GPU runs the replay form,
@@ -37,6 +39,7 @@ from rgp.optim import PaletteAdam
 from rgp.scene import make_scene, orbit_cameras
 from ssim_ref import l1_ssim_ref
 from step_ref import adam_first_step
+from test_gpu_parity import check_strips
 
 pytestmark = pytest.mark.gpu
 N_G, K, W, H = 1_000_000, 5, 1920, 1080
@@ -91,7 +94,7 @@ def test_c1_view_full_frame(device, c1, v):
     layer, mask = gsmodel.WeightGSFunctionTest.apply(args[0], shs, args[1], args[2], args[3], tc, K)
     with torch.no_grad():
         _, depths, st = gsmodel.render_weights(args[0], args[1], args[2], args[3], tc, K, shs=shs.detach(),
-                                               return_state=True)
+                                               return_state=True, strips=True)
     G = np.random.default_rng(100 + v).standard_normal((K, H, W)).astype(np.float32)
     (layer * T(G, device)).sum().backward()
     torch.cuda.synchronize()
@@ -104,6 +107,7 @@ def test_c1_view_full_frame(device, c1, v):
     same(N(layer), o["layer"], "layer")
     same(N(st["contrib"]), o["contrib"], "contrib")
     same(N(st["final_tau"]), o["final_tau"], "final_tau")
+    assert check_strips(st, o, sc, cams[v]) > 1_000_000
     print(f"view {v}: {m} pairs, E = {o['n_eval']}, C = {o['n_contrib']}; oracle backward", flush=True)
     ob = O.backward_weight(sc, o, G, replay=False)
     within(N(shs.grad), ob["dshs"], ob["dshs_abs"], "dshs")

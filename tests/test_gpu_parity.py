@@ -188,6 +188,53 @@ def test_colour_path_bitexact(device):
     same(N(img), ref, "image")
 
 
+def strips_written(ranges, contrib, final_tau, width, height):
+    """The list positions whose strip masks the forward writes: every entry of
+    each batch of 64 a tile stages, i.e. while some pixel of the tile is live
+    at the batch start (a pixel stops at its contrib when final_tau < 1e-4,
+    else runs through the list; pixels outside the image never start)."""
+    gx = (width + 15) // 16
+    keep = np.zeros(int(ranges[:, 1].max(initial=0)), bool)
+    for t in np.flatnonzero(ranges[:, 1] > ranges[:, 0]):
+        a, b = ranges[t]
+        ty, tx = divmod(int(t), gx)
+        c = contrib[16 * ty:16 * ty + 16, 16 * tx:16 * tx + 16]
+        f = final_tau[16 * ty:16 * ty + 16, 16 * tx:16 * tx + 16]
+        stop = np.where(f < np.float32(1e-4), c, b - a).max()
+        keep[a:a + min(b - a, 64 * ((int(stop) + 63) // 64))] = True
+    return keep
+
+
+def check_strips(st, o, sc, cam):
+    _, _, _, _, _, ref = O.draw(cam.width, cam.height, o["ranges"], o["ids"], o["us"], o["cinv"], sc.alphas,
+                                o["weights"], strips=True)
+    keep = strips_written(o["ranges"], o["contrib"], o["final_tau"], cam.width, cam.height)
+    got = N(st["strips"][:len(o["ids"])])
+    same(got[keep], ref[keep], "strips")
+    assert (ref[~keep] == 0).all()  # nothing contributes after every pixel has stopped
+    return int((ref != 0).sum())
+
+
+def test_forward_strip_masks_c0(device, c0):
+    """The forward's per-entry masks of row strips with a contribution, bit
+    for bit against the oracle; the replay backward with them equals the
+    oracle's gradient, as without them."""
+    scene, cam = c0
+    o = O.forward_weight(scene, cam)
+    tc = cam.to_torch(device)
+    args = [T(x, device) for x in (scene.pws, scene.alphas, scene.scales, scene.rots)]
+    layer, depths, st = gsmodel.render_weights(*args, tc, scene.K, shs=T(scene.shs, device), return_state=True,
+                                               strips=True)
+    same(N(layer), o["layer"], "layer")
+    assert check_strips(st, o, scene, cam) > 1000
+    dl = np.random.default_rng(3).normal(size=o["layer"].shape).astype(np.float32)
+    ob = O.backward_weight(scene, o, dl, replay=True)
+    for strips in (st["strips"], None):
+        dw = ops.draw_backward_replay(cam.width, cam.height, scene.K, st["ranges"], st["ids"], st["records"],
+                                      T(dl, device), scene.n, strips)
+        grad_close(N(dw), ob["dweights"], ob["dweights_abs"], f"dweights strips={strips is not None}")
+
+
 def test_mix_matches_bake(device, c0):
     """mix(render(w), P) against the oracle mix, and the linear identity with
     the bake-then-render colour path (gsmodel.py:488-509, 13-26)."""

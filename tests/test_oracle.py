@@ -316,6 +316,7 @@ def test_draw_pure_python_recurrence():
     o = O.forward_weight(sc, cam)
     us, cinv, w, al = o["us"], o["cinv"], o["weights"], sc.alphas
     gx, _ = O.grid_dims(cam.width, cam.height)
+    strips = np.zeros(len(o["ids"]), np.uint8)  # rows strips (py mod 16) // 4 each entry contributed in
     nz = 0
     for py in range(cam.height):
         for px in range(cam.width):
@@ -332,6 +333,7 @@ def test_draw_pure_python_recurrence():
                 if ap < f32(0.002):
                     continue
                 acc = [fma32(tau * ap, w[g, k], acc[k]) for k in range(3)]
+                strips[j] |= 1 << ((py % 16) // 4)
                 cont = j - a + 1
                 tau = tau * (f32(1) - ap)
                 if tau < f32(0.0001):
@@ -342,6 +344,8 @@ def test_draw_pure_python_recurrence():
             assert o["contrib"][py, px] == cont and o["final_tau"][py, px] == tau
             nz += cont > 0
     assert nz > 100
+    got = O.draw(cam.width, cam.height, o["ranges"], o["ids"], us, cinv, al, w, strips=True)[5]
+    assert np.array_equal(got, strips) and (strips != 0).sum() > 50 and len(np.unique(strips)) > 4
 
 
 def test_draw_unit_weights_invariant():

@@ -432,6 +432,116 @@ def measure_vanilla(args, dev, n=1_000_000, views=8, steps=16, warmup=2):
                         f"{views} views, L1+SSIM + final_tau term, geometry gradients, Adam on all five leaves"}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def measure_cpu(args, scene, cams, pal_target):
+    """The CPU oracle (FP32 C++ restatement of the path, OpenMP) timed on this
+    host with the GPU step's stages (oracle.timed_step): the finetune step of
+    the headline workload on one view at all threads and at one thread, plus
+    configs[0] (20k, 256x256; median of 5), the rank-2 low-rank step at
+    1M and 3M Gaussians (1080p) and the 6M / K=7 / 3840x2160 / 4-view step.
+    The one-thread samples of the large workloads and the 4-view step run the
+    compositing stages on a seeded subset of the tiles (1/8; 1/32 at 4K) and
+    scale them by the subset's share of the list entries (labelled
+    'extrapolated'); every other stage runs in full."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from rgp.scene import make_scene, orbit_cameras, workload_c0
+
+    threads = O.num_threads()
+    comp = ("forward_compositing", "backward_compositing")
+
+    def total(t):
+        share = t["entry_share"]
+        return sum(v / share if k in comp else v for k, v in t.items() if k != "entry_share")
+
+    def rounded(t):
+        return {k: round(v * 1e3, 3) if k != "entry_share" else round(v, 4) for k, v in t.items()}
+
+    def target_of(sc, cam):
+        o = O.forward_weight(sc, cam) if sc.n <= 20000 else None
+        if o is None:
+            return None
+        return O.mix(np.clip(sc.palette + 0.05, 0, 1).astype(np.float32), o["layer"])
+
+    def tile_subset(cam, every, seed):
+        gx, gy = O.grid_dims(cam.width, cam.height)
+        return np.random.default_rng(seed).permutation(gx * gy)[: (gx * gy) // every]
+
+    c0 = time.perf_counter()
+    log(f"cpu oracle: headline step on {threads} threads and on 1 thread")
+    # the headline step: 1M full rank, 1080p, L1+SSIM against the perturbed-palette target of view 0
+    o = O.forward_weight(scene, cams[0])
+    target = O.mix(np.asarray(pal_target.cpu()), o["layer"])
+    del o
+    t_all = O.timed_step(scene, cams[0], target)
+    O.set_threads(1)
+    t_one = O.timed_step(scene, cams[0], target, tiles=tile_subset(cams[0], 8, 1))
+    O.set_threads(threads)
+    # configs[0]: median of 5 full steps at each thread count
+    log("cpu oracle: 20k Gaussians 256x256, median of 5")
+    sc0, cam0 = workload_c0()
+    tgt0 = target_of(sc0, cam0)
+    c0s = {}
+    for th in (threads, 1):
+        O.set_threads(th)
+        runs = [O.timed_step(sc0, cam0, tgt0) for _ in range(5)]
+        c0s[th] = sorted(runs, key=total)[2]
+    O.set_threads(threads)
+    # configs[2]: rank-2 low-rank, 1M and 3M, 1080p (one view; no SSIM target: a seeded dL/drgb)
+    c2 = {}
+    for n in (1_000_000, 3_000_000):
+        log(f"cpu oracle: {n} Gaussians rank-2 1920x1080")
+        sc = make_scene(n, seed=args.seed, extent=3.0, K=5, lowrank=True, rank=2, P=5, Q=15)
+        cam = orbit_cameras(8, 6.0, args.width, args.height)[0]
+        ta = O.timed_step(sc, cam)
+        O.set_threads(1)
+        t1 = O.timed_step(sc, cam, tiles=tile_subset(cam, 8, 1))
+        O.set_threads(threads)
+        c2[f"{n // 1_000_000}M"] = {"all_threads_steps_per_s": round(1.0 / total(ta), 4),
+                                     "one_thread_steps_per_s_extrapolated": round(1.0 / total(t1), 5),
+                                     "stages_ms_all_threads": rounded(ta), "stages_ms_one_thread_subset": rounded(t1)}
+        del sc
+    # configs[4]: 6M, K=7 rank-2, 3840x2160, 4 views; compositing on 1/32 of the tiles, extrapolated
+    sc = make_scene(6_000_000, seed=4, extent=3.0, K=7, lowrank=True, rank=2, P=7, Q=15)
+    log("cpu oracle: 6M K=7 rank-2 3840x2160, 4 views")
+    c4_t = [O.timed_step(sc, cam, tiles=tile_subset(cam, 32, 7 + v))
+            for v, cam in enumerate(orbit_cameras(4, 6.0, 3840, 2160))]
+    del sc
+    # one Adam update per 4-view step
+    c4_total = sum(total(t) for t in c4_t) - sum(t["adam"] for t in c4_t[1:])
+    wall = time.perf_counter() - c0
+    return {"value": round(1.0 / total(t_all), 5), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "CPU oracle (FP32 C++ restatement of the path, OpenMP over Gaussians, tiles and rows) "
+                      "timed on one view of the headline workload: preprocess, scan, key duplication, sort, "
+                      f"ranges, forward compositing, mix, L1+SSIM forward and backward, mix backward, backward "
+                      f"compositing, SH backward, Adam; {wall:.1f}s for every sample below",
+            "value_1_thread_extrapolated": round(1.0 / total(t_one), 5),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "stages_ms": {"all_threads": rounded(t_all), "one_thread_subset": rounded(t_one)},
+            "configs0_20k_256": {"all_threads_steps_per_s": round(1.0 / total(c0s[threads]), 3),
+                                 "one_thread_steps_per_s": round(1.0 / total(c0s[1]), 3),
+                                 "stages_ms_all_threads": rounded(c0s[threads]),
+                                 "stages_ms_one_thread": rounded(c0s[1])},
+            "configs2_rank2_1080p": c2,
+            "configs4_6M_4K_4views": {"all_threads_steps_per_s_extrapolated": round(1.0 / c4_total, 5),
+                                      "method": "compositing stages on a seeded 1/32 of the tiles per view, scaled "
+                                                "by 1 / (their share of the list entries); every other stage in "
+                                                "full, per view, with one Adam update per step; no SSIM target "
+                                                "(seeded dL/drgb)",
+                                      "stages_ms_view0_subset": rounded(c4_t[0])}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -602,32 +712,10 @@ def main():
     vanilla = None if args.no_vanilla else measure_vanilla(args, dev)
     c4 = None if args.no_c4 else measure_c4(args, dev)
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        from oracle import oracle as O
-        threads = O.num_threads()
-        sample = []
-        c0 = time.perf_counter()
-        for v in range(args.cpu_views):
-            o = O.forward_weight(scene, cams[v])
-            dl = np.random.default_rng(v).normal(size=o["layer"].shape).astype(np.float32)
-            O.backward_weight(scene, o, dl, parallel=True)
-            sample.append(v)
-        c1 = time.perf_counter()
-        # the same sample on one thread
-        O.set_threads(1)
-        s0 = time.perf_counter()
-        o = O.forward_weight(scene, cams[0])
-        O.backward_weight(scene, o, np.random.default_rng(0).normal(size=o["layer"].shape).astype(np.float32),
-                          parallel=True)
-        s1 = time.perf_counter()
-        O.set_threads(threads)
-        cpu = {"value": round(len(sample) / (c1 - c0), 5), "unit": "steps/s", "cores": threads, "kind": "port",
-               "sample": f"CPU oracle (FP32, OpenMP over tiles) forward + backward rasterization of "
-                         f"{len(sample)} view(s) of the same 1M-Gaussian 1080p K=5 scene, "
-                         f"{c1 - c0:.1f}s; loss and optimizer not included",
-               "value_1_thread": round(1.0 / (s1 - s0), 5)}
-        log(f"cpu oracle: {cpu['value']} steps/s on {threads} threads, {cpu['value_1_thread']} on 1 thread")
+    cpu = None if args.no_cpu_baseline else measure_cpu(args, scene, cams, pal_target)
+    if cpu is not None:
+        log(f"cpu oracle: {cpu['value']} steps/s on {cpu['cores']} threads, "
+            f"{cpu['value_1_thread_extrapolated']} on 1 thread")
 
     out = {
         "metric": METRIC, "value": round(args.steps / (t1 - t0), 3), "unit": "steps/s", "n_gpus": 1,

@@ -84,6 +84,7 @@ def lib():
         _lib.orc_expf.argtypes = [ctypes.c_float]
         for s in ("f", "d"):
             getattr(_lib, "orc_bin_sort_" + s).restype = ctypes.c_int64
+        _lib.orc_l1_ssim_f.restype = ctypes.c_double
     return _lib
 
 
@@ -342,8 +343,10 @@ def get_rects(us, areas, depths, width, height, dt=f32):
     return rects, counts
 
 
-def bin_sort(depths, rects, counts, width, height, dt=f32):
-    """Returns (keys (M,) u64, ids (M,) i32, ranges (T,2) i32)."""
+def bin_sort(depths, rects, counts, width, height, dt=f32, secs=None):
+    """Returns (keys (M,) u64, ids (M,) i32, ranges (T,2) i32); secs (a
+    float64 array of 4, optional) receives the seconds of the scan, the key
+    duplication, the sort and the ranges."""
     depths = _c(depths, dt)
     n = depths.shape[0]
     gx, gy = grid_dims(width, height)
@@ -351,7 +354,7 @@ def bin_sort(depths, rects, counts, width, height, dt=f32):
     keys, ids = np.zeros(max(M, 1), u64), np.zeros(max(M, 1), i32)
     ranges = np.zeros((gx * gy, 2), i32)
     got = getattr(lib(), "orc_bin_sort_" + _sfx(dt))(n, _p(depths), _p(_c(rects, u32)), _p(_c(counts, u32)), gx, gy,
-                                                     _p(keys), _p(ids), ctypes.c_int64(M), _p(ranges))
+                                                     _p(keys), _p(ids), ctypes.c_int64(M), _p(ranges), _p(secs))
     assert got == M
     return keys[:M], ids[:M], ranges
 
@@ -415,6 +418,110 @@ def mix(palette, W, dt=f32):
     rgb = np.zeros((3,) + W.shape[1:], dt)
     getattr(lib(), "orc_mix_" + _sfx(dt))(K, ctypes.c_int64(HW), _p(_c(palette, dt)), _p(W), _p(rgb))
     return rgb
+
+
+def ssim_taps():
+    """The normalised float32 taps of create_window (pytorch_ssim.py:17-29)."""
+    import math
+    g = np.array([math.exp(-(x - 5) ** 2 / float(2 * 1.5 ** 2)) for x in range(11)], np.float32)
+    return (g / g.sum()).astype(np.float32)
+
+
+def l1_ssim(img, tgt, lam=0.2, grad=True):
+    """FP32 (1 - lam) L1 + lam (1 - SSIM) of (C,H,W) images and, with grad,
+    dL/dimg (OpenMP over rows; the CPU timing sample's loss stage)."""
+    img, tgt = _c(img, f32), _c(tgt, f32)
+    C, H, W = img.shape
+    d = np.zeros_like(img) if grad else None
+    loss = lib().orc_l1_ssim_f(C, H, W, _p(img), _p(tgt), _p(ssim_taps()), ctypes.c_float(0.01 ** 2),
+                               ctypes.c_float(0.03 ** 2), ctypes.c_float(lam), _p(d))
+    return loss, d
+
+
+def sh_backward_full(dw, basis):
+    """dL/dshs (N, L*K) = dL/dw (N,K) x Y (N,L) per element, FP32."""
+    dw, basis = _c(dw, f32), _c(basis, f32)
+    n, K = dw.shape
+    L = basis.shape[1]
+    out = np.empty((n, L * K), f32)
+    lib().orc_sh_backward_full_f(ctypes.c_int64(n), K, L, _p(dw), _p(basis), _p(out))
+    return out
+
+
+def adam(p, g, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """torch.optim.Adam's update of p, m, v in place (FP32, OpenMP)."""
+    lib().orc_adam_f(ctypes.c_int64(p.size), _p(p), _p(_c(g, f32)), _p(m), _p(v), ctypes.c_int64(step),
+                     ctypes.c_double(lr), ctypes.c_double(b1), ctypes.c_double(b2), ctypes.c_double(eps))
+
+
+def timed_step(scene, cam, target=None, tiles=None, lr=1e-3):
+    """One finetune step of the palette-weight rasterizer on the CPU with the
+    stages of the GPU step, each timed in seconds: preprocess (project,
+    covariances, SH weights, inverse, rects), scan, key duplication, sort,
+    ranges, forward compositing, palette mix, loss (L1 + SSIM against target,
+    forward and backward, when target is given), mix backward, backward
+    compositing, SH / low-rank backward, Adam.  tiles: composite only these
+    tiles (a seeded subset; the compositing stages then cover that share of
+    the list entries, reported as 'entry_share')."""
+    import time
+    t = {}
+    clock = time.perf_counter
+    c0 = clock()
+    K = scene.K
+    us, pcs, depths = project(scene.pws, cam.Rcw, cam.tcw, cam.fx, cam.fy, cam.cx, cam.cy)
+    c3 = cov3d(scene.rots, scene.scales, depths)
+    c2 = cov2d(c3, pcs, cam.Rcw, depths, cam.fx, cam.fy, cam.width, cam.height)
+    if scene.lowrank:
+        w, ylm = sh2weight_fast(scene.dc, scene.A, scene.B, scene.pws, cam.twc, K, scene.Q)
+    else:
+        w, basis = sh2weight(scene.shs, scene.pws, cam.twc, K)
+    cinv, areas = inverse_cov2d(c2, depths)
+    rects, counts = get_rects(us, areas, depths, cam.width, cam.height)
+    t["preprocess"] = clock() - c0
+    secs = np.zeros(4, f64)
+    _, ids, ranges = bin_sort(depths, rects, counts, cam.width, cam.height, secs=secs)
+    t["scan"], t["key_duplication"], t["sort"], t["ranges"] = (float(x) for x in secs)
+    share = 1.0
+    if tiles is not None:
+        sub = np.zeros_like(ranges)
+        sub[tiles] = ranges[tiles]
+        share = float((sub[:, 1] - sub[:, 0]).sum()) / max(1, len(ids))
+        ranges = sub
+    c0 = clock()
+    layer, contrib, ftau, _, _ = draw(cam.width, cam.height, ranges, ids, us, cinv, scene.alphas, w, 0.0)
+    t["forward_compositing"] = clock() - c0
+    c0 = clock()
+    rgb = mix(scene.palette, layer)
+    t["mix"] = clock() - c0
+    if target is not None:
+        c0 = clock()
+        _, drgb = l1_ssim(rgb, target)
+        t["loss"] = clock() - c0
+    else:
+        drgb = np.random.default_rng(0).standard_normal(rgb.shape).astype(f32)
+    c0 = clock()
+    P = scene.palette.astype(f32)
+    dl = ((P[:, 0, None, None] * drgb[0] + P[:, 1, None, None] * drgb[1]) + P[:, 2, None, None] * drgb[2])
+    dl = dl.astype(f32)
+    t["mix_backward"] = clock() - c0
+    c0 = clock()
+    g = draw_backward(cam.width, cam.height, K, ranges, ids, us, cinv, scene.alphas, contrib, ftau, dl, f32, mode=1)
+    t["backward_compositing"] = clock() - c0
+    c0 = clock()
+    if scene.lowrank:
+        dA, dB = sh2weight_fast_backward(scene.A, scene.B, ylm, g, K, scene.Q)
+        grads = [(g * f32(0.28209479177387814)).astype(f32), dA, dB]
+    else:
+        grads = [sh_backward_full(g, basis)]
+    t["sh_backward"] = clock() - c0
+    params = [scene.dc, scene.A, scene.B] if scene.lowrank else [scene.shs]
+    state = [(pa.astype(f32).copy(), np.zeros(pa.shape, f32), np.zeros(pa.shape, f32)) for pa in params]
+    c0 = clock()
+    for (p_, m_, v_), gr in zip(state, grads):
+        adam(p_, gr, m_, v_, 1, lr)
+    t["adam"] = clock() - c0
+    t["entry_share"] = share
+    return t
 
 
 # --------------------------------------------------------------------------

@@ -35,6 +35,7 @@
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
+#include <parallel/algorithm>
 #endif
 
 namespace orc {
@@ -156,6 +157,7 @@ template <typename T> static inline T fmx(T a, T b) { return std::fmax(a, b); }
 template <typename T>
 void project(int n, const T* pws, const T* R, const T* t, T fx, T fy, T cx, T cy,
              T* us, T* pcs, T* depths) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     T pc[3];
     for (int r = 0; r < 3; ++r) {
@@ -180,6 +182,7 @@ void project(int n, const T* pws, const T* R, const T* t, T fx, T fy, T cx, T cy
 // ---------------------------------------------------------------------------
 template <typename T>
 void cov3d(int n, const T* rots, const T* scales, const T* depths, T* cov3ds) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     if ((double)depths[i] < MIN_DEPTH) continue;
     const T w = rots[i * 4 + 0], x = rots[i * 4 + 1], y = rots[i * 4 + 2], z = rots[i * 4 + 3];
@@ -213,6 +216,7 @@ void cov3d(int n, const T* rots, const T* scales, const T* depths, T* cov3ds) {
 template <typename T>
 void cov2d(int n, const T* cov3ds, const T* pcs, const T* Rcw, const T* depths,
            T fx, T fy, T tan_fovx, T tan_fovy, T* cov2ds) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     if ((double)depths[i] < MIN_DEPTH) continue;
     T x = pcs[i * 3 + 0], y = pcs[i * 3 + 1];
@@ -258,6 +262,7 @@ void cov2d(int n, const T* cov3ds, const T* pcs, const T* Rcw, const T* depths,
 // ---------------------------------------------------------------------------
 template <typename T>
 void inverse_cov2d(int n, const T* cov2ds, T* depths, T* cinv2ds, int32_t* areas) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     if ((double)depths[i] < MIN_DEPTH) continue;
     const T a = cov2ds[i * 3 + 0], b = cov2ds[i * 3 + 1], c = cov2ds[i * 3 + 2];
@@ -308,6 +313,7 @@ static void sh_dir_basis(const T* pw, const T* twc, int sh_dim, T* Y /*[16]*/) {
 template <typename T>
 void sh2weight(int n, int K, int sh_dim, const T* shs, const T* pws, const T* twc,
                T* weights, T* basis /*nullable, (N, sh_dim)*/) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     const T* sh = shs + (size_t)i * sh_dim * K;
     T w[MAX_SPLAT_DIM];
@@ -349,6 +355,7 @@ void sh2weight_fast(int n, int K, int P, int Q, int rank, const T* dc, const T* 
                     const T* B, const T* pws, const T* twc, T* weights, T* ylm /*nullable (N,L)*/) {
   const int L = P * Q / K;
   const int sh_dim = L + 1;
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     T w[MAX_SPLAT_DIM];
     for (int k = 0; k < K; ++k) w[k] = dmul(SH_C0_0, dc[(size_t)i * K + k]);
@@ -385,6 +392,7 @@ template <typename T>
 void sh2weight_fast_backward(int n, int K, int P, int Q, int rank, const T* A, const T* B,
                              const T* ylm, const T* dw, T* dA, T* dB) {
   const int L = P * Q / K;
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     const T* a = A + (size_t)i * P * rank;
     const T* b = B + (size_t)i * rank * Q;
@@ -413,6 +421,7 @@ void sh2weight_fast_backward(int n, int K, int P, int Q, int rank, const T* A, c
 // ---------------------------------------------------------------------------
 template <typename T>
 void sh2color(int n, int sh_dim, const T* shs, const T* pws, const T* twc, T* colors) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     const T* sh = shs + (size_t)i * sh_dim * 3;
     const T c0 = (T)SH_C0_0;
@@ -513,6 +522,7 @@ void sh2color_jac(int n, int sh_dim, const T* shs, const T* pws, const T* twc, T
 template <typename T>
 void get_rects(int n, const T* us, int32_t* areas, T* depths, int gx, int gy,
                uint32_t* rects /*(N,4)*/, uint32_t* counts) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     counts[i] = 0;
     if ((double)depths[i] < MIN_DEPTH) continue;
@@ -543,10 +553,20 @@ void get_rects(int n, const T* us, int32_t* areas, T* depths, int gx, int gy,
 // u64 keys (gausplat.cu:216, a stable radix sort), getRanges (kernel.cu:165-187)
 // including its M == 1 behaviour.  Returns M.
 // ---------------------------------------------------------------------------
+// The stable order is the order of the pairs' positions o (Gaussian order), so
+// the sort orders (key, o), unique, with a parallel comparison sort; secs
+// (nullable) receives the seconds of scan, key duplication, sort and ranges.
+struct KeyPos {
+  uint64_t key;
+  int64_t o;
+  bool operator<(const KeyPos& b) const { return key < b.key || (key == b.key && o < b.o); }
+};
+
 template <typename T>
 int64_t bin_sort(int n, const T* depths, const uint32_t* rects, const uint32_t* counts,
                  int gx, int gy, uint64_t* keys_out /*cap*/, int32_t* ids_out /*cap*/,
-                 int64_t cap, int32_t* ranges /*(T,2)*/) {
+                 int64_t cap, int32_t* ranges /*(T,2)*/, double* secs) {
+  double t0 = omp_get_wtime();
   std::vector<uint64_t> off(n > 0 ? n : 1);
   uint64_t run = 0;
   for (int i = 0; i < n; ++i) { run += counts[i]; off[i] = run; }
@@ -554,8 +574,10 @@ int64_t bin_sort(int n, const T* depths, const uint32_t* rects, const uint32_t* 
   const int tiles = gx * gy;
   for (int t = 0; t < 2 * tiles; ++t) ranges[t] = 0;
   if (M > cap) return M;
+  double t1 = omp_get_wtime();
   std::vector<uint64_t> keys(M);
   std::vector<int32_t> ids(M);
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     const T depth = depths[i];
     if ((double)depth < MIN_DEPTH) continue;
@@ -570,11 +592,14 @@ int64_t bin_sort(int n, const T* depths, const uint32_t* rects, const uint32_t* 
         keys[o] = key; ids[o] = i; ++o;
       }
   }
-  std::vector<int64_t> perm(M);
-  std::iota(perm.begin(), perm.end(), 0);
-  std::stable_sort(perm.begin(), perm.end(),
-                   [&](int64_t a, int64_t b) { return keys[a] < keys[b]; });
-  for (int64_t p = 0; p < M; ++p) { keys_out[p] = keys[perm[p]]; ids_out[p] = ids[perm[p]]; }
+  double t2 = omp_get_wtime();
+  std::vector<KeyPos> kp(M);
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < M; ++p) kp[p] = KeyPos{keys[p], p};
+  __gnu_parallel::sort(kp.begin(), kp.end());
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < M; ++p) { keys_out[p] = kp[p].key; ids_out[p] = ids[kp[p].o]; }
+  double t3 = omp_get_wtime();
   for (int64_t p = 0; p < M; ++p) {
     const int64_t prv = p == 0 ? 0 : p - 1;
     const uint32_t cur_t = (uint32_t)(keys_out[p] >> 32);
@@ -582,6 +607,13 @@ int64_t bin_sort(int n, const T* depths, const uint32_t* rects, const uint32_t* 
     if (p == 0) ranges[2 * cur_t + 0] = 0;
     else if (p == M - 1) ranges[2 * cur_t + 1] = (int32_t)M;
     if (prv_t != cur_t) { ranges[2 * prv_t + 1] = (int32_t)p; ranges[2 * cur_t + 0] = (int32_t)p; }
+  }
+  if (secs) {
+    const double t4 = omp_get_wtime();
+    secs[0] = t1 - t0;
+    secs[1] = t2 - t1;
+    secs[2] = t3 - t2;
+    secs[3] = t4 - t3;
   }
   return M;
 }
@@ -832,6 +864,7 @@ void draw_color_backward(int width, int height, const int32_t* ranges, const int
 template <typename T>
 void mix(int K, int64_t HW, const T* pal /*(K,3)*/, const T* W, T* rgb) {
   for (int c = 0; c < 3; ++c)
+#pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < HW; ++p) {
       T acc = pal[0 * 3 + c] * W[0 * HW + p];
       for (int k = 1; k < K; ++k) acc = acc + pal[k * 3 + c] * W[k * HW + p];
@@ -1130,8 +1163,8 @@ void sh2weight_jac(int n, int K, int sh_dim, const T* shs, const T* pws, const T
   }                                                                                               \
   int64_t orc_bin_sort_##S(int n, const T* depths, const uint32_t* rects, const uint32_t* counts, \
                            int gx, int gy, uint64_t* keys, int32_t* ids, int64_t cap,             \
-                           int32_t* ranges) {                                                     \
-    return orc::bin_sort<T>(n, depths, rects, counts, gx, gy, keys, ids, cap, ranges);            \
+                           int32_t* ranges, double* secs) {                                       \
+    return orc::bin_sort<T>(n, depths, rects, counts, gx, gy, keys, ids, cap, ranges, secs);      \
   }                                                                                               \
   void orc_draw_##S(int w, int h, int dim, const int32_t* ranges, const int32_t* ids,             \
                     const T* us, const T* cinv, const T* alphas, const T* feat, T* layer,         \
@@ -1175,6 +1208,119 @@ void sh2weight_jac(int n, int K, int sh_dim, const T* shs, const T* pws, const T
     orc::sh2weight_jac<T>(n, K, sh_dim, shs, pws, twc, dwdpw);                                    \
   }
 
+namespace orc {
+// ---------------------------------------------------------------------------
+// Stages of the finetune step that only the CPU timing sample needs, in FP32.
+//
+// L1 + SSIM (gsplat/pytorch_ssim.py:17-66 with zero padding, 11x11 window as
+// two separable passes of the float32 taps; (1 - lam) L1 + lam (1 - SSIM),
+// vanilla_3dgs/gsplat/pytorch_ssim.py:64-67): the loss and dL/dimg for
+// dL/dloss = 1, as
+//   dL/dx = (1-lam)/n sign(x-y) - lam/n (w*A + 2x (w*B) + y (w*C)),
+//   A = dS/dmu_x, B = dS/dE[x^2], C = dS/dE[xy] per pixel.
+// OpenMP over rows.
+// ---------------------------------------------------------------------------
+static void blur_plane(int H, int W, const float* w11, const float* in, float* tmp, float* out) {
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      float a = 0.f;
+      for (int t = 0; t < 11; ++t) {
+        const int xx = x + t - 5;
+        if (xx >= 0 && xx < W) a = a + w11[t] * in[(size_t)y * W + xx];
+      }
+      tmp[(size_t)y * W + x] = a;
+    }
+#pragma omp parallel for schedule(static)
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      float a = 0.f;
+      for (int t = 0; t < 11; ++t) {
+        const int yy = y + t - 5;
+        if (yy >= 0 && yy < H) a = a + w11[t] * tmp[(size_t)yy * W + x];
+      }
+      out[(size_t)y * W + x] = a;
+    }
+}
+
+double l1_ssim(int C, int H, int W, const float* img, const float* tgt, const float* w11, float C1, float C2,
+               float lam, float* dimg) {
+  const size_t HW = (size_t)H * W;
+  std::vector<float> xx(HW), yy(HW), xy(HW), tmp(HW), m1(HW), m2(HW), e11(HW), e22(HW), e12(HW);
+  std::vector<float> A(HW), B(HW), Cc(HW), bA(HW), bB(HW), bC(HW);
+  const double n = (double)C * HW;
+  double l1 = 0.0, ss = 0.0;
+  for (int c = 0; c < C; ++c) {
+    const float* x = img + c * HW;
+    const float* y = tgt + c * HW;
+#pragma omp parallel for schedule(static)
+    for (size_t p = 0; p < HW; ++p) {
+      xx[p] = x[p] * x[p];
+      yy[p] = y[p] * y[p];
+      xy[p] = x[p] * y[p];
+    }
+    blur_plane(H, W, w11, x, tmp.data(), m1.data());
+    blur_plane(H, W, w11, y, tmp.data(), m2.data());
+    blur_plane(H, W, w11, xx.data(), tmp.data(), e11.data());
+    blur_plane(H, W, w11, yy.data(), tmp.data(), e22.data());
+    blur_plane(H, W, w11, xy.data(), tmp.data(), e12.data());
+    double l1c = 0.0, ssc = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : l1c, ssc)
+    for (size_t p = 0; p < HW; ++p) {
+      const float mx = m1[p], my = m2[p];
+      const float s1 = e11[p] - mx * mx, s2 = e22[p] - my * my, s12 = e12[p] - mx * my;
+      const float N1 = 2.f * mx * my + C1, N2 = 2.f * s12 + C2;
+      const float D1 = mx * mx + my * my + C1, D2 = s1 + s2 + C2;
+      const float DD = D1 * D2;
+      ssc += (double)((N1 * N2) / DD);
+      l1c += (double)std::fabs(x[p] - y[p]);
+      const float S = (N1 * N2) / DD;
+      A[p] = 2.f * my * N2 / DD - 2.f * mx * S / D1 + 2.f * mx * S / D2 - 2.f * my * N1 / DD;
+      B[p] = -S / D2;
+      Cc[p] = 2.f * N1 / DD;
+    }
+    l1 += l1c;
+    ss += ssc;
+    if (dimg) {
+      blur_plane(H, W, w11, A.data(), tmp.data(), bA.data());
+      blur_plane(H, W, w11, B.data(), tmp.data(), bB.data());
+      blur_plane(H, W, w11, Cc.data(), tmp.data(), bC.data());
+      const float cl = (float)((1.0 - lam) / n), cs = (float)(lam / n);
+      float* d = dimg + c * HW;
+#pragma omp parallel for schedule(static)
+      for (size_t p = 0; p < HW; ++p) {
+        const float sg = x[p] > y[p] ? 1.f : (x[p] < y[p] ? -1.f : 0.f);
+        d[p] = cl * sg - cs * ((bA[p] + 2.f * x[p] * bB[p]) + y[p] * bC[p]);
+      }
+    }
+  }
+  return (1.0 - lam) * l1 / n + lam * (1.0 - ss / n);
+}
+
+// dL/dshs[n, l*K + k] = dL/dw[n, k] * Y_l(n) (gsmodel.py:140-143), in FP32.
+void sh_backward_full(int64_t n, int K, int L, const float* dw, const float* basis, float* dshs) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i)
+    for (int l = 0; l < L; ++l)
+      for (int k = 0; k < K; ++k) dshs[(i * L + l) * K + k] = dw[i * K + k] * basis[i * L + l];
+}
+
+// torch.optim.Adam's element update (no weight decay), scalars in double.
+void adam(int64_t n, float* p, const float* g, float* m, float* v, int64_t step, double lr, double b1,
+          double b2, double eps) {
+  const double bc1 = 1.0 - std::pow(b1, (double)step), bc2 = 1.0 - std::pow(b2, (double)step);
+  const float a = (float)(-lr / bc1), rs = (float)(1.0 / std::sqrt(bc2)), e = (float)eps;
+  const float c1 = (float)(1.0 - b1), c2 = (float)(1.0 - b2), fb2 = (float)b2;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const float gi = g[i];
+    m[i] = m[i] + c1 * (gi - m[i]);
+    v[i] = v[i] * fb2 + (c2 * gi) * gi;
+    p[i] = p[i] + a * (m[i] / (std::sqrt(v[i]) * rs + e));
+  }
+}
+}  // namespace orc
+
 extern "C" {
 ORC_BOTH(f, float, f)
 ORC_BOTH(d, double, d)
@@ -1205,6 +1351,17 @@ void orc_gauss_half_check(float lo, float hi, int64_t* checked, int64_t* differ,
   *checked = (int64_t)b - (int64_t)a + 1;
   *differ = nd;
   *first = nd ? orc::bits2f(fb) : 0.f;
+}
+double orc_l1_ssim_f(int C, int H, int W, const float* img, const float* tgt, const float* w11, float C1, float C2,
+                     float lam, float* dimg) {
+  return orc::l1_ssim(C, H, W, img, tgt, w11, C1, C2, lam, dimg);
+}
+void orc_sh_backward_full_f(int64_t n, int K, int L, const float* dw, const float* basis, float* dshs) {
+  orc::sh_backward_full(n, K, L, dw, basis, dshs);
+}
+void orc_adam_f(int64_t n, float* p, const float* g, float* m, float* v, int64_t step, double lr, double b1,
+                double b2, double eps) {
+  orc::adam(n, p, g, m, v, step, lr, b1, b2, eps);
 }
 int orc_num_threads(void) {
 #ifdef _OPENMP

@@ -1,9 +1,8 @@
-// Tile binning for gfx950: inclusive scan of per-Gaussian tile counts, key
-// duplication, a stable LSD radix sort of the (tile, uint32(depth*1000)) keys
-// and per-tile ranges.  Replaces thrust::inclusive_scan, createKeys,
-// thrust::sort_by_key and getRanges of gsplatcu/gausplat.cu:199-226 /
-// kernel.cu:96-187, producing the same order: tile ascending, quantised depth
-// ascending, Gaussian id ascending (the input order a stable sort keeps).
+// Tile binning for gfx950: the pair count M, the tile lists and per-tile
+// ranges.  Replaces thrust::inclusive_scan, createKeys, thrust::sort_by_key
+// and getRanges of gsplatcu/gausplat.cu:199-226 / kernel.cu:96-187,
+// producing the same order: tile ascending, quantised depth ascending,
+// Gaussian id ascending (the input order a stable sort keeps).
 //
 // Nothing here synchronises the host.  Sizes that the reference reads back
 // (the pair count M, gausplat.cu:201) stay on the device: every kernel is
@@ -11,12 +10,15 @@
 // header; if M exceeds the capacity the overflow flag is raised and all
 // downstream kernels see zero pairs.
 //
-// Sort: 8-bit digits, one pass per key byte; passes whose digit is the same
-// for every key (depth bytes above the largest quantised depth, tile bytes
-// above the tile count) are skipped, decided on the device.  Each pass is
-// histogram -> per-digit scan over blocks -> scatter, where the scatter ranks
-// keys stably inside the block with wavefront ballots, reorders them in LDS
-// and writes coalesced runs.
+// Two paths.  Grids up to 3840x2160 take the tile-bucket path (two-level
+// bucketing into the tiles' lists, then a sort inside each tile; below).
+// Larger grids take an LSD radix sort of the (tile, depth) keys with 8-bit
+// digits, one pass per key byte; passes whose digit is the same for every key
+// (depth bytes above the largest quantised depth, tile bytes above the tile
+// count) are skipped, decided on the device.  Each pass is histogram ->
+// per-digit scan over blocks -> scatter, where the scatter ranks keys stably
+// inside the block with wavefront ballots, reorders them in LDS and writes
+// coalesced runs.
 #include "rgp_internal.h"
 
 namespace rgp {
@@ -113,7 +115,6 @@ __global__ __launch_bounds__(1024) void k_scan_top(int nscan, uint32_t* __restri
   if ((threadIdx.x & 63) == 0) s_dq[threadIdx.x >> 6] = dqm;
   __syncthreads();
   if (threadIdx.x == 0) {
-    hdr[kHdrBig] = 0;
     uint32_t dq = 0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) dq = max(dq, s_dq[w]);
     hdr[kHdrDqMax] = dq;
@@ -432,95 +433,285 @@ __global__ __launch_bounds__(256) void k_ranges(int tiles, const uint32_t* __res
 // produces inside one tile is ascending (uint32(depth*1000), Gaussian id):
 // createKeys writes the pairs of Gaussian i before those of i + 1
 // (kernel.cu:96-124) and the radix sort keeps that order among equal keys.
-// So: count pairs per (tile, Gaussian block) in LDS, scan those counts in
-// tile-major order (which yields M, every tile's range and every block's
-// slots), scatter the Gaussian ids into their tile's slots with LDS atomics
-// (any order inside a tile), then sort each tile by the key
-// (depth_q << 32 | id), whose values are all distinct, so the result is the
-// reference's order.
+// So the tile lists may be filled in any order and each sorted by the key
+// (depth_q << 32 | id), whose values are all distinct:
+//   1. k_st_count: blocks of Gaussians count their (super-tile, Gaussian)
+//      pairs in LDS and reserve slots of each super-tile's coarse list;
+//   2. k_st_plan: M (from the tile counts), the overflow flag, the coarse
+//      lists' offsets;
+//   3. k_st_scatter: each coarse entry carries the Gaussian's id, tile rect
+//      and quantised depth (written by the preprocess in Gaussian order);
+//   4. k_st_fine_count + k_tile_offsets: one block per super-tile counts its
+//      tiles' pairs; one block scans them into the tile ranges;
+//   5. k_st_fine_scatter: one block per super-tile writes its tiles' complete
+//      lists of sort keys (every line of a list written by one block);
+//   6. k_tb_sort (lists <= 1024, registers), k_tb_sort_long (longer ones).
+// Super-tiles are 4 x 4 tiles up to 2560x1440 and 8 x 8 above, a few
+// hundred of them either way.
 // ---------------------------------------------------------------------------
 
-// Calls fn(tile, gaussian) for every pair of the Gaussians [g0, g1): a thread
-// walks the rect of its own Gaussian when it covers at most kTbBig tiles;
-// larger rects are walked by the whole wavefront, 64 tiles a step.
+// Super-tile rect [sx0, sx1) x [sy0, sy1) of a packed tile rect (x1, y1
+// exclusive, non-empty), super-tiles of 2^sts x 2^sts tiles.
+struct StRect {
+  uint32_t x0, y0, x1, y1;
+};
+__device__ __forceinline__ StRect st_rect(uint2 r, uint32_t sts) {
+  const uint32_t x0 = r.x & 0xffffu, y0 = r.x >> 16, x1 = r.y & 0xffffu, y1 = r.y >> 16;
+  return StRect{x0 >> sts, y0 >> sts, ((x1 - 1) >> sts) + 1, ((y1 - 1) >> sts) + 1};
+}
+
+// Calls fn(s, i, rect, w) for every super-tile s that the rect of Gaussian i
+// in [g0, g1) reaches (w = extra[i], or 0 without extra), and returns the
+// thread's sum of the tile counts of its Gaussians: a thread walks its own
+// Gaussian's super-tiles when there are at most kTbBig of them; more are
+// walked by the whole wavefront, 64 a step.
 template <class F>
-__device__ __forceinline__ void tb_for_pairs(int g0, int g1, const uint32_t* __restrict__ counts,
-                                             const uint2* __restrict__ rects, uint32_t gx, F fn) {
+__device__ __forceinline__ uint32_t st_for_pairs(int g0, int g1, const uint32_t* __restrict__ counts,
+                                                 const uint2* __restrict__ rects,
+                                                 const uint32_t* __restrict__ extra, uint32_t sts, uint32_t sgx,
+                                                 F fn) {
   const uint32_t lane = threadIdx.x & 63;
+  uint32_t fine = 0;
   for (int b0 = g0; b0 < g1; b0 += (int)blockDim.x) {
     const int i = b0 + (int)threadIdx.x;
-    uint32_t c = 0;
+    uint32_t c = 0, w = 0;
     uint2 r = make_uint2(0u, 0u);
     if (i < g1) {
       c = counts[i];
-      if (c) r = rects[i];
+      if (c) {
+        r = rects[i];
+        if (extra) w = extra[i];
+      }
     }
-    const uint32_t x0 = r.x & 0xffffu, y0 = r.x >> 16, x1 = r.y & 0xffffu;
-    if (c != 0 && c <= (uint32_t)kTbBig) {
-      uint32_t tx = x0, row = y0 * gx;
-      for (uint32_t k = 0; k < c; ++k) {
-        fn(row + tx, i);
-        if (++tx == x1) {
-          tx = x0;
-          row += gx;
+    fine += c;
+    StRect q = c ? st_rect(r, sts) : StRect{0, 0, 0, 0};
+    const uint32_t sw = q.x1 - q.x0, sc = sw * (q.y1 - q.y0);
+    if (sc != 0 && sc <= (uint32_t)kTbBig) {
+      uint32_t sx = q.x0, row = q.y0 * sgx;
+      for (uint32_t k = 0; k < sc; ++k) {
+        fn(row + sx, i, r, w);
+        if (++sx == q.x1) {
+          sx = q.x0;
+          row += sgx;
         }
       }
     }
-    uint64_t big = __ballot(c > (uint32_t)kTbBig);
+    uint64_t big = __ballot(sc > (uint32_t)kTbBig);
     while (big) {
       const int src = __ffsll((unsigned long long)big) - 1;
       big &= big - 1;
-      const uint32_t bc = __shfl(c, src);
-      const uint32_t bx0 = __shfl(x0, src), by0 = __shfl(y0, src), bx1 = __shfl(x1, src);
+      const uint32_t bc = __shfl(sc, src), bw = __shfl(sw, src);
+      const uint32_t bx0 = __shfl(q.x0, src), by0 = __shfl(q.y0, src);
       const int bi = __shfl(i, src);
-      const uint32_t w = bx1 - bx0;
+      const uint32_t bwd = (uint32_t)__shfl((int)w, src);
+      const uint2 br = make_uint2((uint32_t)__shfl((int)r.x, src), (uint32_t)__shfl((int)r.y, src));
       for (uint32_t k = lane; k < bc; k += 64) {
-        const uint32_t yy = k / w, xx = k - yy * w;
-        fn((by0 + yy) * gx + bx0 + xx, bi);
+        const uint32_t yy = k / bw, xx = k - yy * bw;
+        fn((by0 + yy) * sgx + bx0 + xx, bi, br, bwd);
       }
     }
   }
+  return fine;
 }
 
-// Block b counts the pairs of its Gaussians per tile; tbpart[t * nb + b].
-__global__ __launch_bounds__(kTbThreads) void k_tb_count(int n, int chunk, int tiles, int nb, int gx,
-                                                         const uint32_t* __restrict__ counts,
-                                                         const uint2* __restrict__ rects,
-                                                         uint32_t* __restrict__ tbpart) {
-  extern __shared__ uint32_t s_hist[];  // tiles
-  // consecutive chunks on one XCD: their columns of tbpart, and their slots in
-  // a tile's list, are adjacent words that combine in that XCD's L2
-  const int b = xcd_tile(blockIdx.x, nb);
-  for (int q = threadIdx.x; q < tiles; q += kTbThreads) s_hist[q] = 0;
-  __syncthreads();
-  const int g0 = b * chunk, g1 = min(n, g0 + chunk);
-  tb_for_pairs(g0, g1, counts, rects, (uint32_t)gx,
-               [&](uint32_t tile, int) { atomicAdd(&s_hist[tile], 1u); });
-  __syncthreads();
-  for (int q = threadIdx.x; q < tiles; q += kTbThreads) tbpart[(size_t)q * nb + b] = s_hist[q];
-}
-
-// Block b writes the ids of its pairs into its slots of every tile: tbpart
-// holds inclusive offsets, so block b owns
-// [tbpart[t*nb+b] - count(t, b), tbpart[t*nb+b]).
-__global__ __launch_bounds__(kTbThreads) void k_tb_scatter(int n, int chunk, int tiles, int nb, int gx,
-                                                           const uint32_t* __restrict__ counts,
-                                                           const uint2* __restrict__ rects,
-                                                           const uint32_t* __restrict__ tbpart,
-                                                           const uint32_t* __restrict__ hdr,
-                                                           int32_t* __restrict__ ids) {
-  if (hdr[kHdrMEff] == 0) return;
-  extern __shared__ uint32_t s_cur[];  // tiles
-  const int b = xcd_tile(blockIdx.x, nb);
-  for (int q = threadIdx.x; q < tiles; q += kTbThreads) s_cur[q] = tbpart[(size_t)q * nb + b];
-  __syncthreads();
-  const int g0 = b * chunk, g1 = min(n, g0 + chunk);
-  tb_for_pairs(g0, g1, counts, rects, (uint32_t)gx,
-               [&](uint32_t tile, int g) { ids[atomicSub(&s_cur[tile], 1u) - 1u] = g; });
-}
-
+// In-tile sort key of Gaussian g: (uint32(depth * 1000) << 32) | g.
 __device__ __forceinline__ uint64_t tb_key(const float* __restrict__ depths, int32_t g) {
   return ((uint64_t)sat_u32(depths[g] * 1000.f) << 32) | (uint32_t)g;
+}
+
+// Coarse count: block b counts its Gaussians' (super-tile, Gaussian) pairs per
+// super-tile in LDS, reserves that many slots of each super-tile's coarse list
+// with one atomic on sttot[s] (cleared with the header), and records the end
+// of its slots, relative to the list's start, in boff[b * st + s]; bpart[b] =
+// the tile pairs of its Gaussians.
+__global__ __launch_bounds__(kTbThreads) void k_st_count(int n, int chunk, int st, int sts, int sgx,
+                                                         const uint32_t* __restrict__ counts,
+                                                         const uint2* __restrict__ rects,
+                                                         uint32_t* __restrict__ sttot,
+                                                         uint32_t* __restrict__ boff,
+                                                         uint32_t* __restrict__ bpart) {
+  extern __shared__ uint32_t s_hist[];  // st
+  __shared__ uint32_t lds[16];
+  const int b = blockIdx.x;
+  for (int q = threadIdx.x; q < st; q += kTbThreads) s_hist[q] = 0;
+  __syncthreads();
+  const int g0 = b * chunk, g1 = min(n, g0 + chunk);
+  const uint32_t fine = st_for_pairs(g0, g1, counts, rects, nullptr, (uint32_t)sts, (uint32_t)sgx,
+                                     [&](uint32_t s, int, uint2, uint32_t) { atomicAdd(&s_hist[s], 1u); });
+  uint32_t tot;
+  block_excl_scan(fine, &tot, lds);  // its barriers also order the histogram's atomics before the reads
+  for (int q = threadIdx.x; q < st; q += kTbThreads) {
+    const uint32_t h = s_hist[q];
+    if (h) boff[(size_t)b * st + q] = atomicAdd(&sttot[q], h) + h;
+  }
+  if (threadIdx.x == 0) bpart[b] = tot;
+}
+
+// In-place scan of a[0, n) by one block: thread t owns the contiguous run
+// [t * per, t * per + per) (per a multiple of 4, so the run is read and
+// written as 16-byte groups, a must be 16-byte aligned), sums it with all of
+// its loads in flight, takes its offset from one block scan of the sums and
+// rewrites the run.  Inclusive, or exclusive.  Returns the total.
+template <bool INCLUSIVE>
+__device__ __forceinline__ uint32_t block_scan_inplace(uint32_t* __restrict__ a, int64_t n, uint32_t* lds) {
+  const int64_t per = (n + 4 * (int64_t)blockDim.x - 1) / (4 * (int64_t)blockDim.x) * 4;
+  const int64_t lo = min(n, per * (int64_t)threadIdx.x), hi = min(n, lo + per);
+  auto load4 = [&](int64_t i) {
+    if (i + 4 <= hi) return *reinterpret_cast<const uint4*>(a + i);
+    return make_uint4(a[i], i + 1 < hi ? a[i + 1] : 0u, i + 2 < hi ? a[i + 2] : 0u, 0u);
+  };
+  uint32_t s = 0;
+#pragma unroll 8
+  for (int64_t i = lo; i < hi; i += 4) {
+    const uint4 v = load4(i);
+    s += (v.x + v.y) + (v.z + v.w);
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan(s, &tot, lds);
+#pragma unroll 4
+  for (int64_t i = lo; i < hi; i += 4) {
+    const uint4 v = load4(i);
+    uint4 o;
+    if (INCLUSIVE) o = make_uint4(run + v.x, run + v.x + v.y, run + v.x + v.y + v.z, run + v.x + v.y + v.z + v.w);
+    else o = make_uint4(run, run + v.x, run + v.x + v.y, run + v.x + v.y + v.z);
+    run += (v.x + v.y) + (v.z + v.w);
+    if (i + 4 <= hi) {
+      *reinterpret_cast<uint4*>(a + i) = o;
+    } else {
+      a[i] = o.x;
+      if (i + 1 < hi) a[i + 1] = o.y;
+      if (i + 2 < hi) a[i + 2] = o.z;
+    }
+  }
+  return tot;
+}
+
+// One block: M = the sum of bpart (the pair count of the reference's scan,
+// gausplat.cu:199-201), the overflow flag, and the super-tile totals ->
+// exclusive offsets in place (sttot[st] = the coarse pairs), so super-tile
+// s's coarse list is [sttot[s], sttot[s + 1]).
+__global__ __launch_bounds__(1024) void k_st_plan(int st, uint32_t* __restrict__ sttot, int nb,
+                                                  const uint32_t* __restrict__ bpart,
+                                                  uint32_t* __restrict__ hdr, int64_t cap) {
+  __shared__ uint32_t lds[16];
+  uint32_t m = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) m += bpart[b];
+  uint32_t M;
+  block_excl_scan(m, &M, lds);
+  const uint32_t ms = block_scan_inplace<false>(sttot, st, lds);
+  if (threadIdx.x == 0) {
+    sttot[st] = ms;
+    const bool ovf = (int64_t)M > cap;
+    hdr[kHdrM] = M;
+    hdr[kHdrOverflow] = ovf ? 1u : 0u;
+    hdr[kHdrMEff] = ovf ? 0u : M;
+  }
+}
+
+// Coarse scatter: block b writes, for each (super-tile, Gaussian) pair of its
+// Gaussians, the Gaussian's id, tile rect and quantised depth into its slots
+// of the super-tile's coarse list.  Coarse pairs never outnumber tile pairs,
+// so the lists fit in the pair capacity whenever M does.
+__global__ __launch_bounds__(kTbThreads) void k_st_scatter(int n, int chunk, int st, int sts, int sgx,
+                                                           const uint32_t* __restrict__ counts,
+                                                           const uint2* __restrict__ rects,
+                                                           const uint32_t* __restrict__ dqs,
+                                                           const uint32_t* __restrict__ sttot,
+                                                           const uint32_t* __restrict__ boff,
+                                                           const uint32_t* __restrict__ hdr,
+                                                           uint4* __restrict__ clist) {
+  if (hdr[kHdrMEff] == 0) return;
+  extern __shared__ uint32_t s_cur[];  // st; entries of super-tiles this block has no pair in are not used
+  const int b = blockIdx.x;
+  for (int q = threadIdx.x; q < st; q += kTbThreads) s_cur[q] = sttot[q] + boff[(size_t)b * st + q];
+  __syncthreads();
+  const int g0 = b * chunk, g1 = min(n, g0 + chunk);
+  st_for_pairs(g0, g1, counts, rects, dqs, (uint32_t)sts, (uint32_t)sgx, [&](uint32_t s, int g, uint2 r, uint32_t dq) {
+    clist[atomicSub(&s_cur[s], 1u) - 1u] = make_uint4((uint32_t)g, r.x, r.y, dq);
+  });
+}
+
+// Calls fn(t) for every tile t (0..kst*kst-1, row-major inside the super-tile
+// of edge kst = 2^sts whose first tile is (tx0, ty0)) that the packed tile
+// rect (rx, ry) covers.
+template <class F>
+__device__ __forceinline__ void st_tiles(uint32_t rx, uint32_t ry, uint32_t tx0, uint32_t ty0, uint32_t sts,
+                                         F fn) {
+  const uint32_t kst = 1u << sts;
+  const uint32_t x0 = rx & 0xffffu, y0 = rx >> 16, x1 = ry & 0xffffu, y1 = ry >> 16;
+  const uint32_t ix0 = max(x0, tx0) - tx0, ix1 = min(x1, tx0 + kst) - tx0;
+  const uint32_t iy0 = max(y0, ty0) - ty0, iy1 = min(y1, ty0 + kst) - ty0;
+  for (uint32_t y = iy0; y < iy1; ++y)
+    for (uint32_t x = ix0; x < ix1; ++x) fn((y << sts) + x);
+}
+
+// Fine count: one block per super-tile counts the pairs of each of its tiles
+// from its coarse list; toff[tile] = the count (scanned by k_tile_offsets).
+__global__ __launch_bounds__(1024) void k_st_fine_count(int sts, int sgx, int gx, int gy,
+                                                        const uint32_t* __restrict__ sttot,
+                                                        const uint32_t* __restrict__ hdr,
+                                                        const uint4* __restrict__ clist,
+                                                        uint32_t* __restrict__ toff) {
+  __shared__ uint32_t cnt[kStMax * kStMax];
+  if (hdr[kHdrMEff] == 0) return;
+  const int s = blockIdx.x;
+  const uint32_t kst = 1u << sts, nt = kst * kst;
+  const uint32_t tx0 = (uint32_t)(s % sgx) << sts, ty0 = (uint32_t)(s / sgx) << sts;
+  if (threadIdx.x < nt) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t start = sttot[s], end = sttot[s + 1];
+  for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
+    const uint4 c = clist[e];
+    st_tiles(c.y, c.z, tx0, ty0, sts, [&](uint32_t t) { atomicAdd(&cnt[t], 1u); });
+  }
+  __syncthreads();
+  if (threadIdx.x < nt) {
+    const uint32_t tx = tx0 + (threadIdx.x & (kst - 1)), ty = ty0 + (threadIdx.x >> sts);
+    if (tx < (uint32_t)gx && ty < (uint32_t)gy) toff[ty * gx + tx] = cnt[threadIdx.x];
+  }
+}
+
+// One block: the tile counts -> exclusive offsets in place, toff[tiles] = M
+// (all zero when the pairs overflowed the capacity: k_st_fine_count then
+// wrote nothing).
+__global__ __launch_bounds__(1024) void k_tile_offsets(int tiles, uint32_t* __restrict__ toff,
+                                                       const uint32_t* __restrict__ hdr) {
+  __shared__ uint32_t lds[16];
+  if (hdr[kHdrMEff] == 0) {
+    for (int e = threadIdx.x; e <= tiles; e += blockDim.x) toff[e] = 0u;
+    return;
+  }
+  const uint32_t tot = block_scan_inplace<false>(toff, tiles, lds);
+  if (threadIdx.x == 0) toff[tiles] = tot;
+}
+
+// Fine scatter: one block per super-tile writes the sort keys of its tiles'
+// pairs, each tile's list filled from its start by this block alone (LDS
+// cursors), so the lines of the key list are written whole.  The keys come
+// with the coarse entries, so the in-tile sorts read them in list order
+// instead of gathering depths per pair.
+__global__ __launch_bounds__(1024) void k_st_fine_scatter(int sts, int sgx, int gx, int gy,
+                                                          const uint32_t* __restrict__ sttot,
+                                                          const uint32_t* __restrict__ hdr,
+                                                          const uint4* __restrict__ clist,
+                                                          const uint32_t* __restrict__ toff,
+                                                          uint64_t* __restrict__ tkeys) {
+  __shared__ uint32_t cur[kStMax * kStMax];
+  if (hdr[kHdrMEff] == 0) return;
+  const int s = blockIdx.x;
+  const uint32_t kst = 1u << sts, nt = kst * kst;
+  const uint32_t tx0 = (uint32_t)(s % sgx) << sts, ty0 = (uint32_t)(s / sgx) << sts;
+  if (threadIdx.x < nt) {
+    const uint32_t tx = tx0 + (threadIdx.x & (kst - 1)), ty = ty0 + (threadIdx.x >> sts);
+    cur[threadIdx.x] = (tx < (uint32_t)gx && ty < (uint32_t)gy) ? toff[ty * gx + tx] : 0u;
+  }
+  __syncthreads();
+  const uint32_t start = sttot[s], end = sttot[s + 1];
+  for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
+    const uint4 c = clist[e];
+    const uint64_t key = ((uint64_t)c.w << 32) | c.x;
+    st_tiles(c.y, c.z, tx0, ty0, sts, [&](uint32_t t) { tkeys[atomicAdd(&cur[t], 1u)] = key; });
+  }
 }
 
 // Bitonic network whose comparators all put the smaller key at the lower
@@ -572,10 +763,10 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t n) {
   return P;
 }
 
-__device__ __forceinline__ void tb_range(int tile, int nb, const uint32_t* __restrict__ tbpart,
-                                         uint32_t* start, uint32_t* end) {
-  *start = tile == 0 ? 0u : tbpart[(size_t)tile * nb - 1];
-  *end = tbpart[(size_t)tile * nb + nb - 1];
+__device__ __forceinline__ void tb_range(int tile, const uint32_t* __restrict__ toff, uint32_t* start,
+                                         uint32_t* end) {
+  *start = toff[tile];
+  *end = toff[tile + 1];
 }
 
 // ---------------------------------------------------------------------------
@@ -678,34 +869,52 @@ __device__ __forceinline__ void wave_bitonic_from(uint64_t (&k)[KPL], int lane) 
   }
 }
 
-// Sort the tile list ids[start, start + n) (n <= 64 KPL) by (depth_q, id).
+// Index of sorted position x in the LDS buffers of the in-tile sorts: two
+// spare slots after every 16 keys, so a lane's 16 consecutive keys start 144
+// bytes after its neighbour's and the 16-byte accesses of a wavefront spread
+// over the banks.
+__device__ __forceinline__ uint32_t tb_pad(uint32_t x) { return x + 2 * (x >> 4); }
+constexpr int kTbPadded = kTbSortSmall + 2 * (kTbSortSmall / 16);
+constexpr int kTbPaddedReg = kTbSortReg + 2 * (kTbSortReg / 16);
+
+// Writes sorted keys s[tb_pad(e)], e < n, as ids (and tile-tagged keys)
+// with consecutive threads on consecutive positions.
+__device__ __forceinline__ void tb_emit(const uint64_t* __restrict__ s, uint32_t n, int32_t* __restrict__ ids,
+                                        uint64_t* __restrict__ keys_out, uint64_t thi, uint32_t start) {
+  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const uint64_t key = s[tb_pad(e)];
+    ids[start + e] = (int32_t)(uint32_t)key;
+    if (keys_out) keys_out[start + e] = thi | (key >> 32);
+  }
+}
+
+// Sort the tile list tkeys[start, start + n) (n <= 64 KPL) by (depth_q, id)
+// into ids[start, start + n).  Lane l ends with the keys of positions
+// l*KPL .. l*KPL+KPL-1; they pass through LDS (s) so that the id stores are
+// coalesced.
 template <int KPL>
-__device__ __forceinline__ void tb_wave_sort(const float* __restrict__ depths, int32_t* __restrict__ ids,
+__device__ __forceinline__ void tb_wave_sort(const uint64_t* __restrict__ tkeys, int32_t* __restrict__ ids,
                                              uint64_t* __restrict__ keys_out, uint64_t thi, uint32_t start,
-                                             uint32_t n, int lane) {
+                                             uint32_t n, int lane, uint64_t* __restrict__ s) {
   uint64_t k[KPL];
 #pragma unroll
   for (int r = 0; r < KPL; ++r) {
     const uint32_t e = (uint32_t)(r * 64 + lane);  // coalesced loads; any placement is sorted alike
-    k[r] = e < n ? tb_key(depths, ids[start + e]) : ~0ull;
+    k[r] = e < n ? tkeys[start + e] : ~0ull;
   }
   wave_bitonic_from<KPL, 2>(k, lane);
 #pragma unroll
-  for (int r = 0; r < KPL; ++r) {
-    const uint32_t i = (uint32_t)(lane * KPL + r);
-    if (i < n) {
-      ids[start + i] = (int32_t)(uint32_t)k[r];
-      if (keys_out) keys_out[start + i] = thi | (k[r] >> 32);
-    }
-  }
+  for (int r = 0; r < KPL; ++r) s[tb_pad((uint32_t)(lane * KPL + r))] = k[r];
+  __syncthreads();
+  tb_emit(s, n, ids, keys_out, thi, start);
 }
 
 // One wavefront per tile: its range, and the sort of its ids when the list
 // has at most kTbSortReg entries (in registers, KPL the smallest power of two
-// with 64 KPL >= n).  Longer lists are left to k_tb_sort_mid.
-__global__ __launch_bounds__(64) void k_tb_sort(int tiles, int nb, const uint32_t* __restrict__ hdr,
-                                                const uint32_t* __restrict__ tbpart,
-                                                const float* __restrict__ depths,
+// with 64 KPL >= n).  Longer lists are left to k_tb_sort_long.
+__global__ __launch_bounds__(64) void k_tb_sort(int tiles, const uint32_t* __restrict__ hdr,
+                                                const uint32_t* __restrict__ toff,
+                                                const uint64_t* __restrict__ tkeys,
                                                 int32_t* __restrict__ ids,
                                                 uint64_t* __restrict__ keys_out,
                                                 int2* __restrict__ ranges) {
@@ -714,130 +923,180 @@ __global__ __launch_bounds__(64) void k_tb_sort(int tiles, int nb, const uint32_
   const int lane = threadIdx.x;
   const uint32_t M = hdr[kHdrMEff];
   uint32_t start, end;
-  tb_range(tile, nb, tbpart, &start, &end);
+  tb_range(tile, toff, &start, &end);
   // empty tiles keep (0,0), and so does every tile when M == 1 (kernel.cu:165-187)
   if (lane == 0) ranges[tile] = (M <= 1 || end == start) ? make_int2(0, 0) : make_int2((int)start, (int)end);
   if (M == 0 || end == start) return;
   const uint32_t n = end - start;
   if (n > (uint32_t)kTbSortReg) return;
   const uint64_t thi = (uint64_t)tile << 32;
-  if (n <= 64) tb_wave_sort<1>(depths, ids, keys_out, thi, start, n, lane);
-  else if (n <= 128) tb_wave_sort<2>(depths, ids, keys_out, thi, start, n, lane);
-  else if (n <= 256) tb_wave_sort<4>(depths, ids, keys_out, thi, start, n, lane);
-  else if (n <= 512) tb_wave_sort<8>(depths, ids, keys_out, thi, start, n, lane);
-  else tb_wave_sort<16>(depths, ids, keys_out, thi, start, n, lane);
+  __shared__ uint64_t s[kTbPaddedReg];
+  if (n <= 64) tb_wave_sort<1>(tkeys, ids, keys_out, thi, start, n, lane, s);
+  else if (n <= 128) tb_wave_sort<2>(tkeys, ids, keys_out, thi, start, n, lane, s);
+  else if (n <= 256) tb_wave_sort<4>(tkeys, ids, keys_out, thi, start, n, lane, s);
+  else if (n <= 512) tb_wave_sort<8>(tkeys, ids, keys_out, thi, start, n, lane, s);
+  else tb_wave_sort<16>(tkeys, ids, keys_out, thi, start, n, lane, s);
 }
 
-// Lists of kTbSortReg + 1 .. kTbSortSmall entries: one 256-thread block per
-// tile, the bitonic network in LDS (at these lengths a wavefront would need 32
-// keys per lane, about 200 registers, and ran slower); lists longer than
-// kTbSortSmall go to the queue of k_tb_sort_big.  Blocks of other tiles exit
-// at once.
-__global__ __launch_bounds__(256) void k_tb_sort_mid(int tiles, int nb, uint32_t* __restrict__ hdr,
-                                                     const uint32_t* __restrict__ tbpart,
-                                                     const float* __restrict__ depths,
-                                                     int32_t* __restrict__ ids,
-                                                     uint64_t* __restrict__ keys_out,
-                                                     uint32_t* __restrict__ bigq) {
-  __shared__ uint64_t s[kTbSortSmall];
-  const int tile = blockIdx.x;
-  if (hdr[kHdrMEff] == 0) return;
-  uint32_t start, end;
-  tb_range(tile, nb, tbpart, &start, &end);
-  const uint32_t n = end - start;
-  if (n <= (uint32_t)kTbSortReg) return;
-  if (n > (uint32_t)kTbSortSmall) {
-    if (threadIdx.x == 0) bigq[atomicAdd(&hdr[kHdrBig], 1u)] = (uint32_t)tile;
-    return;
+
+// Merge path over two sorted runs of L keys each, s[a0, a0 + L) and
+// s[b0, b0 + L) (unpadded positions; keys distinct except the ~0 padding):
+// the cnt outputs from diagonal d, handed to out(q, key) in order.  The split
+// point is the number of keys the first d outputs take from the first run,
+// found by binary search; the first run wins ties, so the merge is stable.
+template <class F>
+__device__ __forceinline__ void tb_merge_path(const uint64_t* __restrict__ s, uint32_t a0, uint32_t b0, uint32_t L,
+                                              uint32_t d, int cnt, F out) {
+  uint32_t lo = d > L ? d - L : 0u, hi = d < L ? d : L;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s[tb_pad(a0 + mid)] <= s[tb_pad(b0 + d - 1 - mid)]) lo = mid + 1;
+    else hi = mid;
   }
-  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) s[e] = tb_key(depths, ids[start + e]);
+  uint32_t i = lo, j = d - lo;
+  uint64_t x = i < L ? s[tb_pad(a0 + i)] : ~0ull;
+  uint64_t y = j < L ? s[tb_pad(b0 + j)] : ~0ull;
+  for (int q = 0; q < cnt; ++q) {
+    const bool ta = j >= L || (i < L && x <= y);
+    out(q, ta ? x : y);
+    if (ta) {
+      ++i;
+      x = i < L ? s[tb_pad(a0 + i)] : ~0ull;
+    } else {
+      ++j;
+      y = j < L ? s[tb_pad(b0 + j)] : ~0ull;
+    }
+  }
+}
+
+// Lists of kTbSortReg + 1 .. kTbSortSmall entries, by the 256 threads of a
+// block.  Wavefront w sorts keys [1024 w, 1024 w + 1024) in registers (the
+// runs past the list are ~0), the runs go to LDS, and merge path joins them:
+// two runs in one pass, four in two; each pass holds its outputs in registers
+// until every thread has read its inputs, then writes them back to the same
+// buffer, and the result leaves through tb_emit (coalesced id stores).
+__device__ __forceinline__ void tb_sort_runs(uint64_t* __restrict__ s, const uint64_t* __restrict__ tkeys,
+                                             int32_t* __restrict__ ids, uint64_t* __restrict__ keys_out,
+                                             int tile, uint32_t start, uint32_t n) {
+  static_assert(kTbSortReg == 1024 && kTbSortSmall == 4 * kTbSortReg, "up to four register runs of 1024 keys");
+  const uint32_t w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const uint32_t runs = n <= 2u * kTbSortReg ? 2u : 4u;
+  if (w < runs) {
+    const uint32_t base = w * kTbSortReg;
+    uint64_t k[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const uint32_t e = base + (uint32_t)(r * 64 + lane);
+      k[r] = e < n ? tkeys[start + e] : ~0ull;
+    }
+    if (base < n) wave_bitonic_from<16, 2>(k, lane);
+#pragma unroll
+    for (int r = 0; r < 16; r += 2)
+      *reinterpret_cast<ulonglong2*>(&s[tb_pad(base + (uint32_t)lane * 16 + r)]) = make_ulonglong2(k[r], k[r + 1]);
+  }
   __syncthreads();
-  tb_bitonic(s, n, pow2_at_least(n));
   const uint64_t thi = (uint64_t)tile << 32;
+  uint64_t o[16];
+  if (runs == 2) {
+    const uint32_t d = threadIdx.x * 8;
+    tb_merge_path(s, 0, kTbSortReg, kTbSortReg, d, 8, [&](int q, uint64_t key) { o[q] = key; });
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; q += 2)
+      *reinterpret_cast<ulonglong2*>(&s[tb_pad(d + q)]) = make_ulonglong2(o[q], o[q + 1]);
+  } else {
+    {
+      // runs (0,1) and (2,3), 128 threads each, 16 outputs per thread
+      const uint32_t a0 = (threadIdx.x >> 7) * 2 * kTbSortReg, d = (threadIdx.x & 127) * 16;
+      tb_merge_path(s, a0, a0 + kTbSortReg, kTbSortReg, d, 16, [&](int q, uint64_t key) { o[q] = key; });
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 16; q += 2)
+        *reinterpret_cast<ulonglong2*>(&s[tb_pad(a0 + d + q)]) = make_ulonglong2(o[q], o[q + 1]);
+    }
+    __syncthreads();
+    const uint32_t d = threadIdx.x * 16;
+    tb_merge_path(s, 0, 2 * kTbSortReg, 2 * kTbSortReg, d, 16, [&](int q, uint64_t key) { o[q] = key; });
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q += 2)
+      *reinterpret_cast<ulonglong2*>(&s[tb_pad(d + q)]) = make_ulonglong2(o[q], o[q + 1]);
+  }
+  __syncthreads();
+  tb_emit(s, n, ids, keys_out, thi, start);
+}
+
+// Lists longer than kTbSortSmall: chunks of kTbSortLds keys sorted in LDS,
+// then merged by the network's stages that cross chunks in global memory
+// (tkeys) and the rest of each merge in LDS.
+__device__ __forceinline__ void tb_sort_huge(uint64_t* __restrict__ s, uint64_t* __restrict__ tkeys,
+                                             int32_t* __restrict__ ids, uint64_t* __restrict__ keys_out,
+                                             int tile, uint32_t start, uint32_t n) {
+  constexpr uint32_t C = kTbSortLds;
+  const uint64_t thi = (uint64_t)tile << 32;
+  uint64_t* seg = tkeys + start;
+  const uint32_t P = pow2_at_least(n);
+  // chunks sorted in LDS (the network's blocks of size <= C)
+  for (uint32_t c0 = 0; c0 < n; c0 += C) {
+    const uint32_t nc = min(C, n - c0);
+    for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) s[e] = seg[c0 + e];
+    __syncthreads();
+    tb_bitonic(s, nc, C);
+    for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) seg[c0 + e] = s[e];
+    __threadfence_block();
+    __syncthreads();
+  }
+  for (uint32_t k = 2 * C; k <= P; k <<= 1) {
+    tb_flip(seg, n, P, k);
+    __threadfence_block();
+    __syncthreads();
+    for (uint32_t j = k >> 2; j >= C; j >>= 1) {
+      tb_half(seg, n, P, j);
+      __threadfence_block();
+      __syncthreads();
+    }
+    for (uint32_t c0 = 0; c0 < n; c0 += C) {
+      const uint32_t nc = min(C, n - c0);
+      for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) s[e] = seg[c0 + e];
+      __syncthreads();
+      for (uint32_t j = C >> 1; j > 0; j >>= 1) {
+        tb_half(s, nc, C, j);
+        __syncthreads();
+      }
+      for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) seg[c0 + e] = s[e];
+      __threadfence_block();
+      __syncthreads();
+    }
+  }
   for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-    const uint64_t k = s[e];
+    const uint64_t k = seg[e];
     ids[start + e] = (int32_t)(uint32_t)k;
     if (keys_out) keys_out[start + e] = thi | (k >> 32);
   }
 }
 
-// Lists longer than kTbSortSmall, taken from the queue by a fixed grid.  Up
-// to kTbSortLds keys are sorted in LDS; longer lists are sorted in chunks of
-// kTbSortLds in LDS, then merged by the network's stages that cross chunks
-// in global memory (tkeys) and the rest of each merge in LDS.
-__global__ __launch_bounds__(256) void k_tb_sort_big(int nb, const uint32_t* __restrict__ hdr,
-                                                     const uint32_t* __restrict__ tbpart,
-                                                     const uint32_t* __restrict__ bigq,
-                                                     const float* __restrict__ depths,
-                                                     uint64_t* __restrict__ tkeys,
-                                                     int32_t* __restrict__ ids,
-                                                     uint64_t* __restrict__ keys_out) {
-  constexpr uint32_t C = kTbSortLds;
-  __shared__ uint64_t s[C];
-  const uint32_t nq = hdr[kHdrBig];
-  for (uint32_t w = blockIdx.x; w < nq; w += gridDim.x) {
-    const int tile = (int)bigq[w];
-    uint32_t start, end;
-    tb_range(tile, nb, tbpart, &start, &end);
-    const uint32_t n = end - start;
-    const uint64_t thi = (uint64_t)tile << 32;
-    if (n <= C) {
-      for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) s[e] = tb_key(depths, ids[start + e]);
-      __syncthreads();
-      tb_bitonic(s, n, pow2_at_least(n));
-      for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-        const uint64_t k = s[e];
-        ids[start + e] = (int32_t)(uint32_t)k;
-        if (keys_out) keys_out[start + e] = thi | (k >> 32);
-      }
-      __syncthreads();
-      continue;
-    }
-    uint64_t* seg = tkeys + start;
-    const uint32_t P = pow2_at_least(n);
-    // chunks sorted in LDS (the network's blocks of size <= C)
-    for (uint32_t c0 = 0; c0 < n; c0 += C) {
-      const uint32_t nc = min(C, n - c0);
-      for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) s[e] = tb_key(depths, ids[start + c0 + e]);
-      __syncthreads();
-      tb_bitonic(s, nc, C);
-      for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) seg[c0 + e] = s[e];
-      __threadfence_block();
-      __syncthreads();
-    }
-    for (uint32_t k = 2 * C; k <= P; k <<= 1) {
-      tb_flip(seg, n, P, k);
-      __threadfence_block();
-      __syncthreads();
-      for (uint32_t j = k >> 2; j >= C; j >>= 1) {
-        tb_half(seg, n, P, j);
-        __threadfence_block();
-        __syncthreads();
-      }
-      for (uint32_t c0 = 0; c0 < n; c0 += C) {
-        const uint32_t nc = min(C, n - c0);
-        for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) s[e] = seg[c0 + e];
-        __syncthreads();
-        for (uint32_t j = C >> 1; j > 0; j >>= 1) {
-          tb_half(s, nc, C, j);
-          __syncthreads();
-        }
-        for (uint32_t e = threadIdx.x; e < nc; e += blockDim.x) seg[c0 + e] = s[e];
-        __threadfence_block();
-        __syncthreads();
-      }
-    }
-    for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-      const uint64_t k = seg[e];
-      ids[start + e] = (int32_t)(uint32_t)k;
-      if (keys_out) keys_out[start + e] = thi | (k >> 32);
-    }
-    __syncthreads();
-  }
+// Every list longer than kTbSortReg entries, one 256-thread block per tile
+// (the blocks of tiles with shorter lists, which k_tb_sort sorted, exit at
+// once).
+__global__ __launch_bounds__(256) void k_tb_sort_long(const uint32_t* __restrict__ hdr,
+                                                      const uint32_t* __restrict__ toff,
+                                                      uint64_t* __restrict__ tkeys, int32_t* __restrict__ ids,
+                                                      uint64_t* __restrict__ keys_out) {
+  static_assert(kTbSortLds <= kTbPadded, "the chunk buffer fits in the run buffer");
+  __shared__ __align__(16) uint64_t s[kTbPadded];
+  if (hdr[kHdrMEff] == 0) return;
+  const int tile = blockIdx.x;
+  uint32_t start, end;
+  tb_range(tile, toff, &start, &end);
+  const uint32_t n = end - start;
+  if (n <= (uint32_t)kTbSortReg) return;
+  if (n <= (uint32_t)kTbSortSmall) tb_sort_runs(s, tkeys, ids, keys_out, tile, start, n);
+  else tb_sort_huge(s, tkeys, ids, keys_out, tile, start, n);
 }
 
-int bin_reset_header(void* ws, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, (uint32_t*)ws, 0u, (int64_t)kHdrWords);
+int bin_reset_header(void* ws, int words, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3((words + 255) / 256), dim3(256), 0, s, (uint32_t*)ws, 0u, (int64_t)words);
   return check_launch("bin_reset_header");
 }
 
@@ -852,21 +1111,6 @@ extern "C" size_t rgp_bin_workspace_bytes(int n, int64_t capacity, int width, in
 
 extern "C" const uint32_t* rgp_bin_header(const void* ws) { return (const uint32_t*)ws; }
 
-// Dynamic LDS of the count / scatter kernels beyond the default 64 KiB,
-// requested once per process.
-static int tb_lds_attr() {
-  static int rc = [] {
-    const int bytes = kTbMaxTiles * 4;
-    if (hipFuncSetAttribute((const void*)k_tb_count, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes) != hipSuccess ||
-        hipFuncSetAttribute((const void*)k_tb_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            bytes) != hipSuccess)
-      return set_error(RGP_ERR_LAUNCH, "hipFuncSetAttribute(max dynamic LDS) failed");
-    return (int)RGP_OK;
-  }();
-  return rc;
-}
-
 extern "C" int rgp_bin_scan(int n, int width, int height, void* ws, int64_t capacity,
                             hipStream_t s) {
   if (n < 0 || width <= 0 || height <= 0) return set_error(RGP_ERR_ARG, "rgp_bin_scan: bad size");
@@ -876,19 +1120,25 @@ extern "C" int rgp_bin_scan(int n, int width, int height, void* ws, int64_t capa
   int tile_bytes = 0;
   while (tile_bytes < 4 && ((uint32_t)(L.tiles - 1) >> (8 * tile_bytes)) != 0) ++tile_bytes;
   if (L.tb) {
-    int rc = tb_lds_attr();
-    if (rc) return rc;
-    uint32_t* part = (uint32_t*)(base + L.tbpart);
-    hipLaunchKernelGGL(k_tb_count, dim3(L.nb), dim3(kTbThreads), (size_t)L.tiles * 4, s, n, L.chunk,
-                       L.tiles, L.nb, L.gx, (const uint32_t*)(base + L.counts),
-                       (const uint2*)(base + L.rects), part);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(L.nscan), dim3(256), 0, s, (int)L.npart,
-                       (const uint32_t*)part, 0, (const uint32_t*)nullptr,
-                       (uint32_t*)(base + L.partials), (uint32_t*)nullptr);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, L.nscan, (uint32_t*)(base + L.partials),
-                       (const uint32_t*)nullptr, hdr, L.cap, tile_bytes);
-    hipLaunchKernelGGL(k_scan_down, dim3(L.nscan), dim3(256), 0, s, (int)L.npart, part,
-                       (const uint32_t*)(base + L.partials));
+    // coarse count -> M and the coarse offsets -> coarse lists -> tile counts -> tile offsets
+    uint32_t* sttot = (uint32_t*)(base + L.sttot);
+    uint32_t* boff = (uint32_t*)(base + L.boff);
+    uint32_t* bpart = (uint32_t*)(base + L.bpart);
+    uint4* clist = (uint4*)(base + L.clist);
+    uint32_t* toff = (uint32_t*)(base + L.toff);
+    const size_t st_lds = (size_t)L.st * 4;
+    hipLaunchKernelGGL(k_st_count, dim3(L.nb), dim3(kTbThreads), st_lds, s, n, L.chunk, L.st, L.sts, L.sgx,
+                       (const uint32_t*)(base + L.counts), (const uint2*)(base + L.rects), sttot, boff, bpart);
+    hipLaunchKernelGGL(k_st_plan, dim3(1), dim3(1024), 0, s, L.st, sttot, L.nb, (const uint32_t*)bpart, hdr, L.cap);
+    if (n > 0 && L.cap > 0)
+      hipLaunchKernelGGL(k_st_scatter, dim3(L.nb), dim3(kTbThreads), st_lds, s, n, L.chunk, L.st, L.sts, L.sgx,
+                         (const uint32_t*)(base + L.counts), (const uint2*)(base + L.rects),
+                         (const uint32_t*)(base + L.dqs), (const uint32_t*)sttot, (const uint32_t*)boff,
+                         (const uint32_t*)hdr, clist);
+    hipLaunchKernelGGL(k_st_fine_count, dim3(L.st), dim3(1024), 0, s, L.sts, L.sgx, L.gx, L.gy,
+                       (const uint32_t*)sttot,
+                       (const uint32_t*)hdr, (const uint4*)clist, toff);
+    hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, s, L.tiles, toff, (const uint32_t*)hdr);
     return check_launch("rgp_bin_scan");
   }
   if (L.nscan > 0)
@@ -915,22 +1165,16 @@ extern "C" int rgp_bin_sort(int n, const float* depths, int width, int height, v
   char* base = (char*)ws;
   const uint32_t* hdr = (const uint32_t*)(base + L.header);
   if (L.tb) {
-    uint32_t* hdr_w = (uint32_t*)(base + L.header);
-    int rc = tb_lds_attr();
-    if (rc) return rc;
-    const uint32_t* part = (const uint32_t*)(base + L.tbpart);
+    const uint32_t* toff = (const uint32_t*)(base + L.toff);
     uint64_t* tkeys = (uint64_t*)(base + L.tkeys);
-    if (n > 0 && L.cap > 0)
-      hipLaunchKernelGGL(k_tb_scatter, dim3(L.nb), dim3(kTbThreads), (size_t)L.tiles * 4, s, n, L.chunk,
-                         L.tiles, L.nb, L.gx, (const uint32_t*)(base + L.counts),
-                         (const uint2*)(base + L.rects), part, hdr, ids_out);
-    hipLaunchKernelGGL(k_tb_sort, dim3(L.tiles), dim3(64), 0, s, L.tiles, L.nb, (const uint32_t*)hdr, part,
-                       depths, ids_out, keys_out, (int2*)ranges_out);
-    uint32_t* bigq = (uint32_t*)(base + L.bigq);
-    hipLaunchKernelGGL(k_tb_sort_mid, dim3(L.tiles), dim3(256), 0, s, L.tiles, L.nb, hdr_w, part, depths, ids_out,
-                       keys_out, bigq);
-    hipLaunchKernelGGL(k_tb_sort_big, dim3(kTbBigBlocks), dim3(256), 0, s, L.nb, (const uint32_t*)hdr, part,
-                       (const uint32_t*)bigq, depths, tkeys, ids_out, keys_out);
+    (void)depths;  // the coarse lists carry the quantised depths written by the preprocess
+    hipLaunchKernelGGL(k_st_fine_scatter, dim3(L.st), dim3(1024), 0, s, L.sts, L.sgx, L.gx, L.gy,
+                       (const uint32_t*)(base + L.sttot), (const uint32_t*)hdr, (const uint4*)(base + L.clist),
+                       toff, tkeys);
+    hipLaunchKernelGGL(k_tb_sort, dim3(L.tiles), dim3(64), 0, s, L.tiles, (const uint32_t*)hdr, toff,
+                       (const uint64_t*)tkeys, ids_out, keys_out, (int2*)ranges_out);
+    hipLaunchKernelGGL(k_tb_sort_long, dim3(L.tiles), dim3(256), 0, s, (const uint32_t*)hdr, toff, tkeys, ids_out,
+                       keys_out);
     return check_launch("rgp_bin_sort");
   }
   uint64_t* k0 = (uint64_t*)(base + L.keys0);

@@ -525,8 +525,12 @@ __global__ __launch_bounds__(256) void k_rects(int n, const float* __restrict__ 
                                                float* __restrict__ depths, int gx, int gy,
                                                uint2* __restrict__ rects,
                                                uint32_t* __restrict__ counts,
-                                               uint32_t* __restrict__ dqw) {
+                                               uint32_t* __restrict__ dqw,
+                                               uint32_t* __restrict__ dqs, uint32_t* __restrict__ clr,
+                                               int clr_words) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0)  // the binning header (and the super-tile totals), cleared for the scan
+    for (int q = threadIdx.x; q < clr_words; q += blockDim.x) clr[q] = 0u;
   uint32_t dq = 0;
   if (i < n) {
     uint32_t cnt = 0;
@@ -543,6 +547,7 @@ __global__ __launch_bounds__(256) void k_rects(int n, const float* __restrict__ 
       }
     }
     counts[i] = cnt;
+    dqs[i] = dq;
   }
   // one plain store per group of 64 Gaussians; the scan reduces them
   dq = wave_max_u32(dq);
@@ -721,18 +726,23 @@ __global__ __launch_bounds__(64) void k_preprocess(
     const float* __restrict__ shs, const float* __restrict__ dc, const float* __restrict__ A,
     const float* __restrict__ B, float* __restrict__ depths, float4* __restrict__ recs,
     uint2* __restrict__ rects, uint32_t* __restrict__ counts, uint32_t* __restrict__ dqw,
-    int gx, int gy, float* __restrict__ us_out, float* __restrict__ cinv_out,
-    int32_t* __restrict__ areas_out, float* __restrict__ w_out) {
+    uint32_t* __restrict__ dqs, uint32_t* __restrict__ clr, int clr_words, int gx, int gy,
+    float* __restrict__ us_out, float* __restrict__ cinv_out, int32_t* __restrict__ areas_out,
+    float* __restrict__ w_out) {
   __shared__ float4 s_sh[WMODE == 4 ? 64 * 4 * K : WMODE == 5 ? (64 * 15 * K + 3) / 4 : 1];
   const int t = threadIdx.x;
   const int i = blockIdx.x * 64 + t;
+  if (blockIdx.x == 0)  // the binning header (and the super-tile totals), cleared for the scan
+    for (int q = t; q < clr_words; q += 64) clr[q] = 0u;
   if constexpr (WMODE == 4) stage_sh_rows<K, WMODE>(s_sh, n, shs);
   if constexpr (WMODE == 5) stage_high_rows<K>(reinterpret_cast<float*>(s_sh), n, shs);
   uint32_t dqv = 0;
-  if (i < n)
+  if (i < n) {
     dqv = preprocess_gaussian<K, WMODE>(i, t, s_sh, sh_dim, P, Q, rank, cam, pws, rots, scales, alphas, shs, dc,
                                         A, B, depths, recs, rects, counts, gx, gy, us_out, cinv_out, areas_out,
                                         w_out);
+    dqs[i] = dqv;
+  }
   dqv = wave_max_u32(dqv);
   if (t == 0) dqw[blockIdx.x] = dqv;
 }
@@ -749,21 +759,26 @@ __global__ __launch_bounds__(64) void k_preprocess_views(
     const float* __restrict__ alphas, const float* __restrict__ shs, const float* __restrict__ dc,
     const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ depths,
     float4* __restrict__ recs, char* __restrict__ ws, size_t ws_stride, size_t off_rects, size_t off_counts,
-    size_t off_dqw, int gx, int gy) {
+    size_t off_dqw, size_t off_dqs, int clr_words, int gx, int gy) {
   __shared__ float4 s_sh[WMODE == 4 ? 64 * 4 * K : WMODE == 5 ? (64 * 15 * K + 3) / 4 : 1];
   const int t = threadIdx.x;
   const int i = blockIdx.x * 64 + t;
+  if (blockIdx.x == 0)  // every view's binning header (and super-tile totals), cleared for the scan
+    for (int v = 0; v < views; ++v)
+      for (int q = t; q < clr_words; q += 64) ((uint32_t*)(ws + (size_t)v * ws_stride))[q] = 0u;
   if constexpr (WMODE == 4) stage_sh_rows<K, WMODE>(s_sh, n, shs);
   if constexpr (WMODE == 5) stage_high_rows<K>(reinterpret_cast<float*>(s_sh), n, shs);
   for (int v = 0; v < views; ++v) {
     char* w = ws + (size_t)v * ws_stride;
     uint32_t dqv = 0;
-    if (i < n)
+    if (i < n) {
       dqv = preprocess_gaussian<K, WMODE>(i, t, s_sh, sh_dim, P, Q, rank, cams + v, pws, rots, scales, alphas,
                                           shs, dc, A, B, depths + (size_t)v * n,
                                           recs + (size_t)v * n * RecStride<K>::f4, (uint2*)(w + off_rects),
                                           (uint32_t*)(w + off_counts), gx, gy, nullptr, nullptr, nullptr,
                                           nullptr);
+      ((uint32_t*)(w + off_dqs))[i] = dqv;
+    }
     dqv = wave_max_u32(dqv);
     if (t == 0) ((uint32_t*)(w + off_dqw))[blockIdx.x] = dqv;
   }
@@ -1610,12 +1625,11 @@ extern "C" int rgp_bin_rects(int n, const float* us, int32_t* areas, float* dept
     return set_error(RGP_ERR_ARG, "rgp_bin_rects: bad size");
   BinLayout L = bin_layout(n, capacity, width, height);
   char* base = (char*)ws;
-  int rc = bin_reset_header(ws, s);
-  if (rc) return rc;
-  if (n == 0) return RGP_OK;
+  if (n == 0) return bin_reset_header(ws, L.reset_words, s);
   hipLaunchKernelGGL(k_rects, dim3(grid1(n)), dim3(256), 0, s, n, us, areas, depths, L.gx, L.gy,
                      (uint2*)(base + L.rects), (uint32_t*)(base + L.counts),
-                     (uint32_t*)(base + L.dqw));
+                     (uint32_t*)(base + L.dqw), (uint32_t*)(base + L.dqs), (uint32_t*)(base + L.header),
+                     L.reset_words);
   return check_launch("rgp_bin_rects");
 }
 
@@ -1641,9 +1655,7 @@ extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int r
     return set_error(RGP_ERR_ARG, "rgp_preprocess_weight: sh_dim (P) must be 1, 4, 9 or 16");
   BinLayout L = bin_layout(n, capacity, width, height);
   char* base = (char*)ws;
-  int rc = bin_reset_header(ws, s);
-  if (rc) return rc;
-  if (n == 0) return RGP_OK;
+  if (n == 0) return bin_reset_header(ws, L.reset_words, s);
   int wmode = mode;
   if (mode == 1 && Q == 15 && P == K && (rank == 1 || rank == 2)) wmode = rank + 1;
   if (mode == 0 && sh_dim == 16 && ((uintptr_t)shs & 15) == 0) wmode = 4;
@@ -1653,7 +1665,8 @@ extern "C" int rgp_preprocess_weight(int n, int K, int mode, int P, int Q, int r
                                      sh_dim, P, Q, rank, cam, pws, (const float4*)rots, scales,   \
                                      alphas, shs, dc, A, B, depths, (float4*)records,             \
                                      (uint2*)(base + L.rects), (uint32_t*)(base + L.counts),      \
-                                     (uint32_t*)(base + L.dqw), L.gx, L.gy, us,                   \
+                                     (uint32_t*)(base + L.dqw), (uint32_t*)(base + L.dqs),        \
+                                     (uint32_t*)(base + L.header), L.reset_words, L.gx, L.gy, us, \
                                      cinv2ds, areas, weights))
   switch (wmode) {
     case 0: RGP_PRE(0); break;
@@ -1691,11 +1704,13 @@ extern "C" int rgp_preprocess_weight_views(int views, int n, int K, int mode, in
     return set_error(RGP_ERR_ARG, "rgp_preprocess_weight_views: ws_stride %zu below the workspace size %zu "
                      "or not a multiple of 256", ws_stride, L.total);
   char* base = (char*)ws;
-  for (int v = 0; v < views; ++v) {
-    int rc = bin_reset_header(base + (size_t)v * ws_stride, s);
-    if (rc) return rc;
+  if (n == 0) {
+    for (int v = 0; v < views; ++v) {
+      int rc = bin_reset_header(base + (size_t)v * ws_stride, L.reset_words, s);
+      if (rc) return rc;
+    }
+    return RGP_OK;
   }
-  if (n == 0) return RGP_OK;
   int wmode = mode;
   if (mode == 1 && Q == 15 && P == K && (rank == 1 || rank == 2)) wmode = rank + 1;
   if (mode == 0 && sh_dim == 16 && ((uintptr_t)shs & 15) == 0) wmode = 4;
@@ -1704,7 +1719,7 @@ extern "C" int rgp_preprocess_weight_views(int views, int n, int K, int mode, in
   RGP_K_SWITCH(K, hipLaunchKernelGGL((k_preprocess_views<KK, WM>), dim3((n + 63) / 64), dim3(64), 0, s,  \
                                      views, n, sh_dim, P, Q, rank, cams, pws, (const float4*)rots, scales, \
                                      alphas, shs, dc, A, B, depths, (float4*)records, base, ws_stride,    \
-                                     L.rects, L.counts, L.dqw, L.gx, L.gy))
+                                     L.rects, L.counts, L.dqw, L.dqs, L.reset_words, L.gx, L.gy))
   switch (wmode) {
     case 0: RGP_PREV(0); break;
     case 1: RGP_PREV(1); break;

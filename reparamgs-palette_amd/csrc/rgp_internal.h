@@ -12,20 +12,33 @@ int set_error(int code, const char* fmt, ...);
 int check_launch(const char* who);
 
 // ---------------------------------------------------------------------------
-// Binning workspace layout.  All regions 256-byte aligned.  Two sort paths:
+// Binning workspace layout.  All regions 256-byte aligned.  Two sort paths.
 //
-// Tile-bucket path (tiles <= kTbMaxTiles, every resolution up to 4K):
-//   header   : 64 u32 control words (below)
+// Common regions:
+//   header   : 64 u32 control words (below); on the tile-bucket path followed
+//              by sttot, S + 1 u32 (S super-tiles), cleared with the header
 //   counts   : N u32 per-Gaussian tile counts
 //   rects    : N uint2, packed (x0 | y0 << 16, x1 | y1 << 16)
 //   dqw      : ceil(N / 64) u32, max uint32(depth * 1000) per 64 Gaussians
-//   tbpart   : tiles x nb u32, pairs of tile t from Gaussian block b at
-//              [t * nb + b]; the scan turns it into inclusive offsets
-//   partials : scan block sums of tbpart
-//   tkeys    : capacity u64 (uint32(depth * 1000) << 32 | Gaussian id), the
-//              keys of tile lists too long to sort in LDS
-//   bigq     : tiles u32, the queue of those tiles
-//   (the ids are scattered into the caller's id buffer and sorted there)
+//   dqs      : N u32, uint32(depth * 1000) of each binned Gaussian
+//
+// Tile-bucket path (tiles <= kTbMaxTiles, every resolution up to 4K), two
+// levels: Gaussians are first bucketed into super-tiles of kst x kst tiles
+// (kst = 4 up to kStSmallGrid tiles, 8 above, so that either way there are a
+// few hundred super-tiles),
+// then each super-tile's block writes the complete lists of its tiles.
+//   sttot    : coarse pairs per super-tile (blocks reserve their slots with
+//              atomics), then scanned into exclusive offsets, sttot[S] = total
+//   boff     : nb x S u32, end of Gaussian block b's slots in super-tile s's
+//              coarse list, relative to the list's start, at [b * S + s]
+//   bpart    : nb u32, the tile pairs of Gaussian block b (sum: M)
+//   clist    : capacity uint4 (Gaussian id, rect.x, rect.y, its dqs), the
+//              coarse lists, super-tile major (no more entries than M)
+//   toff     : tiles + 1 u32, exclusive offsets of the tile lists
+//   tkeys    : capacity u64 (uint32(depth * 1000) << 32 | Gaussian id), every
+//              tile list's sort keys, written by the fine scatter (and the
+//              merge space of lists too long to sort in LDS)
+//   (the in-tile sorts read tkeys and write the caller's id buffer)
 //
 // Radix path (larger grids):
 //   counts are turned into inclusive offsets by the scan; keys0/1 hold
@@ -40,32 +53,33 @@ constexpr int kHdrCur = 4;       // 9 words: source buffer of sort pass p (p = 0
 constexpr int kHdrSkip = 13;     // 8 words: pass p is skipped (all digits equal)
 constexpr int kHdrLast = 21;     // index of the last executed pass, or -1
 constexpr int kHdrNblk = 22;     // active sort blocks
-constexpr int kHdrBig = 23;      // tiles queued for the long-list sort (tile-bucket path)
 constexpr int kHdrWords = 64;
 
 constexpr int kScanChunk = 4096;  // counts per scan block (256 threads x 16)
 constexpr int kSortItems = 16;    // keys per thread per sort block
 constexpr int kSortTile = 256 * kSortItems;
 
-constexpr int kTbMaxTiles = 32768;   // per-tile LDS counters of one block: 128 KiB
-constexpr int kTbThreads = 1024;     // threads of the count / scatter blocks
-constexpr int kTbMinChunk = 4096;    // Gaussians per count / scatter block, at least
+constexpr int kTbMaxTiles = 32768;   // largest grid of the tile-bucket path (3840x2160 is 32400)
+constexpr int kStMax = 8;            // largest super-tile edge in tiles (128 x 128 pixels)
+constexpr int kStSmallGrid = 16384;  // grids up to this many tiles (2560x1440 has 14400) use edge 4
+constexpr int kTbThreads = 1024;     // threads of the coarse count / scatter blocks
+constexpr int kTbMinChunk = 4096;    // Gaussians per coarse count / scatter block, at least
 constexpr int kTbMaxBlocks = 256;
-constexpr int kTbBig = 32;           // rects above this many tiles are walked per wavefront
+constexpr int kTbBig = 32;           // rects above this many super-tiles are walked per wavefront
 constexpr int kTbSortReg = 1024;     // lists sorted in registers by the one-wavefront-per-tile kernel
-constexpr int kTbSortSmall = 2048;   // lists sorted in LDS by the one-block-per-tile kernel (16 KiB)
-constexpr int kTbSortLds = 4096;     // keys of one long list sorted in LDS; longer lists
-                                     // are merged through global memory
-constexpr int kTbBigBlocks = 1024;   // blocks of the long-list sort (grid-stride over its queue)
+constexpr int kTbSortSmall = 4096;   // lists sorted as register runs of kTbSortReg keys merged through LDS
+constexpr int kTbSortLds = 4096;     // keys of one longer list sorted in LDS at a time; the rest of
+                                     // its merges run in global memory
 
 struct BinLayout {
-  size_t header, counts, rects, dqw, partials, dqpart, keys0, keys1, vals0, vals1, hist, totals;
-  size_t tbpart, tkeys, bigq, total;
+  size_t header, counts, rects, dqw, dqs, partials, dqpart, keys0, keys1, vals0, vals1, hist, totals;
+  size_t sttot, boff, bpart, clist, toff, tkeys, total;
   int gx, gy, tiles, nscan, nblk_cap;
+  int reset_words;  // words cleared before binning: the header (and sttot)
   bool tb;      // tile-bucket path
-  int nb;       // count / scatter blocks (tile-bucket path)
-  int chunk;    // Gaussians per count / scatter block
-  int64_t npart;  // tiles * nb
+  int sts, sgx, sgy, st;  // super-tile edge log2 and grid (tile-bucket path)
+  int nb;       // coarse count / scatter blocks (tile-bucket path)
+  int chunk;    // Gaussians per coarse count / scatter block
   int64_t cap;
 };
 
@@ -78,28 +92,40 @@ inline BinLayout bin_layout(int n, int64_t cap, int width, int height) {
   L.tiles = L.gx * L.gy;
   L.cap = cap < 0 ? 0 : cap;
   L.tb = L.tiles <= kTbMaxTiles;
+  L.sts = L.sgx = L.sgy = L.st = 0;
+  if (L.tb) {
+    L.sts = L.tiles > kStSmallGrid ? 3 : 2;
+    const int kst = 1 << L.sts;
+    L.sgx = (L.gx + kst - 1) / kst;
+    L.sgy = (L.gy + kst - 1) / kst;
+    L.st = L.sgx * L.sgy;
+  }
   size_t o = 0;
-  L.header = o; o = align256(o + kHdrWords * 4);
+  L.header = o;
+  L.sttot = o + kHdrWords * 4;
+  L.reset_words = kHdrWords + (L.tb ? L.st + 1 : 0);
+  o = align256(o + (size_t)L.reset_words * 4);
   L.counts = o; o = align256(o + (size_t)n * 4);
   L.rects = o; o = align256(o + (size_t)n * 8);
   L.dqw = o; o = align256(o + (size_t)((n + 63) / 64) * 4);
-  L.keys0 = L.keys1 = L.vals0 = L.vals1 = L.hist = L.totals = L.dqpart = L.tbpart = L.tkeys = L.bigq = 0;
+  L.dqs = o; o = align256(o + (size_t)n * 4);
+  L.keys0 = L.keys1 = L.vals0 = L.vals1 = L.hist = L.totals = L.dqpart = L.partials = 0;
+  L.boff = L.bpart = L.clist = L.toff = L.tkeys = 0;
   L.nblk_cap = 0;
   if (L.tb) {
     int nb = (n + kTbMinChunk - 1) / kTbMinChunk;
     nb = nb < 1 ? 1 : (nb > kTbMaxBlocks ? kTbMaxBlocks : nb);
     L.nb = nb;
     L.chunk = (n + nb - 1) / nb;
-    L.npart = (int64_t)L.tiles * nb;
-    L.nscan = (int)((L.npart + kScanChunk - 1) / kScanChunk);
-    L.tbpart = o; o = align256(o + (size_t)(L.npart + 1) * 4);
-    L.partials = o; o = align256(o + (size_t)(L.nscan + 1) * 4);
+    L.nscan = 0;
+    L.boff = o; o = align256(o + (size_t)nb * L.st * 4);
+    L.bpart = o; o = align256(o + (size_t)nb * 4);
+    L.clist = o; o = align256(o + (size_t)L.cap * 16);
+    L.toff = o; o = align256(o + (size_t)(L.tiles + 1) * 4);
     L.tkeys = o; o = align256(o + (size_t)L.cap * 8);
-    L.bigq = o; o = align256(o + (size_t)L.tiles * 4);
   } else {
     L.nb = 0;
     L.chunk = 0;
-    L.npart = 0;
     L.nscan = (n + kScanChunk - 1) / kScanChunk;
     L.nblk_cap = (int)((L.cap + kSortTile - 1) / kSortTile);
     L.partials = o; o = align256(o + (size_t)(L.nscan + 1) * 4);
@@ -115,7 +141,9 @@ inline BinLayout bin_layout(int n, int64_t cap, int width, int height) {
   return L;
 }
 
-int bin_reset_header(void* ws, hipStream_t s);
+// Clears the first `words` words of a binning workspace (L.reset_words).
+int bin_reset_header(void* ws, int words, hipStream_t s);
+
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll

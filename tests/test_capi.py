@@ -36,10 +36,10 @@ def test_pure_host_entry_points():
     assert lib.rgp_record_floats(9) == 0
     lib.rgp_bin_workspace_bytes.restype = ctypes.c_size_t
     lib.rgp_bin_workspace_bytes.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
-    # tile-bucket grids: one u64 key per pair of capacity
+    # tile-bucket grids: per pair of capacity one u64 key and one 16-byte coarse-list entry
     small = lib.rgp_bin_workspace_bytes(1000, 0, 64, 64)
     big = lib.rgp_bin_workspace_bytes(1000, 100000, 64, 64)
-    assert 100000 * 8 <= big - small < 100000 * 9
+    assert 100000 * 24 <= big - small < 100000 * 25
     # grids above 32768 tiles (the radix path): two key and two id buffers
     small = lib.rgp_bin_workspace_bytes(1000, 0, 7680, 4320)
     big = lib.rgp_bin_workspace_bytes(1000, 100000, 7680, 4320)

@@ -454,17 +454,19 @@ def gpu_bin(us, areas, depths, W, H, device, cap):
     return N(keys[:m]), N(ids[:m]), N(ranges), m
 
 
-@pytest.mark.parametrize("case", ["long_lists_ties", "mid_lists", "4k", "8k_radix"])
+@pytest.mark.parametrize("case", ["long_lists_ties", "mid_lists", "two_runs", "4k", "8k_radix"])
 def test_binning_sort_paths(device, case):
     """Keys, ids and ranges against the oracle's stable sort: tile lists far
     longer than one LDS sort (merged through global memory) with many equal
-    quantised depths, lists between the per-tile and the long-list LDS
-    limits, a 3840x2160 grid (32,400 tiles, the largest grid of
-    the tile-bucket path) and a 7680x4320 grid (the LSD radix path)."""
+    quantised depths, lists of two to four register runs of 1024 keys
+    merged in LDS (2049-4096 entries; 1018-1408 entries, on both sides of
+    the one-wavefront limit), a 3840x2160 grid (32,400 tiles, the largest
+    grid of the tile-bucket path) and a 7680x4320 grid (the LSD radix
+    path)."""
     print(f"case: binning {case}", flush=True)
     rng = np.random.default_rng(51)
-    if case in ("long_lists_ties", "mid_lists"):
-        W, H, n = 64, 48, (12_000 if case == "long_lists_ties" else 3_300)
+    if case in ("long_lists_ties", "mid_lists", "two_runs"):
+        W, H, n = 64, 48, {"long_lists_ties": 12_000, "mid_lists": 3_300, "two_runs": 1_500}[case]
         us = rng.uniform(-10, 74, (n, 2)).astype(np.float32)
         areas = rng.integers(30, 60, (n, 2)).astype(np.int32)
         depths = (1.0 + 0.5 * rng.integers(0, 5, n)).astype(np.float32)
@@ -487,6 +489,8 @@ def test_binning_sort_paths(device, case):
         assert lens.max() > 2 * 4096
     if case == "mid_lists":
         assert 2048 < lens.max() <= 4096
+    if case == "two_runs":
+        assert lens.min() <= 1024 < lens.max() <= 2048
 
 
 @pytest.mark.parametrize("sh_dim", [16, 9, 4, 1])

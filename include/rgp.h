@@ -124,9 +124,12 @@ size_t rgp_bin_workspace_bytes(int n, int64_t capacity, int width, int height);
 int rgp_bin_rects(int n, const float* us, int32_t* areas, float* depths, int width, int height,
                   void* ws, int64_t capacity, hipStream_t stream);
 
-/* inclusive scan of the counts (gausplat.cu:199-201).  The pair count M and
- * an overflow flag (M > capacity) are left in the workspace header:
- * rgp_bin_header() gives their device addresses. */
+/* inclusive scan of the counts (gausplat.cu:199-201), and for grids up to
+ * 32768 tiles the bucketing of the pairs that rgp_bin_sort completes.  The
+ * pair count M and an overflow flag (M > capacity) are left in the
+ * workspace header: rgp_bin_header() gives their device addresses.  Runs
+ * after rgp_bin_rects or rgp_preprocess_weight(_views) on the same
+ * workspace (they clear the header and write the counts and rects). */
 int rgp_bin_scan(int n, int width, int height, void* ws, int64_t capacity, hipStream_t stream);
 
 /* createKeys + stable (tile, uint32(depth*1000)) sort + getRanges

@@ -454,7 +454,8 @@ def gpu_bin(us, areas, depths, W, H, device, cap):
     return N(keys[:m]), N(ids[:m]), N(ranges), m
 
 
-@pytest.mark.parametrize("case", ["long_lists_ties", "mid_lists", "two_runs", "4k", "8k_radix"])
+@pytest.mark.parametrize("case", ["long_lists_ties", "mid_lists", "two_runs", "4k", "8k_radix", "huge_1080p",
+                                  "huge_4k"])
 def test_binning_sort_paths(device, case):
     """Keys, ids and ranges against the oracle's stable sort: tile lists far
     longer than one LDS sort (merged through global memory) with many equal
@@ -462,7 +463,9 @@ def test_binning_sort_paths(device, case):
     merged in LDS (2049-4096 entries; 1018-1408 entries, on both sides of
     the one-wavefront limit), a 3840x2160 grid (32,400 tiles, the largest
     grid of the tile-bucket path) and a 7680x4320 grid (the LSD radix
-    path)."""
+    path); splats whose rects reach more than 32 super-tiles (walked by a
+    whole wavefront in the coarse passes) at 1080p (4 x 4-tile super-tiles)
+    and 4K (8 x 8)."""
     print(f"case: binning {case}", flush=True)
     rng = np.random.default_rng(51)
     if case in ("long_lists_ties", "mid_lists", "two_runs"):
@@ -470,6 +473,14 @@ def test_binning_sort_paths(device, case):
         us = rng.uniform(-10, 74, (n, 2)).astype(np.float32)
         areas = rng.integers(30, 60, (n, 2)).astype(np.int32)
         depths = (1.0 + 0.5 * rng.integers(0, 5, n)).astype(np.float32)
+    elif case.startswith("huge"):
+        W, H = (1920, 1080) if case == "huge_1080p" else (3840, 2160)
+        n = 20_000
+        us = np.stack([rng.uniform(-50, W + 50, n), rng.uniform(-50, H + 50, n)], 1).astype(np.float32)
+        areas = rng.integers(1, 40, (n, 2)).astype(np.int32)
+        big = rng.choice(n, 40, replace=False)
+        areas[big] = rng.integers(600, 1600, (40, 2))
+        depths = rng.uniform(0.1, 80.0, n).astype(np.float32)
     else:
         W, H = (3840, 2160) if case == "4k" else (7680, 4320)
         n = 100_000
@@ -491,6 +502,11 @@ def test_binning_sort_paths(device, case):
         assert 2048 < lens.max() <= 4096
     if case == "two_runs":
         assert lens.min() <= 1024 < lens.max() <= 2048
+    if case.startswith("huge"):
+        st = 4 if case == "huge_1080p" else 8
+        sw = (rects[:, 2] - 1) // st - rects[:, 0] // st + 1
+        sh = (rects[:, 3] - 1) // st - rects[:, 1] // st + 1
+        assert ((counts > 0) & (sw * sh > 32)).sum() >= 10
 
 
 @pytest.mark.parametrize("sh_dim", [16, 9, 4, 1])

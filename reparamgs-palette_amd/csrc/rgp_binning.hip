@@ -796,11 +796,33 @@ __device__ __forceinline__ uint64_t lane_xor_u64(uint64_t v) {
   return ((uint64_t)lane_xor_u32<M>((uint32_t)(v >> 32)) << 32) | lane_xor_u32<M>((uint32_t)v);
 }
 
+// Keys below 0x7FEFFFFF << 32 (every key whose quantised depth is below
+// 0x7FEFFFFF) are the bit patterns of finite non-negative doubles, mostly
+// subnormal, and as doubles they order exactly as they do as integers (the
+// kernels run with FP64 subnormals kept, IEEE mode).  Then one v_min_f64 and
+// one v_max_f64 do a register compare-exchange, where the integer form takes
+// a compare and four selects; the padding is the largest finite double,
+// above every such key.  FD selects that form.
+constexpr uint64_t kPadFd = 0x7FEFFFFFFFFFFFFFull;
+constexpr uint32_t kFdMaxHi = 0x7FEFFFFFu;  // keys with a smaller high word qualify
+
+template <bool FD>
+__device__ __forceinline__ uint64_t sort_pad() { return FD ? kPadFd : ~0ull; }
+
+template <bool FD>
 __device__ __forceinline__ void reg_cx(uint64_t& a, uint64_t& b) {
-  const bool sw = b < a;
-  const uint64_t lo = sw ? b : a, hi = sw ? a : b;
-  a = lo;
-  b = hi;
+  if constexpr (FD) {
+    const double x = __longlong_as_double((long long)a), y = __longlong_as_double((long long)b);
+    double lo, hi;
+    asm("v_min_f64 %0, %2, %3\n\tv_max_f64 %1, %2, %3" : "=&v"(lo), "=&v"(hi) : "v"(x), "v"(y));
+    a = (uint64_t)__double_as_longlong(lo);
+    b = (uint64_t)__double_as_longlong(hi);
+  } else {
+    const bool sw = b < a;
+    const uint64_t lo = sw ? b : a, hi = sw ? a : b;
+    a = lo;
+    b = hi;
+  }
 }
 
 // Flip stage of block size K (K > KPL): register r pairs with register
@@ -838,7 +860,7 @@ __device__ __forceinline__ void wave_half(uint64_t (&k)[KPL], int lane) {
   }
 }
 
-template <int KPL, int K, int J>
+template <bool FD, int KPL, int K, int J>
 __device__ __forceinline__ void wave_halves(uint64_t (&k)[KPL], int lane) {
   if constexpr (J >= 1) {
     if constexpr (J >= KPL) {
@@ -846,26 +868,54 @@ __device__ __forceinline__ void wave_halves(uint64_t (&k)[KPL], int lane) {
     } else {
 #pragma unroll
       for (int r = 0; r < KPL; ++r)
-        if ((r & J) == 0) reg_cx(k[r], k[r | J]);
+        if ((r & J) == 0) reg_cx<FD>(k[r], k[r | J]);
     }
-    wave_halves<KPL, K, J / 2>(k, lane);
+    wave_halves<FD, KPL, K, J / 2>(k, lane);
   }
 }
 
-template <int KPL, int K>
+template <bool FD, int KPL, int K>
 __device__ __forceinline__ void wave_bitonic_from(uint64_t (&k)[KPL], int lane) {
   if constexpr (K <= 64 * KPL) {
     if constexpr (K <= KPL) {
 #pragma unroll
       for (int r = 0; r < KPL; ++r) {
         const int q = r ^ (K - 1);
-        if (r < q) reg_cx(k[r], k[q]);
+        if (r < q) reg_cx<FD>(k[r], k[q]);
       }
     } else {
       wave_flip<KPL, K>(k, lane);
     }
-    wave_halves<KPL, K, K / 4>(k, lane);
-    wave_bitonic_from<KPL, K * 2>(k, lane);
+    wave_halves<FD, KPL, K, K / 4>(k, lane);
+    wave_bitonic_from<FD, KPL, K * 2>(k, lane);
+  }
+}
+
+// Loads keys[e] (e = r * 64 + lane below n, coalesced; any placement is
+// sorted alike) into k and sorts them over the wavefront, with the double
+// compare-exchange when every key qualifies (decided for the wavefront) and
+// the integer one otherwise.  Slots past n hold the form's padding.
+template <int KPL>
+__device__ __forceinline__ void wave_sort_keys(const uint64_t* __restrict__ keys, uint32_t n, int lane,
+                                               uint64_t (&k)[KPL]) {
+  uint32_t hmax = 0;
+#pragma unroll
+  for (int r = 0; r < KPL; ++r) {
+    const uint32_t e = (uint32_t)(r * 64 + lane);
+    k[r] = e < n ? keys[e] : 0ull;
+    hmax = max(hmax, (uint32_t)(k[r] >> 32));
+  }
+  hmax = wave_max_u32(hmax);
+  if (hmax < kFdMaxHi) {
+#pragma unroll
+    for (int r = 0; r < KPL; ++r)
+      if ((uint32_t)(r * 64 + lane) >= n) k[r] = kPadFd;
+    wave_bitonic_from<true, KPL, 2>(k, lane);
+  } else {
+#pragma unroll
+    for (int r = 0; r < KPL; ++r)
+      if ((uint32_t)(r * 64 + lane) >= n) k[r] = ~0ull;
+    wave_bitonic_from<false, KPL, 2>(k, lane);
   }
 }
 
@@ -897,12 +947,7 @@ __device__ __forceinline__ void tb_wave_sort(const uint64_t* __restrict__ tkeys,
                                              uint64_t* __restrict__ keys_out, uint64_t thi, uint32_t start,
                                              uint32_t n, int lane, uint64_t* __restrict__ s) {
   uint64_t k[KPL];
-#pragma unroll
-  for (int r = 0; r < KPL; ++r) {
-    const uint32_t e = (uint32_t)(r * 64 + lane);  // coalesced loads; any placement is sorted alike
-    k[r] = e < n ? tkeys[start + e] : ~0ull;
-  }
-  wave_bitonic_from<KPL, 2>(k, lane);
+  wave_sort_keys<KPL>(tkeys + start, n, lane, k);
 #pragma unroll
   for (int r = 0; r < KPL; ++r) s[tb_pad((uint32_t)(lane * KPL + r))] = k[r];
   __syncthreads();
@@ -983,14 +1028,16 @@ __device__ __forceinline__ void tb_sort_runs(uint64_t* __restrict__ s, const uin
   const int lane = threadIdx.x & 63;
   const uint32_t runs = n <= 2u * kTbSortReg ? 2u : 4u;
   if (w < runs) {
+    // each run is sorted with its own form and padding; the merge compares as integers, under which both
+    // paddings sort after every key
     const uint32_t base = w * kTbSortReg;
     uint64_t k[16];
+    if (base < n) {
+      wave_sort_keys<16>(tkeys + start + base, n - base, lane, k);
+    } else {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const uint32_t e = base + (uint32_t)(r * 64 + lane);
-      k[r] = e < n ? tkeys[start + e] : ~0ull;
+      for (int r = 0; r < 16; ++r) k[r] = ~0ull;
     }
-    if (base < n) wave_bitonic_from<16, 2>(k, lane);
 #pragma unroll
     for (int r = 0; r < 16; r += 2)
       *reinterpret_cast<ulonglong2*>(&s[tb_pad(base + (uint32_t)lane * 16 + r)]) = make_ulonglong2(k[r], k[r + 1]);

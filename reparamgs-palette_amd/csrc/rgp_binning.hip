@@ -1028,8 +1028,9 @@ __device__ __forceinline__ void tb_sort_runs(uint64_t* __restrict__ s, const uin
   const int lane = threadIdx.x & 63;
   const uint32_t runs = n <= 2u * kTbSortReg ? 2u : 4u;
   if (w < runs) {
-    // each run is sorted with its own form and padding; the merge compares as integers, under which both
-    // paddings sort after every key
+    // each run is sorted with its own form; its padding, sorted last, leaves as ~0 so that under the merge's
+    // integer compares it follows every key of every run (another run's integer form may hold keys above the
+    // double form's padding)
     const uint32_t base = w * kTbSortReg;
     uint64_t k[16];
     if (base < n) {
@@ -1039,8 +1040,11 @@ __device__ __forceinline__ void tb_sort_runs(uint64_t* __restrict__ s, const uin
       for (int r = 0; r < 16; ++r) k[r] = ~0ull;
     }
 #pragma unroll
-    for (int r = 0; r < 16; r += 2)
-      *reinterpret_cast<ulonglong2*>(&s[tb_pad(base + (uint32_t)lane * 16 + r)]) = make_ulonglong2(k[r], k[r + 1]);
+    for (int r = 0; r < 16; r += 2) {
+      const uint32_t i = base + (uint32_t)lane * 16 + r;
+      *reinterpret_cast<ulonglong2*>(&s[tb_pad(i)]) =
+          make_ulonglong2(i < n ? k[r] : ~0ull, i + 1 < n ? k[r + 1] : ~0ull);
+    }
   }
   __syncthreads();
   const uint64_t thi = (uint64_t)tile << 32;

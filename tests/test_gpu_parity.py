@@ -455,7 +455,7 @@ def gpu_bin(us, areas, depths, W, H, device, cap):
 
 
 @pytest.mark.parametrize("case", ["long_lists_ties", "mid_lists", "two_runs", "4k", "8k_radix", "huge_1080p",
-                                  "huge_4k"])
+                                  "huge_4k", "far_depths"])
 def test_binning_sort_paths(device, case):
     """Keys, ids and ranges against the oracle's stable sort: tile lists far
     longer than one LDS sort (merged through global memory) with many equal
@@ -465,14 +465,22 @@ def test_binning_sort_paths(device, case):
     grid of the tile-bucket path) and a 7680x4320 grid (the LSD radix
     path); splats whose rects reach more than 32 super-tiles (walked by a
     whole wavefront in the coarse passes) at 1080p (4 x 4-tile super-tiles)
-    and 4K (8 x 8)."""
+    and 4K (8 x 8); lists where some keys' quantised depths reach
+    0x7FEFFFFF and above (the in-register sort's integer form, next to runs
+    sorted in the double form)."""
     print(f"case: binning {case}", flush=True)
     rng = np.random.default_rng(51)
-    if case in ("long_lists_ties", "mid_lists", "two_runs"):
-        W, H, n = 64, 48, {"long_lists_ties": 12_000, "mid_lists": 3_300, "two_runs": 1_500}[case]
+    if case in ("long_lists_ties", "mid_lists", "two_runs", "far_depths"):
+        W, H, n = 64, 48, {"long_lists_ties": 12_000, "mid_lists": 3_300, "two_runs": 1_500, "far_depths": 3_300}[case]
         us = rng.uniform(-10, 74, (n, 2)).astype(np.float32)
         areas = rng.integers(30, 60, (n, 2)).astype(np.int32)
         depths = (1.0 + 0.5 * rng.integers(0, 5, n)).astype(np.float32)
+        if case == "far_depths":
+            # quantised depths just below 0x7FEFFFFF (0x7FEFFF80), just above (0x7FF00380) and up to 4e9
+            far = rng.choice(n, 12, replace=False)
+            depths[far[:2]] = np.float32(2146435.0)
+            depths[far[2:4]] = np.float32(2146436.0)
+            depths[far[4:]] = rng.uniform(2.2e6, 4.0e6, 8).astype(np.float32)
     elif case.startswith("huge"):
         W, H = (1920, 1080) if case == "huge_1080p" else (3840, 2160)
         n = 20_000
@@ -502,6 +510,10 @@ def test_binning_sort_paths(device, case):
         assert 2048 < lens.max() <= 4096
     if case == "two_runs":
         assert lens.min() <= 1024 < lens.max() <= 2048
+    if case == "far_depths":
+        dq = keys & np.uint64(0xFFFFFFFF)  # (tile << 32) | quantised depth
+        assert (dq == 0x7FEFFF80).sum() >= 4 and (dq == 0x7FF00380).sum() >= 4
+        assert (dq > 0x7FF00380).sum() >= 10 and (dq < 0x7FEFFFFF).sum() > 30_000
     if case.startswith("huge"):
         st = 4 if case == "huge_1080p" else 8
         sw = (rects[:, 2] - 1) // st - rects[:, 0] // st + 1

@@ -5,9 +5,10 @@
 // Layout.  One wavefront (a 64-thread workgroup) owns one 16x16 tile.  Lane l
 // holds the column px = x0 + (l & 15) in four row slots: slot j covers row
 // y0 + 4j + (l >> 4), so slot j across the wavefront is a 16x4 strip.  The
-// tile's list is staged in LDS 64 entries at a time (one per lane, the next
-// batch prefetched into registers while the current one is composited); no
-// barrier beyond the wavefront is needed.
+// tile's list is staged in LDS 64 entries at a time (one per lane; the
+// backward prefetches the next batch into registers while the current one is
+// composited, the forward only its ids); no barrier beyond the wavefront is
+// needed.
 //
 // Work per entry.  The terms of the Mahalanobis distance that depend only on
 // dx, (cinv.x dx) dx and (2 cinv.y) dx, are formed once per lane and reused by
@@ -105,7 +106,11 @@ __device__ __forceinline__ uint32_t live_slots(const float* tau) {
 // (every entry of each batch it stages), the mask of the tile's four row
 // strips in which some pixel took a contribution from it (strips[e], one
 // byte per list position).  The replay backward, which repeats exactly this
-// arithmetic, then evaluates only those strips.
+// arithmetic, then evaluates only those strips.  The flags are collected in
+// LDS: the contributing lanes of strip j store byte j of the entry's flag
+// word (a store in the contribution branch, at the address already formed for
+// the entry's weights), and after the batch each lane folds the four bytes of
+// its entry into the mask.
 // ---------------------------------------------------------------------------
 template <int DIM, bool COUNT, bool STRIPS = false>
 __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, const int2* __restrict__ ranges,
@@ -119,7 +124,8 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
   const int lane = threadIdx.x;
   const int px = tx * kTile + (lane & 15);
   const int y0 = ty * kTile + (lane >> 4);
-  const float fpx = (float)px;
+  float fpx = (float)px;
+  asm volatile("" : "+v"(fpx));
   const int2 rg = ranges[tile];
 
   float acc[kSlots][DIM];
@@ -133,24 +139,35 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
     const bool in = px < W && y0 + 4 * j < H;
     tau[j] = in ? 1.f : 0.f;
     fpy[j] = (float)(y0 + 4 * j);
+    asm volatile("" : "+v"(fpy[j]));  // kept as a float, not converted again per evaluation
     cont[j] = 0;
     n_eval[j] = 0;
     n_con[j] = 0;
   }
+  // STRIPS: byte j of s_flag[jj] is set to 1 when some pixel of strip j took
+  // a contribution from entry jj of the staged batch (an LDS byte store from
+  // the contributing lanes, at the address of the entry's weights).
+  uint32_t* s_flag = reinterpret_cast<uint32_t*>(s_rec + F4 * 64);
+  if (STRIPS) s_flag[4 * lane] = 0;
   if (rg.y - rg.x > 0) {
-    int nid = 0;
-    RecRegs<F4> nxt = load_rec<F4>(recs, ids, rg.x + lane, rg.x, rg.y, &nid);
+    // Only the next batch's ids are fetched ahead: the records are gathered at
+    // the start of their batch (other wavefronts cover that latency), which
+    // keeps 16 registers free for the per-pixel state.
+    int nid = rg.x + lane < rg.y ? ids[rg.x + lane] : -1;
     for (int b0 = rg.x; b0 < rg.y; b0 += 64) {
       const uint32_t live = live_slots(tau);
       if (live == 0) break;
-      __syncthreads();  // previous batch fully read
-      stage_rec<F4>(s_rec, lane, nxt, ty);
-      __syncthreads();
-      nxt = load_rec<F4>(recs, ids, b0 + 64 + lane, rg.x, rg.y, &nid);
-      const int cnt = min(64, rg.y - b0);
-      uint64_t sbits[kSlots];  // STRIPS: bit jj of sbits[j] = entry b0 + jj contributed in strip j (uniform)
+      RecRegs<F4> cur;
+      {
+        const float4* src = recs + (size_t)max(nid, 0) * F4;
 #pragma unroll
-      for (int j = 0; j < kSlots; ++j) sbits[j] = 0;
+        for (int q = 0; q < F4; ++q) cur.v[q] = nid >= 0 ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      nid = b0 + 64 + lane < rg.y ? ids[b0 + 64 + lane] : -1;
+      __syncthreads();  // previous batch fully read
+      stage_rec<F4>(s_rec, lane, cur, ty);
+      __syncthreads();
+      const int cnt = __builtin_amdgcn_readfirstlane(min(64, rg.y - b0));
       for (int jj = 0; jj < cnt; ++jj) {
         const float4 g0 = s_rec[jj];
         const float4 g1 = s_rec[64 + jj];
@@ -179,24 +196,20 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
           }
           cont[j] = idx;
           tau[j] = tau[j] * (1.f - ap);
+          if (STRIPS) reinterpret_cast<uint8_t*>(s_flag + 4 * jj)[j] = 1;
           if (COUNT) {
             ++n_con[j];
             if (tau[j] < 0.0001f) n_eval[j] = cont[j];
           }
         }
-        if (STRIPS) {
-          // contrib only grows, so it equals idx exactly where this entry contributed (no branch:
-          // a compare per strip and scalar selects)
-          const uint64_t bit = 1ull << jj;
-#pragma unroll
-          for (int j = 0; j < kSlots; ++j) sbits[j] |= __builtin_amdgcn_ballot_w64(cont[j] == idx) != 0 ? bit : 0ull;
-        }
       }
-      if (STRIPS && lane < cnt) {
-        uint32_t mine = 0;
-#pragma unroll
-        for (int j = 0; j < kSlots; ++j) mine |= (uint32_t)((sbits[j] >> lane) & 1ull) << j;
-        strips[b0 + lane] = (uint8_t)mine;
+      if (STRIPS) {
+        // bytes b0..b3 (each 0 or 1) of entry `lane` -> the 4-bit mask b0 | b1<<1 | b2<<2 | b3<<3:
+        // the product with 0x01020408 places byte j's bit at 24 + j and every other partial
+        // product below bit 24 or beyond bit 31
+        const uint32_t f = s_flag[4 * lane];
+        if (lane < cnt) strips[b0 + lane] = (uint8_t)((f * 0x01020408u) >> 24);
+        s_flag[4 * lane] = 0;
       }
     }
   }
@@ -242,7 +255,7 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
                                                  float* __restrict__ final_tau,
                                                  unsigned long long* __restrict__ counts,
                                                  uint8_t* __restrict__ strips) {
-  __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
+  __shared__ float4 s_rec[64 * (RecStride<DIM>::f4 + (STRIPS ? 1 : 0))];
   draw_fwd_tile<DIM, COUNT, STRIPS>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau, layer,
                                     contrib, final_tau, counts, strips, s_rec);
 }
@@ -261,7 +274,7 @@ __global__ __launch_bounds__(64) void k_draw_fwd_views(int W, int H, int gx, int
                                                        int32_t* __restrict__ contrib,
                                                        float* __restrict__ final_tau,
                                                        uint8_t* __restrict__ strips) {
-  __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
+  __shared__ float4 s_rec[64 * (RecStride<DIM>::f4 + (STRIPS ? 1 : 0))];
   const int t = xcd_tile(blockIdx.x, tiles * views);
   const int v = t / tiles, tile = t - v * tiles;
   const size_t HW = (size_t)W * H;

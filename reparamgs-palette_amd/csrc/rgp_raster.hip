@@ -61,11 +61,13 @@ __device__ __forceinline__ RecRegs<F4> load_rec(const float4* __restrict__ recs,
 constexpr uint32_t kWideExp = 1u << 4;
 
 // Stages one record in LDS, replacing its ry by the mask of strips of tile
-// row ty it can reach (bit j: rows ty*16 + 4j .. + 3) and kWideExp.  Float4 q of slot s
+// row ty it can reach (bit j: rows ty*16 + 4j .. + 3) and kWideExp, and
+// returns that mask word (the compositing loops read it from the staging
+// lane's register with v_readlane, not from LDS).  Float4 q of slot s
 // sits at s_rec[q * 64 + s]: the wave's 16-byte stores cover the banks once
 // per quarter-wave, and a read of one slot is a broadcast.
 template <int F4>
-__device__ __forceinline__ void stage_rec(float4* s_rec, int slot, const RecRegs<F4>& r, int ty) {
+__device__ __forceinline__ uint32_t stage_rec(float4* s_rec, int slot, const RecRegs<F4>& r, int ty) {
   uint32_t mask = r.v[1].z <= 170.0f ? 0u : kWideExp;
   const float uy = r.v[0].y, ry = r.v[1].w;
 #pragma unroll
@@ -77,6 +79,13 @@ __device__ __forceinline__ void stage_rec(float4* s_rec, int slot, const RecRegs
   s_rec[64 + slot] = make_float4(r.v[1].x, r.v[1].y, r.v[1].z, __uint_as_float(mask));
 #pragma unroll
   for (int q = 2; q < F4; ++q) s_rec[q * 64 + slot] = r.v[q];
+  return mask;
+}
+
+// The mask word of staged entry jj (wave-uniform jj), from the register of the
+// lane that staged it.
+__device__ __forceinline__ uint32_t entry_bits(uint32_t mask, int jj) {
+  return __builtin_amdgcn_readlane(mask, jj);
 }
 
 // exp(-m/2) for an m that passed m <= mcut; wide (wavefront-uniform) is the
@@ -165,18 +174,29 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
       }
       nid = b0 + 64 + lane < rg.y ? ids[b0 + 64 + lane] : -1;
       __syncthreads();  // previous batch fully read
-      stage_rec<F4>(s_rec, lane, cur, ty);
+      const uint32_t smask = stage_rec<F4>(s_rec, lane, cur, ty);
       __syncthreads();
       const int cnt = __builtin_amdgcn_readfirstlane(min(64, rg.y - b0));
       for (int jj = 0; jj < cnt; ++jj) {
+        const uint32_t slots = entry_bits(smask, jj) & live;
+        if (slots == 0) continue;
         const float4 g0 = s_rec[jj];
         const float4 g1 = s_rec[64 + jj];
-        const uint32_t slots = uniform_bits(g1.w) & live;
-        if (slots == 0) continue;
         const float dx = g0.x - fpx;
         const float ta0 = g0.z * dx * dx;  // (cinv.x dx) dx
         const float tc0 = g0.w * dx;       // (2 cinv.y) dx
         const int idx = b0 + jj - rg.x + 1;  // contrib value of a pixel this entry contributes to
+        // The entry's channel values, read once for its strips: the contribution
+        // step then waits on no LDS read.
+        float wv[DIM];
+#pragma unroll
+        for (int q = 0; q < F4 - 2; ++q) {
+          const float4 f = s_rec[(2 + q) * 64 + jj];
+          if (4 * q + 0 < DIM) wv[4 * q + 0] = f.x;
+          if (4 * q + 1 < DIM) wv[4 * q + 1] = f.y;
+          if (4 * q + 2 < DIM) wv[4 * q + 2] = f.z;
+          if (4 * q + 3 < DIM) wv[4 * q + 3] = f.w;
+        }
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) {
           if (!(slots & (1u << j))) continue;
@@ -187,13 +207,7 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
           if (ap < 0.002f) continue;
           const float ta = tau[j] * ap;
 #pragma unroll
-          for (int q = 0; q < F4 - 2; ++q) {
-            const float4 f = s_rec[(2 + q) * 64 + jj];
-            if (4 * q + 0 < DIM) acc[j][4 * q + 0] = fmaf(ta, f.x, acc[j][4 * q + 0]);
-            if (4 * q + 1 < DIM) acc[j][4 * q + 1] = fmaf(ta, f.y, acc[j][4 * q + 1]);
-            if (4 * q + 2 < DIM) acc[j][4 * q + 2] = fmaf(ta, f.z, acc[j][4 * q + 2]);
-            if (4 * q + 3 < DIM) acc[j][4 * q + 3] = fmaf(ta, f.w, acc[j][4 * q + 3]);
-          }
+          for (int k = 0; k < DIM; ++k) acc[j][k] = fmaf(ta, wv[k], acc[j][k]);
           cont[j] = idx;
           tau[j] = tau[j] * (1.f - ap);
           if (STRIPS) reinterpret_cast<uint8_t*>(s_flag + 4 * jj)[j] = 1;
@@ -550,7 +564,7 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
                                                      const float4* __restrict__ recs,
                                                      const uint8_t* __restrict__ strips,
                                                      const float* __restrict__ dlayer,
-                                                     float* __restrict__ dfeat, float4* s_rec, int32_t* s_id) {
+                                                     float* __restrict__ dfeat, float4* s_rec) {
   constexpr int F4 = RecStride<DIM>::f4;
   const int ty = tile / gx, tx = tile - ty * gx;
   const int lane = threadIdx.x;
@@ -589,24 +603,26 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
     const uint32_t live = live_slots(tau);
     if (live == 0) break;
     __syncthreads();
+    uint32_t smask;
     if (STRIPS) {
       s_rec[lane] = nxt.v[0];
-      s_rec[64 + lane] = make_float4(nxt.v[1].x, nxt.v[1].y, nxt.v[1].z, __uint_as_float(nstr));
+      s_rec[64 + lane] = nxt.v[1];
 #pragma unroll
       for (int q = 2; q < F4; ++q) s_rec[q * 64 + lane] = nxt.v[q];
+      smask = nstr;
     } else {
-      stage_rec<F4>(s_rec, lane, nxt, ty);
+      smask = stage_rec<F4>(s_rec, lane, nxt, ty);
     }
-    s_id[lane] = nid;
+    const int sid = nid;  // the staged entries' Gaussians, read per entry with v_readlane
     __syncthreads();
     nxt = load_rec<F4>(recs, ids, b0 + 64 + lane, rg.x, rg.y, &nid);
     if (STRIPS) nstr = b0 + 64 + lane < rg.y ? strips[b0 + 64 + lane] : 0u;
     const int cnt = min(64, rg.y - b0);
     for (int jj = 0; jj < cnt; ++jj) {
+      const uint32_t slots = entry_bits(smask, jj) & live;
+      if (slots == 0) continue;
       const float4 g0 = s_rec[jj];
       const float4 g1 = s_rec[64 + jj];
-      const uint32_t slots = uniform_bits(g1.w) & live;
-      if (slots == 0) continue;
       const float dx = g0.x - fpx;
       const float ta0 = g0.z * dx * dx;
       const float tc0 = g0.w * dx;
@@ -626,7 +642,7 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
         if (!STRIPS) on = true;
       }
       if (STRIPS || __any(on)) {
-        flush_grad<DIM>(part, dfeat, s_id[jj], lane, ch, owner);
+        flush_grad<DIM>(part, dfeat, __builtin_amdgcn_readlane(sid, jj), lane, ch, owner);
 #pragma unroll
         for (int k = 0; k < DIM; ++k) part[k] = 0.f;
       }
@@ -643,9 +659,8 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
                                                         const float* __restrict__ dlayer,
                                                         float* __restrict__ dfeat) {
   __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
-  __shared__ int32_t s_id[64];
   draw_bwd_replay_tile<DIM, STRIPS>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, strips, dlayer, dfeat,
-                                    s_rec, s_id);
+                                    s_rec);
 }
 
 // The replay backward of `views` images in one launch, laid out as in
@@ -661,13 +676,12 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay_views(int W, int H, int 
                                                               const float* __restrict__ dlayer,
                                                               float* __restrict__ dfeat, int64_t n) {
   __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
-  __shared__ int32_t s_id[64];
   const int t = xcd_tile(blockIdx.x, tiles * views);
   const int v = t / tiles, tile = t - v * tiles;
   const size_t HW = (size_t)W * H;
   draw_bwd_replay_tile<DIM, STRIPS>(W, H, gx, tile, ranges + (size_t)v * tiles, ids + v * id_stride,
                                     recs + v * rec_stride, STRIPS ? strips + v * id_stride : nullptr,
-                                    dlayer + (size_t)v * DIM * HW, dfeat + (size_t)v * n * DIM, s_rec, s_id);
+                                    dlayer + (size_t)v * DIM * HW, dfeat + (size_t)v * n * DIM, s_rec);
 }
 
 // ---------------------------------------------------------------------------

@@ -553,6 +553,99 @@ struct RsLanes {
   }();
 };
 
+// Reduce-scatter of N <= 8 per-lane values over the whole wavefront, with no
+// LDS exchange.  Three scatter levels over lane bits 5, 4 and 0 halve the
+// values a lane holds (of each pair the lanes with the bit clear keep the
+// first value's sum, the others the second's; an odd last value is summed in
+// both): bit 5 with v_permlane32_swap (which exchanges the upper half of its
+// first operand with the lower half of its second, so one swap and one add
+// serve a pair), bit 4 with v_permlane16_swap (odd rows of the first operand
+// with even rows of the second), bit 0 with a quad_perm exchange and two
+// selects.  Bits 1, 2 and 3 are then summed in full with DPP adds (quad_perm
+// [2,3,0,1], row_ror:4, row_ror:8).  Lane l ends with the wavefront sum of
+// channel ws_channel(N, l).
+template <int B>
+__device__ __forceinline__ float ws_pair(float x, float y, bool bit) {
+  if constexpr (B == 5) {
+    auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+  } else if constexpr (B == 4) {
+    auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+  } else {
+    const float keep = bit ? y : x, send = bit ? x : y;
+    return keep + RGP_DPP(send, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  }
+}
+template <int B>
+__device__ __forceinline__ float ws_self(float x) {
+  if constexpr (B == 5) {
+    auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+  } else if constexpr (B == 4) {
+    auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+  } else {
+    return x + RGP_DPP(x, 0xB1, 0xF, 0xF, false);
+  }
+}
+template <int N, int B>
+__device__ __forceinline__ void ws_level(const float* in, float* out, bool bit) {
+#pragma unroll
+  for (int i = 0; i < N / 2; ++i) out[i] = ws_pair<B>(in[2 * i], in[2 * i + 1], bit);
+  if constexpr ((N & 1) != 0) out[N / 2] = ws_self<B>(in[N - 1]);
+}
+template <int N>
+__device__ __forceinline__ float wave_reduce_scatter(const float* v, int lane) {
+  static_assert(N >= 1 && N <= 8, "three scatter levels");
+  constexpr int N1 = (N + 1) / 2, N2 = (N1 + 1) / 2;
+  float a[N1], b[N2], c[1];
+  ws_level<N, 5>(v, a, lane & 32);
+  ws_level<N1, 4>(a, b, lane & 16);
+  ws_level<N2, 0>(b, c, lane & 1);
+  float x = c[0];
+  x = x + RGP_DPP(x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+  x = x + RGP_DPP(x, 0x124, 0xF, 0xF, false);  // row_ror:4
+  x = x + RGP_DPP(x, 0x128, 0xF, 0xF, false);  // row_ror:8
+  return x;
+}
+// The channel wave_reduce_scatter<n> leaves in lane l (it depends on lane
+// bits 5, 4 and 0, taken in that order), and whether l is the lowest lane
+// holding it.
+__host__ __device__ constexpr int ws_channel(int n, int l) {
+  int map[8] = {0};
+  for (int i = 0; i < 8; ++i) map[i] = i;
+  const int bits[3] = {5, 4, 0};
+  for (int t = 0; t < 3; ++t) {
+    const int bit = (l >> bits[t]) & 1;
+    const int h = (n + 1) / 2;
+    for (int i = 0; i < n / 2; ++i) map[i] = map[2 * i + bit];
+    if (n & 1) map[n / 2] = map[n - 1];
+    n = h;
+  }
+  return map[0];
+}
+__host__ __device__ constexpr bool ws_owner(int n, int l) {
+  for (int q = 0; q < l; ++q)
+    if (ws_channel(n, q) == ws_channel(n, l)) return false;
+  return true;
+}
+// Packed at compile time: bit b of `owner` for lane b.
+template <int N>
+struct WsLanes {
+  static constexpr uint64_t owner = [] {
+    uint64_t m = 0;
+    for (int l = 0; l < 64; ++l) m |= ws_owner(N, l) ? 1ull << l : 0ull;
+    return m;
+  }();
+  static constexpr uint32_t chan8 = [] {  // channel of lane (bit0 | bit4 << 1 | bit5 << 2)
+    uint32_t t = 0;
+    for (int i = 0; i < 8; ++i) t |= (uint32_t)ws_channel(N, (i & 1) | ((i >> 1) & 1) << 4 | ((i >> 2) & 1) << 5) << (4 * i);
+    return t;
+  }();
+};
+__device__ __forceinline__ int ws_lane_index(int lane) { return (lane & 1) | ((lane >> 3) & 6); }
+
 // Sum of lanes l, l^16, l^32 and l^48 into every one of them: the gfx950
 // v_permlane16_swap exchanges rows 0<->1 and 2<->3 between its two copies,
 // v_permlane32_swap the two halves of the wavefront.

@@ -299,15 +299,21 @@ __global__ __launch_bounds__(64) void k_draw_fwd_views(int W, int H, int gx, int
 }
 
 // Adds the four-slot lane sums of an entry's gradient terms into dfeat: a
-// reduce-scatter over each 16-lane row leaves channel ch = rs_channel(DIM,
-// lane & 15) in the lane, the four rows are added across, and the owner lanes
-// of row 0 (one per channel) issue one atomic instruction over the entry's
-// DIM floats.  ch and owner are per-lane constants set up once per kernel.
+// reduce-scatter over the wavefront leaves channel ch = ws_channel(DIM, lane)
+// in the lane, and the owner lanes (one per channel) issue one atomic
+// instruction over the entry's DIM floats.  ch and owner are per-lane
+// constants set up once per kernel (grad_lanes).
 template <int DIM>
 __device__ __forceinline__ void flush_grad(const float* part, float* __restrict__ dfeat, int g, int lane,
                                            int ch, bool owner) {
-  const float v = across_rows(row_reduce_scatter<DIM>(part, lane));
+  const float v = wave_reduce_scatter<DIM>(part, lane);
   if (owner && v != 0.f) unsafeAtomicAdd(&dfeat[(size_t)g * DIM + ch], v);
+}
+
+template <int DIM>
+__device__ __forceinline__ void grad_lanes(int lane, int* ch, bool* owner) {
+  *ch = (int)((WsLanes<DIM>::chan8 >> (4 * ws_lane_index(lane))) & 15u);
+  *owner = (WsLanes<DIM>::owner >> lane) & 1ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -334,8 +340,9 @@ __global__ __launch_bounds__(64) void k_draw_bwd(int W, int H, int gx, int tiles
   const int y0 = ty * kTile + (lane >> 4);
   const float fpx = (float)px;
   const int2 rg = ranges[tile];
-  const int ch = (int)((RsLanes<DIM>::chan >> (4 * (lane & 15))) & 15);
-  const bool owner = lane < 16 && ((RsLanes<DIM>::owner >> lane) & 1);
+  int ch;
+  bool owner;
+  grad_lanes<DIM>(lane, &ch, &owner);
   if (rg.y - rg.x <= 0) return;
 
   const size_t HW = (size_t)W * H;
@@ -572,8 +579,9 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
   const int y0 = ty * kTile + (lane >> 4);
   const float fpx = (float)px;
   const int2 rg = ranges[tile];
-  const int ch = (int)((RsLanes<DIM>::chan >> (4 * (lane & 15))) & 15);
-  const bool owner = lane < 16 && ((RsLanes<DIM>::owner >> lane) & 1);
+  int ch;
+  bool owner;
+  grad_lanes<DIM>(lane, &ch, &owner);
   if (rg.y - rg.x <= 0) return;
 
   const size_t HW = (size_t)W * H;
@@ -591,10 +599,16 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
   }
   // The per-entry partial sums are zero outside a flush: they start at zero
   // and are cleared after each one, so an entry without contributions leaves
-  // them untouched and the accumulation needs no per-entry re-zeroing.
+  // them untouched and the accumulation needs no per-entry re-zeroing.  The
+  // zeros are opaque to the compiler (an empty asm), which would otherwise
+  // fold them into the first slot's multiply-adds and rematerialise them on
+  // every path through the slots.
   float part[DIM];
 #pragma unroll
-  for (int k = 0; k < DIM; ++k) part[k] = 0.f;
+  for (int k = 0; k < DIM; ++k) {
+    part[k] = 0.f;
+    asm volatile("" : "+v"(part[k]));
+  }
   int nid = 0;
   RecRegs<F4> nxt = load_rec<F4>(recs, ids, rg.x + lane, rg.x, rg.y, &nid);
   uint32_t nstr = 0;
@@ -644,7 +658,10 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
       if (STRIPS || __any(on)) {
         flush_grad<DIM>(part, dfeat, __builtin_amdgcn_readlane(sid, jj), lane, ch, owner);
 #pragma unroll
-        for (int k = 0; k < DIM; ++k) part[k] = 0.f;
+        for (int k = 0; k < DIM; ++k) {
+          part[k] = 0.f;
+          asm volatile("" : "+v"(part[k]));
+        }
       }
     }
   }

@@ -384,10 +384,15 @@ int rgp_mix_backward(int K, int64_t hw, const float* W, const float* palette, co
  * vanilla_3dgs/gsplat/pytorch_ssim.py:64-67 (no gsplatcu counterpart; the
  * reference computes it with torch conv2d). ------------------------------ */
 
+/* Length (float64 elements) of the sums workspace of rgp_l1_ssim_forward:
+ * one pair per 32x32 output tile and channel. */
+int64_t rgp_l1_ssim_sums(int channels, int height, int width);
+
 /* img, target (C,H,W); window11 a host array of the 11 normalised taps.
  * abc (3,C,H,W) receives the per-pixel SSIM partials the backward needs;
- * sums (float64[128], caller zero-fills) 64 pairs of partial SSIM-map and
- * |img - target| sums;
+ * sums (float64[rgp_l1_ssim_sums(C,H,W)], no initialisation needed) the
+ * per-tile partial sums of the SSIM map and of |img - target|, added in a
+ * fixed order (the loss value is deterministic);
  * loss (device float) the loss value. */
 int rgp_l1_ssim_forward(int channels, int height, int width, const float* img,
                         const float* target, const float* window11, float C1, float C2, float lam,

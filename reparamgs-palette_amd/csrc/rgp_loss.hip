@@ -9,8 +9,9 @@
 // SSIM map with respect to the blurred maps that depend on the image:
 //   a = ds/dmu_x, b = ds/dE[x^2], c = ds/dE[xy]
 // (s = N/D, N = (2 mu_x mu_y + C1)(2 sxy + C2), D = (mu_x^2 + mu_y^2 + C1)
-// (sx^2 + sy^2 + C2)).  Sums of the SSIM map and of |x - y| go to two float64
-// accumulators with one atomic per block.
+// (sx^2 + sy^2 + C2)).  Each block writes its sums of the SSIM map and of
+// |x - y| to its own pair of float64 slots; one block then adds the slots in
+// a fixed order, so the loss value does not depend on the block schedule.
 // Backward: the adjoint of a zero-padded blur with a symmetric window is the
 // same blur, so dS/dx = G*a + 2 x (G*b) + y (G*c), evaluated with the same
 // LDS tiling; the L1 part adds sign(x - y).
@@ -195,7 +196,9 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __r
     const float mx = m3[0][o], my = m3[1][o], exx = m3[2][o], eyy = m2[0][o], exy = m2[1][o];
     const float A1 = 2.f * mx * my + C1, A2 = 2.f * (exy - mx * my) + C2;
     const float B1 = mx * mx + my * my + C1, B2 = (exx - mx * mx) + (eyy - my * my) + C2;
-    const float D = B1 * B2, Dinv = 1.f / D;
+    // D >= C1 (C2 - rounding) > 0 is normal: v_rcp_f32 (1 ulp) in place of a
+    // correctly rounded division, within the loss's float64-reference bar
+    const float D = B1 * B2, Dinv = __builtin_amdgcn_rcpf(D);
     const float sv = A1 * A2 * Dinv;
     const float dN = 2.f * my * (A2 - A1), dD = 2.f * mx * (B2 - B1);
     const size_t o_ = c * HW + (size_t)gy * W + gx;
@@ -209,18 +212,31 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __r
   __syncthreads();
   const float bl = block_sum_f(l1_sum, red);
   if (threadIdx.x == 0) {
-    // 64 accumulator pairs so that blocks do not queue on one address
-    const int slot = blockIdx.x & 63;
-    unsafeAtomicAdd(&sums[2 * slot], (double)bs);
-    unsafeAtomicAdd(&sums[2 * slot + 1], (double)bl);
+    sums[2 * (size_t)blockIdx.x] = (double)bs;
+    sums[2 * (size_t)blockIdx.x + 1] = (double)bl;
   }
 }
 
-__global__ void k_ssim_finish(const double* __restrict__ sums, double n, float lam,
-                              float* __restrict__ loss) {
+// The loss from the blocks' slot pairs: thread t adds the slots t, t + 256,
+// ... in order, then thread 0 adds the 256 partial sums in order.
+__global__ __launch_bounds__(256) void k_ssim_finish(const double* __restrict__ sums, int nblocks, double n,
+                                                     float lam, float* __restrict__ loss) {
+  __shared__ double ps[256], pl[256];
+  double s = 0.0, l = 0.0;
+  for (int i = threadIdx.x; i < nblocks; i += 256) {
+    s += sums[2 * (size_t)i];
+    l += sums[2 * (size_t)i + 1];
+  }
+  ps[threadIdx.x] = s;
+  pl[threadIdx.x] = l;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double s = 0.0, l = 0.0;
-    for (int i = 0; i < 64; ++i) { s += sums[2 * i]; l += sums[2 * i + 1]; }
+    s = 0.0;
+    l = 0.0;
+    for (int i = 0; i < 256; ++i) {
+      s += ps[i];
+      l += pl[i];
+    }
     *loss = (float)((1.0 - lam) * (l / n) + lam * (1.0 - s / n));
   }
 }
@@ -303,6 +319,11 @@ static Win11 make_win(const float* w) {
   return v;
 }
 
+extern "C" int64_t rgp_l1_ssim_sums(int channels, int height, int width) {
+  if (channels <= 0 || height <= 0 || width <= 0) return 0;
+  return 2 * (int64_t)((width + kST - 1) / kST) * ((height + kST - 1) / kST) * channels;
+}
+
 extern "C" int rgp_l1_ssim_forward(int channels, int height, int width, const float* img,
                                    const float* target, const float* window11, float C1, float C2,
                                    float lam, float* abc, double* sums, float* loss,
@@ -315,7 +336,7 @@ extern "C" int rgp_l1_ssim_forward(int channels, int height, int width, const fl
   const dim3 grid((unsigned)blocks);
   hipLaunchKernelGGL(k_ssim_fwd, grid, dim3(256), 0, s, height, width, img, target,
                      make_win(window11), C1, C2, abc, sums);
-  hipLaunchKernelGGL(k_ssim_finish, dim3(1), dim3(64), 0, s, (const double*)sums,
+  hipLaunchKernelGGL(k_ssim_finish, dim3(1), dim3(256), 0, s, (const double*)sums, (int)blocks,
                      (double)channels * height * width, lam, loss);
   return check_launch("rgp_l1_ssim_forward");
 }

@@ -67,6 +67,7 @@ _SIGS = {
     "rgp_mix": (I, [I, I64, P, P, P, P]),
     "rgp_mix_backward": (I, [I, I64, P, P, P, P, P, P]),
     "rgp_fill_u32": (I, [P, U32, I64, P]),
+    "rgp_l1_ssim_sums": (I64, [I, I, I]),
     "rgp_l1_ssim_forward": (I, [I, I, I, P, P, P, F, F, F, P, P, P, P]),
     "rgp_l1_ssim_backward": (I, [I, I, I, P, P, P, P, F, P, P, P]),
     "rgp_bary_workspace_bytes": (ctypes.c_size_t, [I]),

@@ -62,7 +62,7 @@ class L1SSIMLoss(torch.autograd.Function):
     def forward(ctx, img, target, lam=0.2, range_=1.0):
         import ctypes
 
-        from ._lib import call, f32c, ptr, require_cuda, stream
+        from ._lib import call, f32c, lib, ptr, require_cuda, stream
         require_cuda(img, target)
         x, y = f32c(img), f32c(target)
         if x.dim() == 4:
@@ -72,7 +72,7 @@ class L1SSIMLoss(torch.autograd.Function):
         taps = window_taps()
         ctx.taps = taps
         abc = torch.empty((3, C, H, W), dtype=torch.float32, device=x.device)
-        sums = torch.zeros(128, dtype=torch.float64, device=x.device)
+        sums = torch.empty(int(lib().rgp_l1_ssim_sums(C, H, W)), dtype=torch.float64, device=x.device)
         loss = torch.empty((), dtype=torch.float32, device=x.device)
         call("rgp_l1_ssim_forward", C, H, W, ptr(x), ptr(y), taps.ctypes.data_as(ctypes.c_void_p),
              (0.01 * range_) ** 2, (0.03 * range_) ** 2, float(lam), ptr(abc), ptr(sums), ptr(loss), stream(x.device))

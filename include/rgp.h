@@ -161,7 +161,9 @@ int rgp_pack_records(int n, int dim, const float* us, const float* cinv2ds, cons
  * a tile composites, the mask of its four 16x4 row strips (bit j: rows
  * 4j..4j+3 of the tile) in which some pixel took a contribution; entries after
  * the batch of 64 in which the tile's pixels all stopped are not written.
- * rgp_draw_backward_replay reads it back.  No reference counterpart. */
+ * rgp_draw_backward_replay reads it back.  No reference counterpart.
+ * contrib may be null when strips is given: the contribution index is then
+ * neither tracked nor written (the replay backward does not read it). */
 int rgp_draw_forward(int width, int height, int dim, const int32_t* ranges, const int32_t* ids,
                      const float* records, float empty_tau, float* layer, int32_t* contrib,
                      float* final_tau, uint8_t* strips, hipStream_t stream);
@@ -270,7 +272,8 @@ int rgp_bin_sort_views(int views, int n, const float* depths, int width, int hei
 
 /* rgp_draw_forward of every view in one launch (views x tiles blocks);
  * id_stride = capacity, n the Gaussian count (record stride); strips
- * (nullable) has the stride of ids. */
+ * (nullable) has the stride of ids; contrib may be null when strips is
+ * given, as for rgp_draw_forward. */
 int rgp_draw_forward_views(int views, int width, int height, int dim, const int32_t* ranges,
                            const int32_t* ids, int64_t id_stride, const float* records, int64_t n,
                            float empty_tau, float* layer, int32_t* contrib, float* final_tau,
@@ -390,7 +393,7 @@ int64_t rgp_l1_ssim_sums(int channels, int height, int width);
 
 /* img, target (C,H,W); window11 a host array of the 11 normalised taps.
  * abc (3,C,H,W) receives the per-pixel SSIM partials the backward needs;
- * sums (float64[rgp_l1_ssim_sums(C,H,W)], no initialisation needed) the
+ * sums (float64[rgp_l1_ssim_sums(C,H,W)], 16-byte aligned, no initialisation needed) the
  * per-tile partial sums of the SSIM map and of |img - target|, added in a
  * fixed order (the loss value is deterministic);
  * loss (device float) the loss value. */

@@ -218,27 +218,41 @@ __global__ __launch_bounds__(256) void k_ssim_fwd(int H, int W, const float* __r
 }
 
 // The loss from the blocks' slot pairs: thread t adds the slots t, t + 256,
-// ... in order, then thread 0 adds the 256 partial sums in order.
+// ... in order (all its loads issued first), then a pairwise tree over the
+// 256 partial sums, the same tree on every call.
 __global__ __launch_bounds__(256) void k_ssim_finish(const double* __restrict__ sums, int nblocks, double n,
                                                      float lam, float* __restrict__ loss) {
   __shared__ double ps[256], pl[256];
+  const int t = threadIdx.x;
   double s = 0.0, l = 0.0;
-  for (int i = threadIdx.x; i < nblocks; i += 256) {
-    s += sums[2 * (size_t)i];
-    l += sums[2 * (size_t)i + 1];
+  int i = t;
+  for (; i + 3 * 256 < nblocks; i += 4 * 256) {
+    const double2 a = reinterpret_cast<const double2*>(sums)[i];
+    const double2 b = reinterpret_cast<const double2*>(sums)[i + 256];
+    const double2 c = reinterpret_cast<const double2*>(sums)[i + 512];
+    const double2 d = reinterpret_cast<const double2*>(sums)[i + 768];
+    s += a.x; l += a.y;
+    s += b.x; l += b.y;
+    s += c.x; l += c.y;
+    s += d.x; l += d.y;
   }
-  ps[threadIdx.x] = s;
-  pl[threadIdx.x] = l;
+  for (; i < nblocks; i += 256) {
+    const double2 a = reinterpret_cast<const double2*>(sums)[i];
+    s += a.x;
+    l += a.y;
+  }
+  ps[t] = s;
+  pl[t] = l;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    s = 0.0;
-    l = 0.0;
-    for (int i = 0; i < 256; ++i) {
-      s += ps[i];
-      l += pl[i];
+#pragma unroll
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+      ps[t] += ps[t + h];
+      pl[t] += pl[t + h];
     }
-    *loss = (float)((1.0 - lam) * (l / n) + lam * (1.0 - s / n));
+    __syncthreads();
   }
+  if (t == 0) *loss = (float)((1.0 - lam) * (pl[0] / n) + lam * (1.0 - ps[0] / n));
 }
 
 __global__ __launch_bounds__(256) void k_ssim_bwd(int H, int W, const float* __restrict__ img,
@@ -333,6 +347,7 @@ extern "C" int rgp_l1_ssim_forward(int channels, int height, int width, const fl
   if ((int64_t)height * width > 0x7fffffff) return set_error(RGP_ERR_ARG, "rgp_l1_ssim_forward: image too large");
   const int64_t blocks = (int64_t)((width + kST - 1) / kST) * ((height + kST - 1) / kST) * channels;
   if (blocks > 0x7fffffff) return set_error(RGP_ERR_ARG, "rgp_l1_ssim_forward: image too large");
+  if (((uintptr_t)sums & 15) != 0) return set_error(RGP_ERR_ARG, "rgp_l1_ssim_forward: sums must be 16-byte aligned");
   const dim3 grid((unsigned)blocks);
   hipLaunchKernelGGL(k_ssim_fwd, grid, dim3(256), 0, s, height, width, img, target,
                      make_win(window11), C1, C2, abc, sums);

@@ -119,9 +119,11 @@ __device__ __forceinline__ uint32_t live_slots(const float* tau) {
 // LDS: the contributing lanes of strip j store byte j of the entry's flag
 // word (a store in the contribution branch, at the address already formed for
 // the entry's weights), and after the batch each lane folds the four bytes of
-// its entry into the mask.
+// its entry into the mask.  CONT = false: the contribution index is neither
+// tracked nor written (the training forward, whose replay backward reads the
+// strip masks and has no use for contrib).
 // ---------------------------------------------------------------------------
-template <int DIM, bool COUNT, bool STRIPS = false>
+template <int DIM, bool COUNT, bool STRIPS = false, bool CONT = true>
 __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, const int2* __restrict__ ranges,
                                               const int32_t* __restrict__ ids, const float4* __restrict__ recs,
                                               float empty_tau, float* __restrict__ layer,
@@ -186,6 +188,11 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
         const float ta0 = g0.z * dx * dx;  // (cinv.x dx) dx
         const float tc0 = g0.w * dx;       // (2 cinv.y) dx
         const int idx = b0 + jj - rg.x + 1;  // contrib value of a pixel this entry contributes to
+        // Byte offset of the entry's flag word, held in a register for the
+        // strips' flag stores (rematerialised from the scalar jj, it cost one
+        // instruction per contributing strip).
+        int foff = 16 * jj;
+        if (STRIPS) asm volatile("" : "+v"(foff));
         // The entry's channel values, read once for its strips: the contribution
         // step then waits on no LDS read.
         float wv[DIM];
@@ -208,9 +215,9 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
           const float ta = tau[j] * ap;
 #pragma unroll
           for (int k = 0; k < DIM; ++k) acc[j][k] = fmaf(ta, wv[k], acc[j][k]);
-          cont[j] = idx;
+          if (CONT || COUNT) cont[j] = idx;
           tau[j] = tau[j] * (1.f - ap);
-          if (STRIPS) reinterpret_cast<uint8_t*>(s_flag + 4 * jj)[j] = 1;
+          if (STRIPS) reinterpret_cast<uint8_t*>(s_flag)[foff + j] = 1;
           if (COUNT) {
             ++n_con[j];
             if (tau[j] < 0.0001f) n_eval[j] = cont[j];
@@ -236,7 +243,7 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
       const size_t pix = (size_t)py * W + px;
 #pragma unroll
       for (int k = 0; k < DIM; ++k) layer[k * HW + pix] = acc[j][k];
-      contrib[pix] = cont[j];
+      if (CONT) contrib[pix] = cont[j];
       // the reference kernel returns before writing when the list is empty
       final_tau[pix] = rg.y - rg.x > 0 ? tau[j] : empty_tau;
       if (COUNT) {
@@ -259,7 +266,7 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
   }
 }
 
-template <int DIM, bool COUNT = false, bool STRIPS = false>
+template <int DIM, bool COUNT = false, bool STRIPS = false, bool CONT = true>
 __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles,
                                                  const int2* __restrict__ ranges,
                                                  const int32_t* __restrict__ ids,
@@ -270,8 +277,8 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
                                                  unsigned long long* __restrict__ counts,
                                                  uint8_t* __restrict__ strips) {
   __shared__ float4 s_rec[64 * (RecStride<DIM>::f4 + (STRIPS ? 1 : 0))];
-  draw_fwd_tile<DIM, COUNT, STRIPS>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau, layer,
-                                    contrib, final_tau, counts, strips, s_rec);
+  draw_fwd_tile<DIM, COUNT, STRIPS, CONT>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau,
+                                          layer, contrib, final_tau, counts, strips, s_rec);
 }
 
 // `views` images of one resolution in one launch: block b composites tile
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
 // ranges + v*tiles, ids + v*id_stride, recs + v*rec_stride (float4 units)
 // and writes the v-th (DIM,H,W) layer and (H,W) contrib / final_tau (and,
 // STRIPS, its strip masks at strips + v*id_stride).
-template <int DIM, bool STRIPS = false>
+template <int DIM, bool STRIPS = false, bool CONT = true>
 __global__ __launch_bounds__(64) void k_draw_fwd_views(int W, int H, int gx, int tiles, int views,
                                                        const int2* __restrict__ ranges,
                                                        const int32_t* __restrict__ ids, int64_t id_stride,
@@ -292,9 +299,9 @@ __global__ __launch_bounds__(64) void k_draw_fwd_views(int W, int H, int gx, int
   const int t = xcd_tile(blockIdx.x, tiles * views);
   const int v = t / tiles, tile = t - v * tiles;
   const size_t HW = (size_t)W * H;
-  draw_fwd_tile<DIM, false, STRIPS>(W, H, gx, tile, ranges + (size_t)v * tiles, ids + v * id_stride,
-                                    recs + v * rec_stride, empty_tau, layer + (size_t)v * DIM * HW,
-                                    contrib + v * HW, final_tau + v * HW, nullptr,
+  draw_fwd_tile<DIM, false, STRIPS, CONT>(W, H, gx, tile, ranges + (size_t)v * tiles, ids + v * id_stride,
+                                          recs + v * rec_stride, empty_tau, layer + (size_t)v * DIM * HW,
+                                          CONT ? contrib + v * HW : nullptr, final_tau + v * HW, nullptr,
                                     STRIPS ? strips + v * id_stride : nullptr, s_rec);
 }
 
@@ -901,7 +908,13 @@ extern "C" int rgp_draw_forward(int width, int height, int dim, const int32_t* r
                                 hipStream_t s) {
   int gx, tiles;
   if (int rc = draw_grid("rgp_draw_forward", width, height, dim, &gx, &tiles)) return rc;
-  if (strips) {
+  if (!contrib && !strips) return set_error(RGP_ERR_ARG, "rgp_draw_forward: contrib may be null only with strips");
+  if (!contrib) {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false, true, false>), dim3(tiles), dim3(64), 0, s,
+                                           width, height, gx, tiles, (const int2*)ranges, ids,
+                                           (const float4*)records, empty_tau, layer, contrib,
+                                           final_tau, (unsigned long long*)nullptr, strips));
+  } else if (strips) {
     RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false, true>), dim3(tiles), dim3(64), 0, s,
                                            width, height, gx, tiles, (const int2*)ranges, ids,
                                            (const float4*)records, empty_tau, layer, contrib,
@@ -929,7 +942,14 @@ extern "C" int rgp_draw_forward_views(int views, int width, int height, int dim,
                                       float* final_tau, uint8_t* strips, hipStream_t s) {
   int gx, tiles;
   if (int rc = views_grid("rgp_draw_forward_views", views, width, height, dim, id_stride, &gx, &tiles)) return rc;
-  if (strips) {
+  if (!contrib && !strips)
+    return set_error(RGP_ERR_ARG, "rgp_draw_forward_views: contrib may be null only with strips");
+  if (!contrib) {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd_views<DD, true, false>), dim3(tiles * views), dim3(64), 0, s,
+                                           width, height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
+                                           (const float4*)records, n * RecStride<DD>::f4, empty_tau, layer,
+                                           contrib, final_tau, strips));
+  } else if (strips) {
     RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd_views<DD, true>), dim3(tiles * views), dim3(64), 0, s,
                                            width, height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
                                            (const float4*)records, n * RecStride<DD>::f4, empty_tau, layer,

@@ -38,20 +38,23 @@ BACKWARD_MODE = "replay"
 
 
 def _weight_grad(W, H, K, n, records, contrib, final_tau, ranges, ids, dlayer, strips=None):
-    if BACKWARD_MODE == "reference":
+    # the mode in force at the forward decides: a replay-mode forward keeps
+    # strip masks and no contrib image, which the reference form would need
+    if BACKWARD_MODE == "reference" and contrib is not None:
         return ops.draw_backward(W, H, K, ranges, ids, records, contrib, final_tau, dlayer, n)
     return ops.draw_backward_replay(W, H, K, ranges, ids, records, dlayer, n, strips)
 
 
 def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None, B=None, Q=None,
-                   sh_dim=None, return_state=False, exact=False, low=None, high=None, strips=False):
+                   sh_dim=None, return_state=False, exact=False, low=None, high=None, strips=False, contrib=True):
     """Forward of the palette-weight rasterizer.  Full rank when `shs`
     (N, sh_dim*K) is given, or the reference's two leaves low (N,K) and high
     (N,(sh_dim-1)K) (the values of their concatenation, without forming it);
     low rank (dc, A (N,P,r), B (N,r,Q)) otherwise.  The pair buffers keep the
     cached capacity unless exact=True (see ops.splat_pipeline); the results
     do not depend on it.  strips=True also records the strip masks of the
-    replay backward (state["strips"])."""
+    replay backward (state["strips"]); with them, contrib=False skips the
+    contrib image (state["contrib"] is None)."""
     require_cuda(pws, alphas, scales, rots)
     dev = pws.device
     n = pws.shape[0]
@@ -68,7 +71,8 @@ def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None,
              I64(int(cap)), W, H, None, None, None, None, stream(dev))
 
     layer, contrib, final_tau, ranges, ids, m, _, smask = ops.splat_pipeline(n, W, H, K, records, depths, fill, 0.0,
-                                                                             dev, exact=exact, strips=strips)
+                                                                             dev, exact=exact, strips=strips,
+                                                                             contrib=contrib)
     if return_state:
         return layer, depths, dict(records=records, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids,
                                    camd=camd, m=m, mode=mode, P=P_, Q=Q_, rank=r_, strips=smask)
@@ -91,7 +95,7 @@ def _coef_args(K, shs, dc, A, B, Q, sh_dim, low=None, high=None):
 
 
 def render_weights_views(pws, alphas, scales, rots, cams, K, shs=None, dc=None, A=None, B=None, Q=None,
-                         sh_dim=None, return_state=False, exact=False, strips=False):
+                         sh_dim=None, return_state=False, exact=False, strips=False, contrib=True):
     """render_weights for several cameras of one resolution at once (no
     reference counterpart: the reference renders one camera per call).  One
     preprocess launch reads the Gaussians once for all views, and one
@@ -116,7 +120,7 @@ def render_weights_views(pws, alphas, scales, rots, cams, K, shs=None, dc=None, 
              ptr(ws), ctypes.c_size_t(ws_stride), I64(int(cap)), W, H, stream(dev))
 
     layer, contrib, final_tau, ranges, ids, ms, _, _, smask = ops.splat_pipeline_views(
-        n, V, W, H, K, records, depths, fill, 0.0, dev, exact=exact, strips=strips)
+        n, V, W, H, K, records, depths, fill, 0.0, dev, exact=exact, strips=strips, contrib=contrib)
     if return_state:
         return layer, depths, dict(records=records, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids,
                                    camsd=camsd, m=ms, mode=mode, P=P_, Q=Q_, rank=r_, strips=smask)
@@ -133,7 +137,7 @@ class WeightGSFunctionTestViews(torch.autograd.Function):
         K = int(num_palette)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights_views(pws, alphas, scales, rots, cams, K, shs=shs, return_state=True,
-                                                 strips=True)
+                                                 strips=True, contrib=False)
         ctx.meta = (int(cams[0].width), int(cams[0].height), K, pws.shape[0], st["P"], len(cams))
         ctx.save_for_backward(f32c(pws), st["records"], st["ranges"], st["ids"], st["camsd"], st["strips"])
         return layer, depths > 0.2
@@ -162,7 +166,7 @@ class WeightGSFunctionFastViews(torch.autograd.Function):
         K = int(num_palette)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights_views(pws, alphas, scales, rots, cams, K, dc=shs_dc, A=shs_A, B=shs_B,
-                                                 Q=Q, return_state=True, strips=True)
+                                                 Q=Q, return_state=True, strips=True, contrib=False)
         ctx.meta = (int(cams[0].width), int(cams[0].height), K, pws.shape[0], st["P"], st["Q"], st["rank"],
                     len(cams))
         fused = getattr(shs_dc, "_rgp_fused_adam", None)  # PaletteAdam with the triple in fused_lowrank
@@ -205,7 +209,8 @@ class WeightGSFunctionSplit(torch.autograd.Function):
         K = int(num_palette)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, low=low, high=high, return_state=True,
-                                           strips=BACKWARD_MODE == "replay")
+                                           strips=BACKWARD_MODE == "replay",
+                                           contrib=BACKWARD_MODE != "replay")
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"])
         fused = getattr(low, "_rgp_fused_adam", None)
         ctx.fused = (fused, low, high) if fused is not None else None
@@ -239,7 +244,8 @@ class WeightGSFunctionTest(torch.autograd.Function):
         K = int(num_palette)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, shs=shs, return_state=True,
-                                           strips=BACKWARD_MODE == "replay")
+                                           strips=BACKWARD_MODE == "replay",
+                                           contrib=BACKWARD_MODE != "replay")
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"])
         # an optimizer that fuses the SH update (rgp.optim.PaletteAdam) takes dL/dw instead of dL/dshs
         fused = getattr(shs, "_rgp_fused_adam", None)
@@ -275,7 +281,8 @@ class WeightGSFunctionFast(torch.autograd.Function):
         K = int(num_palette)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the unused mask
         layer, depths, st = render_weights(pws, alphas, scales, rots, cam, K, dc=shs_dc, A=shs_A, B=shs_B, Q=Q,
-                                           return_state=True, strips=BACKWARD_MODE == "replay")
+                                           return_state=True, strips=BACKWARD_MODE == "replay",
+                                           contrib=BACKWARD_MODE != "replay")
         ctx.meta = (int(cam.width), int(cam.height), K, pws.shape[0], st["P"], st["Q"], st["rank"])
         # an optimizer that fuses the factor update (rgp.optim.PaletteAdam with the triple in
         # fused_lowrank) takes dL/dw instead of the three gradients

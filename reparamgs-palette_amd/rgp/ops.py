@@ -110,13 +110,16 @@ def pack_records(us, cinv2ds, alphas, feat):
     return rec
 
 
-def draw_forward(width, height, dim, ranges, ids, records, empty_tau, strips=None):
+def draw_forward(width, height, dim, ranges, ids, records, empty_tau, strips=None, contrib=True):
     """Forward compositing; strips (uint8, one per list position) receives,
     when given, each composited entry's mask of row strips with a
-    contribution, which draw_backward_replay then uses."""
+    contribution, which draw_backward_replay then uses.  contrib=False (only
+    with strips): the contrib image is not formed and None is returned for it."""
     dev = records.device
+    if not contrib and strips is None:
+        raise ValueError("draw_forward: contrib=False needs strips")
     layer = torch.empty((dim, height, width), dtype=torch.float32, device=dev)
-    contrib = torch.empty((height, width), dtype=torch.int32, device=dev)
+    contrib = torch.empty((height, width), dtype=torch.int32, device=dev) if contrib else None
     final_tau = torch.empty((height, width), dtype=torch.float32, device=dev)
     with SPANS.span("draw_forward"):
         call("rgp_draw_forward", int(width), int(height), int(dim), ptr(ranges), ptr(ids), ptr(records),
@@ -197,7 +200,7 @@ CAPACITY = CapacityCache()
 
 
 def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, device,
-                   exact=True, strips=False):
+                   exact=True, strips=False, contrib=True):
     """Scan -> sort -> ranges -> draw.  ws_fill(ws, cap) must write
     rects/counts into a workspace (the getRects stage or the fused
     preprocess).
@@ -210,7 +213,8 @@ def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, d
     pipeline runs again with the grown capacity.  Both give the same M,
     ranges, ids[:M] and images.  strips=True: the forward also writes the
     strip masks of the replay backward (uint8, one per pair slot); they are
-    returned last (None otherwise)."""
+    returned last (None otherwise).  contrib=False (with strips): no contrib
+    image (None in its place)."""
     cap = CAPACITY.get(n, width, height)
     while True:
         ws = new_workspace(n, cap, width, height, device)
@@ -232,7 +236,8 @@ def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, d
         with SPANS.span("sort"):
             ids, ranges, _ = bin_sort(n, depths, width, height, ws, cap, device)
         smask = torch.empty(ids.shape[0], dtype=torch.uint8, device=device) if strips else None
-        layer, contrib, final_tau = draw_forward(width, height, dim, ranges, ids, records, empty_tau, smask)
+        layer, contrib_img, final_tau = draw_forward(width, height, dim, ranges, ids, records, empty_tau, smask,
+                                                     contrib)
         if not exact:
             scanned.synchronize()
             m = int(hdr[0])
@@ -240,7 +245,7 @@ def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, d
             if int(hdr[1]) != 0:
                 cap = CAPACITY.get(n, width, height)
                 continue
-        return layer, contrib, final_tau, ranges, ids, m, ws, smask
+        return layer, contrib_img, final_tau, ranges, ids, m, ws, smask
 
 
 def camera_structs(cams, device):
@@ -253,14 +258,17 @@ def _align256(x):
 
 
 def splat_pipeline_views(n, views, width, height, dim, records, depths, ws_fill, empty_tau, device,
-                         exact=False, strips=False):
+                         exact=False, strips=False, contrib=True):
     """splat_pipeline for `views` cameras of one resolution: one preprocess
     launch (ws_fill(ws, stride, cap)), the binning of every view, and one
     compositing launch over all views' tiles.  Returns layers (V,dim,H,W),
     contrib (V,H,W), final_tau (V,H,W), ranges (V,tiles,2), ids (V,cap),
     the pair counts per view, the workspace, its per-view stride and the
-    strip masks (V,cap) uint8 when strips=True (else None).  The overflow
-    check follows splat_pipeline's (exact=False: after the draw)."""
+    strip masks (V,cap) uint8 when strips=True (else None); contrib=False
+    (with strips) returns None for contrib.  The overflow check follows
+    splat_pipeline's (exact=False: after the draw)."""
+    if not contrib and not strips:
+        raise ValueError("splat_pipeline_views: contrib=False needs strips")
     gx, gy = grid_dims(width, height)
     cap = CAPACITY.get(n, width, height)
     while True:
@@ -290,12 +298,12 @@ def splat_pipeline_views(n, views, width, height, dim, records, depths, ws_fill,
             call("rgp_bin_sort_views", int(views), int(n), ptr(depths), int(width), int(height), ptr(ws),
                  ctypes.c_size_t(stride), I64(int(cap)), ptr(ids), ptr(ranges), stream(device))
         layer = torch.empty((views, dim, height, width), dtype=torch.float32, device=device)
-        contrib = torch.empty((views, height, width), dtype=torch.int32, device=device)
+        contrib_img = torch.empty((views, height, width), dtype=torch.int32, device=device) if contrib else None
         final_tau = torch.empty((views, height, width), dtype=torch.float32, device=device)
         smask = torch.empty(ids.shape, dtype=torch.uint8, device=device) if strips else None
         with SPANS.span("draw_forward"):
             call("rgp_draw_forward_views", int(views), int(width), int(height), int(dim), ptr(ranges), ptr(ids),
-                 I64(ids.shape[1]), ptr(records), I64(int(n)), float(empty_tau), ptr(layer), ptr(contrib),
+                 I64(ids.shape[1]), ptr(records), I64(int(n)), float(empty_tau), ptr(layer), ptr(contrib_img),
                  ptr(final_tau), ptr(smask), stream(device))
         if not exact:
             scanned.synchronize()
@@ -305,7 +313,7 @@ def splat_pipeline_views(n, views, width, height, dim, records, depths, ws_fill,
             if int(host[:, 1].max()) != 0:
                 cap = CAPACITY.get(n, width, height)
                 continue
-        return layer, contrib, final_tau, ranges, ids, ms, ws, stride, smask
+        return layer, contrib_img, final_tau, ranges, ids, ms, ws, stride, smask
 
 
 def draw_backward_replay_views(views, width, height, dim, ranges, ids, records, dlayer, n, strips=None):

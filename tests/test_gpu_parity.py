@@ -235,6 +235,37 @@ def test_forward_strip_masks_c0(device, c0):
         grad_close(N(dw), ob["dweights"], ob["dweights_abs"], f"dweights strips={strips is not None}")
 
 
+def test_forward_without_contrib_c0(device, c0):
+    """The training forward (strip masks, no contrib image) writes the same
+    layer, final_tau and masks as the forward that also forms contrib, and the
+    autograd wrapper's gradient in replay mode equals the oracle's; a null
+    contrib without strip masks is refused."""
+    scene, cam = c0
+    o = O.forward_weight(scene, cam)
+    tc = cam.to_torch(device)
+    args = [T(x, device) for x in (scene.pws, scene.alphas, scene.scales, scene.rots)]
+    shs = T(scene.shs, device)
+    _, _, a = gsmodel.render_weights(*args, tc, scene.K, shs=shs, return_state=True, strips=True)
+    layer, _, b = gsmodel.render_weights(*args, tc, scene.K, shs=shs, return_state=True, strips=True, contrib=False)
+    assert b["contrib"] is None
+    same(N(layer), o["layer"], "layer")
+    same(N(b["final_tau"]), N(a["final_tau"]), "final_tau")
+    assert check_strips(b, o, scene, cam) > 1000
+    with pytest.raises(ValueError):
+        ops.draw_forward(cam.width, cam.height, scene.K, a["ranges"], a["ids"], a["records"], 0.0, None, False)
+    from rgp._lib import lib
+    rc = lib().rgp_draw_forward(cam.width, cam.height, scene.K, a["ranges"].data_ptr(), a["ids"].data_ptr(),
+                                a["records"].data_ptr(), 0.0, torch.empty_like(layer).data_ptr(), None,
+                                torch.empty_like(b["final_tau"]).data_ptr(), None, torch.cuda.current_stream().cuda_stream)
+    assert rc != 0
+    x = shs.clone().requires_grad_(True)
+    out, _ = gsmodel.WeightGSFunctionTest.apply(*args[:1], x, *args[1:], tc, scene.K)
+    dl = np.random.default_rng(5).normal(size=o["layer"].shape).astype(np.float32)
+    out.backward(T(dl, device))
+    ob = O.backward_weight(scene, o, dl, replay=True)
+    grad_close(N(x.grad), ob["dshs"], ob["dshs_abs"], "dshs")
+
+
 def test_mix_matches_bake(device, c0):
     """mix(render(w), P) against the oracle mix, and the linear identity with
     the bake-then-render colour path (gsmodel.py:488-509, 13-26)."""

@@ -437,15 +437,16 @@ __global__ __launch_bounds__(256) void k_ranges(int tiles, const uint32_t* __res
 // (depth_q << 32 | id), whose values are all distinct:
 //   1. k_st_count: blocks of Gaussians count their (super-tile, Gaussian)
 //      pairs in LDS and reserve slots of each super-tile's coarse list;
-//   2. k_st_plan: M (from the tile counts), the overflow flag, the coarse
-//      lists' offsets;
-//   3. k_st_scatter: each coarse entry carries the Gaussian's id, tile rect
-//      and quantised depth (written by the preprocess in Gaussian order);
-//   4. k_st_fine_count + k_tile_offsets: one block per super-tile counts its
-//      tiles' pairs; one block scans them into the tile ranges;
-//   5. k_st_fine_scatter: one block per super-tile writes its tiles' complete
+//   2. k_st_scatter: every block forms M (from the tile counts) and the
+//      coarse lists' offsets, block 0 publishes them with the overflow flag,
+//      and each coarse entry carries the Gaussian's id, tile rect and
+//      quantised depth (written by the preprocess in Gaussian order);
+//   3. k_st_fine_count: one block per super-tile counts its tiles' pairs and
+//      adds them to the per-tile-row sums;
+//   4. k_st_fine_scatter: one block per super-tile forms its tiles' list
+//      offsets from the row sums and the counts, and writes the complete
 //      lists of sort keys (every line of a list written by one block);
-//   6. k_tb_sort (lists <= 1024, registers), k_tb_sort_long (longer ones).
+//   5. k_tb_sort (lists <= 1024, registers), k_tb_sort_long (longer ones).
 // Super-tiles are 4 x 4 tiles up to 2560x1440 and 8 x 8 above, a few
 // hundred of them either way.
 // ---------------------------------------------------------------------------
@@ -464,7 +465,10 @@ __device__ __forceinline__ StRect st_rect(uint2 r, uint32_t sts) {
 // in [g0, g1) reaches (w = extra[i], or 0 without extra), and returns the
 // thread's sum of the tile counts of its Gaussians: a thread walks its own
 // Gaussian's super-tiles when there are at most kTbBig of them; more are
-// walked by the whole wavefront, 64 a step.
+// walked by the whole wavefront, 64 a step.  The count, rect and extra word
+// of a thread's next Gaussian are loaded before its current one is walked
+// (the rect and extra unconditionally: words of culled Gaussians are read
+// and ignored), so each step waits on loads issued a step earlier.
 template <class F>
 __device__ __forceinline__ uint32_t st_for_pairs(int g0, int g1, const uint32_t* __restrict__ counts,
                                                  const uint2* __restrict__ rects,
@@ -472,17 +476,22 @@ __device__ __forceinline__ uint32_t st_for_pairs(int g0, int g1, const uint32_t*
                                                  F fn) {
   const uint32_t lane = threadIdx.x & 63;
   uint32_t fine = 0;
+  uint32_t nc = 0, nw = 0;
+  uint2 nr = make_uint2(0u, 0u);
+  auto fetch = [&](int i) {
+    nc = 0;
+    if (i < g1) {
+      nc = counts[i];
+      nr = rects[i];
+      if (extra) nw = extra[i];
+    }
+  };
+  fetch(g0 + (int)threadIdx.x);
   for (int b0 = g0; b0 < g1; b0 += (int)blockDim.x) {
     const int i = b0 + (int)threadIdx.x;
-    uint32_t c = 0, w = 0;
-    uint2 r = make_uint2(0u, 0u);
-    if (i < g1) {
-      c = counts[i];
-      if (c) {
-        r = rects[i];
-        if (extra) w = extra[i];
-      }
-    }
+    const uint32_t c = nc, w = nw;
+    const uint2 r = nr;
+    fetch(i + (int)blockDim.x);
     fine += c;
     StRect q = c ? st_rect(r, sts) : StRect{0, 0, 0, 0};
     const uint32_t sw = q.x1 - q.x0, sc = sw * (q.y1 - q.y0);
@@ -547,83 +556,58 @@ __global__ __launch_bounds__(kTbThreads) void k_st_count(int n, int chunk, int s
   if (threadIdx.x == 0) bpart[b] = tot;
 }
 
-// In-place scan of a[0, n) by one block: thread t owns the contiguous run
-// [t * per, t * per + per) (per a multiple of 4, so the run is read and
-// written as 16-byte groups, a must be 16-byte aligned), sums it with all of
-// its loads in flight, takes its offset from one block scan of the sums and
-// rewrites the run.  Inclusive, or exclusive.  Returns the total.
-template <bool INCLUSIVE>
-__device__ __forceinline__ uint32_t block_scan_inplace(uint32_t* __restrict__ a, int64_t n, uint32_t* lds) {
-  const int64_t per = (n + 4 * (int64_t)blockDim.x - 1) / (4 * (int64_t)blockDim.x) * 4;
-  const int64_t lo = min(n, per * (int64_t)threadIdx.x), hi = min(n, lo + per);
-  auto load4 = [&](int64_t i) {
-    if (i + 4 <= hi) return *reinterpret_cast<const uint4*>(a + i);
-    return make_uint4(a[i], i + 1 < hi ? a[i + 1] : 0u, i + 2 < hi ? a[i + 2] : 0u, 0u);
-  };
-  uint32_t s = 0;
-#pragma unroll 8
-  for (int64_t i = lo; i < hi; i += 4) {
-    const uint4 v = load4(i);
-    s += (v.x + v.y) + (v.z + v.w);
-  }
-  uint32_t tot;
-  uint32_t run = block_excl_scan(s, &tot, lds);
-#pragma unroll 4
-  for (int64_t i = lo; i < hi; i += 4) {
-    const uint4 v = load4(i);
-    uint4 o;
-    if (INCLUSIVE) o = make_uint4(run + v.x, run + v.x + v.y, run + v.x + v.y + v.z, run + v.x + v.y + v.z + v.w);
-    else o = make_uint4(run, run + v.x, run + v.x + v.y, run + v.x + v.y + v.z);
-    run += (v.x + v.y) + (v.z + v.w);
-    if (i + 4 <= hi) {
-      *reinterpret_cast<uint4*>(a + i) = o;
-    } else {
-      a[i] = o.x;
-      if (i + 1 < hi) a[i + 1] = o.y;
-      if (i + 2 < hi) a[i + 2] = o.z;
-    }
-  }
-  return tot;
-}
-
-// One block: M = the sum of bpart (the pair count of the reference's scan,
-// gausplat.cu:199-201), the overflow flag, and the super-tile totals ->
-// exclusive offsets in place (sttot[st] = the coarse pairs), so super-tile
-// s's coarse list is [sttot[s], sttot[s + 1]).
-__global__ __launch_bounds__(1024) void k_st_plan(int st, uint32_t* __restrict__ sttot, int nb,
-                                                  const uint32_t* __restrict__ bpart,
-                                                  uint32_t* __restrict__ hdr, int64_t cap) {
-  __shared__ uint32_t lds[16];
-  uint32_t m = 0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) m += bpart[b];
-  uint32_t M;
-  block_excl_scan(m, &M, lds);
-  const uint32_t ms = block_scan_inplace<false>(sttot, st, lds);
-  if (threadIdx.x == 0) {
-    sttot[st] = ms;
-    const bool ovf = (int64_t)M > cap;
-    hdr[kHdrM] = M;
-    hdr[kHdrOverflow] = ovf ? 1u : 0u;
-    hdr[kHdrMEff] = ovf ? 0u : M;
-  }
-}
-
-// Coarse scatter: block b writes, for each (super-tile, Gaussian) pair of its
-// Gaussians, the Gaussian's id, tile rect and quantised depth into its slots
-// of the super-tile's coarse list.  Coarse pairs never outnumber tile pairs,
-// so the lists fit in the pair capacity whenever M does.
-__global__ __launch_bounds__(kTbThreads) void k_st_scatter(int n, int chunk, int st, int sts, int sgx,
+// Coarse scatter.  Every block first forms M = the sum of bpart (the pair
+// count of the reference's scan, gausplat.cu:199-201) and the exclusive scan
+// of the super-tile totals in LDS (the same values in every block; block 0
+// publishes M, the overflow flag and the offsets, stoff[st] = the coarse
+// pairs, so super-tile s's coarse list is [stoff[s], stoff[s + 1])).  Then
+// block b writes, for each (super-tile, Gaussian) pair of its Gaussians, the
+// Gaussian's id, tile rect and quantised depth into its slots of the
+// super-tile's coarse list.  Coarse pairs never outnumber tile pairs, so the
+// lists fit in the pair capacity whenever M does.
+__global__ __launch_bounds__(kTbThreads) void k_st_scatter(int n, int chunk, int st, int sts, int sgx, int nb,
                                                            const uint32_t* __restrict__ counts,
                                                            const uint2* __restrict__ rects,
                                                            const uint32_t* __restrict__ dqs,
                                                            const uint32_t* __restrict__ sttot,
                                                            const uint32_t* __restrict__ boff,
-                                                           const uint32_t* __restrict__ hdr,
-                                                           uint4* __restrict__ clist) {
-  if (hdr[kHdrMEff] == 0) return;
+                                                           const uint32_t* __restrict__ bpart,
+                                                           uint32_t* __restrict__ hdr, uint32_t* __restrict__ stoff,
+                                                           int64_t cap, uint4* __restrict__ clist) {
   extern __shared__ uint32_t s_cur[];  // st; entries of super-tiles this block has no pair in are not used
+  __shared__ uint32_t lds[16];
   const int b = blockIdx.x;
-  for (int q = threadIdx.x; q < st; q += kTbThreads) s_cur[q] = sttot[q] + boff[(size_t)b * st + q];
+  uint32_t m = 0;
+  for (int q = threadIdx.x; q < nb; q += kTbThreads) m += bpart[q];
+  for (int q = threadIdx.x; q < st; q += kTbThreads) s_cur[q] = sttot[q];
+  uint32_t M;
+  block_excl_scan(m, &M, lds);  // its barriers also order the s_cur stores before the reads below
+  {
+    const int per = (st + kTbThreads - 1) / kTbThreads;
+    const int lo = min(st, per * (int)threadIdx.x), hi = min(st, lo + per);
+    uint32_t sum = 0;
+    for (int q = lo; q < hi; ++q) sum += s_cur[q];
+    uint32_t ctot;
+    uint32_t run = block_excl_scan(sum, &ctot, lds);
+    for (int q = lo; q < hi; ++q) {
+      const uint32_t v = s_cur[q];
+      s_cur[q] = run;
+      run += v;
+    }
+    __syncthreads();
+    const bool ovf = (int64_t)M > cap;
+    if (b == 0) {
+      for (int q = threadIdx.x; q < st; q += kTbThreads) stoff[q] = s_cur[q];
+      if (threadIdx.x == 0) {
+        stoff[st] = ctot;
+        hdr[kHdrM] = M;
+        hdr[kHdrOverflow] = ovf ? 1u : 0u;
+        hdr[kHdrMEff] = ovf ? 0u : M;
+      }
+    }
+    if (ovf || M == 0) return;
+  }
+  for (int q = threadIdx.x; q < st; q += kTbThreads) s_cur[q] += boff[(size_t)b * st + q];
   __syncthreads();
   const int g0 = b * chunk, g1 = min(n, g0 + chunk);
   st_for_pairs(g0, g1, counts, rects, dqs, (uint32_t)sts, (uint32_t)sgx, [&](uint32_t s, int g, uint2 r, uint32_t dq) {
@@ -645,13 +629,19 @@ __device__ __forceinline__ void st_tiles(uint32_t rx, uint32_t ry, uint32_t tx0,
     for (uint32_t x = ix0; x < ix1; ++x) fn((y << sts) + x);
 }
 
+// Coarse entries a thread of the fine count / scatter loads at once (all its
+// loads in flight before the first entry is walked).
+constexpr int kFineBatch = 4;
+
 // Fine count: one block per super-tile counts the pairs of each of its tiles
-// from its coarse list; toff[tile] = the count (scanned by k_tile_offsets).
+// from its coarse list, toff[tile] = the count, and adds each of its tile
+// rows' count to rowsum[row] (cleared with the header).
 __global__ __launch_bounds__(1024) void k_st_fine_count(int sts, int sgx, int gx, int gy,
-                                                        const uint32_t* __restrict__ sttot,
+                                                        const uint32_t* __restrict__ stoff,
                                                         const uint32_t* __restrict__ hdr,
                                                         const uint4* __restrict__ clist,
-                                                        uint32_t* __restrict__ toff) {
+                                                        uint32_t* __restrict__ toff,
+                                                        uint32_t* __restrict__ rowsum) {
   __shared__ uint32_t cnt[kStMax * kStMax];
   if (hdr[kHdrMEff] == 0) return;
   const int s = blockIdx.x;
@@ -659,30 +649,30 @@ __global__ __launch_bounds__(1024) void k_st_fine_count(int sts, int sgx, int gx
   const uint32_t tx0 = (uint32_t)(s % sgx) << sts, ty0 = (uint32_t)(s / sgx) << sts;
   if (threadIdx.x < nt) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t start = sttot[s], end = sttot[s + 1];
-  for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
-    const uint4 c = clist[e];
-    st_tiles(c.y, c.z, tx0, ty0, sts, [&](uint32_t t) { atomicAdd(&cnt[t], 1u); });
+  const uint32_t start = stoff[s], end = stoff[s + 1];
+  for (uint32_t e0 = start + threadIdx.x; e0 < end; e0 += kFineBatch * blockDim.x) {
+    uint4 c[kFineBatch];
+#pragma unroll
+    for (int u = 0; u < kFineBatch; ++u) {
+      const uint32_t e = e0 + u * blockDim.x;
+      c[u] = e < end ? clist[e] : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < kFineBatch; ++u)
+      if (e0 + u * blockDim.x < end)
+        st_tiles(c[u].y, c[u].z, tx0, ty0, sts, [&](uint32_t t) { atomicAdd(&cnt[t], 1u); });
   }
   __syncthreads();
   if (threadIdx.x < nt) {
     const uint32_t tx = tx0 + (threadIdx.x & (kst - 1)), ty = ty0 + (threadIdx.x >> sts);
     if (tx < (uint32_t)gx && ty < (uint32_t)gy) toff[ty * gx + tx] = cnt[threadIdx.x];
+    // tiles past the grid's edge hold no pairs (tile rects are clipped to the grid)
+    if ((threadIdx.x & (kst - 1)) == 0 && ty < (uint32_t)gy) {
+      uint32_t r = 0;
+      for (uint32_t x = 0; x < kst; ++x) r += cnt[threadIdx.x + x];
+      if (r) atomicAdd(&rowsum[ty], r);
+    }
   }
-}
-
-// One block: the tile counts -> exclusive offsets in place, toff[tiles] = M
-// (all zero when the pairs overflowed the capacity: k_st_fine_count then
-// wrote nothing).
-__global__ __launch_bounds__(1024) void k_tile_offsets(int tiles, uint32_t* __restrict__ toff,
-                                                       const uint32_t* __restrict__ hdr) {
-  __shared__ uint32_t lds[16];
-  if (hdr[kHdrMEff] == 0) {
-    for (int e = threadIdx.x; e <= tiles; e += blockDim.x) toff[e] = 0u;
-    return;
-  }
-  const uint32_t tot = block_scan_inplace<false>(toff, tiles, lds);
-  if (threadIdx.x == 0) toff[tiles] = tot;
 }
 
 // Fine scatter: one block per super-tile writes the sort keys of its tiles'
@@ -691,26 +681,59 @@ __global__ __launch_bounds__(1024) void k_tile_offsets(int tiles, uint32_t* __re
 // with the coarse entries, so the in-tile sorts read them in list order
 // instead of gathering depths per pair.
 __global__ __launch_bounds__(1024) void k_st_fine_scatter(int sts, int sgx, int gx, int gy,
-                                                          const uint32_t* __restrict__ sttot,
+                                                          const uint32_t* __restrict__ stoff,
                                                           const uint32_t* __restrict__ hdr,
                                                           const uint4* __restrict__ clist,
                                                           const uint32_t* __restrict__ toff,
+                                                          const uint32_t* __restrict__ rowsum,
+                                                          uint32_t* __restrict__ toffx,
                                                           uint64_t* __restrict__ tkeys) {
   __shared__ uint32_t cur[kStMax * kStMax];
-  if (hdr[kHdrMEff] == 0) return;
+  const uint32_t M = hdr[kHdrMEff];
+  if (M == 0) return;
   const int s = blockIdx.x;
   const uint32_t kst = 1u << sts, nt = kst * kst;
   const uint32_t tx0 = (uint32_t)(s % sgx) << sts, ty0 = (uint32_t)(s / sgx) << sts;
-  if (threadIdx.x < nt) {
-    const uint32_t tx = tx0 + (threadIdx.x & (kst - 1)), ty = ty0 + (threadIdx.x >> sts);
-    cur[threadIdx.x] = (tx < (uint32_t)gx && ty < (uint32_t)gy) ? toff[ty * gx + tx] : 0u;
+  // List offsets of the block's tiles, in row-major tile order: wavefront w
+  // takes tile row ty0 + w and adds the row sums of the rows above it, the
+  // counts of the row's tiles left of the super-tile, and the exclusive scan
+  // of the counts inside it.  They go to cur (the fill cursors) and to toffx
+  // for the in-tile sorts; toffx[tiles] = M.
+  const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (w < kst) {
+    const uint32_t ty = ty0 + w;
+    uint32_t c = 0;
+    if (ty < (uint32_t)gy) {
+      uint32_t part = 0;
+      for (uint32_t r0 = 0; r0 < ty; r0 += 64) part += r0 + lane < ty ? rowsum[r0 + lane] : 0u;
+      for (uint32_t x0 = 0; x0 < tx0; x0 += 64) part += x0 + lane < tx0 ? toff[ty * gx + x0 + lane] : 0u;
+      const uint32_t before = __builtin_amdgcn_readlane(wave_incl_scan(part), 63);
+      c = lane < kst && tx0 + lane < (uint32_t)gx ? toff[ty * gx + tx0 + lane] : 0u;
+      const uint32_t off = before + wave_incl_scan(c) - c;
+      if (lane < kst) {
+        cur[w * kst + lane] = off;
+        if (tx0 + lane < (uint32_t)gx) toffx[ty * gx + tx0 + lane] = off;
+      }
+    } else if (lane < kst) {
+      cur[w * kst + lane] = 0u;  // rows past the grid hold no pairs
+    }
   }
+  if (s == 0 && threadIdx.x == 0) toffx[gx * gy] = M;
   __syncthreads();
-  const uint32_t start = sttot[s], end = sttot[s + 1];
-  for (uint32_t e = start + threadIdx.x; e < end; e += blockDim.x) {
-    const uint4 c = clist[e];
-    const uint64_t key = ((uint64_t)c.w << 32) | c.x;
-    st_tiles(c.y, c.z, tx0, ty0, sts, [&](uint32_t t) { tkeys[atomicAdd(&cur[t], 1u)] = key; });
+  const uint32_t start = stoff[s], end = stoff[s + 1];
+  for (uint32_t e0 = start + threadIdx.x; e0 < end; e0 += kFineBatch * blockDim.x) {
+    uint4 c[kFineBatch];
+#pragma unroll
+    for (int u = 0; u < kFineBatch; ++u) {
+      const uint32_t e = e0 + u * blockDim.x;
+      c[u] = e < end ? clist[e] : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < kFineBatch; ++u) {
+      if (e0 + u * blockDim.x >= end) break;
+      const uint64_t key = ((uint64_t)c[u].w << 32) | c[u].x;
+      st_tiles(c[u].y, c[u].z, tx0, ty0, sts, [&](uint32_t t) { tkeys[atomicAdd(&cur[t], 1u)] = key; });
+    }
   }
 }
 
@@ -973,7 +996,7 @@ __global__ __launch_bounds__(64) void k_tb_sort(int tiles, const uint32_t* __res
   if (lane == 0) ranges[tile] = (M <= 1 || end == start) ? make_int2(0, 0) : make_int2((int)start, (int)end);
   if (M == 0 || end == start) return;
   const uint32_t n = end - start;
-  if (n > (uint32_t)kTbSortReg) return;
+  if (n > (uint32_t)kTbSortReg) return;  // left to k_tb_sort_long
   const uint64_t thi = (uint64_t)tile << 32;
   __shared__ uint64_t s[kTbPaddedReg];
   if (n <= 64) tb_wave_sort<1>(tkeys, ids, keys_out, thi, start, n, lane, s);
@@ -1171,25 +1194,27 @@ extern "C" int rgp_bin_scan(int n, int width, int height, void* ws, int64_t capa
   int tile_bytes = 0;
   while (tile_bytes < 4 && ((uint32_t)(L.tiles - 1) >> (8 * tile_bytes)) != 0) ++tile_bytes;
   if (L.tb) {
-    // coarse count -> M and the coarse offsets -> coarse lists -> tile counts -> tile offsets
+    // coarse count -> M, the coarse offsets and the coarse lists -> tile counts and tile-row sums
+    // (the tile offsets are formed by the fine scatter of rgp_bin_sort); with no Gaussians the
+    // cleared header already says M = 0
     uint32_t* sttot = (uint32_t*)(base + L.sttot);
+    uint32_t* stoff = (uint32_t*)(base + L.stoff);
     uint32_t* boff = (uint32_t*)(base + L.boff);
     uint32_t* bpart = (uint32_t*)(base + L.bpart);
     uint4* clist = (uint4*)(base + L.clist);
     uint32_t* toff = (uint32_t*)(base + L.toff);
     const size_t st_lds = (size_t)L.st * 4;
-    hipLaunchKernelGGL(k_st_count, dim3(L.nb), dim3(kTbThreads), st_lds, s, n, L.chunk, L.st, L.sts, L.sgx,
-                       (const uint32_t*)(base + L.counts), (const uint2*)(base + L.rects), sttot, boff, bpart);
-    hipLaunchKernelGGL(k_st_plan, dim3(1), dim3(1024), 0, s, L.st, sttot, L.nb, (const uint32_t*)bpart, hdr, L.cap);
-    if (n > 0 && L.cap > 0)
+    if (n > 0) {
+      hipLaunchKernelGGL(k_st_count, dim3(L.nb), dim3(kTbThreads), st_lds, s, n, L.chunk, L.st, L.sts, L.sgx,
+                         (const uint32_t*)(base + L.counts), (const uint2*)(base + L.rects), sttot, boff, bpart);
       hipLaunchKernelGGL(k_st_scatter, dim3(L.nb), dim3(kTbThreads), st_lds, s, n, L.chunk, L.st, L.sts, L.sgx,
-                         (const uint32_t*)(base + L.counts), (const uint2*)(base + L.rects),
+                         L.nb, (const uint32_t*)(base + L.counts), (const uint2*)(base + L.rects),
                          (const uint32_t*)(base + L.dqs), (const uint32_t*)sttot, (const uint32_t*)boff,
-                         (const uint32_t*)hdr, clist);
-    hipLaunchKernelGGL(k_st_fine_count, dim3(L.st), dim3(1024), 0, s, L.sts, L.sgx, L.gx, L.gy,
-                       (const uint32_t*)sttot,
-                       (const uint32_t*)hdr, (const uint4*)clist, toff);
-    hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, s, L.tiles, toff, (const uint32_t*)hdr);
+                         (const uint32_t*)bpart, hdr, stoff, L.cap, clist);
+      hipLaunchKernelGGL(k_st_fine_count, dim3(L.st), dim3(1024), 0, s, L.sts, L.sgx, L.gx, L.gy,
+                         (const uint32_t*)stoff, (const uint32_t*)hdr, (const uint4*)clist, toff,
+                         (uint32_t*)(base + L.rowsum));
+    }
     return check_launch("rgp_bin_scan");
   }
   if (L.nscan > 0)
@@ -1216,16 +1241,16 @@ extern "C" int rgp_bin_sort(int n, const float* depths, int width, int height, v
   char* base = (char*)ws;
   const uint32_t* hdr = (const uint32_t*)(base + L.header);
   if (L.tb) {
-    const uint32_t* toff = (const uint32_t*)(base + L.toff);
+    const uint32_t* toffx = (const uint32_t*)(base + L.toffx);
     uint64_t* tkeys = (uint64_t*)(base + L.tkeys);
     (void)depths;  // the coarse lists carry the quantised depths written by the preprocess
     hipLaunchKernelGGL(k_st_fine_scatter, dim3(L.st), dim3(1024), 0, s, L.sts, L.sgx, L.gx, L.gy,
-                       (const uint32_t*)(base + L.sttot), (const uint32_t*)hdr, (const uint4*)(base + L.clist),
-                       toff, tkeys);
-    hipLaunchKernelGGL(k_tb_sort, dim3(L.tiles), dim3(64), 0, s, L.tiles, (const uint32_t*)hdr, toff,
-                       (const uint64_t*)tkeys, ids_out, keys_out, (int2*)ranges_out);
-    hipLaunchKernelGGL(k_tb_sort_long, dim3(L.tiles), dim3(256), 0, s, (const uint32_t*)hdr, toff, tkeys, ids_out,
-                       keys_out);
+                       (const uint32_t*)(base + L.stoff), hdr, (const uint4*)(base + L.clist),
+                       (const uint32_t*)(base + L.toff), (const uint32_t*)(base + L.rowsum),
+                       (uint32_t*)toffx, tkeys);
+    hipLaunchKernelGGL(k_tb_sort, dim3(L.tiles), dim3(64), 0, s, L.tiles, hdr, toffx, (const uint64_t*)tkeys,
+                       ids_out, keys_out, (int2*)ranges_out);
+    hipLaunchKernelGGL(k_tb_sort_long, dim3(L.tiles), dim3(256), 0, s, hdr, toffx, tkeys, ids_out, keys_out);
     return check_launch("rgp_bin_sort");
   }
   uint64_t* k0 = (uint64_t*)(base + L.keys0);

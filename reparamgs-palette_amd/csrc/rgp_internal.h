@@ -73,9 +73,9 @@ constexpr int kTbSortLds = 4096;     // keys of one longer list sorted in LDS at
 
 struct BinLayout {
   size_t header, counts, rects, dqw, dqs, partials, dqpart, keys0, keys1, vals0, vals1, hist, totals;
-  size_t sttot, boff, bpart, clist, toff, tkeys, total;
+  size_t sttot, rowsum, stoff, boff, bpart, clist, toff, toffx, tkeys, total;
   int gx, gy, tiles, nscan, nblk_cap;
-  int reset_words;  // words cleared before binning: the header (and sttot)
+  int reset_words;  // words cleared before binning: the header (and sttot, rowsum)
   bool tb;      // tile-bucket path
   int sts, sgx, sgy, st;  // super-tile edge log2 and grid (tile-bucket path)
   int nb;       // coarse count / scatter blocks (tile-bucket path)
@@ -103,14 +103,15 @@ inline BinLayout bin_layout(int n, int64_t cap, int width, int height) {
   size_t o = 0;
   L.header = o;
   L.sttot = o + kHdrWords * 4;
-  L.reset_words = kHdrWords + (L.tb ? L.st + 1 : 0);
+  L.rowsum = L.sttot + (size_t)(L.tb ? L.st + 1 : 0) * 4;
+  L.reset_words = kHdrWords + (L.tb ? L.st + 1 + L.gy : 0);
   o = align256(o + (size_t)L.reset_words * 4);
   L.counts = o; o = align256(o + (size_t)n * 4);
   L.rects = o; o = align256(o + (size_t)n * 8);
   L.dqw = o; o = align256(o + (size_t)((n + 63) / 64) * 4);
   L.dqs = o; o = align256(o + (size_t)n * 4);
   L.keys0 = L.keys1 = L.vals0 = L.vals1 = L.hist = L.totals = L.dqpart = L.partials = 0;
-  L.boff = L.bpart = L.clist = L.toff = L.tkeys = 0;
+  L.stoff = L.boff = L.bpart = L.clist = L.toff = L.toffx = L.tkeys = 0;
   L.nblk_cap = 0;
   if (L.tb) {
     int nb = (n + kTbMinChunk - 1) / kTbMinChunk;
@@ -118,10 +119,12 @@ inline BinLayout bin_layout(int n, int64_t cap, int width, int height) {
     L.nb = nb;
     L.chunk = (n + nb - 1) / nb;
     L.nscan = 0;
+    L.stoff = o; o = align256(o + (size_t)(L.st + 1) * 4);
     L.boff = o; o = align256(o + (size_t)nb * L.st * 4);
     L.bpart = o; o = align256(o + (size_t)nb * 4);
     L.clist = o; o = align256(o + (size_t)L.cap * 16);
     L.toff = o; o = align256(o + (size_t)(L.tiles + 1) * 4);
+    L.toffx = o; o = align256(o + (size_t)(L.tiles + 1) * 4);
     L.tkeys = o; o = align256(o + (size_t)L.cap * 8);
   } else {
     L.nb = 0;

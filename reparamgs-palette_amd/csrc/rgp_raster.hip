@@ -281,6 +281,20 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
                                           layer, contrib, final_tau, counts, strips, s_rec);
 }
 
+// The training forward (strip masks, no contrib image) held to 64 VGPRs, so
+// that eight wavefronts share each SIMD (at K <= 5 it needs no more
+// registers than that; the default allocation took 66, seven wavefronts).
+template <int DIM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_draw_fwd_train(
+    int W, int H, int gx, int tiles, const int2* __restrict__ ranges, const int32_t* __restrict__ ids,
+    const float4* __restrict__ recs, float empty_tau, float* __restrict__ layer, float* __restrict__ final_tau,
+    uint8_t* __restrict__ strips) {
+  __shared__ float4 s_rec[64 * (RecStride<DIM>::f4 + 1)];
+  draw_fwd_tile<DIM, false, true, false>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau,
+                                         layer, (int32_t*)nullptr, final_tau, (unsigned long long*)nullptr, strips,
+                                         s_rec);
+}
+
 // `views` images of one resolution in one launch: block b composites tile
 // t % tiles of view t / tiles (t the XCD-banded position of b); view v reads
 // ranges + v*tiles, ids + v*id_stride, recs + v*rec_stride (float4 units)
@@ -909,7 +923,13 @@ extern "C" int rgp_draw_forward(int width, int height, int dim, const int32_t* r
   int gx, tiles;
   if (int rc = draw_grid("rgp_draw_forward", width, height, dim, &gx, &tiles)) return rc;
   if (!contrib && !strips) return set_error(RGP_ERR_ARG, "rgp_draw_forward: contrib may be null only with strips");
-  if (!contrib) {
+  if (!contrib && dim <= 5) {
+    RGP_DIM_SWITCH(dim, if constexpr (DD <= 5) hipLaunchKernelGGL((k_draw_fwd_train<DD>), dim3(tiles), dim3(64), 0,
+                                                                    s, width, height, gx, tiles,
+                                                                    (const int2*)ranges, ids,
+                                                                    (const float4*)records, empty_tau, layer,
+                                                                    final_tau, strips));
+  } else if (!contrib) {
     RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false, true, false>), dim3(tiles), dim3(64), 0, s,
                                            width, height, gx, tiles, (const int2*)ranges, ids,
                                            (const float4*)records, empty_tau, layer, contrib,

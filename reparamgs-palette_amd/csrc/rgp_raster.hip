@@ -630,28 +630,40 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
     part[k] = 0.f;
     asm volatile("" : "+v"(part[k]));
   }
-  int nid = 0;
-  RecRegs<F4> nxt = load_rec<F4>(recs, ids, rg.x + lane, rg.x, rg.y, &nid);
+  // Only the next batch's ids (and strip masks) are fetched ahead; the
+  // records are gathered at the start of their batch, where the other
+  // wavefronts of the SIMD cover the latency (with the 16 registers a
+  // prefetched record would hold, the kernel fits eight wavefronts per
+  // SIMD).  With strip masks, entries without one are not gathered at all.
+  int nid = rg.x + lane < rg.y ? ids[rg.x + lane] : -1;
   uint32_t nstr = 0;
   if (STRIPS && rg.x + lane < rg.y) nstr = strips[rg.x + lane];
   for (int b0 = rg.x; b0 < rg.y; b0 += 64) {
     const uint32_t live = live_slots(tau);
     if (live == 0) break;
-    __syncthreads();
-    uint32_t smask;
-    if (STRIPS) {
-      s_rec[lane] = nxt.v[0];
-      s_rec[64 + lane] = nxt.v[1];
+    RecRegs<F4> cur;
+    {
+      const bool ok = nid >= 0 && (!STRIPS || nstr != 0u);
+      const float4* src = recs + (size_t)max(nid, 0) * F4;
 #pragma unroll
-      for (int q = 2; q < F4; ++q) s_rec[q * 64 + lane] = nxt.v[q];
-      smask = nstr;
-    } else {
-      smask = stage_rec<F4>(s_rec, lane, nxt, ty);
+      for (int q = 0; q < F4; ++q) cur.v[q] = ok ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     const int sid = nid;  // the staged entries' Gaussians, read per entry with v_readlane
-    __syncthreads();
-    nxt = load_rec<F4>(recs, ids, b0 + 64 + lane, rg.x, rg.y, &nid);
+    const uint32_t sstr = nstr;
+    nid = b0 + 64 + lane < rg.y ? ids[b0 + 64 + lane] : -1;
     if (STRIPS) nstr = b0 + 64 + lane < rg.y ? strips[b0 + 64 + lane] : 0u;
+    __syncthreads();  // previous batch fully read
+    uint32_t smask;
+    if (STRIPS) {
+      s_rec[lane] = cur.v[0];
+      s_rec[64 + lane] = cur.v[1];
+#pragma unroll
+      for (int q = 2; q < F4; ++q) s_rec[q * 64 + lane] = cur.v[q];
+      smask = sstr;
+    } else {
+      smask = stage_rec<F4>(s_rec, lane, cur, ty);
+    }
+    __syncthreads();
     const int cnt = min(64, rg.y - b0);
     for (int jj = 0; jj < cnt; ++jj) {
       const uint32_t slots = entry_bits(smask, jj) & live;

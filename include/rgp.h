@@ -162,8 +162,10 @@ int rgp_pack_records(int n, int dim, const float* us, const float* cinv2ds, cons
  * 4j..4j+3 of the tile) in which some pixel took a contribution; entries after
  * the batch of 64 in which the tile's pixels all stopped are not written.
  * rgp_draw_backward_replay reads it back.  No reference counterpart.
- * contrib may be null when strips is given: the contribution index is then
- * neither tracked nor written (the replay backward does not read it). */
+ * contrib (nullable): when null the contribution index is neither tracked
+ * nor written (the replay backward with strips does not read it, and frames
+ * that only show the images do not need it); layer and final_tau are the
+ * same either way. */
 int rgp_draw_forward(int width, int height, int dim, const int32_t* ranges, const int32_t* ids,
                      const float* records, float empty_tau, float* layer, int32_t* contrib,
                      float* final_tau, uint8_t* strips, hipStream_t stream);
@@ -272,8 +274,8 @@ int rgp_bin_sort_views(int views, int n, const float* depths, int width, int hei
 
 /* rgp_draw_forward of every view in one launch (views x tiles blocks);
  * id_stride = capacity, n the Gaussian count (record stride); strips
- * (nullable) has the stride of ids; contrib may be null when strips is
- * given, as for rgp_draw_forward. */
+ * (nullable) has the stride of ids; contrib is nullable as for
+ * rgp_draw_forward. */
 int rgp_draw_forward_views(int views, int width, int height, int dim, const int32_t* ranges,
                            const int32_t* ids, int64_t id_stride, const float* records, int64_t n,
                            float empty_tau, float* layer, int32_t* contrib, float* final_tau,

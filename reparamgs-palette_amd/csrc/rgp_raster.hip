@@ -281,18 +281,20 @@ __global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles
                                           layer, contrib, final_tau, counts, strips, s_rec);
 }
 
-// The training forward (strip masks, no contrib image) held to 64 VGPRs, so
-// that eight wavefronts share each SIMD (at K <= 5 it needs no more
-// registers than that; the default allocation took 66, seven wavefronts).
-template <int DIM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_draw_fwd_train(
+// The forwards without a contrib image held to 64 VGPRs, so that eight
+// wavefronts share each SIMD: the training forward (STRIPS, the strip masks
+// of the replay backward) and the frame forward (images only).  At K <= 5
+// neither needs more registers than that (the default allocation took 66
+// and 70, seven wavefronts).
+template <int DIM, bool STRIPS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_draw_fwd_lean(
     int W, int H, int gx, int tiles, const int2* __restrict__ ranges, const int32_t* __restrict__ ids,
     const float4* __restrict__ recs, float empty_tau, float* __restrict__ layer, float* __restrict__ final_tau,
     uint8_t* __restrict__ strips) {
-  __shared__ float4 s_rec[64 * (RecStride<DIM>::f4 + 1)];
-  draw_fwd_tile<DIM, false, true, false>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau,
-                                         layer, (int32_t*)nullptr, final_tau, (unsigned long long*)nullptr, strips,
-                                         s_rec);
+  __shared__ float4 s_rec[64 * (RecStride<DIM>::f4 + (STRIPS ? 1 : 0))];
+  draw_fwd_tile<DIM, false, STRIPS, false>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau,
+                                           layer, (int32_t*)nullptr, final_tau, (unsigned long long*)nullptr,
+                                           strips, s_rec);
 }
 
 // `views` images of one resolution in one launch: block b composites tile
@@ -934,13 +936,23 @@ extern "C" int rgp_draw_forward(int width, int height, int dim, const int32_t* r
                                 hipStream_t s) {
   int gx, tiles;
   if (int rc = draw_grid("rgp_draw_forward", width, height, dim, &gx, &tiles)) return rc;
-  if (!contrib && !strips) return set_error(RGP_ERR_ARG, "rgp_draw_forward: contrib may be null only with strips");
-  if (!contrib && dim <= 5) {
-    RGP_DIM_SWITCH(dim, if constexpr (DD <= 5) hipLaunchKernelGGL((k_draw_fwd_train<DD>), dim3(tiles), dim3(64), 0,
-                                                                    s, width, height, gx, tiles,
+  if (!contrib && dim <= 5 && strips) {
+    RGP_DIM_SWITCH(dim, if constexpr (DD <= 5) hipLaunchKernelGGL((k_draw_fwd_lean<DD, true>), dim3(tiles), dim3(64),
+                                                                    0, s, width, height, gx, tiles,
                                                                     (const int2*)ranges, ids,
                                                                     (const float4*)records, empty_tau, layer,
                                                                     final_tau, strips));
+  } else if (!contrib && dim <= 5) {
+    RGP_DIM_SWITCH(dim, if constexpr (DD <= 5) hipLaunchKernelGGL((k_draw_fwd_lean<DD, false>), dim3(tiles),
+                                                                    dim3(64), 0, s, width, height, gx, tiles,
+                                                                    (const int2*)ranges, ids,
+                                                                    (const float4*)records, empty_tau, layer,
+                                                                    final_tau, (uint8_t*)nullptr));
+  } else if (!contrib && !strips) {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false, false, false>), dim3(tiles), dim3(64), 0, s,
+                                           width, height, gx, tiles, (const int2*)ranges, ids,
+                                           (const float4*)records, empty_tau, layer, contrib,
+                                           final_tau, (unsigned long long*)nullptr, (uint8_t*)nullptr));
   } else if (!contrib) {
     RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false, true, false>), dim3(tiles), dim3(64), 0, s,
                                            width, height, gx, tiles, (const int2*)ranges, ids,
@@ -974,9 +986,12 @@ extern "C" int rgp_draw_forward_views(int views, int width, int height, int dim,
                                       float* final_tau, uint8_t* strips, hipStream_t s) {
   int gx, tiles;
   if (int rc = views_grid("rgp_draw_forward_views", views, width, height, dim, id_stride, &gx, &tiles)) return rc;
-  if (!contrib && !strips)
-    return set_error(RGP_ERR_ARG, "rgp_draw_forward_views: contrib may be null only with strips");
-  if (!contrib) {
+  if (!contrib && !strips) {
+    RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd_views<DD, false, false>), dim3(tiles * views), dim3(64), 0, s,
+                                           width, height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
+                                           (const float4*)records, n * RecStride<DD>::f4, empty_tau, layer,
+                                           contrib, final_tau, (uint8_t*)nullptr));
+  } else if (!contrib) {
     RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd_views<DD, true, false>), dim3(tiles * views), dim3(64), 0, s,
                                            width, height, gx, tiles, views, (const int2*)ranges, ids, id_stride,
                                            (const float4*)records, n * RecStride<DD>::f4, empty_tau, layer,

@@ -53,8 +53,10 @@ def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None,
     low rank (dc, A (N,P,r), B (N,r,Q)) otherwise.  The pair buffers keep the
     cached capacity unless exact=True (see ops.splat_pipeline); the results
     do not depend on it.  strips=True also records the strip masks of the
-    replay backward (state["strips"]); with them, contrib=False skips the
-    contrib image (state["contrib"] is None)."""
+    replay backward (state["strips"]); contrib=False skips the contrib image
+    (state["contrib"] is None), which the replay backward and plain frames do
+    not need (without return_state nothing could read it, so it is skipped
+    then too)."""
     require_cuda(pws, alphas, scales, rots)
     dev = pws.device
     n = pws.shape[0]
@@ -72,7 +74,7 @@ def render_weights(pws, alphas, scales, rots, cam, K, shs=None, dc=None, A=None,
 
     layer, contrib, final_tau, ranges, ids, m, _, smask = ops.splat_pipeline(n, W, H, K, records, depths, fill, 0.0,
                                                                              dev, exact=exact, strips=strips,
-                                                                             contrib=contrib)
+                                                                             contrib=contrib and return_state)
     if return_state:
         return layer, depths, dict(records=records, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids,
                                    camd=camd, m=m, mode=mode, P=P_, Q=Q_, rank=r_, strips=smask)
@@ -120,7 +122,8 @@ def render_weights_views(pws, alphas, scales, rots, cams, K, shs=None, dc=None, 
              ptr(ws), ctypes.c_size_t(ws_stride), I64(int(cap)), W, H, stream(dev))
 
     layer, contrib, final_tau, ranges, ids, ms, _, _, smask = ops.splat_pipeline_views(
-        n, V, W, H, K, records, depths, fill, 0.0, dev, exact=exact, strips=strips, contrib=contrib)
+        n, V, W, H, K, records, depths, fill, 0.0, dev, exact=exact, strips=strips,
+        contrib=contrib and return_state)
     if return_state:
         return layer, depths, dict(records=records, contrib=contrib, final_tau=final_tau, ranges=ranges, ids=ids,
                                    camsd=camsd, m=ms, mode=mode, P=P_, Q=Q_, rank=r_, strips=smask)

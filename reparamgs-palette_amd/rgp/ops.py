@@ -113,11 +113,10 @@ def pack_records(us, cinv2ds, alphas, feat):
 def draw_forward(width, height, dim, ranges, ids, records, empty_tau, strips=None, contrib=True):
     """Forward compositing; strips (uint8, one per list position) receives,
     when given, each composited entry's mask of row strips with a
-    contribution, which draw_backward_replay then uses.  contrib=False (only
-    with strips): the contrib image is not formed and None is returned for it."""
+    contribution, which draw_backward_replay then uses.  contrib=False: the
+    contrib image is not formed and None is returned for it (the training
+    forward with strips, and frames that need only the images)."""
     dev = records.device
-    if not contrib and strips is None:
-        raise ValueError("draw_forward: contrib=False needs strips")
     layer = torch.empty((dim, height, width), dtype=torch.float32, device=dev)
     contrib = torch.empty((height, width), dtype=torch.int32, device=dev) if contrib else None
     final_tau = torch.empty((height, width), dtype=torch.float32, device=dev)
@@ -213,8 +212,8 @@ def splat_pipeline(n, width, height, dim, records, depths, ws_fill, empty_tau, d
     pipeline runs again with the grown capacity.  Both give the same M,
     ranges, ids[:M] and images.  strips=True: the forward also writes the
     strip masks of the replay backward (uint8, one per pair slot); they are
-    returned last (None otherwise).  contrib=False (with strips): no contrib
-    image (None in its place)."""
+    returned last (None otherwise).  contrib=False: no contrib image (None in
+    its place)."""
     cap = CAPACITY.get(n, width, height)
     while True:
         ws = new_workspace(n, cap, width, height, device)
@@ -265,10 +264,8 @@ def splat_pipeline_views(n, views, width, height, dim, records, depths, ws_fill,
     contrib (V,H,W), final_tau (V,H,W), ranges (V,tiles,2), ids (V,cap),
     the pair counts per view, the workspace, its per-view stride and the
     strip masks (V,cap) uint8 when strips=True (else None); contrib=False
-    (with strips) returns None for contrib.  The overflow check follows
-    splat_pipeline's (exact=False: after the draw)."""
-    if not contrib and not strips:
-        raise ValueError("splat_pipeline_views: contrib=False needs strips")
+    returns None for contrib.  The overflow check follows splat_pipeline's
+    (exact=False: after the draw)."""
     gx, gy = grid_dims(width, height)
     cap = CAPACITY.get(n, width, height)
     while True:

@@ -87,12 +87,12 @@ class SceneSpec:
 
     def slot_bytes(self, width, height, cap_per_gaussian=6.0):
         """Render buffers of one resolution: records, depths, workspace at the
-        planning capacity, ids, ranges, weight image, contrib, final_tau."""
+        planning capacity, ids, ranges, weight image, final_tau."""
         rec = self.n * 4 * (8 + 4 * math.ceil(self.K / 4))
         cap = int(cap_per_gaussian * self.n)
         tiles = ((width + 15) // 16) * ((height + 15) // 16)
         ws = 256 + self.n * 16 + cap * 24 + 256 * 4 * (cap // 4096 + 1)
-        img = width * height * 4 * (self.K + 2)
+        img = width * height * 4 * (self.K + 1)
         return rec + self.n * 4 + ws + cap * 4 + tiles * 8 + img
 
     def footprint(self, resolutions=RESOLUTIONS, views=0):
@@ -169,7 +169,6 @@ class RenderSlot:
         self.ids = torch.empty(max(self.cap, 1), dtype=torch.int32, device=device)
         self.ranges = torch.empty((gx * gy, 2), dtype=torch.int32, device=device)
         self.layer = torch.empty((K, height, width), **f)
-        self.contrib = torch.empty((height, width), dtype=torch.int32, device=device)
         self.final_tau = torch.empty((height, width), **f)
         self.graph = None
         if use_graph:
@@ -194,7 +193,7 @@ class RenderSlot:
         call("rgp_bin_sort", sc.n, ptr(self.depths), W, H, ptr(self.ws), I64(self.cap), ptr(self.ids), None,
              ptr(self.ranges), st)
         call("rgp_draw_forward", W, H, sc.K, ptr(self.ranges), ptr(self.ids), ptr(self.records), 0.0,
-             ptr(self.layer), ptr(self.contrib), ptr(self.final_tau), None, st)
+             ptr(self.layer), None, ptr(self.final_tau), None, st)  # no contrib image: frames only
 
     def render(self, cam_host):
         """Enqueue a render on the current stream with the pose in cam_host
@@ -223,7 +222,6 @@ class BatchSlot:
         self.ids = torch.empty((vmax, max(self.cap, 1)), dtype=torch.int32, device=device)
         self.ranges = torch.empty((vmax, gx * gy, 2), dtype=torch.int32, device=device)
         self.layer = torch.empty((vmax, K, height, width), **f)
-        self.contrib = torch.empty((vmax, height, width), dtype=torch.int32, device=device)
         self.final_tau = torch.empty((vmax, height, width), **f)
         self.hdr_dev = self.ws.view(vmax, self.stride)[:, :8]
         self.use_graph = use_graph
@@ -240,7 +238,7 @@ class BatchSlot:
         call("rgp_bin_sort_views", v, sc.n, ptr(self.depths), W, H, ptr(self.ws), sz, I64(self.cap), ptr(self.ids),
              ptr(self.ranges), st)
         call("rgp_draw_forward_views", v, W, H, sc.K, ptr(self.ranges), ptr(self.ids), I64(self.ids.shape[1]),
-             ptr(self.records), I64(sc.n), 0.0, ptr(self.layer), ptr(self.contrib), ptr(self.final_tau), None, st)
+             ptr(self.records), I64(sc.n), 0.0, ptr(self.layer), None, ptr(self.final_tau), None, st)
 
     def capture(self, v):
         """The graph of the v-view render (captured once; the capture

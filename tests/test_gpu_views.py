@@ -66,6 +66,10 @@ def test_views_full_rank(device, views):
                                                       T(scene.scales, device), T(scene.rots, device), cams,
                                                       scene.K, return_state=True, **coef)
     check_views_forward(scene, cams, layers, depths, st, device, **coef)
+    # without the state the views' compositing forms no contrib images; the layers are the same
+    lay0, _ = gsmodel.render_weights_views(T(scene.pws, device), T(scene.alphas, device), T(scene.scales, device),
+                                           T(scene.rots, device), cams, scene.K, **coef)
+    same(N(lay0), N(layers), "layers without contrib")
     shs = T(scene.shs, device).requires_grad_(True)
     lay, mask = gsmodel.WeightGSFunctionTestViews.apply(T(scene.pws, device), shs, T(scene.alphas, device),
                                                          T(scene.scales, device), T(scene.rots, device), cams,

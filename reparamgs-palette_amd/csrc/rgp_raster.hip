@@ -266,35 +266,35 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
   }
 }
 
+// Wavefronts per SIMD the compositing kernels are compiled for: the most at
+// which the register allocation of the instance still needs no spills (every
+// instance compiled at 8, 7, 6 and 5 wavefronts).  These loops are VALU-bound
+// with an LDS round trip per staged batch and a record gather per batch, and
+// more wavefronts hide more of that latency: the training forward and the
+// replay backward at K = 5 went from seven wavefronts (66 and 71 VGPRs) to
+// eight (64) without extra instructions.
+template <int DIM, bool COUNT, bool CONT>
+constexpr int fwd_waves() {
+  if (COUNT) return DIM <= 3 ? 8 : DIM == 4 ? 7 : DIM <= 6 ? 6 : 5;
+  if (!CONT) return DIM <= 5 ? 8 : DIM <= 7 ? 7 : 6;
+  return DIM <= 4 ? 8 : DIM <= 6 ? 7 : 6;
+}
+template <int DIM, bool STRIPS>
+constexpr int bwd_waves() {
+  if (STRIPS) return DIM <= 5 ? 8 : DIM <= 7 ? 7 : 6;
+  return DIM <= 4 ? 8 : DIM <= 6 ? 7 : DIM == 7 ? 6 : 5;
+}
+
 template <int DIM, bool COUNT = false, bool STRIPS = false, bool CONT = true>
-__global__ __launch_bounds__(64) void k_draw_fwd(int W, int H, int gx, int tiles,
-                                                 const int2* __restrict__ ranges,
-                                                 const int32_t* __restrict__ ids,
-                                                 const float4* __restrict__ recs, float empty_tau,
-                                                 float* __restrict__ layer,
-                                                 int32_t* __restrict__ contrib,
-                                                 float* __restrict__ final_tau,
-                                                 unsigned long long* __restrict__ counts,
-                                                 uint8_t* __restrict__ strips) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(fwd_waves<DIM, COUNT, CONT>())))
+void k_draw_fwd(
+    int W, int H, int gx, int tiles,
+    const int2* __restrict__ ranges, const int32_t* __restrict__ ids, const float4* __restrict__ recs,
+    float empty_tau, float* __restrict__ layer, int32_t* __restrict__ contrib, float* __restrict__ final_tau,
+    unsigned long long* __restrict__ counts, uint8_t* __restrict__ strips) {
   __shared__ float4 s_rec[64 * (RecStride<DIM>::f4 + (STRIPS ? 1 : 0))];
   draw_fwd_tile<DIM, COUNT, STRIPS, CONT>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau,
                                           layer, contrib, final_tau, counts, strips, s_rec);
-}
-
-// The forwards without a contrib image held to 64 VGPRs, so that eight
-// wavefronts share each SIMD: the training forward (STRIPS, the strip masks
-// of the replay backward) and the frame forward (images only).  At K <= 5
-// neither needs more registers than that (the default allocation took 66
-// and 70, seven wavefronts).
-template <int DIM, bool STRIPS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_draw_fwd_lean(
-    int W, int H, int gx, int tiles, const int2* __restrict__ ranges, const int32_t* __restrict__ ids,
-    const float4* __restrict__ recs, float empty_tau, float* __restrict__ layer, float* __restrict__ final_tau,
-    uint8_t* __restrict__ strips) {
-  __shared__ float4 s_rec[64 * (RecStride<DIM>::f4 + (STRIPS ? 1 : 0))];
-  draw_fwd_tile<DIM, false, STRIPS, false>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, empty_tau,
-                                           layer, (int32_t*)nullptr, final_tau, (unsigned long long*)nullptr,
-                                           strips, s_rec);
 }
 
 // `views` images of one resolution in one launch: block b composites tile
@@ -303,14 +303,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) void k_
 // and writes the v-th (DIM,H,W) layer and (H,W) contrib / final_tau (and,
 // STRIPS, its strip masks at strips + v*id_stride).
 template <int DIM, bool STRIPS = false, bool CONT = true>
-__global__ __launch_bounds__(64) void k_draw_fwd_views(int W, int H, int gx, int tiles, int views,
-                                                       const int2* __restrict__ ranges,
-                                                       const int32_t* __restrict__ ids, int64_t id_stride,
-                                                       const float4* __restrict__ recs, int64_t rec_stride,
-                                                       float empty_tau, float* __restrict__ layer,
-                                                       int32_t* __restrict__ contrib,
-                                                       float* __restrict__ final_tau,
-                                                       uint8_t* __restrict__ strips) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(fwd_waves<DIM, false, CONT>())))
+void k_draw_fwd_views(
+    int W, int H, int gx, int tiles, int views,
+    const int2* __restrict__ ranges, const int32_t* __restrict__ ids, int64_t id_stride,
+    const float4* __restrict__ recs, int64_t rec_stride, float empty_tau, float* __restrict__ layer,
+    int32_t* __restrict__ contrib, float* __restrict__ final_tau, uint8_t* __restrict__ strips) {
   __shared__ float4 s_rec[64 * (RecStride<DIM>::f4 + (STRIPS ? 1 : 0))];
   const int t = xcd_tile(blockIdx.x, tiles * views);
   const int v = t / tiles, tile = t - v * tiles;
@@ -703,13 +701,11 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
 }
 
 template <int DIM, bool STRIPS>
-__global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, int tiles,
-                                                        const int2* __restrict__ ranges,
-                                                        const int32_t* __restrict__ ids,
-                                                        const float4* __restrict__ recs,
-                                                        const uint8_t* __restrict__ strips,
-                                                        const float* __restrict__ dlayer,
-                                                        float* __restrict__ dfeat) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(bwd_waves<DIM, STRIPS>())))
+void k_draw_bwd_replay(
+    int W, int H, int gx, int tiles,
+    const int2* __restrict__ ranges, const int32_t* __restrict__ ids, const float4* __restrict__ recs,
+    const uint8_t* __restrict__ strips, const float* __restrict__ dlayer, float* __restrict__ dfeat) {
   __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
   draw_bwd_replay_tile<DIM, STRIPS>(W, H, gx, xcd_tile(blockIdx.x, tiles), ranges, ids, recs, strips, dlayer, dfeat,
                                     s_rec);
@@ -720,13 +716,12 @@ __global__ __launch_bounds__(64) void k_draw_bwd_replay(int W, int H, int gx, in
 // strips + v*id_stride) and adds into its own gradient rows dfeat + v*n*DIM
 // (the SH chain behind them depends on the view).
 template <int DIM, bool STRIPS>
-__global__ __launch_bounds__(64) void k_draw_bwd_replay_views(int W, int H, int gx, int tiles, int views,
-                                                              const int2* __restrict__ ranges,
-                                                              const int32_t* __restrict__ ids, int64_t id_stride,
-                                                              const float4* __restrict__ recs, int64_t rec_stride,
-                                                              const uint8_t* __restrict__ strips,
-                                                              const float* __restrict__ dlayer,
-                                                              float* __restrict__ dfeat, int64_t n) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(bwd_waves<DIM, STRIPS>())))
+void k_draw_bwd_replay_views(
+    int W, int H, int gx, int tiles, int views,
+    const int2* __restrict__ ranges, const int32_t* __restrict__ ids, int64_t id_stride,
+    const float4* __restrict__ recs, int64_t rec_stride, const uint8_t* __restrict__ strips,
+    const float* __restrict__ dlayer, float* __restrict__ dfeat, int64_t n) {
   __shared__ float4 s_rec[64 * RecStride<DIM>::f4];
   const int t = xcd_tile(blockIdx.x, tiles * views);
   const int v = t / tiles, tile = t - v * tiles;
@@ -936,19 +931,7 @@ extern "C" int rgp_draw_forward(int width, int height, int dim, const int32_t* r
                                 hipStream_t s) {
   int gx, tiles;
   if (int rc = draw_grid("rgp_draw_forward", width, height, dim, &gx, &tiles)) return rc;
-  if (!contrib && dim <= 5 && strips) {
-    RGP_DIM_SWITCH(dim, if constexpr (DD <= 5) hipLaunchKernelGGL((k_draw_fwd_lean<DD, true>), dim3(tiles), dim3(64),
-                                                                    0, s, width, height, gx, tiles,
-                                                                    (const int2*)ranges, ids,
-                                                                    (const float4*)records, empty_tau, layer,
-                                                                    final_tau, strips));
-  } else if (!contrib && dim <= 5) {
-    RGP_DIM_SWITCH(dim, if constexpr (DD <= 5) hipLaunchKernelGGL((k_draw_fwd_lean<DD, false>), dim3(tiles),
-                                                                    dim3(64), 0, s, width, height, gx, tiles,
-                                                                    (const int2*)ranges, ids,
-                                                                    (const float4*)records, empty_tau, layer,
-                                                                    final_tau, (uint8_t*)nullptr));
-  } else if (!contrib && !strips) {
+  if (!contrib && !strips) {
     RGP_DIM_SWITCH(dim, hipLaunchKernelGGL((k_draw_fwd<DD, false, false, false>), dim3(tiles), dim3(64), 0, s,
                                            width, height, gx, tiles, (const int2*)ranges, ids,
                                            (const float4*)records, empty_tau, layer, contrib,

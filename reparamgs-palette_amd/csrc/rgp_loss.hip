@@ -261,8 +261,12 @@ __global__ __launch_bounds__(256) void k_ssim_bwd(int H, int W, const float* __r
                                                   float lam, double n,
                                                   const float* __restrict__ gout,
                                                   float* __restrict__ dimg) {
+  // The horizontal pass writes its sums over the staged maps they were read
+  // from (every thread holds its inputs in registers across a barrier), so
+  // the block needs 22 KB of LDS instead of 38: seven blocks per CU, not four.
   __shared__ float sa[3][kSI][kSI + 1];
-  __shared__ float hb[3][kSI][kST + 1];
+  float (*const hb)[kSI][kST + 1] = reinterpret_cast<float (*)[kSI][kST + 1]>(&sa[0][0][0]);
+  static_assert(3 * kSI * (kST + 1) <= 3 * kSI * (kSI + 1), "the sums fit in the staged maps' space");
   int bx, by, c;
   ssim_block(H, W, &bx, &by, &c);
   const size_t HW = (size_t)H * W;
@@ -273,13 +277,18 @@ __global__ __launch_bounds__(256) void k_ssim_bwd(int H, int W, const float* __r
     ssim_stage<3>(planes, H, W, x0, y0, dst);
   }
   __syncthreads();
-  if (threadIdx.x < kSI * kHG) {
-    const int r = threadIdx.x % kSI, q0 = kHO * (threadIdx.x / kSI);
-    float v[3][kHO + 10];
+  const bool hitem = threadIdx.x < kSI * kHG;
+  const int hr = threadIdx.x % kSI, q0 = kHO * (threadIdx.x / kSI);
+  float hv[3][kHO + 10];
+  if (hitem) {
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
-      for (int j = 0; j < kHO + 10; ++j) v[m][j] = sa[m][r][q0 + j];
+      for (int j = 0; j < kHO + 10; ++j) hv[m][j] = sa[m][hr][q0 + j];
+  }
+  __syncthreads();  // every staged value read before the sums overwrite them
+  if (hitem) {
+    const int r = hr;
 #pragma unroll
     for (int o = 0; o < kHO; ++o) {
       float acc[3] = {0.f, 0.f, 0.f};
@@ -287,7 +296,7 @@ __global__ __launch_bounds__(256) void k_ssim_bwd(int H, int W, const float* __r
       for (int j = 0; j < 11; ++j) {
         const float g = win.g[j];
 #pragma unroll
-        for (int m = 0; m < 3; ++m) acc[m] = fmaf(g, v[m][o + j], acc[m]);
+        for (int m = 0; m < 3; ++m) acc[m] = fmaf(g, hv[m][o + j], acc[m]);
       }
 #pragma unroll
       for (int m = 0; m < 3; ++m) hb[m][r][q0 + o] = acc[m];

@@ -328,7 +328,34 @@ template <int DIM>
 __device__ __forceinline__ void flush_grad(const float* part, float* __restrict__ dfeat, int g, int lane,
                                            int ch, bool owner) {
   const float v = wave_reduce_scatter<DIM>(part, lane);
-  if (owner && v != 0.f) unsafeAtomicAdd(&dfeat[(size_t)g * DIM + ch], v);
+  // The Gaussian's row (g is wavefront-uniform) is formed in scalar registers
+  // and pinned there, so the atomic takes it as its scalar base with the
+  // lane's 32-bit channel offset; otherwise the compiler folds the lane's
+  // channel into a 64-bit vector multiply-add per flush.
+  typedef __attribute__((address_space(1))) float gfloat;
+  gfloat* row = (gfloat*)(dfeat + (size_t)__builtin_amdgcn_readfirstlane(g) * DIM);
+  asm volatile("" : "+s"(row));
+  if (owner && v != 0.f)
+    __hip_atomic_fetch_add(row + (uint32_t)ch, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sets the per-entry partial sums to zero, two registers per v_mov_b64.  The
+// zeros come from asm, so the compiler neither folds them into the first
+// slot's multiply-adds nor rematerialises them on every path through the
+// slots.
+template <int DIM>
+__device__ __forceinline__ void zero_part(float* part) {
+#pragma unroll
+  for (int k = 0; k + 1 < DIM; k += 2) {
+    uint64_t z;
+    asm volatile("v_mov_b64 %0, 0" : "=v"(z));
+    part[k] = __uint_as_float((uint32_t)z);
+    part[k + 1] = __uint_as_float((uint32_t)(z >> 32));
+  }
+  if constexpr ((DIM & 1) != 0) {
+    part[DIM - 1] = 0.f;
+    asm volatile("" : "+v"(part[DIM - 1]));
+  }
 }
 
 template <int DIM>
@@ -619,17 +646,11 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
     fpy[j] = (float)py;
   }
   // The per-entry partial sums are zero outside a flush: they start at zero
-  // and are cleared after each one, so an entry without contributions leaves
-  // them untouched and the accumulation needs no per-entry re-zeroing.  The
-  // zeros are opaque to the compiler (an empty asm), which would otherwise
-  // fold them into the first slot's multiply-adds and rematerialise them on
-  // every path through the slots.
+  // and are cleared after each one (zero_part), so an entry without
+  // contributions leaves them untouched and the accumulation needs no
+  // per-entry re-zeroing.
   float part[DIM];
-#pragma unroll
-  for (int k = 0; k < DIM; ++k) {
-    part[k] = 0.f;
-    asm volatile("" : "+v"(part[k]));
-  }
+  zero_part<DIM>(part);
   // Only the next batch's ids (and strip masks) are fetched ahead; the
   // records are gathered at the start of their batch, where the other
   // wavefronts of the SIMD cover the latency (with the 16 registers a
@@ -690,11 +711,7 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
       }
       if (STRIPS || __any(on)) {
         flush_grad<DIM>(part, dfeat, __builtin_amdgcn_readlane(sid, jj), lane, ch, owner);
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-          part[k] = 0.f;
-          asm volatile("" : "+v"(part[k]));
-        }
+        zero_part<DIM>(part);
       }
     }
   }

@@ -173,6 +173,16 @@ def gauss_half_check(lo, hi):
     return n.value, d.value, f.value
 
 
+def gauss_half_fast_check(lo, hi):
+    """(checked, differing, first differing m) over every float m in [lo, hi]:
+    the kernels' exponent-add form of exp(-m/2) (entries whose cutoff is at
+    most 170) against the range-test-free form."""
+    n, d, f = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_float()
+    lib().orc_gauss_half_fast_check(ctypes.c_float(lo), ctypes.c_float(hi), ctypes.byref(n), ctypes.byref(d),
+                                    ctypes.byref(f))
+    return n.value, d.value, f.value
+
+
 # --------------------------------------------------------------------------
 # per-stage wrappers (shapes follow gsplatcu/ext.cpp:10-124)
 # --------------------------------------------------------------------------

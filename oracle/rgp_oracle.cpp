@@ -87,6 +87,7 @@ template <typename T> static inline T dmul2(double c, T x, T y) {
 static inline float bits2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 
 const float kC2 = 0x1.fffddcp-2f, kC3 = 0x1.5557acp-3f, kC4 = 0x1.57347cp-5f, kC5 = 0x1.106baap-7f;
+const float kRound = 0x1.8p23f;  // 1.5 * 2^23
 
 float rgp_expf(float x) {
   if (!(x > -104.0f)) return (x != x) ? x : 0.0f;
@@ -94,7 +95,9 @@ float rgp_expf(float x) {
   const float LOG2E = 1.44269504088896341f;
   const float LN2_HI = 0.693145751953125f;        // 0x3f317200
   const float LN2_LO = 1.42860676533018570e-06f;  // ln2 - LN2_HI
-  float n = rintf(x * LOG2E);
+  // n: the exact product x log2e rounded once to the nearest integer, read off
+  // kRound + n (floats in [2^23, 2^24) are the integers)
+  const float n = fmaf(x, LOG2E, kRound) - kRound;
   float r = fmaf(n, -LN2_HI, x);
   r = fmaf(n, -LN2_LO, r);
   float p = kC5;
@@ -110,16 +113,18 @@ float rgp_expf(float x) {
 }
 
 // exp(-m/2) as the compositing kernels form it from the Mahalanobis distance
-// m >= 0 without a range test: n = rint(m (-log2e/2)), the reduction against
-// 2 ln2 (head and tail), the Horner partials of rgp_expf with the
+// m >= 0 without a range test: m capped at 1000 (exp(-m/2) is 0 from m = 208
+// on), n = m (-log2e/2) rounded once to the nearest integer, the reduction
+// against 2 ln2 (head and tail), the Horner partials of rgp_expf with the
 // coefficients scaled by (-1/2)^k, and ldexp by n.  This is rgp_expf(-0.5 m)
 // with every intermediate scaled by a power of two; orc_gauss_half_check
 // compares the two on every float of an interval (test infrastructure: it
 // pins the kernels' use of this form for every distance below the largest
 // cutoff a finite opacity can have).
 float gauss_half_form(float m) {
-  const float n = rintf(m * (-0.5f * 1.44269504088896341f));
-  float r = fmaf(n, 2.0f * 0.693145751953125f, m);
+  const float mm = fminf(m, 1000.0f);
+  const float n = fmaf(mm, -0.5f * 1.44269504088896341f, kRound) - kRound;
+  float r = fmaf(n, 2.0f * 0.693145751953125f, mm);
   r = fmaf(n, 2.0f * 1.42860676533018570e-06f, r);
   float t = kC5 * -0.03125f;
   t = fmaf(t, r, kC4 * 0.0625f);
@@ -128,6 +133,28 @@ float gauss_half_form(float m) {
   t = fmaf(t, r, -0.5f);
   t = fmaf(t, r, 1.0f);
   return std::ldexp(t, (int)n);
+}
+
+// The kernels' form of gauss_half_form for entries whose cutoff is at most 170:
+// the same n and polynomial t, scaled by adding the low bits of
+// fl(m (-log2e/2) + kRound), shifted left by 23, to t's bit pattern (an
+// exponent-field add, exact while 2^n t is normal).  orc_gauss_half_fast_check
+// compares it with gauss_half_form on every float of an interval.
+float gauss_half_fast_form(float m) {
+  const float s = fmaf(m, -0.5f * 1.44269504088896341f, kRound);
+  const float n = s - kRound;
+  float r = fmaf(n, 2.0f * 0.693145751953125f, m);
+  r = fmaf(n, 2.0f * 1.42860676533018570e-06f, r);
+  float t = kC5 * -0.03125f;
+  t = fmaf(t, r, kC4 * 0.0625f);
+  t = fmaf(t, r, kC3 * -0.125f);
+  t = fmaf(t, r, kC2 * 0.25f);
+  t = fmaf(t, r, -0.5f);
+  t = fmaf(t, r, 1.0f);
+  uint32_t tb, sb;
+  std::memcpy(&tb, &t, 4);
+  std::memcpy(&sb, &s, 4);
+  return bits2f(tb + (sb << 23));
 }
 
 template <typename T> static inline T ex(T x);
@@ -1342,6 +1369,30 @@ void orc_gauss_half_check(float lo, float hi, int64_t* checked, int64_t* differ,
   for (int64_t u = a; u <= (int64_t)b; ++u) {
     const float m = orc::bits2f((uint32_t)u);
     const float g1 = orc::gauss_half_form(m), g2 = orc::rgp_expf(-0.5f * m);
+    uint32_t x1, x2;
+    std::memcpy(&x1, &g1, 4);
+    std::memcpy(&x2, &g2, 4);
+    if (x1 != x2) {
+      ++nd;
+      if ((uint32_t)u < fb) fb = (uint32_t)u;
+    }
+  }
+  *checked = (int64_t)b - (int64_t)a + 1;
+  *differ = nd;
+  *first = nd ? orc::bits2f(fb) : 0.f;
+}
+// The same over every float m in [lo, hi] for gauss_half_fast_form(m) against
+// gauss_half_form(m).
+void orc_gauss_half_fast_check(float lo, float hi, int64_t* checked, int64_t* differ, float* first) {
+  uint32_t a, b;
+  std::memcpy(&a, &lo, 4);
+  std::memcpy(&b, &hi, 4);
+  int64_t nd = 0;
+  uint32_t fb = 0xffffffffu;
+#pragma omp parallel for reduction(+ : nd) reduction(min : fb) schedule(static)
+  for (int64_t u = a; u <= (int64_t)b; ++u) {
+    const float m = orc::bits2f((uint32_t)u);
+    const float g1 = orc::gauss_half_fast_form(m), g2 = orc::gauss_half_form(m);
     uint32_t x1, x2;
     std::memcpy(&x1, &g1, 4);
     std::memcpy(&x2, &g2, 4);

@@ -53,11 +53,12 @@ __device__ __forceinline__ RecRegs<F4> load_rec(const float4* __restrict__ recs,
 
 // Bit of the staged mask word set when the record's cutoff mcut is not at
 // most 170 (or is NaN).  Below it every distance that passes m <= mcut lies
-// in [0, 170], where gauss_of(m) is gauss_half(m), so the range test is a
-// scalar branch per entry instead of a per-lane one.  The forward and the
+// in [0, 170], where gauss_of(m) is gauss_half_fast(m), so the range test is
+// a scalar branch per entry instead of a per-lane one.  The forward and the
 // replay backward use only alpha' = min(0.99, alpha exp(-m/2)), which
-// gauss_half gives exactly for every record (see mcut_of), so they need no
-// test at all; the colour backward also uses exp(-m/2) itself.
+// gauss_half gives exactly for every record (see mcut_of), so a wide entry
+// takes gauss_half and every other one gauss_half_fast; the colour backward
+// also uses exp(-m/2) itself.
 constexpr uint32_t kWideExp = 1u << 4;
 
 // Stages one record in LDS, replacing its ry by the mask of strips of tile
@@ -92,8 +93,9 @@ __device__ __forceinline__ uint32_t entry_bits(uint32_t mask, int jj) {
 // entry's kWideExp bit, so the branch is a scalar one.
 __device__ __forceinline__ float gauss_in(float m, bool wide) {
   if (__builtin_expect(wide, 0)) return gauss_of(m);
-  return gauss_half(m);
+  return gauss_half_fast(m);
 }
+
 
 __device__ __forceinline__ uint32_t uniform_bits(float v) {
   return __builtin_amdgcn_readfirstlane(__float_as_uint(v));
@@ -180,7 +182,8 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
       __syncthreads();
       const int cnt = __builtin_amdgcn_readfirstlane(min(64, rg.y - b0));
       for (int jj = 0; jj < cnt; ++jj) {
-        const uint32_t slots = entry_bits(smask, jj) & live;
+        const uint32_t ebits = entry_bits(smask, jj);
+        const uint32_t slots = ebits & live;
         if (slots == 0) continue;
         const float4 g0 = s_rec[jj];
         const float4 g1 = s_rec[64 + jj];
@@ -210,7 +213,11 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
           const float dy = g0.y - fpy[j];
           const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
           if (tau[j] < 0.0001f || m > g1.z) continue;  // stopped pixel, or beyond the cutoff
-          const float ap = fminf(0.99f, g1.y * gauss_half(m));
+          // exp(-m/2) in the form the entry's cutoff allows (a scalar branch;
+          // the loop stays one copy, whose registers the compiler keeps in place)
+          float e = gauss_half_fast(m);
+          if (__builtin_expect((ebits & kWideExp) != 0u, 0)) e = gauss_half(m);
+          const float ap = fminf(0.99f, g1.y * e);
           if (ap < 0.002f) continue;
           const float ta = tau[j] * ap;
 #pragma unroll
@@ -680,14 +687,15 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
       s_rec[64 + lane] = cur.v[1];
 #pragma unroll
       for (int q = 2; q < F4; ++q) s_rec[q * 64 + lane] = cur.v[q];
-      smask = sstr;
+      smask = sstr | (cur.v[1].z <= 170.0f ? 0u : kWideExp);
     } else {
       smask = stage_rec<F4>(s_rec, lane, cur, ty);
     }
     __syncthreads();
     const int cnt = min(64, rg.y - b0);
     for (int jj = 0; jj < cnt; ++jj) {
-      const uint32_t slots = entry_bits(smask, jj) & live;
+      const uint32_t ebits = entry_bits(smask, jj);
+      const uint32_t slots = ebits & live;
       if (slots == 0) continue;
       const float4 g0 = s_rec[jj];
       const float4 g1 = s_rec[64 + jj];
@@ -700,8 +708,14 @@ __device__ __forceinline__ void draw_bwd_replay_tile(int W, int H, int gx, int t
         if (!(slots & (1u << j))) continue;
         const float dy = g0.y - fpy[j];
         const float m = fmaxf(0.0f, fmaf(tc0, dy, fmaf(g1.x * dy, dy, ta0)));
+        // exp(-m/2) in the form the entry's cutoff allows, formed before the
+        // lanes' tests so that the scalar branch lies outside their divergent
+        // region (a replayed strip has a contributing pixel, so there is no
+        // all-lanes-skip to lose)
+        float e = gauss_half_fast(m);
+        if (__builtin_expect((ebits & kWideExp) != 0u, 0)) e = gauss_half(m);
         if (tau[j] < 0.0001f || m > g1.z) continue;  // stopped pixel, or beyond the cutoff
-        const float ap = fminf(0.99f, g1.y * gauss_half(m));
+        const float ap = fminf(0.99f, g1.y * e);
         if (ap < 0.002f) continue;
         const float ta = tau[j] * ap;
 #pragma unroll

@@ -58,6 +58,20 @@ def test_gauss_half_form_exhaustive():
     assert d == 0, f"{d} differing values, first at m = {first}"
 
 
+def test_gauss_half_fast_form_exhaustive():
+    """For entries whose cutoff is at most 170 the kernels scale the
+    polynomial by adding n << 23 to its bit pattern instead of ldexp.  On
+    every float of [0, 170] (1.12e9 values) that must equal the range-test-
+    free form bit for bit; just above, where the result leaves the normal
+    range, the exponent add must stop agreeing (a check that the interval
+    really is where the shortcut holds, not a vacuous comparison)."""
+    n, d, first = O.gauss_half_fast_check(0.0, 170.0)
+    assert n > 1_100_000_000
+    assert d == 0, f"{d} differing values, first at m = {first}"
+    n, d, first = O.gauss_half_fast_check(170.0, 200.0)
+    assert d > 0 and 174.0 < first < 176.0
+
+
 # --------------------------------------------------------------------------
 # geometry
 # --------------------------------------------------------------------------

@@ -185,23 +185,25 @@ __device__ __forceinline__ void draw_fwd_tile(int W, int H, int gx, int tile, co
         const uint32_t ebits = entry_bits(smask, jj);
         const uint32_t slots = ebits & live;
         if (slots == 0) continue;
-        const float4 g0 = s_rec[jj];
-        const float4 g1 = s_rec[64 + jj];
+        // Byte offset of the entry's staged record (and of its flag word), held
+        // in one register for the record reads and the strips' flag stores
+        // (rematerialised from the scalar jj, it cost one instruction per
+        // contributing strip, and a second copy for the reads one per entry).
+        int foff = 16 * jj;
+        if (STRIPS) asm volatile("" : "+v"(foff));
+        const float4* s_ent = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_rec) + foff);
+        const float4 g0 = s_ent[0];
+        const float4 g1 = s_ent[64];
         const float dx = g0.x - fpx;
         const float ta0 = g0.z * dx * dx;  // (cinv.x dx) dx
         const float tc0 = g0.w * dx;       // (2 cinv.y) dx
         const int idx = b0 + jj - rg.x + 1;  // contrib value of a pixel this entry contributes to
-        // Byte offset of the entry's flag word, held in a register for the
-        // strips' flag stores (rematerialised from the scalar jj, it cost one
-        // instruction per contributing strip).
-        int foff = 16 * jj;
-        if (STRIPS) asm volatile("" : "+v"(foff));
         // The entry's channel values, read once for its strips: the contribution
         // step then waits on no LDS read.
         float wv[DIM];
 #pragma unroll
         for (int q = 0; q < F4 - 2; ++q) {
-          const float4 f = s_rec[(2 + q) * 64 + jj];
+          const float4 f = s_ent[(2 + q) * 64];
           if (4 * q + 0 < DIM) wv[4 * q + 0] = f.x;
           if (4 * q + 1 < DIM) wv[4 * q + 1] = f.y;
           if (4 * q + 2 < DIM) wv[4 * q + 2] = f.z;

@@ -8,6 +8,10 @@ reference call chains of gsplat/gsmodel.py:104-200 on numpy arrays.
 How each function is pinned (tests/test_oracle.py):
   rgp_expf            against libm exp in double: max error <= 2 ulp over a
                       dense sweep of [-104, 88.7]; exact at 0 and on 2^n.
+  gauss_half_form     bit for bit equal to rgp_expf(-m/2) on every float of
+                      [0, 1e7]; gauss_half_fast_form equal to it on every float
+                      of [0, 170] and differing just above 174 (gauss_half_check,
+                      gauss_half_fast_check).
   project             closed form: an unprojection of (u, depth) returns the
                       camera-space point; z < 0.2 culls; FP32 vs FP64.
   cov3d               det(Sigma) = (s0 s1 s2)^2 for unit quaternions, Sigma
